@@ -1,0 +1,197 @@
+/*
+ * puzzlesearch_amd.h -- C ABI of the MI355X (gfx950) state-expansion and
+ * bit-array frontier library.
+ *
+ * This is the boundary the reference's PuzzleGpu plugin crosses: every
+ * function takes plain pointers and sizes, returns an int status (0 = PS_OK)
+ * and never lets a C++ exception escape.  ps_last_error() returns the message
+ * of the most recent failure on the calling thread.  Device pointers are HIP
+ * device allocations (ps_alloc); streams are hipStream_t values passed as
+ * void* (NULL = the default stream).
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   runtime   Puzzles/GPU.h:8-32, Puzzles/GPU.cu:5-58 (ERR, CreateGPUBuffer,
+ *             CreateCudaStream, CopyToGpu, CopyFromGpu)
+ *   expand    PuzzleGpu::ExpandGpu (Puzzles/PuzzleGpu.h:30) and the kernels it
+ *             launches: PancakeExpand / PancakeExpandInSegment /
+ *             PancakeExpandCrossSegment (Puzzles/PancakeGpu.h:5-25),
+ *             GpuSlidingTilePuzzleSimpleExpand and
+ *             GpuSlidingTilePuzzleOptimizedExpand<w,h> (Puzzles/SlidingPuzzleGpu.h:5-18),
+ *             GpuHanoiTowersExpand / GpuHanoiTowersExpandOptimized
+ *             (Puzzles/HanoiTowersGPU.h:5-19)
+ *   bits      BitArray::AndNot / AndNotAndCount / Clear / ScanBits
+ *             (Common/BitArray.h:97-165)
+ *   drivers   InMemoryThreeBitBFS / InMemoryTwoBitBFS (Common/InMemoryBFS.h:7-9)
+ *             run device-resident; DiskBasedOptThreeBitBFS's per-segment
+ *             layer structure (Common/DiskBasedOptThreeBitBFS.cpp:54-144)
+ *             as the segmented/batched mode.
+ */
+#ifndef PUZZLESEARCH_AMD_H
+#define PUZZLESEARCH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define PS_OK 0
+#define PS_ERR_INVALID_ARGUMENT 1
+#define PS_ERR_HIP 2
+#define PS_ERR_OUT_OF_MEMORY 3
+#define PS_ERR_NOT_INITIALIZED 4
+#define PS_ERR_UNSUPPORTED 5
+#define PS_ERR_CAPACITY 6
+#define PS_ERR_INDEX_RANGE 7
+#define PS_ERR_CANCELLED 8
+
+typedef void* ps_stream;
+
+/* ---------------------------------------------------------------- puzzles */
+/* Puzzle descriptor.  kind selects the reference class whose device I/O layout
+ * is reproduced:
+ *   PS_PANCAKE         PancakeOptimizedGPU(size, flags & PS_FLAG_INV_INDEX)
+ *   PS_SLIDING         SlidingTilePuzzleGpu(size = width, height)
+ *   PS_SLIDING_PACKED  SlidingTilePuzzleOptimized<width = size, height>
+ *   PS_HANOI4          FourPegHanoiGPU(size = disks, flags & PS_FLAG_SYMMETRY)
+ *   PS_HANOI4_OPT      FourPegHanoiOptimizedGPU(size = disks, flags & PS_FLAG_SYMMETRY) */
+typedef struct ps_puzzle {
+    int32_t kind;
+    int32_t size;
+    int32_t height;
+    int32_t flags;
+} ps_puzzle;
+
+enum { PS_PANCAKE = 1, PS_SLIDING = 2, PS_SLIDING_PACKED = 3, PS_HANOI4 = 4, PS_HANOI4_OPT = 5 };
+enum { PS_FLAG_SYMMETRY = 1, PS_FLAG_INV_INDEX = 2 };
+/* Pancake n > 12 with 29-bit segments: PancakeOptimizedGPU::Expand picks the
+ * in-segment (ops 0..10) or cross-segment (ops 11..n-2) kernel from ExpandHint
+ * (Puzzles/PancakeOptimizedGpu.cpp:57-79). */
+enum { PS_EXPAND_FULL = 0, PS_EXPAND_IN_SEGMENT = 1, PS_EXPAND_CROSS_SEGMENT = 2 };
+
+typedef struct ps_puzzle_info {
+    int32_t operators_count;     /* Puzzle::OperatorsCount() */
+    int32_t branching_factor;    /* Puzzle::BranchingFactor() */
+    int32_t has_odd_cycles;      /* Puzzle::HasOddLengthCycles() */
+    int32_t reserved;
+    uint64_t indexes_count;      /* Puzzle::IndexesCount() */
+} ps_puzzle_info;
+
+/* Host-only query (no GPU needed).  Mirrors Common/Puzzle.h:22-26. */
+int ps_puzzle_query(const ps_puzzle* p, ps_puzzle_info* out);
+/* Slots per parent written by ps_expand for this mode, or -1 if invalid. */
+int ps_expand_slots(const ps_puzzle* p, int32_t mode);
+
+/* ---------------------------------------------------------------- runtime */
+/* Select and initialise a device (Puzzles/GPU.cu has no explicit init; the
+ * CUDA runtime initialises lazily -- here it is explicit so errors surface). */
+int ps_init(int32_t device);
+int ps_shutdown(void);
+const char* ps_last_error(void);
+int ps_device_count(int32_t* count);
+/* Device name and properties of the current device. */
+int ps_device_props(char* name, int32_t name_cap, int32_t* cu_count, int32_t* lds_bytes_per_block,
+                    uint64_t* hbm_bytes);
+
+int ps_stream_create(ps_stream* out);           /* CreateCudaStream  (Puzzles/GPU.cu:27-31) */
+int ps_stream_destroy(ps_stream s);             /* DestroyCudaStream (Puzzles/GPU.cu:33-35) */
+int ps_stream_sync(ps_stream s);
+int ps_device_sync(void);
+int ps_alloc(void** out, uint64_t bytes);       /* CreateGPUBuffer   (Puzzles/GPU.cu:12-22) */
+int ps_free(void* p);                           /* DestroyGPUBuffer  (Puzzles/GPU.cu:24-26) */
+int ps_host_alloc(void** out, uint64_t bytes);  /* pinned host memory */
+int ps_host_free(void* p);
+/* CopyToGpu (Puzzles/GPU.cu:37-40) */
+int ps_memcpy_h2d_async(void* dst, const void* src, uint64_t bytes, ps_stream s);
+/* CopyFromGpu without the trailing synchronize (Puzzles/GPU.cu:42-45) */
+int ps_memcpy_d2h_async(void* dst, const void* src, uint64_t bytes, ps_stream s);
+int ps_memcpy_d2d_async(void* dst, const void* src, uint64_t bytes, ps_stream s);
+/* Fill with a byte value using a kernel (safe inside captured graphs). */
+int ps_fill_async(void* dst, uint8_t value, uint64_t bytes, ps_stream s);
+
+/* ----------------------------------------------------------- expansion */
+/* PuzzleGpu::ExpandGpu (Puzzles/PuzzleGpu.h:30): d_in[i] = index << ops |
+ * usedOpBits; d_out[i*S + k] = child << ops | opLabel, or ~0 for an empty
+ * slot, S = ps_expand_slots(p, mode).  Bit-identical to the reference kernels
+ * (Puzzles/PancakeGpu.cu:92-177, Puzzles/SlidingPuzzleGpu.cu:25-81,210-270,
+ * Puzzles/HanoiTowersGPU.cu:114-144,250-281). */
+int ps_expand(const ps_puzzle* p, int32_t mode, const uint64_t* d_in, uint64_t* d_out, uint64_t count,
+              ps_stream s);
+/* PancakeOptimizedGPU::CrossSegmentPostProcess -> PancakeCrossSegmentPostProcessGPU
+ * (Puzzles/PancakeOptimizedGpu.cpp:21-34, Puzzles/PancakeGpu.cu:179-193). */
+int ps_pancake_xseg_postprocess(int32_t n, int32_t segment, uint32_t* d_idx, uint64_t count, ps_stream s);
+
+/* ----------------------------------------------------------- bit arrays */
+/* Bit i of an array lives in 64-bit word i/64 at bit i%64 (Common/BitArray.h:81-83). */
+/* a &= ~excl1 [& ~excl2]; *d_count (u64, device) += popcount(a).  excl2 may be NULL.
+ * BitArray::AndNot + AndNotAndCount (Common/BitArray.h:97-125). */
+int ps_bits_andnot_count(uint64_t* d_a, const uint64_t* d_excl1, const uint64_t* d_excl2, uint64_t nwords,
+                         uint64_t* d_count, ps_stream s);
+/* *d_count (device) += popcount of the array (BitArray::BitsCount). */
+int ps_bits_count(const uint64_t* d_a, uint64_t nwords, uint64_t* d_count, ps_stream s);
+/* *d_hash (device) += sum over nonzero words w_j of splitmix64(splitmix64(j + word_offset) ^ w_j). */
+int ps_bits_hash(const uint64_t* d_a, uint64_t nwords, uint64_t word_offset, uint64_t* d_hash, ps_stream s);
+/* BitArray::ScanBits (Common/BitArray.h:137-144) into an index list: writes
+ * the set bit indexes (ascending) of [0, nbits) to d_out and their number to
+ * *d_count (device u64).  Fails with PS_ERR_CAPACITY if more than cap. */
+int ps_bits_scan(const uint64_t* d_a, uint64_t nbits, uint64_t* d_out, uint64_t cap, uint64_t* d_count,
+                 ps_stream s);
+/* Sets bit idx[i] for every i (BitArray::Set in a loop). */
+int ps_bits_set_list(uint64_t* d_a, const uint64_t* d_idx, uint64_t count, ps_stream s);
+
+/* ----------------------------------------------------------- device BFS */
+typedef struct ps_bfs ps_bfs;
+
+enum { PS_BFS_BATCHED = 0, PS_BFS_PLAIN = 1 };
+
+typedef struct ps_bfs_options {
+    int32_t bits;           /* 3: InMemoryThreeBitBFS semantics, 2: InMemoryTwoBitBFS */
+    int32_t segment_bits;   /* 0: one segment; else segments of 2^segment_bits states */
+    int32_t mode;           /* PS_BFS_BATCHED or PS_BFS_PLAIN */
+    int32_t max_layers;     /* 0: until a layer is empty */
+    uint64_t index_count;   /* 0: Puzzle::IndexesCount() */
+    int32_t tile_bits_log2; /* 0: tuned default; else LDS window of 2^k states (-1: no window) */
+    int32_t block_threads;  /* 0: tuned default */
+    int32_t grid_blocks;    /* 0: tuned default (persistent grid) */
+    int32_t hash;           /* 1: per-layer frontier hashes */
+    int32_t graph;          /* 1: capture the batched layer loop in a hipGraph (default on) */
+    int32_t graph_layers;   /* layers per captured graph (0: default 4) */
+    int32_t reserved[2];
+} ps_bfs_options;
+
+typedef struct ps_bfs_stats {
+    uint64_t layers;
+    uint64_t expanded;            /* parents expanded, all layers */
+    uint64_t children;            /* children generated (marks), all layers */
+    uint64_t algorithmic_bytes;   /* see the header of bfs.hip */
+    uint64_t expand_bytes;        /* algorithmic bytes of the expand kernels alone */
+    uint64_t scan_words;          /* frontier words read by the expand kernels */
+    uint64_t expand_launches;
+    uint64_t finalize_launches;
+    double expand_ms;             /* summed event time of the expand kernels */
+    double finalize_ms;
+    double total_ms;              /* event time of the whole run */
+} ps_bfs_stats;
+
+void ps_bfs_options_default(ps_bfs_options* o);
+int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt, ps_stream s, ps_bfs** out);
+int ps_bfs_destroy(ps_bfs* b);
+/* Runs the search from start_index to exhaustion (or max_layers).  counts[l]
+ * and hashes[l] (may be NULL) receive layer l; *n_layers the layer count. */
+int ps_bfs_run(ps_bfs* b, uint64_t start_index, uint64_t* counts, uint64_t* hashes, int32_t cap,
+               int32_t* n_layers);
+/* Stepwise use: reset to a start state, then one layer per call (*count = 0
+ * when the search is exhausted). */
+int ps_bfs_reset(ps_bfs* b, uint64_t start_index);
+int ps_bfs_step(ps_bfs* b, uint64_t* count, uint64_t* hash);
+int ps_bfs_stats_get(const ps_bfs* b, ps_bfs_stats* out);
+/* Device bit array of the current frontier (three-bit) or the 2-bit array. */
+int ps_bfs_frontier(const ps_bfs* b, const uint64_t** d_words, uint64_t* nwords);
+/* HBM bytes ps_bfs_create would allocate for this puzzle and options. */
+int ps_bfs_footprint(const ps_puzzle* p, const ps_bfs_options* opt, uint64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,715 @@
+// Device-resident BFS driver: InMemoryThreeBitBFS / InMemoryTwoBitBFS
+// (Common/InMemoryThreeBitBFS.cpp:4-48, Common/InMemoryTwoBitBFS.cpp:35-80)
+// with the whole layer loop on the GPU, optionally split into segments of
+// 2^segment_bits states as DiskBasedOptThreeBitBFS does
+// (Common/DiskBasedOptThreeBitBFS.cpp:54-144).
+//
+// Two execution modes:
+//  * PS_BFS_BATCHED: per layer one persistent expand kernel over every tile of
+//    every live segment, one persistent finalize kernel, one bookkeeping kernel;
+//    the layer loop is captured in a hipGraph (graph_layers layers per replay)
+//    and the host only polls the done flag between replays.
+//  * PS_BFS_PLAIN: the host walks the segments like the reference's segment
+//    loop, launching one expand and one finalize kernel per live segment per
+//    layer.
+//
+// Algorithmic bytes (for the roofline) per layer: 8 bytes per cur word scanned,
+// 8 bytes per child generated (one 32-bit read-modify-write), and for each
+// finalized word 8*(4 or 5) bytes (read new, old [, cur], write new, old), or 8
+// bytes for a word of old that is only cleared.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <vector>
+
+#include "bfs_launch.h"
+#include "ps_internal.h"
+
+using psk::BfsDevState;
+
+namespace {
+
+uint64_t host_splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+uint64_t host_word_hash(uint64_t j, uint64_t w) { return w ? host_splitmix64(host_splitmix64(j) ^ w) : 0; }
+
+constexpr int MAX_LAYERS_CAP = 1 << 16;
+
+}  // namespace
+
+struct ps_bfs {
+    ps_puzzle puzzle{};
+    ps_bfs_options opt{};
+    ps_puzzle_info info{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int bits = 3;
+    uint64_t nbits = 0, nwords = 0;
+    uint64_t* arr[3] = {nullptr, nullptr, nullptr};
+    uint32_t nseg = 1;
+    int seg_shift = 63;
+    uint64_t seg_words = 0;
+    unsigned long long* seg_count = nullptr;
+    uint32_t* touched = nullptr;
+    unsigned long long* layer_counts = nullptr;
+    unsigned long long* layer_hashes = nullptr;
+    BfsDevState* st = nullptr;
+    BfsDevState* h_st = nullptr;  // pinned mirror
+    unsigned long long* h_seg = nullptr;  // pinned, 3*nseg
+    uint32_t* h_touched = nullptr;        // pinned
+    int max_layers = MAX_LAYERS_CAP;
+    // expand geometry
+    int use_window = 1;
+    uint64_t tile_words = 0, ntiles = 0;
+    int tpb_expand = 512;
+    uint32_t grid_expand = 0;
+    size_t lds_expand = 0;
+    // finalize geometry
+    int tpb_fin = 256;
+    uint32_t grid_fin = 0;
+    uint64_t chunks_per_seg = 0, nchunks = 0;
+    // two-bit geometry
+    uint32_t grid_bfs2 = 0;
+    // graph
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_layers = 4;
+    // events for timing
+    hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<size_t, size_t>> expand_spans, fin_spans;
+    bool collect_kernel_times = true;
+    ps_bfs_stats stats{};
+    int layers_done = 0;
+    uint64_t start_index = 0;
+};
+
+namespace ps {
+
+namespace {
+
+int hip_cu_count() {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+    return prop.multiProcessorCount;
+}
+
+size_t hip_lds_limit() {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 65536;
+    return prop.sharedMemPerBlock;
+}
+
+int take_event(ps_bfs* b, hipEvent_t* out, size_t* idx) {
+    if (b->ev_used == b->ev_pool.size()) {
+        hipEvent_t e;
+        PS_HIP_CHECK(hipEventCreate(&e));
+        b->ev_pool.push_back(e);
+    }
+    *idx = b->ev_used;
+    *out = b->ev_pool[b->ev_used++];
+    return PS_OK;
+}
+
+int record(ps_bfs* b, size_t* idx) {
+    if (!b->collect_kernel_times) return PS_OK;
+    hipEvent_t e;
+    if (int r = take_event(b, &e, idx)) return r;
+    PS_HIP_CHECK(hipEventRecord(e, b->stream));
+    return PS_OK;
+}
+
+psk::ExpandArgs expand_args(ps_bfs* b) {
+    psk::ExpandArgs a{};
+    a.prm = params_of(b->puzzle);
+    for (int i = 0; i < 3; ++i) a.bits[i] = b->arr[i];
+    a.nbits = b->nbits;
+    a.nwords = b->nwords;
+    a.tile_words = b->tile_words;
+    a.tile_begin = 0;
+    a.tile_end = b->ntiles;
+    a.word_lo = 0;
+    a.word_hi = b->nwords;
+    a.seg_shift = b->seg_shift;
+    a.nseg = b->nseg;
+    a.seg_count = b->seg_count;
+    a.touched = b->touched;
+    a.st = b->st;
+    a.persistent = 1;
+    a.use_window = b->use_window;
+    return a;
+}
+
+psk::FinalizeArgs finalize_args(ps_bfs* b) {
+    psk::FinalizeArgs f{};
+    for (int i = 0; i < 3; ++i) f.bits[i] = b->arr[i];
+    f.nwords = b->nwords;
+    f.seg_words = b->seg_words;
+    f.chunks_per_seg = b->chunks_per_seg;
+    f.chunk_begin = 0;
+    f.chunk_end = b->nchunks;
+    f.nseg = b->nseg;
+    f.odd = b->info.has_odd_cycles;
+    f.hash = b->opt.hash;
+    f.persistent = 1;
+    f.seg_count = b->seg_count;
+    f.touched = b->touched;
+    f.st = b->st;
+    return f;
+}
+
+psk::LayerEndArgs layer_end_args(ps_bfs* b) {
+    psk::LayerEndArgs e{};
+    e.nseg = b->nseg;
+    e.seg_count = b->seg_count;
+    e.touched = b->touched;
+    e.layer_counts = b->layer_counts;
+    e.layer_hashes = b->layer_hashes;
+    e.st = b->st;
+    e.bits = b->bits;
+    return e;
+}
+
+int launch_layer_end(ps_bfs* b) {
+    hipLaunchKernelGGL(psk::k_bfs_layer_end<1024>, dim3(1), dim3(1024), 0, b->stream, layer_end_args(b));
+    PS_HIP_CHECK(hipGetLastError());
+    return PS_OK;
+}
+
+// One batched layer: persistent expand, persistent finalize, bookkeeping.
+int launch_batched_layer(ps_bfs* b, bool timed) {
+    if (b->bits == 2) {
+        psk::Bfs2Args a2{};
+        a2.prm = params_of(b->puzzle);
+        a2.arr = b->arr[0];
+        a2.nbits = b->nbits;
+        a2.nwords = b->nwords;
+        a2.tile_words = b->tile_words;
+        a2.ntiles = b->ntiles;
+        a2.st = b->st;
+        size_t e0 = 0, e1 = 0;
+        if (timed)
+            if (int r = record(b, &e0)) return r;
+        LaunchCfg c{256, b->grid_bfs2, (size_t)4 * psk::QCAP * 4, b->stream};
+        if (int r = bfs2_layer(b->puzzle, a2, c, nullptr)) return r;
+        if (timed) {
+            if (int r = record(b, &e1)) return r;
+            if (b->collect_kernel_times) b->expand_spans.push_back({e0, e1});
+        }
+        b->stats.expand_launches++;
+        if (b->opt.hash) {
+            uint32_t g = (uint32_t)std::min<uint64_t>((b->nwords + 1023) / 1024, 4096);
+            if (g == 0) g = 1;
+            hipLaunchKernelGGL(psk::k_bfs2_hash<256>, dim3(g), dim3(256), 0, b->stream, (const uint64_t*)b->arr[0],
+                               b->nwords, b->st);
+            PS_HIP_CHECK(hipGetLastError());
+        }
+        return launch_layer_end(b);
+    }
+    size_t e0 = 0, e1 = 0, e2 = 0;
+    if (timed)
+        if (int r = record(b, &e0)) return r;
+    LaunchCfg ce{b->tpb_expand, b->grid_expand, b->lds_expand, b->stream};
+    if (int r = bfs3_expand(b->puzzle, expand_args(b), ce, nullptr)) return r;
+    if (timed)
+        if (int r = record(b, &e1)) return r;
+    hipLaunchKernelGGL(psk::k_bfs3_finalize<256>, dim3(b->grid_fin), dim3(256), 0, b->stream, finalize_args(b));
+    PS_HIP_CHECK(hipGetLastError());
+    if (timed) {
+        if (int r = record(b, &e2)) return r;
+        if (b->collect_kernel_times) {
+            b->expand_spans.push_back({e0, e1});
+            b->fin_spans.push_back({e1, e2});
+        }
+    }
+    b->stats.expand_launches++;
+    b->stats.finalize_launches++;
+    return launch_layer_end(b);
+}
+
+// One plain layer: host reads the per-segment counts and launches one expand
+// and one finalize kernel per segment that needs it.
+int run_plain_layer(ps_bfs* b, bool* done) {
+    const uint32_t L = b->h_st->layer;
+    (void)L;
+    PS_HIP_CHECK(hipMemcpyAsync(b->h_st, b->st, sizeof(BfsDevState), hipMemcpyDeviceToHost, b->stream));
+    PS_HIP_CHECK(hipMemcpyAsync(b->h_seg, b->seg_count, (size_t)3 * b->nseg * 8, hipMemcpyDeviceToHost, b->stream));
+    PS_HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (b->h_st->done) {
+        *done = true;
+        return PS_OK;
+    }
+    const uint32_t layer = b->h_st->layer;
+    const unsigned long long* sc_cur = b->h_seg + (uint64_t)(layer % 3) * b->nseg;
+    const unsigned long long* sc_old = b->h_seg + (uint64_t)((layer + 2) % 3) * b->nseg;
+    size_t e0 = 0, e1 = 0, e2 = 0;
+    if (int r = record(b, &e0)) return r;
+    psk::ExpandArgs a = expand_args(b);
+    a.persistent = 0;
+    for (uint32_t s = 0; s < b->nseg; ++s) {
+        if (b->nseg > 1 && sc_cur[s] == 0) continue;
+        const uint64_t w0 = (uint64_t)s * b->seg_words;
+        const uint64_t w1 = std::min(w0 + b->seg_words, b->nwords);
+        a.word_lo = w0;
+        a.word_hi = w1;
+        a.tile_begin = w0 / b->tile_words;
+        a.tile_end = (w1 + b->tile_words - 1) / b->tile_words;
+        LaunchCfg c{b->tpb_expand, (uint32_t)(a.tile_end - a.tile_begin), b->lds_expand, b->stream};
+        if (int r = bfs3_expand(b->puzzle, a, c, nullptr)) return r;
+        b->stats.expand_launches++;
+    }
+    if (int r = record(b, &e1)) return r;
+    if (b->nseg > 1) {
+        PS_HIP_CHECK(hipMemcpyAsync(b->h_touched, b->touched, (size_t)((b->nseg + 31) / 32) * 4,
+                                    hipMemcpyDeviceToHost, b->stream));
+        PS_HIP_CHECK(hipStreamSynchronize(b->stream));
+    }
+    psk::FinalizeArgs f = finalize_args(b);
+    f.persistent = 0;
+    for (uint32_t s = 0; s < b->nseg; ++s) {
+        if (b->nseg > 1) {
+            const bool touched = (b->h_touched[s >> 5] >> (s & 31)) & 1u;
+            if (!touched && sc_old[s] == 0) continue;
+        }
+        f.chunk_begin = (uint64_t)s * b->chunks_per_seg;
+        f.chunk_end = f.chunk_begin + b->chunks_per_seg;
+        hipLaunchKernelGGL(psk::k_bfs3_finalize<256>, dim3((uint32_t)b->chunks_per_seg), dim3(256), 0, b->stream, f);
+        PS_HIP_CHECK(hipGetLastError());
+        b->stats.finalize_launches++;
+    }
+    if (int r = record(b, &e2)) return r;
+    if (b->collect_kernel_times) {
+        b->expand_spans.push_back({e0, e1});
+        b->fin_spans.push_back({e1, e2});
+    }
+    return launch_layer_end(b);
+}
+
+int build_graph(ps_bfs* b) {
+    if (b->graph_exec) return PS_OK;
+    hipGraph_t graph = nullptr;
+    PS_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = PS_OK;
+    for (int i = 0; i < b->graph_layers && rc == PS_OK; ++i) rc = launch_batched_layer(b, false);
+    hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (rc != PS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    PS_HIP_CHECK(e);
+    hipError_t ie = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    PS_HIP_CHECK(ie);
+    // the launches above were only captured; do not count them
+    b->stats.expand_launches = 0;
+    b->stats.finalize_launches = 0;
+    return PS_OK;
+}
+
+int destroy_bfs(ps_bfs* b) {
+    if (!b) return PS_OK;
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    for (auto e : b->ev_pool) (void)hipEventDestroy(e);
+    if (b->ev_run0) (void)hipEventDestroy(b->ev_run0);
+    if (b->ev_run1) (void)hipEventDestroy(b->ev_run1);
+    for (int i = 0; i < 3; ++i)
+        if (b->arr[i]) (void)hipFree(b->arr[i]);
+    if (b->seg_count) (void)hipFree(b->seg_count);
+    if (b->touched) (void)hipFree(b->touched);
+    if (b->layer_counts) (void)hipFree(b->layer_counts);
+    if (b->layer_hashes) (void)hipFree(b->layer_hashes);
+    if (b->st) (void)hipFree(b->st);
+    if (b->h_st) (void)hipHostFree(b->h_st);
+    if (b->h_seg) (void)hipHostFree(b->h_seg);
+    if (b->h_touched) (void)hipHostFree(b->h_touched);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return PS_OK;
+}
+
+struct Geometry {
+    uint64_t nbits, nwords, seg_words;
+    uint32_t nseg;
+    int seg_shift;
+};
+
+int geometry(const ps_puzzle* p, const ps_bfs_options* o, const ps_puzzle_info& info, Geometry* g) {
+    g->nbits = o->index_count ? o->index_count : info.indexes_count;
+    if (g->nbits == 0) return fail(PS_ERR_INVALID_ARGUMENT, "empty index space");
+    if (o->index_count && o->index_count < info.indexes_count && p->kind != PS_PANCAKE && p->kind != PS_SLIDING)
+        return fail(PS_ERR_INVALID_ARGUMENT, "index_count smaller than the puzzle's index space");
+    if (o->bits == 2) {
+        g->nwords = (g->nbits + 31) / 32;
+        g->nseg = 1;
+        g->seg_words = g->nwords;
+        g->seg_shift = 63;
+        if (o->segment_bits) return fail(PS_ERR_UNSUPPORTED, "the two-bit search runs as one segment");
+        return PS_OK;
+    }
+    g->nwords = (g->nbits + 63) / 64;
+    if (o->segment_bits == 0) {
+        g->nseg = 1;
+        g->seg_words = g->nwords;
+        g->seg_shift = 63;
+    } else {
+        if (o->segment_bits < 12 || o->segment_bits > 40)
+            return fail(PS_ERR_INVALID_ARGUMENT, "segment_bits %d not in [12, 40]", o->segment_bits);
+        g->seg_shift = o->segment_bits;
+        g->seg_words = 1ull << (o->segment_bits - 6);
+        const uint64_t ns = (g->nwords + g->seg_words - 1) / g->seg_words;
+        if (ns > (1ull << 22)) return fail(PS_ERR_INVALID_ARGUMENT, "too many segments (%llu)", (unsigned long long)ns);
+        g->nseg = (uint32_t)ns;
+    }
+    return PS_OK;
+}
+
+uint64_t footprint_bytes(const ps_bfs_options* o, const Geometry& g, int max_layers) {
+    const uint64_t arrays = (o->bits == 2 ? 1 : 3) * g.nwords * 8;
+    const uint64_t meta = 3ull * g.nseg * 8 + ((g.nseg + 31) / 32) * 4 + 2ull * max_layers * 8 + sizeof(BfsDevState);
+    return arrays + meta;
+}
+
+}  // namespace
+
+}  // namespace ps
+
+using namespace ps;
+
+extern "C" {
+
+void ps_bfs_options_default(ps_bfs_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->bits = 3;
+    o->mode = PS_BFS_BATCHED;
+    o->hash = 1;
+    o->graph = 1;
+    o->graph_layers = 4;
+}
+
+int ps_bfs_footprint(const ps_puzzle* p, const ps_bfs_options* opt, uint64_t* bytes) {
+    PS_API_BEGIN
+    if (!p || !opt || !bytes) return fail(PS_ERR_INVALID_ARGUMENT, "NULL argument");
+    ps_puzzle_info info;
+    if (int r = puzzle_query(p, &info)) return r;
+    Geometry g;
+    if (int r = geometry(p, opt, info, &g)) return r;
+    const int ml = opt->max_layers > 0 ? std::min(opt->max_layers, MAX_LAYERS_CAP) : MAX_LAYERS_CAP;
+    *bytes = footprint_bytes(opt, g, ml);
+    return PS_OK;
+    PS_API_END
+}
+
+int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s, ps_bfs** out) {
+    PS_API_BEGIN
+    if (int r = require_runtime()) return r;
+    if (!p || !out) return fail(PS_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    ps_bfs_options opt;
+    if (opt_in) opt = *opt_in;
+    else ps_bfs_options_default(&opt);
+    if (opt.bits != 2 && opt.bits != 3) return fail(PS_ERR_INVALID_ARGUMENT, "bits must be 2 or 3");
+    if (opt.mode != PS_BFS_BATCHED && opt.mode != PS_BFS_PLAIN) return fail(PS_ERR_INVALID_ARGUMENT, "bad mode");
+    if (opt.bits == 2 && opt.mode == PS_BFS_PLAIN) opt.mode = PS_BFS_BATCHED;
+    ps_puzzle_info info;
+    if (int r = puzzle_query(p, &info)) return r;
+    Geometry g;
+    if (int r = geometry(p, &opt, info, &g)) return r;
+    // the puzzle must have a device expander
+    if (int r = dispatch<FAM_ALL>(*p, PS_EXPAND_FULL, [](auto) { return PS_OK; })) return r;
+
+    ps_bfs* b = new ps_bfs();
+    b->puzzle = *p;
+    b->opt = opt;
+    b->info = info;
+    b->bits = opt.bits;
+    b->nbits = g.nbits;
+    b->nwords = g.nwords;
+    b->nseg = g.nseg;
+    b->seg_shift = g.seg_shift;
+    b->seg_words = g.seg_words;
+    b->max_layers = opt.max_layers > 0 ? std::min(opt.max_layers, MAX_LAYERS_CAP) : MAX_LAYERS_CAP;
+    b->graph_layers = opt.graph_layers > 0 ? opt.graph_layers : 4;
+    auto cleanup = [&](int rc) {
+        destroy_bfs(b);
+        return rc;
+    };
+    if (s) {
+        b->stream = stream_of(s);
+    } else {
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(fail(PS_ERR_HIP, "stream creation failed"));
+        b->own_stream = true;
+    }
+    auto dalloc = [&](void** ptr, uint64_t bytes) -> int {
+        hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(PS_ERR_OUT_OF_MEMORY, "hipMalloc(%llu) failed: %s", (unsigned long long)bytes,
+                        hipGetErrorString(e));
+        }
+        return PS_OK;
+    };
+    const int narr = b->bits == 2 ? 1 : 3;
+    for (int i = 0; i < narr; ++i)
+        if (int r = dalloc((void**)&b->arr[i], b->nwords * 8)) return cleanup(r);
+    if (b->bits == 2) b->arr[1] = b->arr[2] = b->arr[0];
+    if (int r = dalloc((void**)&b->seg_count, (uint64_t)3 * b->nseg * 8)) return cleanup(r);
+    if (int r = dalloc((void**)&b->touched, (uint64_t)((b->nseg + 31) / 32) * 4)) return cleanup(r);
+    if (int r = dalloc((void**)&b->layer_counts, (uint64_t)b->max_layers * 8)) return cleanup(r);
+    if (int r = dalloc((void**)&b->layer_hashes, (uint64_t)b->max_layers * 8)) return cleanup(r);
+    if (int r = dalloc((void**)&b->st, sizeof(BfsDevState))) return cleanup(r);
+    if (hipHostMalloc((void**)&b->h_st, sizeof(BfsDevState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_seg, (size_t)3 * b->nseg * 8, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_touched, (size_t)((b->nseg + 31) / 32) * 4 + 4, hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(PS_ERR_OUT_OF_MEMORY, "pinned host allocation failed"));
+    std::memset(b->h_st, 0, sizeof(BfsDevState));
+    if (hipEventCreate(&b->ev_run0) != hipSuccess || hipEventCreate(&b->ev_run1) != hipSuccess)
+        return cleanup(fail(PS_ERR_HIP, "event creation failed"));
+
+    const int cus = hip_cu_count();
+    if (b->bits == 2) {
+        b->use_window = 0;
+        b->tile_words = std::min<uint64_t>(b->nwords, 2048);
+        if (b->tile_words == 0) b->tile_words = 1;
+        b->ntiles = (b->nwords + b->tile_words - 1) / b->tile_words;
+        int occ = 0;
+        psk::Bfs2Args a2{};
+        LaunchCfg c{256, 0, (size_t)4 * psk::QCAP * 4, b->stream};
+        if (int r = bfs2_layer(*p, a2, c, &occ)) return cleanup(r);
+        b->grid_bfs2 = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)occ * cus, (int64_t)b->ntiles));
+    } else {
+        // tile (LDS window) geometry
+        uint64_t block_bits;
+        if (opt.tile_bits_log2 > 0) block_bits = 1ull << std::min(opt.tile_bits_log2, 40);
+        else block_bits = puzzle_locality_block(p);
+        b->use_window = opt.tile_bits_log2 >= 0 ? 1 : 0;
+        if (!b->use_window) block_bits = 1ull << 18;
+        b->tile_words = std::max<uint64_t>(1, block_bits / 64);
+        b->tpb_expand = opt.block_threads ? opt.block_threads : 512;
+        if (b->tpb_expand != 256 && b->tpb_expand != 512)
+            return cleanup(fail(PS_ERR_INVALID_ARGUMENT, "block_threads must be 256 or 512"));
+        const size_t lds_limit = hip_lds_limit();
+        while (b->use_window && b->tile_words > 1 &&
+               psk::bfs3_lds_bytes(b->tpb_expand, b->tile_words, 1, b->nseg) + 2048 > lds_limit)
+            b->tile_words /= 2;
+        if (b->tile_words > b->nwords) b->tile_words = b->nwords;
+        b->ntiles = (b->nwords + b->tile_words - 1) / b->tile_words;
+        b->lds_expand = psk::bfs3_lds_bytes(b->tpb_expand, b->tile_words, b->use_window, b->nseg);
+        int occ = 0;
+        LaunchCfg c{b->tpb_expand, 0, b->lds_expand, b->stream};
+        if (int r = bfs3_expand(*p, psk::ExpandArgs{}, c, &occ)) return cleanup(r);
+        if (occ < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "expand kernel does not fit (LDS %zu bytes)", b->lds_expand));
+        const int64_t want = (int64_t)occ * cus;
+        b->grid_expand = (uint32_t)std::max<int64_t>(1, opt.grid_blocks > 0 ? opt.grid_blocks : std::min<int64_t>(want, (int64_t)b->ntiles));
+        b->chunks_per_seg = (b->seg_words + psk::FIN_CHUNK_WORDS - 1) / psk::FIN_CHUNK_WORDS;
+        b->nchunks = (uint64_t)b->nseg * b->chunks_per_seg;
+        b->grid_fin = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * 8, b->nchunks));
+    }
+    *out = b;
+    return PS_OK;
+    PS_API_END
+}
+
+int ps_bfs_destroy(ps_bfs* b) {
+    PS_API_BEGIN
+    if (!b) return PS_OK;
+    if (runtime_ready()) (void)hipStreamSynchronize(b->stream);
+    return destroy_bfs(b);
+    PS_API_END
+}
+
+int ps_bfs_reset(ps_bfs* b, uint64_t start) {
+    PS_API_BEGIN
+    if (!b) return fail(PS_ERR_INVALID_ARGUMENT, "NULL bfs");
+    if (start >= b->nbits)
+        return fail(PS_ERR_INDEX_RANGE, "start index %llu outside [0, %llu)", (unsigned long long)start,
+                    (unsigned long long)b->nbits);
+    const int narr = b->bits == 2 ? 1 : 3;
+    for (int i = 0; i < narr; ++i)
+        if (int r = launch_fill(b->arr[i], 0, b->nwords * 8, b->stream)) return r;
+    if (int r = launch_fill(b->seg_count, 0, (uint64_t)3 * b->nseg * 8, b->stream)) return r;
+    if (int r = launch_fill(b->touched, 0, (uint64_t)((b->nseg + 31) / 32) * 4, b->stream)) return r;
+    PS_HIP_CHECK(hipStreamSynchronize(b->stream));
+    // start state, layer 0
+    uint64_t word_idx, word;
+    if (b->bits == 2) {
+        word_idx = start / 32;
+        word = 2ull << (2 * (start % 32));  // CUR of layer 0
+    } else {
+        word_idx = start / 64;
+        word = 1ull << (start % 64);
+    }
+    PS_HIP_CHECK(hipMemcpy(b->arr[0] + word_idx, &word, 8, hipMemcpyHostToDevice));
+    if (b->bits == 3) {
+        const uint64_t seg = b->nseg > 1 ? (start >> b->seg_shift) : 0;
+        const unsigned long long one = 1;
+        PS_HIP_CHECK(hipMemcpy(b->seg_count + seg, &one, 8, hipMemcpyHostToDevice));
+    }
+    const unsigned long long c0 = 1, h0 = host_word_hash(word_idx, word);
+    PS_HIP_CHECK(hipMemcpy(b->layer_counts, &c0, 8, hipMemcpyHostToDevice));
+    PS_HIP_CHECK(hipMemcpy(b->layer_hashes, &h0, 8, hipMemcpyHostToDevice));
+    BfsDevState st{};
+    st.max_layers = (uint32_t)b->max_layers;
+    st.layers_recorded = 1;
+    st.done = b->max_layers <= 1 ? 1 : 0;
+    PS_HIP_CHECK(hipMemcpy(b->st, &st, sizeof(st), hipMemcpyHostToDevice));
+    *b->h_st = st;
+    b->stats = ps_bfs_stats{};
+    b->ev_used = 0;
+    b->expand_spans.clear();
+    b->fin_spans.clear();
+    b->layers_done = 1;
+    b->start_index = start;
+    return PS_OK;
+    PS_API_END
+}
+
+// Reads the device state into the stats (call after the stream is idle).
+static int collect_stats(ps_bfs* b) {
+    PS_HIP_CHECK(hipMemcpy(b->h_st, b->st, sizeof(BfsDevState), hipMemcpyDeviceToHost));
+    const BfsDevState& st = *b->h_st;
+    if (st.error) return fail(PS_ERR_INDEX_RANGE, "a child index exceeded the index space (%llu states)",
+                              (unsigned long long)b->nbits);
+    b->stats.layers = st.layers_recorded;
+    b->stats.expanded = st.expanded;
+    b->stats.children = st.children;
+    if (b->bits == 3) {
+        const uint64_t fin_words_bytes = b->info.has_odd_cycles ? 40 : 32;
+        b->stats.algorithmic_bytes =
+            8ull * st.scan_words + 8ull * st.children + fin_words_bytes * st.fin_full_words + 8ull * st.fin_clear_words;
+        b->stats.expand_bytes = 8ull * st.scan_words + 8ull * st.children;
+        b->stats.scan_words = st.scan_words;
+    } else {
+        // two-bit: every layer reads and writes the whole 2-bit array, 8 bytes per child
+        b->stats.algorithmic_bytes = 16ull * b->nwords * (st.layers_recorded) + 8ull * st.children;
+        b->stats.expand_bytes = b->stats.algorithmic_bytes;
+        b->stats.scan_words = b->nwords * st.layers_recorded;
+    }
+    double ems = 0, fms = 0;
+    for (auto& sp : b->expand_spans) {
+        float ms = 0;
+        PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
+        ems += ms;
+    }
+    for (auto& sp : b->fin_spans) {
+        float ms = 0;
+        PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
+        fms += ms;
+    }
+    b->stats.expand_ms = ems;
+    b->stats.finalize_ms = fms;
+    return PS_OK;
+}
+
+int ps_bfs_step(ps_bfs* b, uint64_t* count, uint64_t* hash) {
+    PS_API_BEGIN
+    if (!b) return fail(PS_ERR_INVALID_ARGUMENT, "NULL bfs");
+    if (count) *count = 0;
+    if (hash) *hash = 0;
+    bool done = false;
+    if (b->opt.mode == PS_BFS_PLAIN && b->bits == 3) {
+        if (int r = run_plain_layer(b, &done)) return r;
+    } else {
+        PS_HIP_CHECK(hipMemcpyAsync(b->h_st, b->st, sizeof(BfsDevState), hipMemcpyDeviceToHost, b->stream));
+        PS_HIP_CHECK(hipStreamSynchronize(b->stream));
+        done = b->h_st->done != 0;
+        if (!done)
+            if (int r = launch_batched_layer(b, true)) return r;
+    }
+    PS_HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (int r = collect_stats(b)) return r;
+    const uint64_t rec = b->h_st->layers_recorded;
+    if (done || (int)rec <= b->layers_done) return PS_OK;
+    b->layers_done = (int)rec;
+    unsigned long long c = 0, h = 0;
+    PS_HIP_CHECK(hipMemcpy(&c, b->layer_counts + rec - 1, 8, hipMemcpyDeviceToHost));
+    PS_HIP_CHECK(hipMemcpy(&h, b->layer_hashes + rec - 1, 8, hipMemcpyDeviceToHost));
+    if (count) *count = c;
+    if (hash) *hash = h;
+    return PS_OK;
+    PS_API_END
+}
+
+int ps_bfs_run(ps_bfs* b, uint64_t start, uint64_t* counts, uint64_t* hashes, int32_t cap, int32_t* n_layers) {
+    PS_API_BEGIN
+    if (!b) return fail(PS_ERR_INVALID_ARGUMENT, "NULL bfs");
+    if (int r = ps_bfs_reset(b, start)) return r;
+    const bool use_graph = b->opt.mode == PS_BFS_BATCHED && b->opt.graph;
+    b->collect_kernel_times = !use_graph;
+    if (use_graph)
+        if (int r = build_graph(b)) return r;
+    PS_HIP_CHECK(hipEventRecord(b->ev_run0, b->stream));
+    bool done = false;
+    while (!done) {
+        if (b->opt.mode == PS_BFS_PLAIN && b->bits == 3) {
+            if (int r = run_plain_layer(b, &done)) return r;
+            continue;
+        }
+        if (use_graph) {
+            PS_HIP_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
+            b->stats.expand_launches += b->graph_layers;
+            b->stats.finalize_launches += b->bits == 3 ? b->graph_layers : 0;
+        } else {
+            if (int r = launch_batched_layer(b, true)) return r;
+        }
+        PS_HIP_CHECK(hipMemcpyAsync(b->h_st, b->st, sizeof(BfsDevState), hipMemcpyDeviceToHost, b->stream));
+        PS_HIP_CHECK(hipStreamSynchronize(b->stream));
+        done = b->h_st->done != 0;
+    }
+    PS_HIP_CHECK(hipEventRecord(b->ev_run1, b->stream));
+    PS_HIP_CHECK(hipStreamSynchronize(b->stream));
+    const uint64_t launches_e = b->stats.expand_launches, launches_f = b->stats.finalize_launches;
+    if (int r = collect_stats(b)) return r;
+    b->stats.expand_launches = launches_e;
+    b->stats.finalize_launches = launches_f;
+    float total = 0;
+    PS_HIP_CHECK(hipEventElapsedTime(&total, b->ev_run0, b->ev_run1));
+    b->stats.total_ms = total;
+    const int nl = (int)b->h_st->layers_recorded;
+    b->layers_done = nl;
+    if (n_layers) *n_layers = nl;
+    const int ncopy = std::min(nl, cap);
+    if (counts && ncopy > 0)
+        PS_HIP_CHECK(hipMemcpy(counts, b->layer_counts, (size_t)ncopy * 8, hipMemcpyDeviceToHost));
+    if (hashes && ncopy > 0)
+        PS_HIP_CHECK(hipMemcpy(hashes, b->layer_hashes, (size_t)ncopy * 8, hipMemcpyDeviceToHost));
+    if (nl > cap) return fail(PS_ERR_CAPACITY, "%d layers do not fit in %d entries", nl, cap);
+    return PS_OK;
+    PS_API_END
+}
+
+int ps_bfs_stats_get(const ps_bfs* b, ps_bfs_stats* out) {
+    PS_API_BEGIN
+    if (!b || !out) return fail(PS_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = b->stats;
+    return PS_OK;
+    PS_API_END
+}
+
+int ps_bfs_frontier(const ps_bfs* b, const uint64_t** d_words, uint64_t* nwords) {
+    PS_API_BEGIN
+    if (!b || !d_words || !nwords) return fail(PS_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (b->bits == 2) {
+        *d_words = b->arr[0];
+    } else {
+        const uint32_t L = b->h_st->layer;
+        *d_words = b->arr[L % 3];
+    }
+    *nwords = b->nwords;
+    return PS_OK;
+    PS_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,544 @@
+// Device-resident BFS kernels (three-bit and two-bit), templated on the puzzle
+// functor of device_puzzles.h.  Included by the per-family translation units,
+// which instantiate them; bfs.hip drives them.
+//
+// Three-bit layer L (Common/InMemoryThreeBitBFS.cpp:25-44) uses three bit
+// arrays A[0..2]: cur = A[L%3], new = A[(L+1)%3], old = A[(L+2)%3].
+//   k_bfs3_expand   scan cur -> expand every parent -> set children in new
+//   k_bfs3_finalize new &= ~cur (odd cycles); new &= ~old; count; hash; old = 0
+//   k_bfs3_layer_end bookkeeping: record the layer, rotate roles (layer += 1)
+// The layer index lives in device memory, so the same launches (or a captured
+// graph of them) serve every layer.
+//
+// Expand: the index space is cut into tiles of `tile_words` 64-bit words.  A
+// workgroup takes a tile, keeps a bitmap of that tile's states in LDS, and
+// expands every parent in it.  Children that land inside the tile are set with
+// LDS atomics; the bitmap is OR-ed into HBM once at the end of the tile.  Other
+// children go straight to HBM with a no-return atomicOr.  Parents are found
+// with a per-wave compaction: each lane owns one word, popcounts are prefix
+// summed across the wave, set-bit offsets go into a 256-entry LDS queue, and the
+// queue is expanded 64 parents at a time so every lane does useful work.
+#pragma once
+
+#include "device_puzzles.h"
+#include "ps_internal.h"
+
+namespace psk {
+
+constexpr int QCAP = 256;                 // per-wave compaction queue (entries)
+constexpr uint32_t TOUCH_LDS_MAX = 65536;  // segments tracked in LDS per workgroup
+constexpr uint64_t FIN_CHUNK_WORDS = 8192;  // words per finalize work item
+
+struct BfsDevState {
+    uint32_t layer;      // index of the current frontier layer
+    uint32_t done;       // set when a layer came out empty or max_layers reached
+    uint32_t error;      // bit0: a child index >= nbits was generated
+    uint32_t max_layers;
+    unsigned long long tile_counter;
+    unsigned long long chunk_counter;
+    unsigned long long children;
+    unsigned long long expanded;
+    unsigned long long acc_count;
+    unsigned long long acc_hash;
+    unsigned long long layers_recorded;
+    unsigned long long scan_words;       // cur words read by the expand kernels
+    unsigned long long fin_full_words;   // words finalized (new/cur/old read, new/old written)
+    unsigned long long fin_clear_words;  // words of old cleared without a finalize
+};
+
+struct ExpandArgs {
+    psd::PuzzleParams prm;
+    uint64_t* bits[3];
+    uint64_t nbits;
+    uint64_t nwords;
+    uint64_t tile_words;
+    uint64_t tile_begin, tile_end;
+    uint64_t word_lo, word_hi;
+    int32_t seg_shift;  // log2(bits per segment)
+    uint32_t nseg;
+    unsigned long long* seg_count;  // [3][nseg]
+    uint32_t* touched;              // nseg bits (global)
+    BfsDevState* st;
+    int32_t persistent;
+    int32_t use_window;
+};
+
+struct FinalizeArgs {
+    uint64_t* bits[3];
+    uint64_t nwords;
+    uint64_t seg_words;      // words per segment (nwords when one segment)
+    uint64_t chunks_per_seg;
+    uint64_t chunk_begin, chunk_end;
+    uint32_t nseg;
+    int32_t odd;
+    int32_t hash;
+    int32_t persistent;
+    unsigned long long* seg_count;
+    uint32_t* touched;
+    BfsDevState* st;
+};
+
+struct LayerEndArgs {
+    uint32_t nseg;
+    unsigned long long* seg_count;
+    uint32_t* touched;
+    unsigned long long* layer_counts;
+    unsigned long long* layer_hashes;
+    BfsDevState* st;
+    int32_t bits;  // 3 or 2
+};
+
+struct Bfs2Args {
+    psd::PuzzleParams prm;
+    uint64_t* arr;  // 2 bits per state, state i at bits 2(i%32).. of word i/32
+    uint64_t nbits;
+    uint64_t nwords;
+    uint64_t tile_words;
+    uint64_t ntiles;
+    BfsDevState* st;
+};
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ uint64_t word_hash(uint64_t j, uint64_t w) {
+    return w ? splitmix64(splitmix64(j) ^ w) : 0ull;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ void gor(uint32_t* p, uint32_t v) {
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lor(uint32_t* p, uint32_t v) {
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Block-wide sums of up to three u64 counters, then one atomicAdd each (a
+// NULL destination skips that counter).  Every thread of the block must call it.
+template <int TPB>
+__device__ __forceinline__ void block_add3(unsigned long long a, unsigned long long b, unsigned long long c,
+                                           unsigned long long* ga, unsigned long long* gb,
+                                           unsigned long long* gc) {
+    __shared__ unsigned long long red[3][TPB / 64];
+    __syncthreads();
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+        red[0][wave] = a;
+        red[1][wave] = b;
+        red[2][wave] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long sa = 0, sb = 0, sc = 0;
+        for (int w = 0; w < TPB / 64; ++w) {
+            sa += red[0][w];
+            sb += red[1][w];
+            sc += red[2][w];
+        }
+        if (sa && ga) atomicAdd(ga, sa);
+        if (sb && gb) atomicAdd(gb, sb);
+        if (sc && gc) atomicAdd(gc, sc);
+    }
+}
+
+// LDS bytes the expand kernel needs besides its static shared variables.
+inline size_t bfs3_lds_bytes(int tpb, uint64_t tile_words, int use_window, uint32_t nseg) {
+    size_t b = (size_t)(tpb / 64) * QCAP * 4;
+    if (use_window) b += (size_t)tile_words * 8;
+    if (nseg > 1 && nseg <= TOUCH_LDS_MAX) b += (size_t)((nseg + 31) / 32) * 4;
+    return b;
+}
+
+template <class PZ, int TPB>
+__global__ __launch_bounds__(TPB) void k_bfs3_expand(ExpandArgs a) {
+    extern __shared__ uint32_t smem[];
+    constexpr int NW = TPB / 64;
+    BfsDevState* st = a.st;
+    if (st->done) return;
+    const uint32_t L = st->layer;
+    const uint64_t* __restrict__ cur = a.bits[L % 3];
+    uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
+    const unsigned long long* sc_cur = a.seg_count + (uint64_t)(L % 3) * a.nseg;
+
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const uint32_t win_u32 = a.use_window ? (uint32_t)(a.tile_words * 2) : 0u;
+    uint32_t* win = smem;
+    uint32_t* q = smem + win_u32 + wave * QCAP;
+    const bool track = a.nseg > 1;
+    const uint32_t tl_words = (track && a.nseg <= TOUCH_LDS_MAX) ? (a.nseg + 31) / 32 : 0u;
+    uint32_t* tl = smem + win_u32 + NW * QCAP;
+    __shared__ unsigned long long s_tile;
+    __shared__ uint32_t s_local;
+
+    for (uint32_t i = tid; i < tl_words; i += TPB) tl[i] = 0;
+
+    unsigned long long my_children = 0, my_parents = 0, my_scan = 0;
+    uint64_t iter = 0;
+    for (;;) {
+        if (tid == 0) {
+            if (a.persistent) s_tile = a.tile_begin + atomicAdd(&st->tile_counter, 1ull);
+            else s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
+            s_local = 0;
+        }
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        ++iter;
+        if (tile >= a.tile_end) break;
+        const uint64_t w0 = tile * a.tile_words;
+        const uint64_t w1 = (w0 + a.tile_words < a.nwords) ? w0 + a.tile_words : a.nwords;
+        const uint64_t pw0 = w0 > a.word_lo ? w0 : a.word_lo;
+        const uint64_t pw1 = w1 < a.word_hi ? w1 : a.word_hi;
+        bool live = pw0 < pw1;
+        if (live && track) {
+            const uint64_t s0 = (pw0 * 64) >> a.seg_shift, s1 = (pw1 * 64 - 1) >> a.seg_shift;
+            live = false;
+            for (uint64_t s = s0; s <= s1; ++s) live |= sc_cur[s] != 0;
+        }
+        if (!live) {
+            __syncthreads();
+            continue;
+        }
+        const uint32_t nwin = (uint32_t)(w1 - w0) * 2;
+        if (a.use_window)
+            for (uint32_t i = tid; i < nwin; i += TPB) win[i] = 0;
+        __syncthreads();
+
+        if (tid == 0) my_scan += pw1 - pw0;
+        const uint64_t win_lo = w0 * 64, win_hi = w1 * 64;
+        bool local_any = false;
+        auto mark = [&](uint64_t child) {
+            if (child >= a.nbits) {
+                atomicOr(&st->error, 1u);
+                return;
+            }
+            ++my_children;
+            if (a.use_window && child >= win_lo && child < win_hi) {
+                lor(&win[(child - win_lo) >> 5], 1u << (child & 31));
+                local_any = true;
+            } else {
+                gor(&nxt[child >> 5], 1u << (child & 31));
+                if (track) {
+                    const uint64_t s = child >> a.seg_shift;
+                    if (tl_words) lor(&tl[s >> 5], 1u << (s & 31));
+                    else gor(&a.touched[s >> 5], 1u << (s & 31));
+                }
+            }
+        };
+        auto run_batch = [&](uint32_t n) {
+            for (uint32_t j = lane; j < n; j += 64) {
+                const uint64_t parent = win_lo + q[j];
+                ++my_parents;
+                PZ::expand(parent, 0u, a.prm, [&](int, uint64_t child, uint32_t) { mark(child); });
+            }
+        };
+
+        int qn = 0;  // wave-uniform queue fill
+        for (uint64_t g = pw0 + (uint64_t)wave * 64; g < pw1; g += (uint64_t)NW * 64) {
+            const uint64_t wi = g + lane;
+            uint64_t bits = (wi < pw1) ? cur[wi] : 0ull;
+            const uint32_t c = (uint32_t)__popcll(bits);
+            const uint32_t incl = wave_incl_scan(c, lane);
+            const int total = (int)__shfl(incl, 63, 64);
+            int off = qn + (int)(incl - c);
+            int end_all = qn + total;
+            const uint32_t base = (uint32_t)((wi - w0) * 64);
+            for (;;) {
+                while (bits && off < QCAP) {
+                    q[off++] = base + (uint32_t)(__ffsll((unsigned long long)bits) - 1);
+                    bits &= bits - 1;
+                }
+                if (end_all < QCAP) {
+                    qn = end_all;
+                    break;
+                }
+                wave_sync();
+                run_batch(QCAP);
+                wave_sync();
+                off -= QCAP;
+                end_all -= QCAP;
+                qn = 0;
+                if (end_all == 0) break;
+            }
+        }
+        wave_sync();
+        run_batch((uint32_t)qn);
+        if (local_any) s_local = 1;
+        __syncthreads();
+        if (a.use_window) {
+            uint32_t* dst = nxt + w0 * 2;
+            for (uint32_t i = tid; i < nwin; i += TPB) {
+                const uint32_t v = win[i];
+                if (v) gor(&dst[i], v);
+            }
+        }
+        if (tid == 0 && track && s_local) {
+            const uint64_t s0 = win_lo >> a.seg_shift, s1 = (win_hi - 1) >> a.seg_shift;
+            for (uint64_t s = s0; s <= s1; ++s) {
+                if (tl_words) lor(&tl[s >> 5], 1u << (s & 31));
+                else gor(&a.touched[s >> 5], 1u << (s & 31));
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < tl_words; i += TPB) {
+        const uint32_t v = tl[i];
+        if (v) gor(&a.touched[i], v);
+    }
+    block_add3<TPB>(my_children, my_parents, my_scan, &st->children, &st->expanded, &st->scan_words);
+}
+
+// Finalize: new &= ~cur [odd]; new &= ~old; per-segment and layer counts; hash; old = 0.
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_bfs3_finalize(FinalizeArgs a) {
+    BfsDevState* st = a.st;
+    if (st->done) return;
+    const uint32_t L = st->layer;
+    const uint64_t* __restrict__ cur = a.bits[L % 3];
+    uint64_t* nw = a.bits[(L + 1) % 3];
+    uint64_t* old = a.bits[(L + 2) % 3];
+    unsigned long long* sc_new = a.seg_count + (uint64_t)((L + 1) % 3) * a.nseg;
+    const unsigned long long* sc_old = a.seg_count + (uint64_t)((L + 2) % 3) * a.nseg;
+    const bool track = a.nseg > 1;
+    __shared__ unsigned long long s_chunk;
+    __shared__ unsigned long long red[2][TPB / 64];
+    unsigned long long tot_cnt = 0, tot_hash = 0, tot_full = 0, tot_clear = 0;
+    uint64_t iter = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            if (a.persistent) s_chunk = a.chunk_begin + atomicAdd(&st->chunk_counter, 1ull);
+            else s_chunk = (iter == 0) ? a.chunk_begin + blockIdx.x : ~0ull;
+        }
+        __syncthreads();
+        const uint64_t c = s_chunk;
+        ++iter;
+        __syncthreads();
+        if (c >= a.chunk_end) break;
+        const uint64_t s = c / a.chunks_per_seg;
+        const uint64_t w0 = s * a.seg_words + (c % a.chunks_per_seg) * FIN_CHUNK_WORDS;
+        uint64_t w1 = w0 + FIN_CHUNK_WORDS;
+        const uint64_t seg_end = (s + 1) * a.seg_words;
+        if (w1 > seg_end) w1 = seg_end;
+        if (w1 > a.nwords) w1 = a.nwords;
+        if (w0 >= w1) continue;
+        const bool touched = !track || ((a.touched[s >> 5] >> (s & 31)) & 1u);
+        const bool old_live = !track || sc_old[s] != 0;
+        unsigned long long cnt = 0, h = 0;
+        if (touched) {
+            for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) {
+                uint64_t v = nw[j] & ~old[j];
+                if (a.odd) v &= ~cur[j];
+                nw[j] = v;
+                old[j] = 0;
+                cnt += (unsigned long long)__popcll(v);
+                if (a.hash) h += word_hash(j, v);
+            }
+        } else if (old_live) {
+            for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) old[j] = 0;
+        }
+        if (threadIdx.x == 0) {
+            if (touched) tot_full += w1 - w0;
+            else if (old_live) tot_clear += w1 - w0;
+        }
+        if (touched) {
+            cnt = wave_sum(cnt);
+            h = wave_sum(h);
+            const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+            if (lane == 0) {
+                red[0][wave] = cnt;
+                red[1][wave] = h;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                unsigned long long sc = 0, sh = 0;
+                for (int w = 0; w < TPB / 64; ++w) {
+                    sc += red[0][w];
+                    sh += red[1][w];
+                }
+                if (sc) atomicAdd(&sc_new[s], sc);
+                tot_cnt += sc;
+                tot_hash += sh;
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (tot_cnt) atomicAdd(&st->acc_count, tot_cnt);
+        if (tot_hash) atomicAdd(&st->acc_hash, tot_hash);
+        if (tot_full) atomicAdd(&st->fin_full_words, tot_full);
+        if (tot_clear) atomicAdd(&st->fin_clear_words, tot_clear);
+    }
+}
+
+// Single-block bookkeeping at the end of a layer.
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_bfs_layer_end(LayerEndArgs a) {
+    BfsDevState* st = a.st;
+    if (st->done) return;
+    const uint32_t L = st->layer;
+    if (a.bits == 3) {
+        unsigned long long* sc_next_new = a.seg_count + (uint64_t)((L + 2) % 3) * a.nseg;
+        for (uint32_t i = threadIdx.x; i < a.nseg; i += TPB) sc_next_new[i] = 0;
+        if (a.nseg > 1)
+            for (uint32_t i = threadIdx.x; i < (a.nseg + 31) / 32; i += TPB) a.touched[i] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long cnt = st->acc_count;
+        if (cnt == 0) {
+            st->done = 1;
+        } else {
+            a.layer_counts[L + 1] = cnt;
+            a.layer_hashes[L + 1] = st->acc_hash;
+            st->layers_recorded = L + 2;
+            st->layer = L + 1;
+            if (L + 2 >= st->max_layers) st->done = 1;
+        }
+        st->acc_count = 0;
+        st->acc_hash = 0;
+        st->tile_counter = 0;
+        st->chunk_counter = 0;
+    }
+}
+
+// ---------------------------------------------------------------- two-bit
+
+// Common/InMemoryTwoBitBFS.cpp:52-69: UNVISITED=0, OLD=1, CUR/NEXT = 2/3 swapped
+// every layer.  One persistent pass per layer: CUR fields of a word become OLD
+// (atomic xor, other lanes may be setting NEXT in the same word), their states
+// are expanded, and each child that is UNVISITED is moved to NEXT with a
+// compare-and-swap that also counts it.
+template <class PZ, int TPB>
+__global__ __launch_bounds__(TPB) void k_bfs2_layer(Bfs2Args a) {
+    extern __shared__ uint32_t smem[];
+    BfsDevState* st = a.st;
+    if (st->done) return;
+    const uint32_t L = st->layer;
+    const uint32_t CUR = (L & 1) ? 3u : 2u, NEXT = (L & 1) ? 2u : 3u, OLD = 1u;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    constexpr int NW = TPB / 64;
+    uint32_t* q = smem + wave * QCAP;
+    uint32_t* a32 = reinterpret_cast<uint32_t*>(a.arr);
+    __shared__ unsigned long long s_tile;
+    unsigned long long my_count = 0, my_parents = 0, my_children = 0, my_scan = 0;
+    const uint64_t m55 = 0x5555555555555555ull;
+    const uint64_t pat = (CUR == 2u) ? 0xAAAAAAAAAAAAAAAAull : 0xFFFFFFFFFFFFFFFFull;
+    const uint64_t flip = CUR ^ OLD;  // 3 (CUR=2) or 2 (CUR=3)
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(&st->tile_counter, 1ull);
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        __syncthreads();
+        if (tile >= a.ntiles) break;
+        const uint64_t w0 = tile * a.tile_words;
+        const uint64_t w1 = (w0 + a.tile_words < a.nwords) ? w0 + a.tile_words : a.nwords;
+        auto mark = [&](uint64_t child) {
+            if (child >= a.nbits) {
+                atomicOr(&st->error, 1u);
+                return;
+            }
+            ++my_children;
+            uint32_t* p = a32 + (child >> 4);
+            const uint32_t sh = (uint32_t)(child & 15) * 2;
+            uint32_t old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (((old >> sh) & 3u) == 0u) {
+                const uint32_t prev = atomicCAS(p, old, old | (NEXT << sh));
+                if (prev == old) {
+                    ++my_count;
+                    break;
+                }
+                old = prev;
+            }
+        };
+        auto run_batch = [&](uint32_t n) {
+            for (uint32_t j = lane; j < n; j += 64) {
+                const uint64_t parent = w0 * 32 + q[j];
+                ++my_parents;
+                PZ::expand(parent, 0u, a.prm, [&](int, uint64_t child, uint32_t) { mark(child); });
+            }
+        };
+        int qn = 0;
+        for (uint64_t g = w0 + (uint64_t)wave * 64; g < w1; g += (uint64_t)NW * 64) {
+            const uint64_t wi = g + lane;
+            uint64_t m = 0;
+            if (wi < w1) {
+                const uint64_t v = __hip_atomic_load(&a.arr[wi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t x = v ^ pat;               // CUR fields become 00
+                m = ~(x | (x >> 1)) & m55;                // low bit of each CUR field
+                if (m) {
+                    const uint64_t fx = (flip & 1 ? m : 0) | (flip & 2 ? (m << 1) : 0);
+                    uint32_t* p = reinterpret_cast<uint32_t*>(&a.arr[wi]);
+                    if ((uint32_t)fx) atomicXor(p, (uint32_t)fx);
+                    if ((uint32_t)(fx >> 32)) atomicXor(p + 1, (uint32_t)(fx >> 32));
+                }
+            }
+            const uint32_t c = (uint32_t)__popcll(m);
+            const uint32_t incl = wave_incl_scan(c, lane);
+            const int total = (int)__shfl(incl, 63, 64);
+            int off = qn + (int)(incl - c);
+            int end_all = qn + total;
+            const uint32_t base = (uint32_t)((wi - w0) * 32);
+            for (;;) {
+                while (m && off < QCAP) {
+                    q[off++] = base + (uint32_t)((__ffsll((unsigned long long)m) - 1) >> 1);
+                    m &= m - 1;
+                }
+                if (end_all < QCAP) {
+                    qn = end_all;
+                    break;
+                }
+                wave_sync();
+                run_batch(QCAP);
+                wave_sync();
+                off -= QCAP;
+                end_all -= QCAP;
+                qn = 0;
+                if (end_all == 0) break;
+            }
+        }
+        wave_sync();
+        run_batch((uint32_t)qn);
+        __syncthreads();
+    }
+    block_add3<TPB>(my_count, my_parents, my_children, &st->acc_count, &st->expanded, &st->children);
+}
+
+// Whole-array hash for the two-bit search (run after a layer when hashes are wanted).
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_bfs2_hash(const uint64_t* arr, uint64_t nwords, BfsDevState* st) {
+    if (st->done) return;
+    unsigned long long h = 0;
+    for (uint64_t j = (uint64_t)blockIdx.x * TPB + threadIdx.x; j < nwords; j += (uint64_t)gridDim.x * TPB)
+        h += word_hash(j, arr[j]);
+    block_add3<TPB>(h, 0ull, 0ull, &st->acc_hash, nullptr, nullptr);
+}
+
+}  // namespace psk

@@ -1,0 +1,490 @@
+// Device move generators for the puzzles of the reference's GPU plugins.
+//
+// Each functor has
+//   static constexpr int SLOTS;                       // children slots per parent
+//   template <class F> static void expand(index, op_bits, prm, F&& f);
+// and calls f(slot, child_index, op_label) for every child that exists and whose
+// operator bit is not set in op_bits, in slot order.  The slot/label layout is
+// the reference kernels' layout; the arithmetic is this library's own:
+//
+//  * pancake: a flip of the last k elements changes only the last k Lehmer
+//    digits, so child = rank - (rank mod k!) + rank(reversed suffix pattern).
+//    All k are produced from one right-to-left sweep over the digits; pairwise
+//    order comes from the digits themselves (a[s] < a[r]  <=>  c(s+1,r) + d[r]
+//    >= d[s]), so the permutation is never materialised.
+//  * sliding tile (lexicographic rank): a horizontal blank move changes two
+//    adjacent digits (closed form); vertical moves re-rank the child.
+//  * sliding tile (packed 32-bit segment encoding): the reference's
+//    pack/unpack/rotate steps with every loop unrolled over the board size.
+//  * 4-peg Hanoi: base-4 index -> four peg bit-planes; moving the top disk t
+//    from peg p to q is index ^ ((p ^ q) << t); symmetry relabels pegs 1..3.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace psd {
+
+constexpr uint64_t kInvalid = ~0ull;
+
+struct PuzzleParams {
+    int32_t kind, size, height, flags;
+};
+
+constexpr uint64_t kFact[21] = {1ull,
+                                1ull,
+                                2ull,
+                                6ull,
+                                24ull,
+                                120ull,
+                                720ull,
+                                5040ull,
+                                40320ull,
+                                362880ull,
+                                3628800ull,
+                                39916800ull,
+                                479001600ull,
+                                6227020800ull,
+                                87178291200ull,
+                                1307674368000ull,
+                                20922789888000ull,
+                                355687428096000ull,
+                                6402373705728000ull,
+                                121645100408832000ull,
+                                2432902008176640000ull};
+
+// c * f for a small count c (< 2^8) and a constant factorial f: 24-bit multiply
+// (full rate) when f fits in 24 bits.
+__device__ __forceinline__ uint32_t mul_small(uint32_t c, uint32_t f) {
+    return (f < (1u << 24)) ? __umul24(c, f) : c * f;
+}
+
+// ------------------------------------------------------------------ pancake
+
+// Lehmer digits of the PancakeOptimized rank (Puzzles/PancakeOptimized.cpp:54-77):
+// n <= 12 plain lexicographic rank; n > 12 segment = index >> 29 holds digits
+// 0..n-13 (radices n..13), the low 29 bits digits n-12..n-1 (radices 12..1).
+template <int N>
+__device__ __forceinline__ void pancake_digits(uint64_t index, uint32_t (&d)[N]) {
+    if constexpr (N <= 12) {
+        uint32_t r = (uint32_t)index;
+#pragma unroll
+        for (int i = N - 1; i >= 0; --i) {
+            const uint32_t m = (uint32_t)(N - i);
+            d[i] = r % m;
+            r /= m;
+        }
+    } else {
+        uint32_t seg = (uint32_t)(index >> 29);
+        uint32_t low = (uint32_t)(index & ((1u << 29) - 1));
+#pragma unroll
+        for (int i = N - 1; i >= N - 12; --i) {
+            const uint32_t m = (uint32_t)(N - i);
+            d[i] = low % m;
+            low /= m;
+        }
+#pragma unroll
+        for (int i = N - 13; i >= 0; --i) {
+            const uint32_t m = (uint32_t)(N - i);
+            d[i] = seg % m;
+            seg /= m;
+        }
+    }
+}
+
+// Index from Lehmer digits in the same encoding.
+template <int N>
+__device__ __forceinline__ uint64_t pancake_index_from_digits(const uint32_t (&d)[N]) {
+    if constexpr (N <= 12) {
+        uint32_t r = 0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) r += mul_small(d[i], (uint32_t)kFact[N - 1 - i]);
+        return r;
+    } else {
+        uint32_t low = 0;
+#pragma unroll
+        for (int i = N - 12; i < N; ++i) low += mul_small(d[i], (uint32_t)kFact[N - 1 - i]);
+        uint32_t seg = 0;
+#pragma unroll
+        for (int i = 0; i < N - 12; ++i) seg = seg * (uint32_t)(N - i) + d[i];
+        return ((uint64_t)seg << 29) | low;
+    }
+}
+
+// Digits <-> permutation (slow path, only for the inverse-index variant).
+template <int N>
+__device__ __forceinline__ void digits_to_perm(const uint32_t (&d)[N], uint32_t (&a)[N]) {
+    uint64_t tiles = 0xFEDCBA9876543210ull;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const uint32_t sh = d[i] * 4;
+        a[i] = (uint32_t)((tiles >> sh) & 15);
+        const uint64_t lo = tiles & ((1ull << sh) - 1ull);
+        const uint64_t hi = (tiles >> sh >> 4) << sh;
+        tiles = hi | lo;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void perm_to_digits(const uint32_t (&a)[N], uint32_t (&d)[N]) {
+    uint32_t seen = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        d[i] = a[i] - __popc(seen & ((1u << a[i]) - 1u));
+        seen |= 1u << a[i];
+    }
+}
+
+template <int N, int MODE>
+struct Pancake {
+    // MODE 0: all flips (ops 0..N-2); 1: in-segment ops 0..10; 2: cross-segment ops 11..N-2
+    static constexpr int OPS = N - 1;
+    static constexpr int OP_LO = (MODE == 2) ? 11 : 0;
+    static constexpr int OP_HI = (MODE == 1) ? 11 : N - 1;
+    static constexpr int SLOTS = OP_HI - OP_LO;
+
+    template <class F>
+    __device__ __forceinline__ static void expand(uint64_t index, uint32_t op_bits, const PuzzleParams& prm,
+                                                  F&& f) {
+        uint32_t d[N];
+        pancake_digits<N>(index, d);
+        const bool inv = (MODE == 2) && (prm.flags & 2) && N > 12;
+        // cc[r] = #{q in [s, r): a[q] < a[r]} for the current suffix start s
+        uint32_t cc[N];
+#pragma unroll
+        for (int r = 0; r < N; ++r) cc[r] = 0;
+        // low-part value of the parent and its suffix digit sums
+        const uint32_t low = (N <= 12) ? (uint32_t)index : (uint32_t)(index & ((1u << 29) - 1));
+        const uint64_t high = (N <= 12) ? 0ull : (index & ~((1ull << 29) - 1ull));
+        uint32_t suffix = 0;  // sum_{i >= s} d[i] * (N-1-i)!   (valid while N-s <= 12)
+#pragma unroll
+        for (int s = N - 2; s >= 0; --s) {
+            const int k = N - s;  // flip length
+            const int op = k - 2;
+            uint32_t rho = 0;  // rank of the reversed suffix pattern: sum_r c(s,r) (r-s)!
+#pragma unroll
+            for (int r = s + 1; r < N; ++r) {
+                const uint32_t lt = (cc[r] + d[r] >= d[s]) ? 1u : 0u;
+                cc[r] += lt;
+                if (k <= 12) rho += mul_small(cc[r], (uint32_t)kFact[r - s]);
+            }
+            if (k <= 12) suffix += mul_small(d[s], (uint32_t)kFact[N - 1 - s]);
+            if (op < OP_LO || op >= OP_HI) continue;
+            if (op_bits & (1u << op)) continue;
+            uint64_t child;
+            if (!inv && k <= 12) {
+                child = high | (uint64_t)(low - suffix + rho);
+            } else {
+                // child digits: prefix unchanged; position P >= s holds c(s, s + N-1-P)
+                uint32_t cd[N];
+#pragma unroll
+                for (int P = 0; P < N; ++P) cd[P] = (P < s) ? d[P] : cc[s + N - 1 - P];
+                if (inv) {
+                    uint32_t a[N];
+                    digits_to_perm<N>(cd, a);
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) {  // reverse the last 12 elements
+                        const uint32_t t = a[N - 12 + i];
+                        a[N - 12 + i] = a[N - 1 - i];
+                        a[N - 1 - i] = t;
+                    }
+                    perm_to_digits<N>(a, cd);
+                }
+                child = pancake_index_from_digits<N>(cd);
+            }
+            f(op - OP_LO, child, (uint32_t)op);
+        }
+    }
+};
+
+// ---------------------------------------------------------- sliding (simple)
+
+constexpr int OP_UP = 0, OP_LEFT = 1, OP_RIGHT = 2, OP_DOWN = 3;
+
+// SlidingTilePuzzleGpu: lexicographic rank over N! (Puzzles/PermutationMap.cpp).
+template <int N>
+struct Sliding {
+    static constexpr int OPS = 4;
+    static constexpr int SLOTS = 4;
+    using R = typename std::conditional<(N <= 12), uint32_t, uint64_t>::type;
+
+    __device__ __forceinline__ static void digits(R r, uint32_t (&d)[N]) {
+#pragma unroll
+        for (int i = N - 1; i >= 0; --i) {
+            const R m = (R)(N - i);
+            d[i] = (uint32_t)(r % m);
+            r /= m;
+        }
+    }
+
+    // rank of the permutation a with positions i and j swapped
+    __device__ __forceinline__ static R rank_swapped(const uint32_t (&a)[N], int i, int j) {
+        uint32_t seen = 0;
+        R r = 0;
+        uint32_t ai = 0, aj = 0;
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            ai = (p == i) ? a[p] : ai;
+            aj = (p == j) ? a[p] : aj;
+        }
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            const uint32_t v = (p == i) ? aj : (p == j) ? ai : a[p];
+            const uint32_t dig = v - __popc(seen & ((1u << v) - 1u));
+            seen |= 1u << v;
+            r = r * (R)(N - p) + (R)dig;
+        }
+        return r;
+    }
+
+    template <class F>
+    __device__ __forceinline__ static void expand(uint64_t index, uint32_t op_bits, const PuzzleParams& prm,
+                                                  F&& f) {
+        const int width = prm.size;
+        const R rank = (R)index;
+        uint32_t d[N];
+        digits(rank, d);
+        // the blank (tile 0) is the leftmost position whose digit is 0
+        int blank = N - 1;
+#pragma unroll
+        for (int p = N - 1; p >= 0; --p) blank = (d[p] == 0) ? p : blank;
+        const int col = blank % width;
+        const bool up = blank >= width && !(op_bits & (1u << OP_UP));
+        const bool dn = blank < N - width && !(op_bits & (1u << OP_DOWN));
+        const bool lf = col > 0 && !(op_bits & (1u << OP_LEFT));
+        const bool rt = col < width - 1 && !(op_bits & (1u << OP_RIGHT));
+        if (up || dn) {
+            uint32_t a[N];
+            digits_to_perm<N>(d, a);
+            if (up) f(0, (uint64_t)rank_swapped(a, blank - width, blank), OP_DOWN);
+            if (dn) f(1, (uint64_t)rank_swapped(a, blank, blank + width), OP_UP);
+        }
+        // horizontal moves touch two adjacent digits; w_p = (N-1-p)!
+        uint32_t dl = 0, dr = 0;
+        R wl = 0, wb = 0, wr = 0;
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            dl = (p == blank - 1) ? d[p] : dl;
+            dr = (p == blank + 1) ? d[p] : dr;
+            wl = (p == blank - 1) ? (R)kFact[N - 1 - p] : wl;
+            wb = (p == blank) ? (R)kFact[N - 1 - p] : wb;
+            wr = (p == blank + 1) ? (R)kFact[N - 1 - p] : wr;
+        }
+        // blank at b moves left: digit at b-1 becomes 0, digit at b becomes d[b-1]-1
+        if (lf) f(2, (uint64_t)(rank - (R)dl * wl + (R)(dl - 1) * wb), OP_RIGHT);
+        // blank at b moves right: digit at b becomes d[b+1]+1, digit at b+1 becomes 0
+        if (rt) f(3, (uint64_t)(rank + (R)(dr + 1) * wb - (R)dr * wr), OP_LEFT);
+    }
+};
+
+// ---------------------------------------------------------- sliding (packed)
+
+// SlidingTilePuzzleOptimized<W,H>: tiles[0..size-2] are the non-blank tiles
+// (tile-1) in cell order, tiles[15] the blank cell; packed value of a tile =
+// number of smaller tiles to its left; tiles[0], tiles[1] are implied by parity
+// (Puzzles/SlidingPuzzleGpu.cu:108-207).
+template <int W, int H>
+struct Packed {
+    static constexpr int OPS = 4;
+    static constexpr int SLOTS = 4;
+    static constexpr int S = W * H;
+
+    __device__ __forceinline__ static void from_index(uint64_t index, uint32_t (&t)[16]) {
+        const uint32_t seg = (uint32_t)(index >> 32);
+        uint32_t idx = (uint32_t)index;
+        t[12] = seg & 15;
+        t[13] = (seg >> 4) & 15;
+        t[14] = seg >> 8;
+        t[15] = idx % 16;
+        idx /= 16;
+        t[0] = 0;
+        t[1] = 0;
+#pragma unroll
+        for (int i = 2; i < 12; ++i) {
+            t[i] = idx % (uint32_t)(i + 1);
+            idx /= (uint32_t)(i + 1);
+        }
+    }
+
+    __device__ __forceinline__ static uint64_t to_index(const uint32_t (&t)[16]) {
+        const uint64_t seg = t[12] | (t[13] << 4) | (t[14] << 8);
+        uint32_t idx = 0;
+#pragma unroll
+        for (int i = 11; i >= 2; --i) idx = idx * (uint32_t)(i + 1) + t[i];
+        return (seg << 32) | (uint64_t)(idx * 16u + t[15]);
+    }
+
+    __device__ __forceinline__ static void unpack(uint32_t (&a)[16]) {
+        uint32_t inv = 0;
+#pragma unroll
+        for (int i = 1; i < S - 1; ++i)
+#pragma unroll
+            for (int j = 0; j < i; ++j) {
+                const uint32_t x = (a[j] >= a[i]) ? 1u : 0u;
+                a[j] += x;
+                inv ^= x;
+            }
+        const bool inv_even = (inv & 1) == 0;
+        const bool row_even = ((a[15] / W) & 1) == 0;
+        const bool restore = (W % 2 == 0) ? ((!inv_even) == row_even) : !inv_even;
+        if (restore) {
+            const uint32_t x = a[0];
+            a[0] = a[1];
+            a[1] = x;
+        }
+    }
+
+    __device__ __forceinline__ static void pack(uint32_t (&a)[16]) {
+        a[0] = 0;
+        a[1] = 0;
+#pragma unroll
+        for (int i = S - 2; i > 2; --i)
+#pragma unroll
+            for (int j = 2; j < i; ++j) a[j] -= (a[j] >= a[i]) ? 1u : 0u;
+    }
+
+    // blank moves up: the tile above (list position z-W) is re-inserted at z-1
+    __device__ __forceinline__ static void rotate_up(uint32_t (&a)[16]) {
+        const int z = (int)a[15];
+        uint32_t b[16];
+        uint32_t moved = 0;
+#pragma unroll
+        for (int i = 0; i < 15; ++i) moved = (i == z - W) ? a[i] : moved;
+#pragma unroll
+        for (int i = 0; i < 15; ++i) {
+            const uint32_t nxt = (i + 1 < 15) ? a[i + 1] : 0;
+            b[i] = (i >= z - W && i < z - 1) ? nxt : (i == z - 1) ? moved : a[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 15; ++i) a[i] = b[i];
+        a[15] = (uint32_t)(z - W);
+    }
+
+    __device__ __forceinline__ static void rotate_dn(uint32_t (&a)[16]) {
+        const int z = (int)a[15];
+        uint32_t b[16];
+        uint32_t moved = 0;
+#pragma unroll
+        for (int i = 0; i < 15; ++i) moved = (i == z + W - 1) ? a[i] : moved;
+#pragma unroll
+        for (int i = 0; i < 15; ++i) {
+            const uint32_t prv = (i > 0) ? a[i - 1] : 0;
+            b[i] = (i > z && i <= z + W - 1) ? prv : (i == z) ? moved : a[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 15; ++i) a[i] = b[i];
+        a[15] = (uint32_t)(z + W);
+    }
+
+    template <class F>
+    __device__ __forceinline__ static void expand(uint64_t index, uint32_t op_bits, const PuzzleParams&, F&& f) {
+        uint32_t a[16];
+        from_index(index, a);
+        unpack(a);
+        const int blank = (int)a[15];
+        if (blank >= W && !(op_bits & (1u << OP_UP))) {
+            uint32_t b[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) b[i] = a[i];
+            rotate_up(b);
+            pack(b);
+            f(0, to_index(b), OP_DOWN);
+        }
+        if (blank < S - W && !(op_bits & (1u << OP_DOWN))) {
+            uint32_t b[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) b[i] = a[i];
+            rotate_dn(b);
+            pack(b);
+            f(1, to_index(b), OP_UP);
+        }
+        if (blank % W > 0 && !(op_bits & (1u << OP_LEFT))) f(2, index - 1, OP_RIGHT);
+        if (blank % W < W - 1 && !(op_bits & (1u << OP_RIGHT))) f(3, index + 1, OP_LEFT);
+    }
+};
+
+// ------------------------------------------------------------ 4-peg Hanoi
+
+// FourPegHanoiGPU / FourPegHanoiOptimizedGPU (Puzzles/HanoiTowersGPU.cu:114-144,
+// 250-281): both kernels produce the same slots; one implementation serves both.
+template <class U>
+struct Hanoi {
+    static constexpr int OPS = 4;
+    static constexpr int SLOTS = 6;
+
+    __device__ __forceinline__ static int ctz_(U x) {
+        if constexpr (sizeof(U) == 4) return __ffs((int)x) - 1;
+        else return __ffsll((unsigned long long)x) - 1;
+    }
+    __device__ __forceinline__ static int top_bit(U x) {
+        if constexpr (sizeof(U) == 4) return x ? 31 - __clz((int)x) : -1;
+        else return x ? 63 - __clzll((long long)x) : -1;
+    }
+
+    template <class F>
+    __device__ __forceinline__ static void expand(uint64_t index64, uint32_t op_bits, const PuzzleParams& prm, F&& f) {
+        const U index = (U)index64;
+        const int disks = prm.size;
+        const U m55 = (U)0x5555555555555555ull;
+        const U bm = (2 * disks >= (int)(8 * sizeof(U))) ? m55 : (U)(((U)1 << (2 * disks)) - 1) & m55;
+        const U p0 = index >> 1, p1 = index;
+        U pg[4];
+        pg[0] = ~p0 & ~p1 & bm;
+        pg[1] = ~p0 & p1 & bm;
+        pg[2] = p0 & ~p1 & bm;
+        pg[3] = p0 & p1 & bm;
+        int top[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) top[p] = pg[p] ? ctz_(pg[p]) : 255;
+        const bool sym = (prm.flags & 1) != 0;
+        int slot = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = a + 1; b < 4; ++b, ++slot) {
+                const bool fwd = top[a] < top[b];
+                const int src = fwd ? a : b, dst = fwd ? b : a;
+                const int ts = fwd ? top[a] : top[b];
+                const int td = fwd ? top[b] : top[a];
+                if (op_bits & (1u << src)) continue;
+                if (!(ts < td)) continue;  // both pegs empty
+                U child = index ^ ((U)(src ^ dst) << ts);
+                uint32_t label = (uint32_t)dst;
+                if (sym) {
+                    // sort pegs 1..3 of the child by bottom disk, largest first
+                    U q[4];
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) q[p] = pg[p];
+                    const U bit = (U)1 << ts;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) {
+                        q[p] = (p == src) ? (q[p] & ~bit) : q[p];
+                        q[p] = (p == dst) ? (q[p] | bit) : q[p];
+                    }
+                    int bot[4], lab[4] = {0, 1, 2, 3};
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) bot[p] = top_bit(q[p]);
+                    const int order[3][2] = {{2, 3}, {1, 2}, {2, 3}};
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) {
+                        const int i = order[o][0], j = order[o][1];
+                        if (bot[i] < bot[j]) {
+                            int t = bot[i]; bot[i] = bot[j]; bot[j] = t;
+                            U u = q[i]; q[i] = q[j]; q[j] = u;
+                            t = lab[i]; lab[i] = lab[j]; lab[j] = t;
+                        }
+                    }
+                    child = q[1] | (q[2] << 1) | (q[3] | (q[3] << 1));
+                    uint32_t nl = 0;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) nl = (lab[p] == dst) ? (uint32_t)p : nl;
+                    label = nl;
+                }
+                f(slot, (uint64_t)child, label);
+            }
+    }
+};
+
+}  // namespace psd

@@ -1,0 +1,52 @@
+// Internal helpers shared by the HIP translation units of libpuzzlesearch_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "puzzlesearch_amd.h"
+
+namespace ps {
+
+// Records the message for ps_last_error() on this thread and returns code.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+bool runtime_ready();
+int require_runtime();
+
+// Host-side puzzle facts (Common/Puzzle.h:22-26 for each reference class).
+int puzzle_query(const ps_puzzle* p, ps_puzzle_info* out);
+int slots_per_parent(const ps_puzzle* p, int mode);
+// Default LDS window (states per tile) for the device BFS: children that stay
+// within the parent's aligned window of this size are marked in LDS.
+uint64_t puzzle_locality_block(const ps_puzzle* p);
+
+inline hipStream_t stream_of(ps_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+int launch_fill(void* dst, uint8_t value, uint64_t bytes, hipStream_t s);
+
+}  // namespace ps
+
+#define PS_HIP_CHECK(expr)                                                                               \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess)                                                                            \
+            return ::ps::fail(PS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, \
+                              __LINE__);                                                                 \
+    } while (0)
+
+// Wraps a C-ABI body so no C++ exception crosses the boundary.
+#define PS_API_BEGIN try {
+#define PS_API_END                                                              \
+    }                                                                           \
+    catch (const std::bad_alloc&) {                                             \
+        return ::ps::fail(PS_ERR_OUT_OF_MEMORY, "host allocation failed");      \
+    }                                                                           \
+    catch (const std::exception& e) {                                           \
+        return ::ps::fail(PS_ERR_INVALID_ARGUMENT, "exception: %s", e.what());  \
+    }                                                                           \
+    catch (...) {                                                               \
+        return ::ps::fail(PS_ERR_INVALID_ARGUMENT, "unknown exception");        \
+    }

@@ -1,0 +1,235 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+
+PANCAKE, SLIDING, SLIDING_PACKED, HANOI4, HANOI4_OPT = 1, 2, 3, 4, 5
+FLAG_SYMMETRY, FLAG_INV_INDEX = 1, 2
+EXPAND_FULL, EXPAND_IN_SEGMENT, EXPAND_CROSS_SEGMENT = 0, 1, 2
+INVALID = (1 << 64) - 1
+
+
+class Puzzle(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("size", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+def puzzle(kind, size, height=0, flags=0):
+    return Puzzle(kind, size, height, flags)
+
+
+def pancake(n, inv_index=False):
+    return Puzzle(PANCAKE, n, 0, FLAG_INV_INDEX if inv_index else 0)
+
+
+def sliding(w, h):
+    return Puzzle(SLIDING, w, h, 0)
+
+
+def sliding_packed(w, h):
+    return Puzzle(SLIDING_PACKED, w, h, 0)
+
+
+def hanoi(disks, symmetry=False, opt=False):
+    return Puzzle(HANOI4_OPT if opt else HANOI4, disks, 0, FLAG_SYMMETRY if symmetry else 0)
+
+
+def from_spec(spec):
+    """Puzzle from a reference_vectors.json 'puzzle' entry."""
+    k = spec["kind"]
+    if k == "pancake":
+        return pancake(spec["n"], spec.get("inv_index", False))
+    if k == "sliding":
+        return sliding(spec["width"], spec["height"])
+    if k == "sliding_packed":
+        return sliding_packed(spec["width"], spec["height"])
+    if k == "hanoi":
+        return hanoi(spec["disks"], spec.get("symmetry", False), spec.get("opt", False))
+    raise ValueError(k)
+
+
+_lib = None
+
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_P = ctypes.POINTER(Puzzle)
+_i = ctypes.c_int
+_u64 = ctypes.c_uint64
+
+_SIGS = {
+    "or_splitmix64": (_u64, [_u64]),
+    "or_bits_hash": (_u64, [_u64p, _u64]),
+    "or_children_hash": (_u64, [_u64p, _i32p, _u64]),
+    "or_perm_compact": (None, [_i32p, _i]),
+    "or_perm_uncompact": (None, [_i32p, _i]),
+    "or_perm_rank": (_u64, [_i32p, _i]),
+    "or_perm_unrank": (None, [_u64, _i32p, _i]),
+    "or_pancake_rank": (_u64, [_i32p, _i]),
+    "or_pancake_unrank": (None, [_u64, _i32p, _i]),
+    "or_packed_from_standard": (ctypes.c_int64, [_i, _i, _i32p]),
+    "or_packed_to_standard": (None, [_i, _i, _u64, _i32p]),
+    "or_operators_count": (_i, [_P]),
+    "or_branching_factor": (_i, [_P]),
+    "or_indexes_count": (_u64, [_P]),
+    "or_has_odd_cycles": (_i, [_P]),
+    "or_parse": (_u64, [_P, ctypes.c_char_p]),
+    "or_to_string": (_i, [_P, _u64, ctypes.c_char_p, _i]),
+    "or_expand_slots": (_i, [_P, _i, _u64p, _u64, _u64p]),
+    "or_slots_per_parent": (_i, [_P, _i]),
+    "or_expand_one": (_i, [_P, _i, _u64, _i, _u64p, _i32p]),
+    "or_hanoi_expand_cpu_opt": (_i, [_i, _i, _u64, _i, _u64p, _i32p]),
+    "or_combine_index_opbits": (None, [_i, _u64p, _i32p, _u64, _u64p]),
+    "or_split_index_ops": (None, [_i, _u64p, _u64, _u64p, _i32p]),
+    "or_pancake_xseg_postprocess": (None, [_i, _i, _u32p, _u64]),
+    "or_bits_andnot_count": (_u64, [_u64p, _u64p, _u64]),
+    "or_bits_count": (_u64, [_u64p, _u64]),
+    "or_bfs3": (_i, [_P, _u64, _u64, _i, _u64p, _u64p]),
+    "or_bfs2": (_i, [_P, _u64, _u64, _i, _u64p, _u64p]),
+    "or_bfs3_segmented": (_i, [_P, _u64, _i, _i, _u64p]),
+    "or_bfs3_expand_range": (_u64, [_P, _u64p, _u64p, _u64, _u64, ctypes.POINTER(_u64)]),
+    "or_svint_encode_tuple": (_i, [_u32p, _u8p]),
+    "or_svint_decode_tuple": (_i, [_u8p, _u32p]),
+    "or_svint_encode_bounds_tuple": (None, [_u8p, _u8p]),
+    "or_svint_decode_bounds_tuple": (None, [_u8p, _u8p]),
+    "or_svint_encode_indexes": (_i, [_i, _u32p, _u8p]),
+    "or_svint_decode_indexes": (_i, [_i, _u8p, _u32p]),
+    "or_svint_decode_indexes_diff": (_i, [_i, _u8p, _u32p]),
+    "or_svint_encode": (_i, [_i, _u32p, _u8p]),
+    "or_svint_decode": (_i, [ctypes.POINTER(_i), _u8p, _u32p]),
+    "or_svint_encode_with_bounds": (_i, [_i, _u32p, _u8p, _u8p]),
+    "or_svint_decode_with_bounds": (_i, [ctypes.POINTER(_i), _u8p, _u32p, _u8p]),
+    "or_opbits_encode": (ctypes.c_int64, [_i, _u64, _u8p, _u8p]),
+    "or_opbits_decode": (ctypes.c_int64, [_i, _u64, _u8p, _u8p]),
+    "or_fc_is_bitmap": (_i, [_u8p]),
+    "or_fc_bitmap_size": (ctypes.c_int64, [_i, _u32p]),
+    "or_fc_encode_bitmap": (_i, [_i, _u32p, _u8p]),
+    "or_fc_encode_bitmap_with_check": (_i, [_i, _u32p, _u8p]),
+    "or_fc_decode_bitmap": (_i, [ctypes.POINTER(_i), _u8p, _u32p]),
+    "or_fc_encode": (_i, [_i, _u32p, _u8p]),
+    "or_fc_encode_with_check": (_i, [_i, _u32p, _u8p]),
+    "or_fc_decode": (_i, [ctypes.POINTER(_i), _u8p, _u32p]),
+}
+
+
+def build():
+    """Compile oracle/liboracle.so (and oracle/_ref where the reference exists)."""
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------- helpers
+
+def parse(p, text):
+    return lib().or_parse(ctypes.byref(p), text.encode())
+
+
+def to_string(p, index):
+    buf = ctypes.create_string_buffer(512)
+    n = lib().or_to_string(ctypes.byref(p), index, buf, 512)
+    assert n >= 0
+    return buf.value.decode()
+
+
+def ops_count(p):
+    return lib().or_operators_count(ctypes.byref(p))
+
+
+def indexes_count(p):
+    return lib().or_indexes_count(ctypes.byref(p))
+
+
+def slots_per_parent(p, mode=EXPAND_FULL):
+    return lib().or_slots_per_parent(ctypes.byref(p), mode)
+
+
+def expand_slots(p, packed, mode=EXPAND_FULL):
+    packed = np.ascontiguousarray(packed, dtype=np.uint64)
+    s = slots_per_parent(p, mode)
+    out = np.zeros(len(packed) * max(s, 0), dtype=np.uint64)
+    r = lib().or_expand_slots(ctypes.byref(p), mode, packed, len(packed), out)
+    assert r == s
+    return out
+
+
+def expand_one(p, index, op_bits=0, mode=EXPAND_FULL):
+    ch = np.zeros(16, dtype=np.uint64)
+    ops = np.zeros(16, dtype=np.int32)
+    n = lib().or_expand_one(ctypes.byref(p), mode, index, op_bits, ch, ops)
+    assert n >= 0
+    return [int(x) for x in ch[:n]], [int(x) for x in ops[:n]]
+
+
+def bfs3(p, start, index_count=0, max_layers=4096, with_hashes=True):
+    counts = np.zeros(max_layers, dtype=np.uint64)
+    hashes = np.zeros(max_layers, dtype=np.uint64)
+    n = lib().or_bfs3(ctypes.byref(p), start, index_count, max_layers, counts, hashes)
+    assert n > 0, "oracle bfs3 failed"
+    return [int(x) for x in counts[:n]], [int(x) for x in hashes[:n]]
+
+
+def bfs2(p, start, index_count=0, max_layers=4096):
+    counts = np.zeros(max_layers, dtype=np.uint64)
+    hashes = np.zeros(max_layers, dtype=np.uint64)
+    n = lib().or_bfs2(ctypes.byref(p), start, index_count, max_layers, counts, hashes)
+    assert n > 0, "oracle bfs2 failed"
+    return [int(x) for x in counts[:n]], [int(x) for x in hashes[:n]]
+
+
+def bfs3_segmented(p, start, segment_bits, max_layers=4096):
+    counts = np.zeros(max_layers, dtype=np.uint64)
+    n = lib().or_bfs3_segmented(ctypes.byref(p), start, segment_bits, max_layers, counts)
+    assert n > 0, "oracle segmented bfs failed"
+    return [int(x) for x in counts[:n]]
+
+
+def bits_hash(words):
+    words = np.ascontiguousarray(words, dtype=np.uint64)
+    return int(lib().or_bits_hash(words, len(words)))
+
+
+def children_hash(children, ops):
+    children = np.ascontiguousarray(children, dtype=np.uint64)
+    ops = np.ascontiguousarray(ops, dtype=np.int32)
+    return int(lib().or_children_hash(children, ops, len(children)))
+
+
+def split_index_ops(ops, packed):
+    packed = np.ascontiguousarray(packed, dtype=np.uint64)
+    idx = np.zeros(len(packed), dtype=np.uint64)
+    op = np.zeros(len(packed), dtype=np.int32)
+    lib().or_split_index_ops(ops, packed, len(packed), idx, op)
+    return idx, op
+
+
+def start_index(p, start):
+    """Start state text, or 'index:N'."""
+    if start.startswith("index:"):
+        return int(start.split(":")[1])
+    r = parse(p, start)
+    assert r != INVALID, start
+    return r

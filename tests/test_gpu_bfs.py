@@ -1,0 +1,221 @@
+"""GPU parity of the device-resident BFS (ps_bfs_*) with the reference
+drivers: per-layer counts must equal the reference's golden vectors, and
+per-layer frontier hashes must equal the oracle's, in every execution mode:
+batched (persistent kernels, with and without the captured hipGraph), plain
+(one launch per segment per layer), several segment sizes, LDS window on and
+off, and the two-bit driver.  Full-size workloads are compared with the oracle
+fixtures in tests/golden/oracle_fixtures.json."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import puzzlesearch_amd as psa
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+SMALL = ["sliding_3x2", "sliding_4x2_edge", "sliding_3x3", "sliding_3x3_center", "sliding_5x2",
+         "pancake_5", "pancake_9", "hanoi_10", "hanoi_10_sym"]
+
+MODES = {
+    "batched_graph": dict(mode=psa.BFS_BATCHED, graph=1),
+    "batched_eager": dict(mode=psa.BFS_BATCHED, graph=0),
+    "plain": dict(mode=psa.BFS_PLAIN),
+    "batched_seg12": dict(mode=psa.BFS_BATCHED, segment_bits=12),
+    "plain_seg12": dict(mode=psa.BFS_PLAIN, segment_bits=12),
+    "batched_nowindow": dict(mode=psa.BFS_BATCHED, tile_bits_log2=-1),
+    "batched_tile12_256": dict(mode=psa.BFS_BATCHED, tile_bits_log2=12, block_threads=256),
+}
+
+
+def _ps_puzzle(op):
+    return psa.Puzzle(op.kind, op.size, op.height, op.flags)
+
+
+_oracle_cache = {}
+
+
+def oracle_run(vid, golden, bits=3):
+    key = (vid, bits)
+    if key not in _oracle_cache:
+        r = golden[vid]
+        op = O.from_spec(r["puzzle"])
+        s = O.start_index(op, r["start"])
+        _oracle_cache[key] = (O.bfs3 if bits == 3 else O.bfs2)(op, s)
+    return _oracle_cache[key]
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+@pytest.mark.parametrize("vid", SMALL)
+def test_bfs3_golden(ps, golden, vid, mode):
+    r = golden[vid]
+    op = O.from_spec(r["puzzle"])
+    start = O.start_index(op, r["start"])
+    with psa.Bfs(_ps_puzzle(op), **MODES[mode]) as b:
+        counts, hashes = b.run(start)
+        st = b.stats()
+    assert counts == r["counts"], r["source"]
+    _, ohash = oracle_run(vid, golden)
+    assert hashes == ohash
+    assert st.expanded == sum(r["counts"])
+
+
+@pytest.mark.parametrize("vid", SMALL)
+def test_bfs2_golden(ps, golden, vid):
+    r = golden[vid]
+    op = O.from_spec(r["puzzle"])
+    start = O.start_index(op, r["start"])
+    with psa.Bfs(_ps_puzzle(op), bits=2) as b:
+        counts, hashes = b.run(start)
+    assert counts == r["counts"], r["source"]
+    _, ohash = oracle_run(vid, golden, bits=2)
+    assert hashes == ohash
+
+
+def test_graph_replays_repeat(ps, golden):
+    """The captured layer graph gives identical results on every replay, with
+    eager runs and synchronisations in between."""
+    r = golden["pancake_9"]
+    p = psa.pancake(9)
+    with psa.Bfs(p, graph=1) as g, psa.Bfs(p, graph=0) as e:
+        ref = e.run(0)
+        for _ in range(4):
+            assert g.run(0) == ref
+            assert e.run(0) == ref
+            psa.sync()
+    assert ref[0] == r["counts"]
+
+
+def test_step_api_matches_run(ps, golden):
+    r = golden["hanoi_10_sym"]
+    with psa.Bfs(psa.hanoi(10, True)) as b:
+        b.reset(0)
+        counts = [1]
+        while True:
+            c, _ = b.step()
+            if c == 0:
+                break
+            counts.append(c)
+    assert counts == r["counts"]
+
+
+def test_frontier_array_matches_oracle(ps):
+    """Final frontier bit array equals the oracle's last layer."""
+    p = psa.pancake(8)
+    with psa.Bfs(p, graph=0) as b:
+        counts, hashes = b.run(0)
+        words = b.frontier()
+    assert O.bits_hash(words) == hashes[-1]
+    assert int(np.unpackbits(words.view(np.uint8)).sum()) == counts[-1]
+
+
+def test_max_layers(ps, golden):
+    r = golden["pancake_9"]
+    with psa.Bfs(psa.pancake(9), max_layers=4) as b:
+        counts, _ = b.run(0)
+    assert counts == r["counts"][:4]
+
+
+def test_start_out_of_range(ps):
+    with psa.Bfs(psa.pancake(7)) as b:
+        with pytest.raises(psa.PSError) as e:
+            b.run(math.factorial(7))
+        assert e.value.code == psa.PS_ERR_INDEX_RANGE
+
+
+def test_index_count_too_small_is_reported(ps):
+    # an index space smaller than the puzzle's: children beyond it are refused
+    with psa.Bfs(psa.pancake(8), index_count=1000) as b:
+        with pytest.raises(psa.PSError) as e:
+            b.run(0)
+        assert e.value.code == psa.PS_ERR_INDEX_RANGE
+
+
+def _fixtures():
+    path = os.path.join(HERE, "golden", "oracle_fixtures.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+FIX = _fixtures()
+
+
+@pytest.mark.parametrize("name", [n for n in ("pancake_10", "pancake_11", "hanoi_12", "hanoi_12_sym",
+                                                "sliding_packed_5x2") if n in FIX])
+@pytest.mark.parametrize("mode", ["batched_graph", "plain_seg20"])
+def test_medium_against_oracle_fixture(ps, name, mode):
+    rec = FIX[name]
+    p = psa.Puzzle(**rec["puzzle"])
+    opts = dict(MODES.get(mode, {}))
+    if mode == "plain_seg20":
+        opts = dict(mode=psa.BFS_PLAIN, segment_bits=20)
+    with psa.Bfs(p, index_count=rec["index_count"], **opts) as b:
+        counts, hashes = b.run(rec["start"])
+    assert counts == rec["counts"]
+    assert [str(h) for h in hashes] == rec["hashes"]
+
+
+@pytest.mark.parametrize("name", [n for n in ("pancake_10", "hanoi_12", "sliding_packed_5x2") if n in FIX])
+def test_two_bit_against_oracle_fixture(ps, name):
+    rec = FIX[name]
+    p = psa.Puzzle(**rec["puzzle"])
+    with psa.Bfs(p, bits=2, index_count=rec["index_count"]) as b:
+        counts, hashes = b.run(rec["start"])
+    assert counts == rec["counts"]
+    assert [str(h) for h in hashes] == rec["hashes2"]
+
+
+@pytest.mark.parametrize("seg", [0, 20, 26])
+def test_pancake12_full(ps, seg):
+    """12-pancake, 12! states, three-bit, identity start: counts and frontier
+    hashes equal the oracle; sum = 12!, radius = P(12) = 14."""
+    if "pancake_12" not in FIX:
+        pytest.skip("oracle fixture pancake_12 not generated")
+    rec = FIX["pancake_12"]
+    with psa.Bfs(psa.pancake(12), index_count=math.factorial(12), segment_bits=seg) as b:
+        counts, hashes = b.run(0)
+    assert sum(counts) == math.factorial(12)
+    assert len(counts) == 15
+    assert counts == rec["counts"]
+    assert [str(h) for h in hashes] == rec["hashes"]
+
+
+def test_pancake12_reference_sizing(ps):
+    """Same search with PancakeOptimized's 2^29 index space: identical result."""
+    if "pancake_12" not in FIX:
+        pytest.skip("oracle fixture pancake_12 not generated")
+    rec = FIX["pancake_12"]
+    with psa.Bfs(psa.pancake(12), mode=psa.BFS_PLAIN, segment_bits=24) as b:
+        counts, hashes = b.run(0)
+    assert counts == rec["counts"]
+    assert [str(h) for h in hashes] == rec["hashes"]
+
+
+@pytest.mark.parametrize("shape", ["sliding_4x3", "sliding_3x4"])
+def test_sliding_4x3_full(ps, golden, shape):
+    """4x3 sliding tile, 12! indexes (half reachable): counts equal
+    SlidingPuzzleTests/TestFrontierSearch.cpp:71 (3x4 by transposition)."""
+    w, h = (4, 3) if shape == "sliding_4x3" else (3, 4)
+    p = psa.sliding(w, h)
+    with psa.Bfs(p, segment_bits=24) as b:
+        counts, hashes = b.run(0)
+    assert counts == golden["sliding_4x3"]["counts"]
+    if shape in FIX:
+        assert [str(x) for x in hashes] == FIX[shape]["hashes"]
+
+
+def test_hanoi16_full(ps):
+    """4-peg Hanoi, 16 disks, 4^16 states: every state is reached once."""
+    with psa.Bfs(psa.hanoi(16), segment_bits=24) as b:
+        counts, hashes = b.run(0)
+        st = b.stats()
+    assert sum(counts) == 4 ** 16
+    assert st.expanded == 4 ** 16
+    assert counts[:11] == [1, 3, 6, 12, 30, 30, 66, 96, 126, 210, 330]  # shared prefix with the 21-disk vector
