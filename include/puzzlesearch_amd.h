@@ -140,6 +140,47 @@ int ps_bits_scan(const uint64_t* d_a, uint64_t nbits, uint64_t* d_out, uint64_t 
 /* Sets bit idx[i] for every i (BitArray::Set in a loop). */
 int ps_bits_set_list(uint64_t* d_a, const uint64_t* d_idx, uint64_t count, ps_stream s);
 
+/* ----------------------------------------------------------- codecs */
+/* Values are handled in chunks: chunk c is values[starts[c], starts[c+1]),
+ * at most 2^18 values (StreamVInt::MAX_INDEXES_COUNT).  Encoded chunks are
+ * concatenated; chunk c starts at out + offsets[c] and offsets[nchunks] is
+ * the total, written by the encoders (device arrays of nchunks+1 u64).  A
+ * chunk with no values encodes to no bytes, as in the reference. */
+
+/* StreamVInt::Encode (Common/StreamVInt.cpp:278-300) or, with d_bounds,
+ * Encode(count, indexes, bounds, ...) (:322-347): u32 count, values delta-coded
+ * and padded with zeros to a multiple of 8 in 4-value tuples (descriptor byte +
+ * 1..4 little-endian bytes each), then 4-bit bounds two per byte.  The input
+ * values are not modified. */
+int ps_svint_encode(const uint32_t* d_values, const uint8_t* d_bounds, const uint64_t* d_starts, uint32_t nchunks,
+                    uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, ps_stream s);
+/* StreamVInt::Decode (Common/StreamVInt.cpp:302-368) of nchunks concatenated
+ * chunks (chunk c at d_in + d_in_offsets[c]).  Chunk c's values go to
+ * d_values + value_starts[c]; value_starts (nchunks+1, device) is written. */
+int ps_svint_decode(const uint8_t* d_in, const uint64_t* d_in_offsets, uint32_t nchunks, int32_t with_bounds,
+                    uint32_t* d_values, uint8_t* d_bounds, uint64_t values_cap, uint64_t* d_value_starts,
+                    ps_stream s);
+/* OpBitsCompression<bits>::Encode / Decode, bits in {2, 4, 8}; count % 8 == 0
+ * (Common/OpBitsCompression.cpp:65-83). */
+int ps_opbits_encode(int32_t bits, const uint8_t* d_ops, uint64_t count, uint8_t* d_out, ps_stream s);
+int ps_opbits_decode(int32_t bits, const uint8_t* d_in, uint64_t count, uint8_t* d_ops, ps_stream s);
+/* FrontierCompression::EncodeWithCheck (with_check = 1) or Encode per chunk:
+ * a bitmap chunk (u32 words | 0x80000000, u32 first, bitmap of v - first - 1)
+ * when it is smaller, else StreamVInt (Common/FrontierCompression.cpp:28-115). */
+int ps_fc_encode(const uint32_t* d_values, const uint64_t* d_starts, uint32_t nchunks, int32_t with_check,
+                 uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, ps_stream s);
+/* FrontierCompression::Decode per chunk (Common/FrontierCompression.cpp:117-125). */
+int ps_fc_decode(const uint8_t* d_in, const uint64_t* d_in_offsets, uint32_t nchunks, uint32_t* d_values,
+                 uint64_t values_cap, uint64_t* d_value_starts, ps_stream s);
+/* Groups cross-segment children by (op, target segment) like
+ * SmartMultiplexor::Add (Common/Multiplexor.cpp:341-353), in canonical order:
+ * bucket b = op * nseg + segment holds the sorted in-segment indexes
+ * d_sorted_idx[starts[b] .. starts[b+1]); d_bucket_starts has nops*nseg+1
+ * entries. */
+int ps_bucket_by_segment(const uint64_t* d_children, const uint8_t* d_ops, uint64_t count, int32_t segment_bits,
+                         uint32_t nseg, int32_t nops, uint32_t* d_sorted_idx, uint64_t* d_bucket_starts,
+                         ps_stream s);
+
 /* ----------------------------------------------------------- device BFS */
 typedef struct ps_bfs ps_bfs;
 

@@ -108,6 +108,24 @@ def build_host(jobs):
     return HOST_LIB
 
 
+HOST_TESTS_SRC = os.path.join(ROOT, "tests", "cpp", "host_tests.cpp")
+HOST_TESTS_BIN = os.path.join(ROOT, "tests", "cpp", "host_tests")
+
+
+def build_host_tests():
+    """The C++ test program of the host mirror (tests/cpp/host_tests)."""
+    if not os.path.exists(HOST_TESTS_SRC):
+        return None
+    tmp = HOST_TESTS_BIN + ".tmp"
+    rc, log = _run([CXX, "-O1", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{HOST}", HOST_TESTS_SRC, "-o", tmp,
+                    f"-L{OUT}", "-lpuzzlesearch_host", "-lpuzzlesearch_amd", "-Wl,-rpath,$ORIGIN/../../puzzlesearch_amd/lib", "-lpthread"])
+    if rc != 0:
+        print(log, file=sys.stderr)
+        raise RuntimeError("build of tests/cpp/host_tests failed")
+    os.replace(tmp, HOST_TESTS_BIN)
+    return HOST_TESTS_BIN
+
+
 def build(jobs=None, only=None):
     jobs = jobs or min(8, os.cpu_count() or 4)
     out = []
@@ -117,6 +135,9 @@ def build(jobs=None, only=None):
         h = build_host(jobs)
         if h:
             out.append(h)
+            t = build_host_tests()
+            if t:
+                out.append(t)
     return out
 
 
