@@ -81,19 +81,37 @@ def cpu_baseline(psa, puzzle, target_seconds=12.0):
     nxt = np.zeros_like(frontier)
     L = O.lib()
     children = ctypes.c_uint64(0)
-    # calibrate on a small span, then run a span sized for target_seconds
-    span = 1 << 20
-    t0 = time.perf_counter()
-    L.or_bfs3_expand_range(ctypes.byref(op), frontier, nxt, 0, span, ctypes.byref(children))
-    dt = time.perf_counter() - t0
-    span = int(min(N, max(span, span * target_seconds / max(dt, 1e-3))))
-    nxt[:] = 0
-    t0 = time.perf_counter()
-    parents = L.or_bfs3_expand_range(ctypes.byref(op), frontier, nxt, 0, span, ctypes.byref(children))
-    dt = time.perf_counter() - t0
+    # consecutive slices of the layer until target_seconds of CPU work are done
+    step = 1 << 22
+    first, parents, nchildren, dt = 0, 0, 0, 0.0
+    while first < N and dt < target_seconds:
+        span = min(step, N - first)
+        t0 = time.perf_counter()
+        parents += L.or_bfs3_expand_range(ctypes.byref(op), frontier, nxt, first, span, ctypes.byref(children))
+        dt += time.perf_counter() - t0
+        nchildren += children.value
+        first += span
+    span = first
+    children = ctypes.c_uint64(nchildren)
     return {"value": parents / dt, "unit": "expanded states/s", "cores": 1, "kind": "port",
             "sample": f"first {span} indexes of layer {best_layer} of the {n}-pancake BFS "
                       f"({parents} parents, {children.value} children marked, {dt:.1f} s, 1 thread)"}
+
+
+def copy_calibration(psa, nbytes, reps=5):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE readings
+    per dispatch give the scale between those counters and bytes moved."""
+    src = psa.DeviceBuffer(nbytes)
+    dst = psa.DeviceBuffer(nbytes)
+    src.zero()
+    psa.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        psa.check(psa.lib().ps_copy_async(dst.ptr, src.ptr, nbytes, None))
+    psa.sync()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+                      "ms_per_copy": dt * 1e3, "GBps": 2 * nbytes / dt / 1e9}))
 
 
 def main():
@@ -105,6 +123,8 @@ def main():
     ap.add_argument("--segment-bits", type=int, default=22)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a kernel copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -113,6 +133,9 @@ def main():
 
     import puzzlesearch_amd as psa
     psa.init(0)
+    if args.copy_calibration:
+        copy_calibration(psa, args.copy_calibration << 20)
+        return
     props = psa.device_props()
     log("device:", props)
     n = args.n

@@ -109,6 +109,10 @@ int ps_memcpy_d2h_async(void* dst, const void* src, uint64_t bytes, ps_stream s)
 int ps_memcpy_d2d_async(void* dst, const void* src, uint64_t bytes, ps_stream s);
 /* Fill with a byte value using a kernel (safe inside captured graphs). */
 int ps_fill_async(void* dst, uint8_t value, uint64_t bytes, ps_stream s);
+/* Device-to-device copy by a kernel (16-byte aligned pointers and size); no
+ * reference counterpart: it is the known-size copy that calibrates the HBM
+ * traffic counters against which the other kernels are measured. */
+int ps_copy_async(void* dst, const void* src, uint64_t bytes, ps_stream s);
 
 /* ----------------------------------------------------------- expansion */
 /* PuzzleGpu::ExpandGpu (Puzzles/PuzzleGpu.h:30): d_in[i] = index << ops |

@@ -152,6 +152,42 @@ int or_fc_encode(int count, uint32_t* idx, uint8_t* buf);
 int or_fc_encode_with_check(int count, uint32_t* idx, uint8_t* buf);
 int or_fc_decode(int* size, const uint8_t* buf, uint32_t* idx);
 
+/* ---- one segment step of the disk-based frontier search ----
+ * DB_OptFS_Solver<4>::Expand(segment) (Common/DiskBasedOptFrontierSearch.cpp:53-132)
+ * with in-memory streams.  Inputs: the segment's frontier (idx ascending,
+ * opBits), and optionally the cross-segment marks of the previous step
+ * (xseg_idx[xseg_op_starts[op] .. xseg_op_starts[op+1]) per op).  Outputs
+ * (caller buffers): the new frontier list and its StreamVInt-with-bounds
+ * stream (2^18-value chunks); the cross-segment buckets in (op, seg) order,
+ * each as its FrontierCompression::EncodeWithCheck chunks of 4096 values in
+ * emission order, concatenated in xs_bytes.  Returns the layer count of the
+ * segment or -1 (capacity or argument error). */
+typedef struct or_fs_io {
+    uint32_t* fr_idx;
+    uint8_t* fr_ops;
+    uint64_t fr_cap;
+    uint64_t fr_n;
+    uint8_t* fr_bytes;
+    uint64_t fr_bytes_cap;
+    uint64_t fr_nbytes;
+    uint32_t* bk_op;
+    uint32_t* bk_seg;
+    uint64_t* bk_values;
+    uint64_t* bk_nbytes;
+    uint64_t bk_cap;
+    uint64_t bk_n;
+    uint8_t* xs_bytes;
+    uint64_t xs_cap;
+    uint64_t xs_nbytes;
+    uint64_t children_in;
+    uint64_t children_cross;
+    uint64_t count;
+} or_fs_io;
+
+int64_t or_fs_segment_step(const or_puzzle* p, int segment_bits, uint32_t segment, uint32_t nseg,
+                           const uint32_t* idx, const uint8_t* opbits, uint64_t n,
+                           const uint32_t* xseg_idx, const uint64_t* xseg_op_starts, or_fs_io* io);
+
 #ifdef __cplusplus
 }
 #endif

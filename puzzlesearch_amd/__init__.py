@@ -121,6 +121,7 @@ _SIGS = {
     "ps_memcpy_d2h_async": (_i32, [_vp, _vp, _u64, _vp]),
     "ps_memcpy_d2d_async": (_i32, [_vp, _vp, _u64, _vp]),
     "ps_fill_async": (_i32, [_vp, ctypes.c_uint8, _u64, _vp]),
+    "ps_copy_async": (_i32, [_vp, _vp, _u64, _vp]),
     "ps_expand": (_i32, [_P, _i32, _vp, _vp, _u64, _vp]),
     "ps_pancake_xseg_postprocess": (_i32, [_i32, _i32, _vp, _u64, _vp]),
     "ps_bits_andnot_count": (_i32, [_vp, _vp, _vp, _u64, _vp, _vp]),
@@ -128,6 +129,13 @@ _SIGS = {
     "ps_bits_hash": (_i32, [_vp, _u64, _u64, _vp, _vp]),
     "ps_bits_scan": (_i32, [_vp, _u64, _vp, _u64, _vp, _vp]),
     "ps_bits_set_list": (_i32, [_vp, _vp, _u64, _vp]),
+    "ps_svint_encode": (_i32, [_vp, _vp, _vp, ctypes.c_uint32, _vp, _u64, _vp, _vp]),
+    "ps_svint_decode": (_i32, [_vp, _vp, ctypes.c_uint32, _i32, _vp, _vp, _u64, _vp, _vp]),
+    "ps_opbits_encode": (_i32, [_i32, _vp, _u64, _vp, _vp]),
+    "ps_opbits_decode": (_i32, [_i32, _vp, _u64, _vp, _vp]),
+    "ps_fc_encode": (_i32, [_vp, _vp, ctypes.c_uint32, _i32, _vp, _u64, _vp, _vp]),
+    "ps_fc_decode": (_i32, [_vp, _vp, ctypes.c_uint32, _vp, _u64, _vp, _vp]),
+    "ps_bucket_by_segment": (_i32, [_vp, _vp, _u64, _i32, ctypes.c_uint32, _i32, _vp, _vp, _vp]),
     "ps_bfs_options_default": (None, [ctypes.POINTER(BfsOptions)]),
     "ps_bfs_create": (_i32, [_P, ctypes.POINTER(BfsOptions), _vp, ctypes.POINTER(_vp)]),
     "ps_bfs_destroy": (_i32, [_vp]),

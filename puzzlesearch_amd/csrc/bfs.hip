@@ -95,22 +95,6 @@ namespace ps {
 
 namespace {
 
-int hip_cu_count() {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
-    return prop.multiProcessorCount;
-}
-
-size_t hip_lds_limit() {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 65536;
-    return prop.sharedMemPerBlock;
-}
-
 int take_event(ps_bfs* b, hipEvent_t* out, size_t* idx) {
     if (b->ev_used == b->ev_pool.size()) {
         hipEvent_t e;
@@ -323,8 +307,12 @@ int destroy_bfs(ps_bfs* b) {
     for (auto e : b->ev_pool) (void)hipEventDestroy(e);
     if (b->ev_run0) (void)hipEventDestroy(b->ev_run0);
     if (b->ev_run1) (void)hipEventDestroy(b->ev_run1);
-    for (int i = 0; i < 3; ++i)
-        if (b->arr[i]) (void)hipFree(b->arr[i]);
+    // the two-bit search aliases arr[1] and arr[2] to arr[0]
+    for (int i = 0; i < 3; ++i) {
+        bool alias = false;
+        for (int j = 0; j < i; ++j) alias = alias || b->arr[j] == b->arr[i];
+        if (b->arr[i] && !alias) (void)hipFree(b->arr[i]);
+    }
     if (b->seg_count) (void)hipFree(b->seg_count);
     if (b->touched) (void)hipFree(b->touched);
     if (b->layer_counts) (void)hipFree(b->layer_counts);
@@ -335,6 +323,8 @@ int destroy_bfs(ps_bfs* b) {
     if (b->h_touched) (void)hipHostFree(b->h_touched);
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+    // a failed release above must not surface as the error of a later, unrelated call
+    (void)hipGetLastError();
     return PS_OK;
 }
 

@@ -799,6 +799,7 @@ int ps_bucket_by_segment(const uint64_t* d_children, const uint8_t* d_ops, uint6
     if (segment_bits < 1 || segment_bits > 32) return fail(PS_ERR_INVALID_ARGUMENT, "segment_bits must be 1..32");
     if (nops < 1 || nops > 16) return fail(PS_ERR_INVALID_ARGUMENT, "nops must be 1..16");
     if (nseg == 0 || (uint64_t)nseg * nops > (1ull << 24)) return fail(PS_ERR_INVALID_ARGUMENT, "bad segment count");
+    if (count > 0x7FFFFFFFull) return fail(PS_ERR_INVALID_ARGUMENT, "at most 2^31-1 children per call");
     if (!d_bucket_starts || (count && (!d_children || !d_ops || !d_sorted_idx)))
         return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
     hipStream_t s = stream_of(st);

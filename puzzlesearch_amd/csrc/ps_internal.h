@@ -15,6 +15,9 @@ int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
 bool runtime_ready();
 int require_runtime();
+// compute units and LDS bytes per block of the current device
+int hip_cu_count();
+size_t hip_lds_limit();
 
 // Host-side puzzle facts (Common/Puzzle.h:22-26 for each reference class).
 int puzzle_query(const ps_puzzle* p, ps_puzzle_info* out);
@@ -32,9 +35,11 @@ int launch_fill(void* dst, uint8_t value, uint64_t bytes, hipStream_t s);
 #define PS_HIP_CHECK(expr)                                                                               \
     do {                                                                                                 \
         hipError_t _e = (expr);                                                                          \
-        if (_e != hipSuccess)                                                                            \
+        if (_e != hipSuccess) {                                                                          \
+            (void)hipGetLastError(); /* reported here; not again by a later call */                      \
             return ::ps::fail(PS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, \
                               __LINE__);                                                                 \
+        }                                                                                                \
     } while (0)
 
 // Wraps a C-ABI body so no C++ exception crosses the boundary.

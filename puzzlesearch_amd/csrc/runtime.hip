@@ -2,6 +2,7 @@
 // error reporting.  Replaces the reference's CUDA shim (Puzzles/GPU.cu:5-58),
 // whose ERR() throws; here every failure becomes a status code plus a
 // per-thread message.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -52,6 +53,32 @@ __global__ void k_fill_bytes(uint8_t* dst, uint8_t v, uint64_t n) {
     for (uint64_t i = i0; i < nvec; i += stride) vd[i] = make_uint4(w, w, w, w);
     const uint64_t tail0 = h + nvec * 16;
     if (i0 < n - tail0) dst[tail0 + i0] = v;
+}
+
+int hip_cu_count() {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    return v;
+}
+
+size_t hip_lds_limit() {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) {
+        (void)hipGetLastError();
+        return 65536;
+    }
+    return (size_t)v;
+}
+
+// 16-byte vector copy; both pointers and the size must be multiples of 16
+__global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
 int launch_fill(void* dst, uint8_t value, uint64_t bytes, hipStream_t s) {
@@ -242,6 +269,22 @@ int ps_fill_async(void* dst, uint8_t value, uint64_t bytes, ps_stream s) {
     if (int r = require_runtime()) return r;
     if (bytes && !dst) return fail(PS_ERR_INVALID_ARGUMENT, "NULL pointer");
     return launch_fill(dst, value, bytes, stream_of(s));
+    PS_API_END
+}
+
+int ps_copy_async(void* dst, const void* src, uint64_t bytes, ps_stream s) {
+    PS_API_BEGIN
+    if (int r = require_runtime()) return r;
+    if (bytes == 0) return PS_OK;
+    if (!dst || !src) return fail(PS_ERR_INVALID_ARGUMENT, "NULL pointer");
+    if (((uintptr_t)dst | (uintptr_t)src | bytes) & 15)
+        return fail(PS_ERR_INVALID_ARGUMENT, "pointers and size must be multiples of 16 bytes");
+    const uint64_t n16 = bytes / 16;
+    const uint64_t blocks = std::min<uint64_t>((n16 + 255) / 256, (uint64_t)hip_cu_count() * 16);
+    hipLaunchKernelGGL(k_copy16, dim3((uint32_t)std::max<uint64_t>(1, blocks)), dim3(256), 0, stream_of(s),
+                       (uint4*)dst, (const uint4*)src, n16);
+    PS_HIP_CHECK(hipGetLastError());
+    return PS_OK;
     PS_API_END
 }
 

@@ -68,7 +68,19 @@ _P = ctypes.POINTER(Puzzle)
 _i = ctypes.c_int
 _u64 = ctypes.c_uint64
 
+class FsIo(ctypes.Structure):
+    """or_fs_io (oracle.h)."""
+    _fields_ = [("fr_idx", ctypes.c_void_p), ("fr_ops", ctypes.c_void_p), ("fr_cap", _u64), ("fr_n", _u64),
+                ("fr_bytes", ctypes.c_void_p), ("fr_bytes_cap", _u64), ("fr_nbytes", _u64),
+                ("bk_op", ctypes.c_void_p), ("bk_seg", ctypes.c_void_p), ("bk_values", ctypes.c_void_p),
+                ("bk_nbytes", ctypes.c_void_p), ("bk_cap", _u64), ("bk_n", _u64),
+                ("xs_bytes", ctypes.c_void_p), ("xs_cap", _u64), ("xs_nbytes", _u64),
+                ("children_in", _u64), ("children_cross", _u64), ("count", _u64)]
+
+
 _SIGS = {
+    "or_fs_segment_step": (ctypes.c_int64, [_P, _i, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u8p, _u64,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(FsIo)]),
     "or_splitmix64": (_u64, [_u64]),
     "or_bits_hash": (_u64, [_u64p, _u64]),
     "or_children_hash": (_u64, [_u64p, _i32p, _u64]),
@@ -233,3 +245,46 @@ def start_index(p, start):
     r = parse(p, start)
     assert r != INVALID, start
     return r
+
+
+def fs_segment_step(p, segment_bits, segment, nseg, idx, opbits, xseg=None):
+    """One frontier-search segment step (or_fs_segment_step); returns a dict with
+    count, the new frontier (idx, ops), its stream bytes, and the buckets
+    [(op, seg, nvalues, bytes)] in (op, seg) order."""
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    opbits = np.ascontiguousarray(opbits, dtype=np.uint8)
+    bf = int(lib().or_branching_factor(ctypes.byref(p)))
+    ops = int(lib().or_operators_count(ctypes.byref(p)))
+    xs_idx = xs_st = None
+    nx = 0
+    if xseg is not None:
+        xs_idx = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint32) for x in xseg]), dtype=np.uint32)
+        xs_st = np.zeros(ops + 1, np.uint64)
+        xs_st[1:] = np.cumsum([len(x) for x in xseg])
+        nx = len(xs_idx)
+    cap = len(idx) * bf + nx + 8
+    fr_idx = np.zeros(cap, np.uint32)
+    fr_ops = np.zeros(cap, np.uint8)
+    fr_bytes = np.zeros(5 * cap + 64 * (cap // (1 << 18) + 1) + 64, np.uint8)
+    bk_cap = cap
+    bk_op = np.zeros(bk_cap, np.uint32)
+    bk_seg = np.zeros(bk_cap, np.uint32)
+    bk_values = np.zeros(bk_cap, np.uint64)
+    bk_nbytes = np.zeros(bk_cap, np.uint64)
+    xs_bytes = np.zeros(5 * cap * bf + 64 * (cap * bf // 4096 + 1) + 64, np.uint8)
+    io = FsIo(fr_idx.ctypes.data, fr_ops.ctypes.data, cap, 0, fr_bytes.ctypes.data, len(fr_bytes), 0,
+              bk_op.ctypes.data, bk_seg.ctypes.data, bk_values.ctypes.data, bk_nbytes.ctypes.data, bk_cap, 0,
+              xs_bytes.ctypes.data, len(xs_bytes), 0, 0, 0, 0)
+    r = lib().or_fs_segment_step(ctypes.byref(p), segment_bits, segment, nseg, idx, opbits, len(idx),
+                                 xs_idx.ctypes.data if xs_idx is not None else None,
+                                 xs_st.ctypes.data if xs_st is not None else None, ctypes.byref(io))
+    assert r >= 0, "oracle segment step failed"
+    buckets = []
+    off = 0
+    for b in range(io.bk_n):
+        nb = int(bk_nbytes[b])
+        buckets.append((int(bk_op[b]), int(bk_seg[b]), int(bk_values[b]), xs_bytes[off:off + nb].copy()))
+        off += nb
+    return {"count": int(r), "fr_idx": fr_idx[:io.fr_n].copy(), "fr_ops": fr_ops[:io.fr_n].copy(),
+            "fr_bytes": fr_bytes[:io.fr_nbytes].copy(), "buckets": buckets,
+            "children_in": int(io.children_in), "children_cross": int(io.children_cross)}

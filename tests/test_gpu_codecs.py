@@ -1,0 +1,189 @@
+"""Device codecs against the CPU oracle codecs, byte for byte: StreamVInt with
+and without bounds (Common/StreamVInt.cpp:278-368), OpBits<2/4/8>
+(Common/OpBitsCompression.cpp:15-83), FrontierCompression Encode /
+EncodeWithCheck / Decode (Common/FrontierCompression.cpp:92-125), and bucketing
+of cross-segment children by (op, segment) (Common/Multiplexor.cpp:341-353)
+against a numpy sort.  Edge cases: empty chunks, counts not a multiple of 8,
+the 2^18-value maximum chunk, unsorted chunks, the reference's bitmap
+selection cases."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def C(ps):
+    from puzzlesearch_amd import codecs
+    return codecs
+
+
+def oracle_svi(chunk, bounds=None):
+    n = len(chunk)
+    if n == 0:
+        return np.zeros(0, np.uint8)
+    adj = (n + 7) // 8 * 8
+    v = np.zeros(adj, np.uint32)
+    v[:n] = chunk
+    buf = np.zeros(4 + 17 * adj // 4 + adj // 2 + 32, np.uint8)
+    if bounds is None:
+        size = O.lib().or_svint_encode(n, v, buf)
+    else:
+        b = np.zeros(adj, np.uint8)
+        b[:n] = bounds
+        size = O.lib().or_svint_encode_with_bounds(n, v, b, buf)
+    return buf[:size]
+
+
+def oracle_fc(chunk, check):
+    n = len(chunk)
+    if n == 0:
+        return np.zeros(0, np.uint8)
+    v = np.zeros((n + 7) // 8 * 8 + 8, np.uint32)
+    v[:n] = chunk
+    buf = np.zeros(4 + 17 * len(v) // 4 + 64 + ((int(max(chunk)) - int(min(chunk))) // 8 + 64), np.uint8)
+    size = (O.lib().or_fc_encode_with_check if check else O.lib().or_fc_encode)(n, v, buf)
+    return buf[:size]
+
+
+def sorted_unique(rng, n, hi):
+    return np.sort(rng.choice(hi, size=n, replace=False)).astype(np.uint32)
+
+
+def svi_chunks(rng):
+    sizes = [0, 1, 3, 7, 8, 9, 17, 1000, 0, 4096, 1 << 18, 5]
+    chunks = []
+    for k, n in enumerate(sizes):
+        if k % 3 == 0:
+            chunks.append(sorted_unique(rng, n, 1 << 32) if n else np.zeros(0, np.uint32))
+        elif k % 3 == 1:
+            chunks.append(rng.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32))  # unsorted: wrapping deltas
+        else:
+            chunks.append(sorted_unique(rng, n, max(4 * n, 16)))  # small deltas
+    return chunks
+
+
+def test_svint_matches_oracle(C):
+    rng = np.random.default_rng(11)
+    chunks = svi_chunks(rng)
+    data, off = C.svint_encode(chunks)
+    expect = np.concatenate([oracle_svi(c) for c in chunks])
+    assert len(data) == len(expect)
+    assert np.array_equal(data, expect)
+    for c, ch in enumerate(chunks):
+        assert off[c + 1] - off[c] == len(oracle_svi(ch))
+    back, _ = C.svint_decode(data, off)
+    for a, b in zip(back, chunks):
+        assert np.array_equal(a, b)
+
+
+def test_svint_bounds_matches_oracle(C):
+    rng = np.random.default_rng(12)
+    chunks = svi_chunks(rng)
+    bounds = [rng.integers(0, 16, size=len(c), dtype=np.uint8) for c in chunks]
+    data, off = C.svint_encode(chunks, bounds)
+    expect = np.concatenate([oracle_svi(c, b) for c, b in zip(chunks, bounds)])
+    assert np.array_equal(data, expect)
+    back, bb = C.svint_decode(data, off, with_bounds=True)
+    for a, b, x, y in zip(back, chunks, bb, bounds):
+        assert np.array_equal(a, b)
+        assert np.array_equal(x, y)
+
+
+def test_svint_reference_frontier_case(C):
+    # CommonTests/StreamVIntTests.cpp:130-146
+    i = np.arange(65536, dtype=np.uint64)
+    vals = (i * i).astype(np.uint32)
+    bnd = (i % 16).astype(np.uint8)
+    data, off = C.svint_encode([vals], [bnd])
+    assert np.array_equal(data, oracle_svi(vals, bnd))
+    back, bb = C.svint_decode(data, off, with_bounds=True)
+    assert np.array_equal(back[0], vals) and np.array_equal(bb[0], bnd)
+
+
+def test_svint_rejects_oversized_chunk(C, ps):
+    with pytest.raises(ps.PSError):
+        C.svint_encode([np.arange((1 << 18) + 1, dtype=np.uint32)])
+
+
+@pytest.mark.parametrize("bits", [2, 4, 8])
+def test_opbits_matches_oracle(C, bits):
+    rng = np.random.default_rng(bits)
+    for n in (8, 64, 8 * 100003):
+        ops = rng.integers(0, 1 << bits, size=n, dtype=np.uint8)
+        enc = C.opbits_encode(bits, ops)
+        ref = np.zeros(n * bits // 8, np.uint8)
+        assert O.lib().or_opbits_encode(bits, n, ops, ref) == len(ref)
+        assert np.array_equal(enc, ref)
+        assert np.array_equal(C.opbits_decode(bits, enc, n), ops)
+
+
+def fc_chunks(rng):
+    chunks = [np.zeros(0, np.uint32),
+              np.arange(150, dtype=np.uint32) * 11,                    # StreamVInt (CompressionTests.cpp:25)
+              np.arange(150, dtype=np.uint32) * 7,                     # bitmap (:43)
+              np.array([i * 7 for i in range(150)] + [700], np.uint32),  # check rejects the bitmap (:61)
+              np.arange(100, dtype=np.uint32) * 2,                     # <= 128 values: StreamVInt
+              sorted_unique(rng, 4096, 16384) + np.uint32(1 << 30),    # dense: bitmap
+              sorted_unique(rng, 4096, 1 << 31),                       # sparse: StreamVInt
+              sorted_unique(rng, 1 << 18, 1 << 19),                    # dense maximum chunk
+              sorted_unique(rng, 129, 140),                            # smallest count with a bitmap
+              ]
+    return chunks
+
+
+@pytest.mark.parametrize("check", [1, 0])
+def test_fc_matches_oracle(C, check):
+    rng = np.random.default_rng(21 + check)
+    chunks = fc_chunks(rng)
+    if not check:
+        chunks = [c for k, c in enumerate(chunks) if k != 3]  # Encode without the check needs sorted chunks
+    data, off = C.fc_encode(chunks, with_check=bool(check))
+    expect = [oracle_fc(c, check) for c in chunks]
+    for c in range(len(chunks)):
+        got = data[int(off[c]):int(off[c + 1])]
+        assert np.array_equal(got, expect[c]), f"chunk {c}"
+    back = C.fc_decode(data, off)
+    for a, b in zip(back, chunks):
+        assert np.array_equal(a, b)
+    # the oracle decodes the device bytes too
+    for c in range(len(chunks)):
+        if len(chunks[c]) == 0:
+            continue
+        seg = np.concatenate([data[int(off[c]):int(off[c + 1])], np.zeros(64, np.uint8)])
+        out = np.zeros(len(chunks[c]) + 64, np.uint32)
+        sz = ctypes.c_int(int(off[c + 1] - off[c]))
+        n = O.lib().or_fc_decode(ctypes.byref(sz), seg, out)
+        assert n == len(chunks[c]) and np.array_equal(out[:n], chunks[c])
+
+
+def test_fc_bitmap_selected_where_reference_selects_it(C):
+    data, off = C.fc_encode([np.arange(150, dtype=np.uint32) * 7, np.arange(150, dtype=np.uint32) * 11])
+    assert data[int(off[0]) + 3] & 0x80
+    assert not data[int(off[1]) + 3] & 0x80
+
+
+def test_bucket_by_segment(C):
+    rng = np.random.default_rng(31)
+    seg_bits, nseg, nops = 20, 37, 4
+    n = 300000
+    children = rng.integers(0, nseg << seg_bits, size=n, dtype=np.uint64)
+    ops = rng.integers(0, nops, size=n, dtype=np.uint8)
+    idx, starts = C.bucket_by_segment(children, ops, seg_bits, nseg, nops)
+    key = ops.astype(np.uint64) * np.uint64(nseg) + (children >> np.uint64(seg_bits))
+    order = np.lexsort((children, key))
+    expect_idx = (children[order] & np.uint64((1 << seg_bits) - 1)).astype(np.uint32)
+    counts = np.bincount(key.astype(np.int64), minlength=nops * nseg)
+    assert np.array_equal(np.diff(starts.astype(np.int64)), counts)
+    assert np.array_equal(idx, expect_idx)
+
+
+def test_bucket_rejects_out_of_range(C, ps):
+    children = np.array([1, 2, 5 << 10], np.uint64)
+    ops = np.array([0, 1, 0], np.uint8)
+    with pytest.raises(ps.PSError):
+        C.bucket_by_segment(children, ops, 10, 4, 2)
