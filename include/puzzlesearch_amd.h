@@ -159,8 +159,10 @@ int ps_bits_set_list(uint64_t* d_a, const uint64_t* d_idx, uint64_t count, ps_st
 int ps_svint_encode(const uint32_t* d_values, const uint8_t* d_bounds, const uint64_t* d_starts, uint32_t nchunks,
                     uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, ps_stream s);
 /* StreamVInt::Decode (Common/StreamVInt.cpp:302-368) of nchunks concatenated
- * chunks (chunk c at d_in + d_in_offsets[c]).  Chunk c's values go to
- * d_values + value_starts[c]; value_starts (nchunks+1, device) is written. */
+ * chunks: chunk c is d_in[d_in_offsets[c] .. d_in_offsets[c+1]) (nchunks+1
+ * device offsets, as the encoders write them; an empty chunk has no bytes).
+ * Chunk c's values go to d_values + value_starts[c]; value_starts (nchunks+1,
+ * device) is written. */
 int ps_svint_decode(const uint8_t* d_in, const uint64_t* d_in_offsets, uint32_t nchunks, int32_t with_bounds,
                     uint32_t* d_values, uint8_t* d_bounds, uint64_t values_cap, uint64_t* d_value_starts,
                     ps_stream s);
@@ -173,7 +175,8 @@ int ps_opbits_decode(int32_t bits, const uint8_t* d_in, uint64_t count, uint8_t*
  * when it is smaller, else StreamVInt (Common/FrontierCompression.cpp:28-115). */
 int ps_fc_encode(const uint32_t* d_values, const uint64_t* d_starts, uint32_t nchunks, int32_t with_check,
                  uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, ps_stream s);
-/* FrontierCompression::Decode per chunk (Common/FrontierCompression.cpp:117-125). */
+/* FrontierCompression::Decode per chunk (Common/FrontierCompression.cpp:117-125);
+ * chunks delimited by nchunks+1 device offsets as for ps_svint_decode. */
 int ps_fc_decode(const uint8_t* d_in, const uint64_t* d_in_offsets, uint32_t nchunks, uint32_t* d_values,
                  uint64_t values_cap, uint64_t* d_value_starts, ps_stream s);
 /* Groups cross-segment children by (op, target segment) like
@@ -185,10 +188,60 @@ int ps_bucket_by_segment(const uint64_t* d_children, const uint8_t* d_ops, uint6
                          uint32_t nseg, int32_t nops, uint32_t* d_sorted_idx, uint64_t* d_bucket_starts,
                          ps_stream s);
 
+/* ------------------------------------------------- disk-segment step */
+/* One segment step of the disk-based frontier search with 4-bit op flags,
+ * DB_OptFS_Solver<4>::Expand(segment) (Common/DiskBasedOptFrontierSearch.cpp:53-132),
+ * for puzzles with 1..4 moves.  The workspace holds the segment's op-flag
+ * array (2^segment_bits / 2 bytes) and grows its output buffers as needed;
+ * one step at a time per workspace. */
+typedef struct ps_fs_segment ps_fs_segment;
+
+typedef struct ps_fs_result {
+    uint64_t count;           /* states of the new layer in the segment (Expand's return value) */
+    uint64_t frontier_count;  /* entries written to the new frontier (a move left unused) */
+    uint64_t frontier_bytes;  /* their StreamVInt-with-bounds stream */
+    uint64_t frontier_chunks; /* chunks of 2^18 values */
+    uint64_t children_in;     /* in-segment children marked */
+    uint64_t children_cross;  /* children bucketed for other segments */
+    uint64_t buckets;         /* nonempty (op, segment) buckets */
+    uint64_t cross_chunks;    /* FrontierCompression chunks of 4096 values over all buckets */
+    uint64_t cross_bytes;
+    uint64_t scanned_words;   /* op-flag words in the regions the scan visited */
+    uint64_t algorithmic_bytes; /* inputs + outputs + 8 per child */
+    double device_ms;         /* first to last kernel of the step */
+    double wall_ms;
+} ps_fs_result;
+
+int ps_fs_segment_create(const ps_puzzle* p, int32_t segment_bits, ps_stream s, ps_fs_segment** out);
+int ps_fs_segment_destroy(ps_fs_segment* w);
+int ps_fs_segment_info(const ps_fs_segment* w, uint32_t* segments, int32_t* operators, uint64_t* footprint_bytes);
+/* Inputs (device): the segment's frontier d_idx[n] (in-segment ids, ascending)
+ * with op bits d_ops[n], as CompressedFrontierReader yields them; optionally the
+ * cross-segment marks of the previous step, d_xseg_idx[xseg_op_starts[op] ..
+ * xseg_op_starts[op+1]) for each op (xseg_op_starts: host, operators+1
+ * entries).  Outputs stay in the workspace until the next step. */
+int ps_fs_segment_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const uint8_t* d_ops, uint64_t n,
+                       const uint32_t* d_xseg_idx, const uint64_t* xseg_op_starts, ps_fs_result* res);
+/* The new frontier: device list (ids ascending, op bits) and its stream;
+ * chunk_offsets (host, frontier_chunks + 1 entries) receives chunk byte offsets. */
+int ps_fs_segment_frontier(const ps_fs_segment* w, const uint32_t** d_idx, const uint8_t** d_ops,
+                           const uint8_t** d_bytes, uint64_t* chunk_offsets, uint64_t cap);
+/* The cross-segment buckets in (op, segment) order: bucket b holds values[b]
+ * children whose compressed chunks are d_bytes[byte_offsets[b] .. byte_offsets[b+1])
+ * (host arrays of cap entries; byte_offsets needs cap + 1). */
+int ps_fs_segment_buckets(const ps_fs_segment* w, uint32_t* op, uint32_t* segment, uint64_t* values,
+                          uint64_t* byte_offsets, uint64_t cap, const uint8_t** d_bytes);
+
 /* ----------------------------------------------------------- device BFS */
 typedef struct ps_bfs ps_bfs;
 
 enum { PS_BFS_BATCHED = 0, PS_BFS_PLAIN = 1 };
+/* How the three-bit expand kernel marks a child outside its parent's LDS
+ * window: PS_CROSS_BYTES stores 1 into a byte-per-state array that the
+ * finalize pass folds into the next layer (no read-modify-write traffic; costs
+ * one byte of HBM per state), PS_CROSS_ATOMIC ORs the bit into the next-layer
+ * array with a device-scope atomic. */
+enum { PS_CROSS_BYTES = 0, PS_CROSS_ATOMIC = 1 };
 
 typedef struct ps_bfs_options {
     int32_t bits;           /* 3: InMemoryThreeBitBFS semantics, 2: InMemoryTwoBitBFS */
@@ -202,7 +255,8 @@ typedef struct ps_bfs_options {
     int32_t hash;           /* 1: per-layer frontier hashes */
     int32_t graph;          /* 1: capture the batched layer loop in a hipGraph (default on) */
     int32_t graph_layers;   /* layers per captured graph (0: default 4) */
-    int32_t reserved[2];
+    int32_t cross_marks;    /* PS_CROSS_BYTES (default) or PS_CROSS_ATOMIC, see below */
+    int32_t reserved;
 } ps_bfs_options;
 
 typedef struct ps_bfs_stats {

@@ -99,12 +99,18 @@ int64_t or_fs_segment_step(const or_puzzle* p, int segment_bits, uint32_t segmen
     }
     qsort(marks, nm, sizeof(uint64_t), cmp_u64);
 
-    /* 3 scan in index order */
-    uint64_t count = 0, fr = 0;
+    /* 3 scan in index order; with odd cycles, entries of the current frontier
+     * (FrontierArray, :93,117) are dropped.  idx[] is ascending. */
+    const int odd = or_has_odd_cycles(p);
+    uint64_t count = 0, fr = 0, cur = 0;
     for (uint64_t i = 0; i < nm;) {
         const uint64_t v = marks[i] >> 8;
         uint32_t bits = 0;
         while (i < nm && (marks[i] >> 8) == v) bits |= (uint32_t)(marks[i++] & 0xFF);
+        if (odd) {
+            while (cur < n && idx[cur] < v) cur++;
+            if (cur < n && idx[cur] == v) continue;
+        }
         count++;
         if (bits < mask) {
             if (fr >= io->fr_cap) return -1;

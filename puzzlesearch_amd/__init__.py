@@ -32,6 +32,7 @@ PANCAKE, SLIDING, SLIDING_PACKED, HANOI4, HANOI4_OPT = 1, 2, 3, 4, 5
 FLAG_SYMMETRY, FLAG_INV_INDEX = 1, 2
 EXPAND_FULL, EXPAND_IN_SEGMENT, EXPAND_CROSS_SEGMENT = 0, 1, 2
 BFS_BATCHED, BFS_PLAIN = 0, 1
+CROSS_BYTES, CROSS_ATOMIC = 0, 1
 INVALID_INDEX = (1 << 64) - 1
 
 
@@ -65,7 +66,7 @@ class BfsOptions(ctypes.Structure):
                 ("max_layers", ctypes.c_int32), ("index_count", ctypes.c_uint64),
                 ("tile_bits_log2", ctypes.c_int32), ("block_threads", ctypes.c_int32),
                 ("grid_blocks", ctypes.c_int32), ("hash", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("graph_layers", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("graph_layers", ctypes.c_int32), ("cross_marks", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class BfsStats(ctypes.Structure):
@@ -99,6 +100,16 @@ _vp = ctypes.c_void_p
 _u64 = ctypes.c_uint64
 _i32 = ctypes.c_int32
 _P = ctypes.POINTER(Puzzle)
+
+
+class FsResult(ctypes.Structure):
+    """ps_fs_result"""
+    _fields_ = [(n, _u64) for n in ("count", "frontier_count", "frontier_bytes", "frontier_chunks", "children_in",
+                                    "children_cross", "buckets", "cross_chunks", "cross_bytes", "scanned_words",
+                                    "algorithmic_bytes")] + [("device_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 _SIGS = {
     "ps_last_error": (ctypes.c_char_p, []),
@@ -136,6 +147,13 @@ _SIGS = {
     "ps_fc_encode": (_i32, [_vp, _vp, ctypes.c_uint32, _i32, _vp, _u64, _vp, _vp]),
     "ps_fc_decode": (_i32, [_vp, _vp, ctypes.c_uint32, _vp, _u64, _vp, _vp]),
     "ps_bucket_by_segment": (_i32, [_vp, _vp, _u64, _i32, ctypes.c_uint32, _i32, _vp, _vp, _vp]),
+    "ps_fs_segment_create": (_i32, [_P, _i32, _vp, ctypes.POINTER(_vp)]),
+    "ps_fs_segment_destroy": (_i32, [_vp]),
+    "ps_fs_segment_info": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_i32), ctypes.POINTER(_u64)]),
+    "ps_fs_segment_step": (_i32, [_vp, ctypes.c_uint32, _vp, _vp, _u64, _vp, _vp, ctypes.POINTER(FsResult)]),
+    "ps_fs_segment_frontier": (_i32, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
+                                      _u64]),
+    "ps_fs_segment_buckets": (_i32, [_vp, _vp, _vp, _vp, _vp, _u64, ctypes.POINTER(_vp)]),
     "ps_bfs_options_default": (None, [ctypes.POINTER(BfsOptions)]),
     "ps_bfs_create": (_i32, [_P, ctypes.POINTER(BfsOptions), _vp, ctypes.POINTER(_vp)]),
     "ps_bfs_destroy": (_i32, [_vp]),

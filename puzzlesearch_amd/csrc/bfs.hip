@@ -53,6 +53,7 @@ struct ps_bfs {
     int bits = 3;
     uint64_t nbits = 0, nwords = 0;
     uint64_t* arr[3] = {nullptr, nullptr, nullptr};
+    uint8_t* xmark = nullptr;  // byte per state, three-bit search with PS_CROSS_BYTES
     uint32_t nseg = 1;
     int seg_shift = 63;
     uint64_t seg_words = 0;
@@ -132,6 +133,7 @@ psk::ExpandArgs expand_args(ps_bfs* b) {
     a.st = b->st;
     a.persistent = 1;
     a.use_window = b->use_window;
+    a.xmark = b->xmark;
     return a;
 }
 
@@ -150,6 +152,7 @@ psk::FinalizeArgs finalize_args(ps_bfs* b) {
     f.seg_count = b->seg_count;
     f.touched = b->touched;
     f.st = b->st;
+    f.xmark = b->xmark;
     return f;
 }
 
@@ -313,6 +316,7 @@ int destroy_bfs(ps_bfs* b) {
         for (int j = 0; j < i; ++j) alias = alias || b->arr[j] == b->arr[i];
         if (b->arr[i] && !alias) (void)hipFree(b->arr[i]);
     }
+    if (b->xmark) (void)hipFree(b->xmark);
     if (b->seg_count) (void)hipFree(b->seg_count);
     if (b->touched) (void)hipFree(b->touched);
     if (b->layer_counts) (void)hipFree(b->layer_counts);
@@ -365,7 +369,8 @@ int geometry(const ps_puzzle* p, const ps_bfs_options* o, const ps_puzzle_info& 
 }
 
 uint64_t footprint_bytes(const ps_bfs_options* o, const Geometry& g, int max_layers) {
-    const uint64_t arrays = (o->bits == 2 ? 1 : 3) * g.nwords * 8;
+    uint64_t arrays = (o->bits == 2 ? 1 : 3) * g.nwords * 8;
+    if (o->bits == 3 && o->cross_marks == PS_CROSS_BYTES) arrays += g.nwords * 64;
     const uint64_t meta = 3ull * g.nseg * 8 + ((g.nseg + 31) / 32) * 4 + 2ull * max_layers * 8 + sizeof(BfsDevState);
     return arrays + meta;
 }
@@ -411,6 +416,8 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
     else ps_bfs_options_default(&opt);
     if (opt.bits != 2 && opt.bits != 3) return fail(PS_ERR_INVALID_ARGUMENT, "bits must be 2 or 3");
     if (opt.mode != PS_BFS_BATCHED && opt.mode != PS_BFS_PLAIN) return fail(PS_ERR_INVALID_ARGUMENT, "bad mode");
+    if (opt.cross_marks != PS_CROSS_BYTES && opt.cross_marks != PS_CROSS_ATOMIC)
+        return fail(PS_ERR_INVALID_ARGUMENT, "cross_marks must be PS_CROSS_BYTES or PS_CROSS_ATOMIC");
     if (opt.bits == 2 && opt.mode == PS_BFS_PLAIN) opt.mode = PS_BFS_BATCHED;
     ps_puzzle_info info;
     if (int r = puzzle_query(p, &info)) return r;
@@ -455,6 +462,8 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
     for (int i = 0; i < narr; ++i)
         if (int r = dalloc((void**)&b->arr[i], b->nwords * 8)) return cleanup(r);
     if (b->bits == 2) b->arr[1] = b->arr[2] = b->arr[0];
+    if (b->bits == 3 && opt.cross_marks == PS_CROSS_BYTES)
+        if (int r = dalloc((void**)&b->xmark, b->nwords * 64)) return cleanup(r);
     if (int r = dalloc((void**)&b->seg_count, (uint64_t)3 * b->nseg * 8)) return cleanup(r);
     if (int r = dalloc((void**)&b->touched, (uint64_t)((b->nseg + 31) / 32) * 4)) return cleanup(r);
     if (int r = dalloc((void**)&b->layer_counts, (uint64_t)b->max_layers * 8)) return cleanup(r);
@@ -529,6 +538,8 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     const int narr = b->bits == 2 ? 1 : 3;
     for (int i = 0; i < narr; ++i)
         if (int r = launch_fill(b->arr[i], 0, b->nwords * 8, b->stream)) return r;
+    if (b->xmark)
+        if (int r = launch_fill(b->xmark, 0, b->nwords * 64, b->stream)) return r;
     if (int r = launch_fill(b->seg_count, 0, (uint64_t)3 * b->nseg * 8, b->stream)) return r;
     if (int r = launch_fill(b->touched, 0, (uint64_t)((b->nseg + 31) / 32) * 4, b->stream)) return r;
     PS_HIP_CHECK(hipStreamSynchronize(b->stream));

@@ -61,6 +61,7 @@ struct ExpandArgs {
     BfsDevState* st;
     int32_t persistent;
     int32_t use_window;
+    uint8_t* xmark;  // byte per state for children outside the window; NULL: atomic OR into next
 };
 
 struct FinalizeArgs {
@@ -76,6 +77,7 @@ struct FinalizeArgs {
     unsigned long long* seg_count;
     uint32_t* touched;
     BfsDevState* st;
+    uint8_t* xmark;  // folded into next and cleared (NULL when the expand kernel uses atomics)
 };
 
 struct LayerEndArgs {
@@ -169,6 +171,31 @@ __device__ __forceinline__ void block_add3(unsigned long long a, unsigned long l
     }
 }
 
+// The 64 mark bytes (0 or 1) of one bit-array word as its bits, clearing
+// them; bytes are read and written as four 16-byte vectors.
+__device__ __forceinline__ uint64_t fold_marks(uint8_t* p) {
+    uint4* v = reinterpret_cast<uint4*>(p);
+    uint64_t bits = 0;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 x = v[k];
+        const uint32_t u[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            // bytes b0..b3 in {0,1}: the product gathers them at bits 28..31
+            const uint32_t nib = ((u[e] & 0x01010101u) * 0x10204080u) >> 28;
+            bits |= (uint64_t)nib << (16 * k + 4 * e);
+            any |= u[e] != 0u;
+        }
+    }
+    if (any) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    return bits;
+}
+
 // LDS bytes the expand kernel needs besides its static shared variables.
 inline size_t bfs3_lds_bytes(int tpb, uint64_t tile_words, int use_window, uint32_t nseg) {
     size_t b = (size_t)(tpb / 64) * QCAP * 4;
@@ -244,7 +271,8 @@ __global__ __launch_bounds__(TPB) void k_bfs3_expand(ExpandArgs a) {
                 lor(&win[(child - win_lo) >> 5], 1u << (child & 31));
                 local_any = true;
             } else {
-                gor(&nxt[child >> 5], 1u << (child & 31));
+                if (a.xmark) a.xmark[child] = 1;
+                else gor(&nxt[child >> 5], 1u << (child & 31));
                 if (track) {
                     const uint64_t s = child >> a.seg_shift;
                     if (tl_words) lor(&tl[s >> 5], 1u << (s & 31));
@@ -293,10 +321,16 @@ __global__ __launch_bounds__(TPB) void k_bfs3_expand(ExpandArgs a) {
         if (local_any) s_local = 1;
         __syncthreads();
         if (a.use_window) {
+            // with byte marks nothing else writes this range of next, so a tile
+            // that owns its whole range (processed once per layer) stores plainly
             uint32_t* dst = nxt + w0 * 2;
+            const bool own = a.xmark && pw0 == w0 && pw1 == w1;
             for (uint32_t i = tid; i < nwin; i += TPB) {
                 const uint32_t v = win[i];
-                if (v) gor(&dst[i], v);
+                if (v) {
+                    if (own) dst[i] = v;
+                    else gor(&dst[i], v);
+                }
             }
         }
         if (tid == 0 && track && s_local) {
@@ -354,7 +388,9 @@ __global__ __launch_bounds__(TPB) void k_bfs3_finalize(FinalizeArgs a) {
         unsigned long long cnt = 0, h = 0;
         if (touched) {
             for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) {
-                uint64_t v = nw[j] & ~old[j];
+                uint64_t v = nw[j];
+                if (a.xmark) v |= fold_marks(a.xmark + j * 64);
+                v &= ~old[j];
                 if (a.odd) v &= ~cur[j];
                 nw[j] = v;
                 old[j] = 0;

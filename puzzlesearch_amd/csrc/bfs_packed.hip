@@ -1,6 +1,8 @@
-// Instantiates the device BFS kernels for the packed puzzle family.
+// Instantiates the device BFS and segment-step kernels for the packed puzzle family.
 #include "bfs_launch.h"
+#include "fs_launch.h"
 
 namespace ps {
 PS_DEFINE_BFS_FAMILY(packed, FAM_PACKED)
+PS_DEFINE_FS_FAMILY(packed, FAM_PACKED)
 }  // namespace ps

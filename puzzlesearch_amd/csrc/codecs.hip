@@ -21,55 +21,16 @@
 #include <stdexcept>
 #include <vector>
 
+#include "codec_internal.h"
+#include "devbuf.h"
 #include "ps_internal.h"
 
 namespace ps {
 namespace {
 
-constexpr uint32_t MAX_CHUNK_VALUES = 256 * 1024;  // StreamVInt::MAX_INDEXES_COUNT
 constexpr uint32_t CODEC_MAP = 0x80000000u;
 
 __device__ __forceinline__ uint32_t vlen(uint32_t v) { return (v >> 24) ? 4u : (v >> 16) ? 3u : (v >> 8) ? 2u : 1u; }
-
-struct DevBuf {
-    void* p = nullptr;
-    hipStream_t s = nullptr;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFreeAsync(p, s);
-    }
-    int alloc(size_t bytes, hipStream_t st) {
-        s = st;
-        hipError_t e = hipMallocAsync(&p, bytes ? bytes : 16, st);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            return fail(PS_ERR_OUT_OF_MEMORY, "device scratch of %zu bytes: %s", bytes, hipGetErrorString(e));
-        }
-        return PS_OK;
-    }
-    template <class T>
-    T* as() const {
-        return reinterpret_cast<T*>(p);
-    }
-};
-
-template <class T>
-int to_device(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
-    if (int r = b.alloc(v.size() * sizeof(T), s)) return r;
-    if (!v.empty()) PS_HIP_CHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    return PS_OK;
-}
-
-template <class T>
-int to_host(std::vector<T>& v, const void* d, size_t n, hipStream_t s) {
-    v.resize(n);
-    if (n) PS_HIP_CHECK(hipMemcpyAsync(v.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, s));
-    PS_HIP_CHECK(hipStreamSynchronize(s));
-    return PS_OK;
-}
 
 // chunk of global tuple t (tuple_base has nchunks+1 entries)
 __device__ __forceinline__ uint32_t chunk_of(const uint64_t* tuple_base, uint32_t nchunks, uint64_t t) {
@@ -191,6 +152,11 @@ __global__ void k_read_counts(const uint8_t* in, const uint64_t* in_off, uint32_
                               uint32_t* is_bitmap) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nchunks) return;
+    if (in_off[c + 1] == in_off[c]) {  // an empty chunk encodes to no bytes
+        counts[c] = 0;
+        if (is_bitmap) is_bitmap[c] = 0;
+        return;
+    }
     const uint8_t* p = in + in_off[c];
     const uint32_t h = rd32(p);
     if (is_bitmap && (h & CODEC_MAP)) {
@@ -456,7 +422,6 @@ __global__ void k_bucket_split(const uint64_t* keys, uint64_t n, int seg_bits, u
     if (op < nops && seg < nseg) atomicAdd(&bucket_counts[op * nseg + seg], 1ull);
 }
 
-uint32_t blocks_for(uint64_t n, uint32_t tpb) { return (uint32_t)std::max<uint64_t>(1, (n + tpb - 1) / tpb); }
 
 // ------------------------------------------------------------ host helpers
 
@@ -565,6 +530,104 @@ int svi_decode_chunks(const uint8_t* d_in, const uint64_t* d_in_off, const std::
 }
 
 }  // namespace
+
+int svi_encode_chunks(const uint32_t* d_values, const uint8_t* d_bounds, const std::vector<uint64_t>& starts,
+                      uint8_t* d_out, uint64_t out_cap, std::vector<uint64_t>& off, hipStream_t s) {
+    const uint32_t nchunks = (uint32_t)(starts.size() - 1);
+    std::vector<uint32_t> sel(nchunks);
+    for (uint32_t c = 0; c < nchunks; ++c) sel[c] = c;
+    DevBuf d_st, d_tb, d_cnt, d_toff, d_pay;
+    std::vector<uint64_t> bytes;
+    SviPlan pl{};
+    if (int r = svi_sizes(d_values, d_bounds, starts, sel, s, d_st, d_tb, d_cnt, d_toff, d_pay, bytes, pl)) return r;
+    off.assign(nchunks + 1, 0);
+    for (uint32_t c = 0; c < nchunks; ++c) off[c + 1] = off[c] + bytes[c];
+    if (off[nchunks] > out_cap)
+        return fail(PS_ERR_CAPACITY, "encoded size %llu exceeds output capacity %llu",
+                    (unsigned long long)off[nchunks], (unsigned long long)out_cap);
+    std::vector<uint64_t> off_sel(off.begin(), off.end() - 1);
+    return svi_write(d_values, d_bounds, pl, d_toff, d_pay, off_sel, d_out, s);
+}
+
+int fc_encode_chunks(const uint32_t* d_values, const std::vector<uint64_t>& starts, int with_check, uint8_t* d_out,
+                     uint64_t out_cap, std::vector<uint64_t>& off, uint64_t* d_offsets, hipStream_t s) {
+    const uint32_t nchunks = (uint32_t)(starts.size() - 1);
+    DevBuf d_starts, d_first, d_last, d_ok, d_off_own;
+    if (int r = to_device(d_starts, starts, s)) return r;
+    if (!d_offsets) {
+        if (int r = d_off_own.alloc(((size_t)nchunks + 1) * 8, s)) return r;
+        d_offsets = d_off_own.as<uint64_t>();
+    }
+    if (int r = d_first.alloc((size_t)nchunks * 4 + 4, s)) return r;
+    if (int r = d_last.alloc((size_t)nchunks * 4 + 4, s)) return r;
+    if (int r = d_ok.alloc((size_t)nchunks * 4 + 4, s)) return r;
+    if (nchunks)
+        hipLaunchKernelGGL(k_fc_inspect<256>, dim3(nchunks), dim3(256), 0, s, d_values, d_starts.as<uint64_t>(),
+                           d_first.as<uint32_t>(), d_last.as<uint32_t>(), d_ok.as<uint32_t>());
+    PS_HIP_CHECK(hipGetLastError());
+    std::vector<uint32_t> first, last, ok;
+    if (int r = to_host(first, d_first.p, nchunks, s)) return r;
+    if (int r = to_host(last, d_last.p, nchunks, s)) return r;
+    if (int r = to_host(ok, d_ok.p, nchunks, s)) return r;
+    // Common/FrontierCompression.cpp:92-115: bitmap when count > 128 and the
+    // bitmap has fewer bits than 10 per value (minus 8 without the check)
+    std::vector<uint32_t> bm, sv;
+    std::vector<uint64_t> bm_words;
+    for (uint32_t c = 0; c < nchunks; ++c) {
+        const uint64_t count = starts[c + 1] - starts[c];
+        bool use_bitmap = false;
+        if (count > 128) {
+            const uint64_t min_bits = count * 10;
+            const uint64_t f = first[c], l = last[c];
+            const uint64_t bits = (l < f) ? ~0ull : ((l - f + 63) / 64 * 8 + 8) * 8;
+            if (with_check) use_bitmap = bits < min_bits && ok[c];
+            else use_bitmap = bits < min_bits - 8 && l >= f;
+            if (!with_check && use_bitmap && !ok[c])
+                return fail(PS_ERR_INVALID_ARGUMENT, "chunk %u is not sorted; use the checked encoder", c);
+        }
+        if (use_bitmap) {
+            bm.push_back(c);
+            bm_words.push_back(((uint64_t)last[c] - first[c] + 63) / 64);
+        } else {
+            sv.push_back(c);
+        }
+    }
+    DevBuf d_st, d_tb, d_cnt, d_toff, d_pay;
+    std::vector<uint64_t> sv_bytes;
+    SviPlan pl{};
+    if (int r = svi_sizes(d_values, nullptr, starts, sv, s, d_st, d_tb, d_cnt, d_toff, d_pay, sv_bytes, pl)) return r;
+    std::vector<uint64_t> bytes(nchunks, 0);
+    for (size_t k = 0; k < sv.size(); ++k) bytes[sv[k]] = sv_bytes[k];
+    for (size_t k = 0; k < bm.size(); ++k) bytes[bm[k]] = 8 + bm_words[k] * 8;
+    off.assign(nchunks + 1, 0);
+    for (uint32_t c = 0; c < nchunks; ++c) off[c + 1] = off[c] + bytes[c];
+    if (off[nchunks] > out_cap)
+        return fail(PS_ERR_CAPACITY, "encoded size %llu exceeds output capacity %llu",
+                    (unsigned long long)off[nchunks], (unsigned long long)out_cap);
+    PS_HIP_CHECK(hipMemcpyAsync(d_offsets, off.data(), off.size() * 8, hipMemcpyHostToDevice, s));
+    std::vector<uint64_t> sv_off(sv.size());
+    for (size_t k = 0; k < sv.size(); ++k) sv_off[k] = off[sv[k]];
+    if (int r = svi_write(d_values, nullptr, pl, d_toff, d_pay, sv_off, d_out, s)) return r;
+    if (!bm.empty()) {
+        std::vector<uint64_t> wb(bm.size() + 1, 0);
+        for (size_t k = 0; k < bm.size(); ++k) wb[k + 1] = wb[k] + bm_words[k];
+        DevBuf d_sel, d_wb, d_words;
+        if (int r = to_device(d_sel, bm, s)) return r;
+        if (int r = to_device(d_wb, wb, s)) return r;
+        if (int r = d_words.alloc(wb.back() * 8 + 8, s)) return r;
+        if (int r = launch_fill(d_words.p, 0, wb.back() * 8 + 8, s)) return r;
+        hipLaunchKernelGGL(k_fc_bitmap_set, dim3((uint32_t)bm.size()), dim3(256), 0, s, d_values,
+                           d_starts.as<uint64_t>(), d_sel.as<uint32_t>(), (uint32_t)bm.size(), d_wb.as<uint64_t>(),
+                           d_first.as<uint32_t>(), d_words.as<unsigned long long>());
+        hipLaunchKernelGGL(k_fc_bitmap_write, dim3((uint32_t)bm.size()), dim3(256), 0, s, d_sel.as<uint32_t>(),
+                           (uint32_t)bm.size(), d_wb.as<uint64_t>(), d_first.as<uint32_t>(),
+                           d_words.as<unsigned long long>(), d_offsets, d_out);
+        PS_HIP_CHECK(hipGetLastError());
+        PS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return PS_OK;
+}
+
 }  // namespace ps
 
 using namespace ps;
@@ -578,22 +641,9 @@ int ps_svint_encode(const uint32_t* d_values, const uint8_t* d_bounds, const uin
     if (!d_starts || !d_offsets || (nchunks && (!d_values || !d_out)))
         return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
     hipStream_t s = stream_of(st);
-    std::vector<uint64_t> starts;
+    std::vector<uint64_t> starts, off;
     if (int r = read_starts(starts, d_starts, nchunks, s)) return r;
-    std::vector<uint32_t> sel(nchunks);
-    for (uint32_t c = 0; c < nchunks; ++c) sel[c] = c;
-    DevBuf d_st, d_tb, d_cnt, d_toff, d_pay;
-    std::vector<uint64_t> bytes;
-    SviPlan pl{};
-    if (int r = svi_sizes(d_values, d_bounds, starts, sel, s, d_st, d_tb, d_cnt, d_toff, d_pay, bytes, pl)) return r;
-    std::vector<uint64_t> off(nchunks + 1, 0);
-    for (uint32_t c = 0; c < nchunks; ++c) off[c + 1] = off[c] + bytes[c];
-    // StreamVInt writes a whole tuple past the data; keep the same slack rule
-    if (off[nchunks] > out_cap)
-        return fail(PS_ERR_CAPACITY, "encoded size %llu exceeds output capacity %llu",
-                    (unsigned long long)off[nchunks], (unsigned long long)out_cap);
-    std::vector<uint64_t> off_sel(off.begin(), off.end() - 1);
-    if (int r = svi_write(d_values, d_bounds, pl, d_toff, d_pay, off_sel, d_out, s)) return r;
+    if (int r = svi_encode_chunks(d_values, d_bounds, starts, d_out, out_cap, off, s)) return r;
     PS_HIP_CHECK(hipMemcpyAsync(d_offsets, off.data(), off.size() * 8, hipMemcpyHostToDevice, s));
     PS_HIP_CHECK(hipStreamSynchronize(s));
     return PS_OK;
@@ -667,76 +717,10 @@ int ps_fc_encode(const uint32_t* d_values, const uint64_t* d_starts, uint32_t nc
     if (!d_starts || !d_offsets || (nchunks && (!d_values || !d_out)))
         return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
     hipStream_t s = stream_of(st);
-    std::vector<uint64_t> starts;
+    std::vector<uint64_t> starts, off;
     if (int r = read_starts(starts, d_starts, nchunks, s)) return r;
-    DevBuf d_first, d_last, d_ok;
-    if (int r = d_first.alloc((size_t)nchunks * 4 + 4, s)) return r;
-    if (int r = d_last.alloc((size_t)nchunks * 4 + 4, s)) return r;
-    if (int r = d_ok.alloc((size_t)nchunks * 4 + 4, s)) return r;
-    if (nchunks)
-        hipLaunchKernelGGL(k_fc_inspect<256>, dim3(nchunks), dim3(256), 0, s, d_values, d_starts,
-                           d_first.as<uint32_t>(), d_last.as<uint32_t>(), d_ok.as<uint32_t>());
-    PS_HIP_CHECK(hipGetLastError());
-    std::vector<uint32_t> first, last, ok;
-    if (int r = to_host(first, d_first.p, nchunks, s)) return r;
-    if (int r = to_host(last, d_last.p, nchunks, s)) return r;
-    if (int r = to_host(ok, d_ok.p, nchunks, s)) return r;
-    // Common/FrontierCompression.cpp:92-115: bitmap when count > 128 and the
-    // bitmap has fewer bits than 10 per value (minus 8 without the check)
-    std::vector<uint32_t> bm, sv;
-    std::vector<uint64_t> bm_words;
-    for (uint32_t c = 0; c < nchunks; ++c) {
-        const uint64_t count = starts[c + 1] - starts[c];
-        bool use_bitmap = false;
-        if (count > 128) {
-            const uint64_t min_bits = count * 10;
-            const uint64_t f = first[c], l = last[c];
-            const uint64_t bits = (l < f) ? ~0ull : ((l - f + 63) / 64 * 8 + 8) * 8;
-            if (with_check) use_bitmap = bits < min_bits && ok[c];
-            else use_bitmap = bits < min_bits - 8 && l >= f;
-            if (!with_check && use_bitmap && !ok[c])
-                return fail(PS_ERR_INVALID_ARGUMENT, "chunk %u is not sorted; use the checked encoder", c);
-        }
-        if (use_bitmap) {
-            bm.push_back(c);
-            bm_words.push_back(((uint64_t)last[c] - first[c] + 63) / 64);
-        } else {
-            sv.push_back(c);
-        }
-    }
-    DevBuf d_st, d_tb, d_cnt, d_toff, d_pay;
-    std::vector<uint64_t> sv_bytes;
-    SviPlan pl{};
-    if (int r = svi_sizes(d_values, nullptr, starts, sv, s, d_st, d_tb, d_cnt, d_toff, d_pay, sv_bytes, pl)) return r;
-    std::vector<uint64_t> bytes(nchunks, 0);
-    for (size_t k = 0; k < sv.size(); ++k) bytes[sv[k]] = sv_bytes[k];
-    for (size_t k = 0; k < bm.size(); ++k) bytes[bm[k]] = 8 + bm_words[k] * 8;
-    std::vector<uint64_t> off(nchunks + 1, 0);
-    for (uint32_t c = 0; c < nchunks; ++c) off[c + 1] = off[c] + bytes[c];
-    if (off[nchunks] > out_cap)
-        return fail(PS_ERR_CAPACITY, "encoded size %llu exceeds output capacity %llu",
-                    (unsigned long long)off[nchunks], (unsigned long long)out_cap);
-    PS_HIP_CHECK(hipMemcpyAsync(d_offsets, off.data(), off.size() * 8, hipMemcpyHostToDevice, s));
-    std::vector<uint64_t> sv_off(sv.size());
-    for (size_t k = 0; k < sv.size(); ++k) sv_off[k] = off[sv[k]];
-    if (int r = svi_write(d_values, nullptr, pl, d_toff, d_pay, sv_off, d_out, s)) return r;
-    if (!bm.empty()) {
-        std::vector<uint64_t> wb(bm.size() + 1, 0);
-        for (size_t k = 0; k < bm.size(); ++k) wb[k + 1] = wb[k] + bm_words[k];
-        DevBuf d_sel, d_wb, d_words;
-        if (int r = to_device(d_sel, bm, s)) return r;
-        if (int r = to_device(d_wb, wb, s)) return r;
-        if (int r = d_words.alloc(wb.back() * 8 + 8, s)) return r;
-        if (int r = launch_fill(d_words.p, 0, wb.back() * 8 + 8, s)) return r;
-        hipLaunchKernelGGL(k_fc_bitmap_set, dim3((uint32_t)bm.size()), dim3(256), 0, s, d_values, d_starts,
-                           d_sel.as<uint32_t>(), (uint32_t)bm.size(), d_wb.as<uint64_t>(), d_first.as<uint32_t>(),
-                           d_words.as<unsigned long long>());
-        hipLaunchKernelGGL(k_fc_bitmap_write, dim3((uint32_t)bm.size()), dim3(256), 0, s, d_sel.as<uint32_t>(),
-                           (uint32_t)bm.size(), d_wb.as<uint64_t>(), d_first.as<uint32_t>(),
-                           d_words.as<unsigned long long>(), d_offsets, d_out);
-        PS_HIP_CHECK(hipGetLastError());
-        PS_HIP_CHECK(hipStreamSynchronize(s));
-    }
+    if (int r = fc_encode_chunks(d_values, starts, with_check, d_out, out_cap, off, d_offsets, s)) return r;
+    PS_HIP_CHECK(hipStreamSynchronize(s));
     return PS_OK;
     PS_API_END
 }

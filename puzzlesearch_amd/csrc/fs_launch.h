@@ -1,0 +1,76 @@
+// Launchers for the templated segment-step kernels; each puzzle family's
+// instantiations live in that family's translation unit (bfs_<family>.hip).
+// Only puzzles with at most 4 moves have a 4-bit op-flag array
+// (Common/DiskBasedOptFrontierSearch.cpp:257-260).
+#pragma once
+
+#include "dispatch.h"
+#include "fs_kernels.h"
+
+namespace ps {
+
+constexpr int FS_TPB = 256;
+
+#define PS_DECLARE_FS_FAMILY(NAME)                                                                       \
+    int fs_mark_in_##NAME(const ps_puzzle& p, const psk::FsMarkArgs& a, uint32_t grid, hipStream_t s); \
+    int fs_cross_##NAME(const ps_puzzle& p, const psk::FsCrossArgs& a, uint32_t grid, hipStream_t s, int* slots);
+PS_DECLARE_FS_FAMILY(pancake)
+PS_DECLARE_FS_FAMILY(sliding)
+PS_DECLARE_FS_FAMILY(packed)
+PS_DECLARE_FS_FAMILY(hanoi)
+
+#define PS_DEFINE_FS_FAMILY(NAME, FAMV)                                                                       \
+    int fs_mark_in_##NAME(const ps_puzzle& p, const psk::FsMarkArgs& a, uint32_t grid, hipStream_t s) {    \
+        return dispatch<FAMV>(p, PS_EXPAND_FULL, [&](auto tag) -> int {                                     \
+            using PZ = typename decltype(tag)::type;                                                        \
+            if constexpr (PZ::OPS <= 4) {                                                                   \
+                hipLaunchKernelGGL((psk::k_fs_mark_in<PZ, FS_TPB>), dim3(grid), dim3(FS_TPB), 0, s, a);     \
+                PS_HIP_CHECK(hipGetLastError());                                                            \
+                return PS_OK;                                                                               \
+            } else {                                                                                        \
+                return fail(PS_ERR_UNSUPPORTED, "the segment step needs at most 4 moves (puzzle has %d)",   \
+                            PZ::OPS);                                                                       \
+            }                                                                                               \
+        });                                                                                                 \
+    }                                                                                                       \
+    int fs_cross_##NAME(const ps_puzzle& p, const psk::FsCrossArgs& a, uint32_t grid, hipStream_t s,       \
+                        int* slots) {                                                                       \
+        return dispatch<FAMV>(p, PS_EXPAND_FULL, [&](auto tag) -> int {                                     \
+            using PZ = typename decltype(tag)::type;                                                        \
+            if constexpr (PZ::OPS <= 4) {                                                                   \
+                if (slots) {                                                                                \
+                    *slots = PZ::SLOTS;                                                                     \
+                    return PS_OK;                                                                           \
+                }                                                                                           \
+                hipLaunchKernelGGL((psk::k_fs_cross<PZ, FS_TPB>), dim3(grid), dim3(FS_TPB), 0, s, a);       \
+                PS_HIP_CHECK(hipGetLastError());                                                            \
+                return PS_OK;                                                                               \
+            } else {                                                                                        \
+                return fail(PS_ERR_UNSUPPORTED, "the segment step needs at most 4 moves (puzzle has %d)",   \
+                            PZ::OPS);                                                                       \
+            }                                                                                               \
+        });                                                                                                 \
+    }
+
+inline int fs_mark_in(const ps_puzzle& p, const psk::FsMarkArgs& a, uint32_t grid, hipStream_t s) {
+    switch (family_of(p)) {
+    case FAM_PANCAKE: return fs_mark_in_pancake(p, a, grid, s);
+    case FAM_SLIDING: return fs_mark_in_sliding(p, a, grid, s);
+    case FAM_PACKED: return fs_mark_in_packed(p, a, grid, s);
+    case FAM_HANOI: return fs_mark_in_hanoi(p, a, grid, s);
+    }
+    return fail(PS_ERR_UNSUPPORTED, "unknown puzzle family");
+}
+
+// slots != NULL: only report the slots per parent
+inline int fs_cross(const ps_puzzle& p, const psk::FsCrossArgs& a, uint32_t grid, hipStream_t s, int* slots) {
+    switch (family_of(p)) {
+    case FAM_PANCAKE: return fs_cross_pancake(p, a, grid, s, slots);
+    case FAM_SLIDING: return fs_cross_sliding(p, a, grid, s, slots);
+    case FAM_PACKED: return fs_cross_packed(p, a, grid, s, slots);
+    case FAM_HANOI: return fs_cross_hanoi(p, a, grid, s, slots);
+    }
+    return fail(PS_ERR_UNSUPPORTED, "unknown puzzle family");
+}
+
+}  // namespace ps

@@ -1,0 +1,558 @@
+// One disk-segment step of the frontier search on the device:
+// DB_OptFS_Solver<4>::Expand(segment) (Common/DiskBasedOptFrontierSearch.cpp:53-132)
+// with the segment's frontier and incoming cross-segment marks already in HBM.
+//
+//   mark     in-segment children of the frontier and the incoming marks are
+//            OR-ed into the 4-bit op-flag array                     (:63-99)
+//   scan     ScanBitsAndClear in two passes over the touched regions: counts
+//            per region, an exclusive scan, then the ordered write of
+//            (id, op bits) for entries with an unused move and the clear (:116-123)
+//   stream   the new frontier as StreamVInt with bounds in 2^18-value chunks,
+//            as CompressedFrontierWriter writes it (Common/CompressedFrontier.cpp:54-75)
+//   cross    children of the written entries in other segments, one slot per
+//            move, compacted in emission order, stably sorted by
+//            (op, segment), and each (op, segment) bucket cut into 4096-value
+//            chunks compressed with FrontierCompression::EncodeWithCheck, as
+//            SmartMultiplexor flushes them (Common/Multiplexor.cpp:125-161)
+//
+// Emission order inside a bucket is (parent, slot) order, the order in which
+// ExpandBuffer hands children to SmartMultiplexor::Add for one segment, so the
+// bucket bytes equal the reference's for a segment step run on its own.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <vector>
+
+#include "codec_internal.h"
+#include "devbuf.h"
+#include "fs_launch.h"
+#include "ps_internal.h"
+
+namespace {
+
+constexpr uint64_t FS_FRONTIER_CHUNK = 1ull << 18;  // StreamVInt::MAX_INDEXES_COUNT
+constexpr uint64_t FS_BUCKET_CHUNK = 1ull << 12;    // SmartMultiplexor small sector
+
+struct GrowBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes, hipStream_t s) {
+        if (bytes <= cap && p) return PS_OK;
+        if (p) (void)hipFreeAsync(p, s);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 4, 256);
+        hipError_t e = hipMallocAsync(&p, want, s);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return ps::fail(PS_ERR_OUT_OF_MEMORY, "segment step buffer of %zu bytes: %s", want, hipGetErrorString(e));
+        }
+        cap = want;
+        return PS_OK;
+    }
+    void release(hipStream_t s) {
+        if (p) (void)hipFreeAsync(p, s);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+struct NotNone {
+    __host__ __device__ bool operator()(const unsigned long long& x) const { return x != psk::FS_NONE; }
+};
+
+// [0] in-segment children [1] count [2] written [3] cross children [4] children
+// outside the segments [5] words scanned [6] frontier ids outside the segment
+constexpr int NCOUNTERS = 8;
+
+uint64_t svi_worst(uint64_t m, bool bounds) {
+    const uint64_t adj = (m + 7) / 8 * 8;
+    return 4 + 17 * (adj / 4) + (bounds ? adj / 2 : 0);
+}
+
+}  // namespace
+
+namespace psk {
+
+// incoming cross-segment marks: NextArray.Set(idx, op) for every (op, idx)
+__global__ void k_fs_mark_xseg(FsXsegArgs a) {
+    const uint64_t total = a.op_starts[a.ops];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        int op = 0;
+        while (op + 1 < a.ops && a.op_starts[op + 1] <= i) ++op;
+        const uint64_t c = a.idx[i] & a.seg_mask;
+        gor(&a.flags[c >> 3], (1u << op) << ((c & 7) * 4));
+        uint8_t* d = &a.dirty[c >> FS_REGION_BITS];
+        if (!*d) *d = 1;
+    }
+}
+
+__global__ void k_fs_clear_odd(const uint32_t* idx, uint64_t n, uint64_t in_mask, unsigned long long* odd) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t id = idx[i];
+        if ((uint64_t)id <= in_mask) odd[id >> 6] = 0ull;
+    }
+}
+
+// sorted (key << 32 | value) -> key and value arrays
+__global__ void k_fs_split(const unsigned long long* kv, uint64_t n, uint32_t* keys, uint32_t* vals) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long x = kv[i];
+        keys[i] = (uint32_t)(x >> 32);
+        vals[i] = (uint32_t)x;
+    }
+}
+
+}  // namespace psk
+
+struct ps_fs_segment {
+    ps_puzzle puzzle{};
+    ps_puzzle_info info{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int seg_bits = 0;
+    uint32_t nseg = 0;
+    int ops = 0, slots = 0, odd = 0, key_bits = 0;
+    uint64_t nwords = 0, nregions = 0;
+    uint32_t* flags = nullptr;
+    uint8_t* dirty = nullptr;
+    unsigned long long* odd_bits = nullptr;
+    unsigned long long* cnt_out = nullptr;
+    unsigned long long* out_off = nullptr;
+    unsigned long long* counters = nullptr;
+    unsigned long long* h_counters = nullptr;  // pinned
+    GrowBuf out_idx, out_ops, fr_bytes, slot_buf, compact, sorted, keys, vals, uniq, runs, nruns, xs_bytes, tmp,
+        xseg_starts;
+    uint32_t cus = 256;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // results of the last step (host)
+    std::vector<uint64_t> fr_chunk_off;
+    std::vector<uint32_t> bk_op, bk_seg;
+    std::vector<uint64_t> bk_values, bk_byte_off;
+    ps_fs_result last{};
+    bool has_result = false;
+};
+
+namespace ps {
+namespace {
+
+int destroy_fs(ps_fs_segment* w) {
+    if (!w) return PS_OK;
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    for (GrowBuf* g : {&w->out_idx, &w->out_ops, &w->fr_bytes, &w->slot_buf, &w->compact, &w->sorted, &w->keys,
+                       &w->vals, &w->uniq, &w->runs, &w->nruns, &w->xs_bytes, &w->tmp, &w->xseg_starts})
+        g->release(w->stream);
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    if (w->flags) (void)hipFree(w->flags);
+    if (w->dirty) (void)hipFree(w->dirty);
+    if (w->odd_bits) (void)hipFree(w->odd_bits);
+    if (w->cnt_out) (void)hipFree(w->cnt_out);
+    if (w->out_off) (void)hipFree(w->out_off);
+    if (w->counters) (void)hipFree(w->counters);
+    if (w->h_counters) (void)hipHostFree(w->h_counters);
+    if (w->ev0) (void)hipEventDestroy(w->ev0);
+    if (w->ev1) (void)hipEventDestroy(w->ev1);
+    if (w->own_stream && w->stream) (void)hipStreamDestroy(w->stream);
+    delete w;
+    (void)hipGetLastError();
+    return PS_OK;
+}
+
+int read_counters(ps_fs_segment* w) {
+    PS_HIP_CHECK(hipMemcpyAsync(w->h_counters, w->counters, NCOUNTERS * 8, hipMemcpyDeviceToHost, w->stream));
+    PS_HIP_CHECK(hipStreamSynchronize(w->stream));
+    return PS_OK;
+}
+
+int tmp_for(ps_fs_segment* w, size_t bytes) { return w->tmp.ensure(bytes, w->stream); }
+
+int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const uint8_t* d_ops, uint64_t n,
+             const uint32_t* d_xidx, const uint64_t* xstarts, ps_fs_result* res) {
+    hipStream_t s = w->stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    w->has_result = false;
+    PS_HIP_CHECK(hipEventRecord(w->ev0, s));
+    if (int r = launch_fill(w->counters, 0, NCOUNTERS * 8, s)) return r;
+    const uint64_t base = (uint64_t)segment << w->seg_bits;
+    const uint64_t seg_mask = (1ull << w->seg_bits) - 1ull;
+    const uint32_t grid_par = (uint32_t)std::min<uint64_t>(blocks_for(n, FS_TPB), (uint64_t)w->cus * 8);
+    uint64_t bad_ids = 0;
+
+    // 1 mark: in-segment children of the frontier, then the incoming marks
+    if (n) {
+        psk::FsMarkArgs a{};
+        a.prm = params_of(w->puzzle);
+        a.idx = d_idx;
+        a.ops = d_ops;
+        a.n = n;
+        a.base = base;
+        a.segment = segment;
+        a.seg_bits = w->seg_bits;
+        a.flags = w->flags;
+        a.dirty = w->dirty;
+        a.odd = w->odd ? w->odd_bits : nullptr;
+        a.counters = w->counters;
+        if (int r = fs_mark_in(w->puzzle, a, grid_par, s)) return r;
+    }
+    uint64_t nx = xstarts ? xstarts[w->ops] : 0;
+    if (nx) {
+        if (int r = w->xseg_starts.ensure((size_t)(w->ops + 1) * 8, s)) return r;
+        PS_HIP_CHECK(hipMemcpyAsync(w->xseg_starts.p, xstarts, (size_t)(w->ops + 1) * 8, hipMemcpyHostToDevice, s));
+        psk::FsXsegArgs x{};
+        x.idx = d_xidx;
+        x.op_starts = w->xseg_starts.as<uint64_t>();
+        x.ops = w->ops;
+        x.seg_mask = seg_mask;
+        x.flags = w->flags;
+        x.dirty = w->dirty;
+        hipLaunchKernelGGL(psk::k_fs_mark_xseg, dim3((uint32_t)std::min<uint64_t>(blocks_for(nx, 256), w->cus * 8ull)),
+                           dim3(256), 0, s, x);
+        PS_HIP_CHECK(hipGetLastError());
+    }
+
+    // 2 scan: count per region, exclusive scan, ordered write + clear
+    psk::FsScanArgs sa{};
+    sa.flags_c = w->flags;
+    sa.flags = w->flags;
+    sa.dirty = w->dirty;
+    sa.odd = w->odd ? w->odd_bits : nullptr;
+    sa.nwords = w->nwords;
+    sa.ops = w->ops;
+    sa.cnt_out = w->cnt_out;
+    sa.counters = w->counters;
+    sa.out_off = w->out_off;
+    hipLaunchKernelGGL(psk::k_fs_count<FS_TPB>, dim3((uint32_t)w->nregions), dim3(FS_TPB), 0, s, sa);
+    PS_HIP_CHECK(hipGetLastError());
+    size_t tb = 0;
+    PS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w->cnt_out, w->out_off, (int)w->nregions, s));
+    if (int r = tmp_for(w, tb)) return r;
+    PS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w->tmp.p, tb, w->cnt_out, w->out_off, (int)w->nregions, s));
+    if (int r = read_counters(w)) return r;
+    if (w->h_counters[6]) {
+        // the marks of the valid ids are still cleared by the write pass below
+        bad_ids = w->h_counters[6];
+    }
+    const uint64_t children_in = w->h_counters[0];
+    const uint64_t count = w->h_counters[1];
+    const uint64_t n_out = w->h_counters[2];
+    const uint64_t scanned = w->h_counters[5];
+    if (int r = w->out_idx.ensure(std::max<uint64_t>(n_out, 1) * 4, s)) return r;
+    if (int r = w->out_ops.ensure(std::max<uint64_t>(n_out, 1), s)) return r;
+    sa.out_idx = w->out_idx.as<uint32_t>();
+    sa.out_ops = w->out_ops.as<uint8_t>();
+    hipLaunchKernelGGL(psk::k_fs_write<FS_TPB>, dim3((uint32_t)w->nregions), dim3(FS_TPB), 0, s, sa);
+    PS_HIP_CHECK(hipGetLastError());
+    if (w->odd && n) {
+        hipLaunchKernelGGL(psk::k_fs_clear_odd, dim3(grid_par), dim3(256), 0, s, d_idx, n, seg_mask, w->odd_bits);
+        PS_HIP_CHECK(hipGetLastError());
+    }
+
+    if (bad_ids) {
+        PS_HIP_CHECK(hipStreamSynchronize(s));
+        return fail(PS_ERR_INDEX_RANGE, "%llu frontier ids are not below 2^%d", (unsigned long long)bad_ids,
+                    w->seg_bits);
+    }
+
+    // 3 the new frontier stream
+    std::vector<uint64_t> starts;
+    for (uint64_t c = 0; c < n_out; c += FS_FRONTIER_CHUNK) starts.push_back(c);
+    starts.push_back(n_out);  // no chunk at all for an empty frontier
+    uint64_t fr_cap = 0;
+    for (size_t c = 0; c + 1 < starts.size(); ++c) fr_cap += svi_worst(starts[c + 1] - starts[c], true);
+    if (int r = w->fr_bytes.ensure(std::max<uint64_t>(fr_cap, 16), s)) return r;
+    if (int r = svi_encode_chunks(w->out_idx.as<uint32_t>(), w->out_ops.as<uint8_t>(), starts, w->fr_bytes.as<uint8_t>(),
+                                  w->fr_bytes.cap, w->fr_chunk_off, s))
+        return r;
+
+    // 4 cross-segment children of the written entries, bucketed by (op, segment)
+    const uint64_t nslots = n_out * (uint64_t)w->slots;
+    if (nslots > 0x7FFFFFFFull)
+        return fail(PS_ERR_CAPACITY, "%llu cross-segment slots in one step exceed 2^31",
+                    (unsigned long long)nslots);
+    uint64_t m = 0;
+    if (n_out) {
+        if (int r = w->slot_buf.ensure(nslots * 8, s)) return r;
+        psk::FsCrossArgs ca{};
+        ca.prm = params_of(w->puzzle);
+        ca.idx = w->out_idx.as<uint32_t>();
+        ca.ops = w->out_ops.as<uint8_t>();
+        ca.n = n_out;
+        ca.base = base;
+        ca.segment = segment;
+        ca.nseg = w->nseg;
+        ca.seg_bits = w->seg_bits;
+        ca.slots = w->slot_buf.as<unsigned long long>();
+        ca.counters = w->counters;
+        const uint32_t grid_out = (uint32_t)std::min<uint64_t>(blocks_for(n_out, FS_TPB), (uint64_t)w->cus * 8);
+        if (int r = fs_cross(w->puzzle, ca, grid_out, s, nullptr)) return r;
+        if (int r = w->compact.ensure(nslots * 8, s)) return r;
+        if (int r = w->nruns.ensure(16, s)) return r;
+        PS_HIP_CHECK(hipcub::DeviceSelect::If(nullptr, tb, w->slot_buf.as<unsigned long long>(),
+                                              w->compact.as<unsigned long long>(), w->nruns.as<int>(), (int)nslots,
+                                              NotNone(), s));
+        if (int r = tmp_for(w, tb)) return r;
+        PS_HIP_CHECK(hipcub::DeviceSelect::If(w->tmp.p, tb, w->slot_buf.as<unsigned long long>(),
+                                              w->compact.as<unsigned long long>(), w->nruns.as<int>(), (int)nslots,
+                                              NotNone(), s));
+        if (int r = read_counters(w)) return r;
+        if (w->h_counters[4])
+            return fail(PS_ERR_INDEX_RANGE, "%llu children fall outside the %u segments",
+                        (unsigned long long)w->h_counters[4], w->nseg);
+        m = w->h_counters[3];
+        int selected = 0;
+        PS_HIP_CHECK(hipMemcpy(&selected, w->nruns.p, 4, hipMemcpyDeviceToHost));
+        if ((uint64_t)selected != m)
+            return fail(PS_ERR_HIP, "compaction kept %d of %llu cross-segment children", selected,
+                        (unsigned long long)m);
+    }
+    w->bk_op.clear();
+    w->bk_seg.clear();
+    w->bk_values.clear();
+    w->bk_byte_off.assign(1, 0);
+    std::vector<uint64_t> xoff;
+    uint64_t xbytes = 0, nchunks_x = 0;
+    if (m) {
+        if (int r = w->sorted.ensure(m * 8, s)) return r;
+        PS_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, w->compact.as<unsigned long long>(),
+                                                       w->sorted.as<unsigned long long>(), (int)m, 32,
+                                                       32 + w->key_bits, s));
+        if (int r = tmp_for(w, tb)) return r;
+        PS_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(w->tmp.p, tb, w->compact.as<unsigned long long>(),
+                                                       w->sorted.as<unsigned long long>(), (int)m, 32,
+                                                       32 + w->key_bits, s));
+        if (int r = w->keys.ensure(m * 4, s)) return r;
+        if (int r = w->vals.ensure(m * 4 + 64, s)) return r;
+        hipLaunchKernelGGL(psk::k_fs_split, dim3((uint32_t)std::min<uint64_t>(blocks_for(m, 256), w->cus * 8ull)),
+                           dim3(256), 0, s, w->sorted.as<unsigned long long>(), m, w->keys.as<uint32_t>(),
+                           w->vals.as<uint32_t>());
+        PS_HIP_CHECK(hipGetLastError());
+        const uint64_t maxb = std::min<uint64_t>(m, (uint64_t)w->ops * w->nseg);
+        if (int r = w->uniq.ensure(maxb * 4, s)) return r;
+        if (int r = w->runs.ensure(maxb * 4, s)) return r;
+        PS_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, w->keys.as<uint32_t>(), w->uniq.as<uint32_t>(),
+                                                           w->runs.as<uint32_t>(), w->nruns.as<int>(), (int)m, s));
+        if (int r = tmp_for(w, tb)) return r;
+        PS_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(w->tmp.p, tb, w->keys.as<uint32_t>(), w->uniq.as<uint32_t>(),
+                                                           w->runs.as<uint32_t>(), w->nruns.as<int>(), (int)m, s));
+        int nb = 0;
+        PS_HIP_CHECK(hipMemcpyAsync(&nb, w->nruns.p, 4, hipMemcpyDeviceToHost, s));
+        PS_HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<uint32_t> uk, rc;
+        if (int r = to_host(uk, w->uniq.p, (size_t)nb, s)) return r;
+        if (int r = to_host(rc, w->runs.p, (size_t)nb, s)) return r;
+        // 4096-value chunks inside every bucket
+        std::vector<uint64_t> cstarts;
+        std::vector<uint64_t> first_chunk(nb + 1);
+        uint64_t pos = 0, cap = 0;
+        for (int b = 0; b < nb; ++b) {
+            first_chunk[b] = cstarts.size();
+            for (uint64_t c = 0; c < rc[b]; c += FS_BUCKET_CHUNK) {
+                cstarts.push_back(pos + c);
+                cap += svi_worst(std::min<uint64_t>(FS_BUCKET_CHUNK, rc[b] - c), false);
+            }
+            pos += rc[b];
+        }
+        first_chunk[nb] = cstarts.size();
+        cstarts.push_back(pos);
+        if (pos != m) return fail(PS_ERR_HIP, "bucket sizes %llu != %llu children", (unsigned long long)pos,
+                                  (unsigned long long)m);
+        if (int r = w->xs_bytes.ensure(cap + 16, s)) return r;
+        if (int r = fc_encode_chunks(w->vals.as<uint32_t>(), cstarts, 1, w->xs_bytes.as<uint8_t>(), w->xs_bytes.cap,
+                                     xoff, nullptr, s))
+            return r;
+        nchunks_x = cstarts.size() - 1;
+        xbytes = xoff.back();
+        for (int b = 0; b < nb; ++b) {
+            w->bk_op.push_back(uk[b] / w->nseg);
+            w->bk_seg.push_back(uk[b] % w->nseg);
+            w->bk_values.push_back(rc[b]);
+            w->bk_byte_off.push_back(xoff[first_chunk[b + 1]]);
+        }
+    }
+    PS_HIP_CHECK(hipEventRecord(w->ev1, s));
+    PS_HIP_CHECK(hipEventSynchronize(w->ev1));
+    float ms = 0.f;
+    PS_HIP_CHECK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+
+    ps_fs_result r{};
+    r.count = count;
+    r.frontier_count = n_out;
+    r.frontier_bytes = w->fr_chunk_off.back();
+    r.frontier_chunks = w->fr_chunk_off.size() - 1;
+    r.children_in = children_in;
+    r.children_cross = m;
+    r.buckets = w->bk_op.size();
+    r.cross_chunks = nchunks_x;
+    r.cross_bytes = xbytes;
+    r.scanned_words = scanned;
+    // inputs + outputs + 8 B per child
+    r.algorithmic_bytes = 5 * n + 4 * nx + 5 * n_out + r.frontier_bytes + xbytes + 8 * (children_in + nx + m);
+    r.device_ms = ms;
+    r.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    w->last = r;
+    w->has_result = true;
+    if (res) *res = r;
+    return PS_OK;
+}
+
+}  // namespace
+}  // namespace ps
+
+using namespace ps;
+
+extern "C" {
+
+int ps_fs_segment_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st, ps_fs_segment** out) {
+    PS_API_BEGIN
+    if (int r = require_runtime()) return r;
+    if (!p || !out) return fail(PS_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    ps_puzzle_info info{};
+    if (int r = puzzle_query(p, &info)) return r;
+    if (info.operators_count < 1 || info.operators_count > 4)
+        return fail(PS_ERR_UNSUPPORTED, "the segment step needs 1..4 moves (puzzle has %d)", info.operators_count);
+    if (segment_bits < 6 || segment_bits > 32) return fail(PS_ERR_INVALID_ARGUMENT, "segment_bits must be 6..32");
+    const uint64_t seg_size = 1ull << segment_bits;
+    const uint64_t nseg64 = (info.indexes_count + seg_size - 1) / seg_size;
+    if (nseg64 == 0 || nseg64 * (uint64_t)info.operators_count > (1ull << 31))
+        return fail(PS_ERR_INVALID_ARGUMENT, "%llu segments of 2^%d states are too many",
+                    (unsigned long long)nseg64, segment_bits);
+    ps_fs_segment* w = new (std::nothrow) ps_fs_segment();
+    if (!w) return fail(PS_ERR_OUT_OF_MEMORY, "host allocation failed");
+    auto cleanup = [&](int rc) {
+        destroy_fs(w);
+        return rc;
+    };
+    w->puzzle = *p;
+    w->info = info;
+    w->seg_bits = segment_bits;
+    w->nseg = (uint32_t)nseg64;
+    w->ops = info.operators_count;
+    w->odd = info.has_odd_cycles;
+    int kb = 1;
+    while ((1ull << kb) <= (uint64_t)w->ops * w->nseg) ++kb;  // all-ones stays above every key
+    w->key_bits = kb;
+    w->nwords = seg_size / 8;
+    w->nregions = (seg_size + (1ull << psk::FS_REGION_BITS) - 1) >> psk::FS_REGION_BITS;
+    w->cus = (uint32_t)hip_cu_count();
+    if (int r = fs_cross(*p, psk::FsCrossArgs{}, 0, nullptr, &w->slots)) return cleanup(r);
+    if (st) {
+        w->stream = stream_of(st);
+    } else {
+        if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(fail(PS_ERR_HIP, "stream creation failed"));
+        w->own_stream = true;
+    }
+    auto dalloc = [&](void** ptr, uint64_t bytes) -> int {
+        hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *ptr = nullptr;
+            return fail(PS_ERR_OUT_OF_MEMORY, "hipMalloc(%llu) failed: %s", (unsigned long long)bytes,
+                        hipGetErrorString(e));
+        }
+        return PS_OK;
+    };
+    if (int r = dalloc((void**)&w->flags, w->nwords * 4)) return cleanup(r);
+    if (int r = dalloc((void**)&w->dirty, w->nregions)) return cleanup(r);
+    if (w->odd)
+        if (int r = dalloc((void**)&w->odd_bits, (seg_size + 63) / 64 * 8)) return cleanup(r);
+    if (int r = dalloc((void**)&w->cnt_out, w->nregions * 8)) return cleanup(r);
+    if (int r = dalloc((void**)&w->out_off, w->nregions * 8 + 8)) return cleanup(r);
+    if (int r = dalloc((void**)&w->counters, NCOUNTERS * 8)) return cleanup(r);
+    if (hipHostMalloc((void**)&w->h_counters, NCOUNTERS * 8, hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(PS_ERR_OUT_OF_MEMORY, "pinned host allocation failed"));
+    if (hipEventCreate(&w->ev0) != hipSuccess || hipEventCreate(&w->ev1) != hipSuccess)
+        return cleanup(fail(PS_ERR_HIP, "event creation failed"));
+    if (int r = launch_fill(w->flags, 0, w->nwords * 4, w->stream)) return cleanup(r);
+    if (int r = launch_fill(w->dirty, 0, w->nregions, w->stream)) return cleanup(r);
+    if (w->odd)
+        if (int r = launch_fill(w->odd_bits, 0, (seg_size + 63) / 64 * 8, w->stream)) return cleanup(r);
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return cleanup(fail(PS_ERR_HIP, "stream synchronize failed"));
+    *out = w;
+    return PS_OK;
+    PS_API_END
+}
+
+int ps_fs_segment_destroy(ps_fs_segment* w) {
+    PS_API_BEGIN
+    if (!w) return PS_OK;
+    return destroy_fs(w);
+    PS_API_END
+}
+
+int ps_fs_segment_info(const ps_fs_segment* w, uint32_t* segments, int32_t* operators, uint64_t* footprint_bytes) {
+    PS_API_BEGIN
+    if (!w) return fail(PS_ERR_INVALID_ARGUMENT, "NULL segment workspace");
+    if (segments) *segments = w->nseg;
+    if (operators) *operators = w->ops;
+    if (footprint_bytes)
+        *footprint_bytes = w->nwords * 4 + w->nregions * 17 + (w->odd ? ((1ull << w->seg_bits) + 63) / 64 * 8 : 0);
+    return PS_OK;
+    PS_API_END
+}
+
+int ps_fs_segment_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const uint8_t* d_ops, uint64_t n,
+                       const uint32_t* d_xseg_idx, const uint64_t* xseg_op_starts, ps_fs_result* res) {
+    PS_API_BEGIN
+    if (int r = require_runtime()) return r;
+    if (!w) return fail(PS_ERR_INVALID_ARGUMENT, "NULL segment workspace");
+    if (segment >= w->nseg) return fail(PS_ERR_INDEX_RANGE, "segment %u outside [0, %u)", segment, w->nseg);
+    if (n && (!d_idx || !d_ops)) return fail(PS_ERR_INVALID_ARGUMENT, "NULL frontier pointer");
+    if (xseg_op_starts) {
+        if (xseg_op_starts[0] != 0) return fail(PS_ERR_INVALID_ARGUMENT, "xseg_op_starts[0] must be 0");
+        for (int op = 0; op < w->ops; ++op)
+            if (xseg_op_starts[op + 1] < xseg_op_starts[op])
+                return fail(PS_ERR_INVALID_ARGUMENT, "xseg_op_starts must not decrease");
+        if (xseg_op_starts[w->ops] && !d_xseg_idx) return fail(PS_ERR_INVALID_ARGUMENT, "NULL cross-segment pointer");
+    }
+    return run_step(w, segment, d_idx, d_ops, n, d_xseg_idx, xseg_op_starts, res);
+    PS_API_END
+}
+
+int ps_fs_segment_frontier(const ps_fs_segment* w, const uint32_t** d_idx, const uint8_t** d_ops,
+                           const uint8_t** d_bytes, uint64_t* chunk_offsets, uint64_t cap) {
+    PS_API_BEGIN
+    if (!w) return fail(PS_ERR_INVALID_ARGUMENT, "NULL segment workspace");
+    if (!w->has_result) return fail(PS_ERR_INVALID_ARGUMENT, "no completed step");
+    if (d_idx) *d_idx = w->out_idx.as<uint32_t>();
+    if (d_ops) *d_ops = w->out_ops.as<uint8_t>();
+    if (d_bytes) *d_bytes = w->fr_bytes.as<uint8_t>();
+    if (chunk_offsets) {
+        if (cap < w->fr_chunk_off.size())
+            return fail(PS_ERR_CAPACITY, "need %zu chunk offsets", w->fr_chunk_off.size());
+        std::memcpy(chunk_offsets, w->fr_chunk_off.data(), w->fr_chunk_off.size() * 8);
+    }
+    return PS_OK;
+    PS_API_END
+}
+
+int ps_fs_segment_buckets(const ps_fs_segment* w, uint32_t* op, uint32_t* segment, uint64_t* values,
+                          uint64_t* byte_offsets, uint64_t cap, const uint8_t** d_bytes) {
+    PS_API_BEGIN
+    if (!w) return fail(PS_ERR_INVALID_ARGUMENT, "NULL segment workspace");
+    if (!w->has_result) return fail(PS_ERR_INVALID_ARGUMENT, "no completed step");
+    const size_t nb = w->bk_op.size();
+    if (cap < nb) return fail(PS_ERR_CAPACITY, "need room for %zu buckets", nb);  // byte_offsets: cap + 1
+    for (size_t b = 0; b < nb; ++b) {
+        if (op) op[b] = w->bk_op[b];
+        if (segment) segment[b] = w->bk_seg[b];
+        if (values) values[b] = w->bk_values[b];
+    }
+    if (byte_offsets) std::memcpy(byte_offsets, w->bk_byte_off.data(), (nb + 1) * 8);
+    if (d_bytes) *d_bytes = w->xs_bytes.as<uint8_t>();
+    return PS_OK;
+    PS_API_END
+}
+
+}  // extern "C"

@@ -30,6 +30,8 @@ MODES = {
     "plain_seg12": dict(mode=psa.BFS_PLAIN, segment_bits=12),
     "batched_nowindow": dict(mode=psa.BFS_BATCHED, tile_bits_log2=-1),
     "batched_tile12_256": dict(mode=psa.BFS_BATCHED, tile_bits_log2=12, block_threads=256),
+    "batched_atomic": dict(mode=psa.BFS_BATCHED, cross_marks=psa.CROSS_ATOMIC),
+    "plain_seg12_atomic": dict(mode=psa.BFS_PLAIN, segment_bits=12, cross_marks=psa.CROSS_ATOMIC),
 }
 
 
@@ -172,14 +174,15 @@ def test_two_bit_against_oracle_fixture(ps, name):
     assert [str(h) for h in hashes] == rec["hashes2"]
 
 
-@pytest.mark.parametrize("seg", [0, 20, 26])
-def test_pancake12_full(ps, seg):
+@pytest.mark.parametrize("seg,cross", [(0, 0), (20, 0), (26, 0), (22, 1)])
+def test_pancake12_full(ps, seg, cross):
     """12-pancake, 12! states, three-bit, identity start: counts and frontier
-    hashes equal the oracle; sum = 12!, radius = P(12) = 14."""
+    hashes equal the oracle; sum = 12!, radius = P(12) = 14; byte and atomic
+    cross-window marks."""
     if "pancake_12" not in FIX:
         pytest.skip("oracle fixture pancake_12 not generated")
     rec = FIX["pancake_12"]
-    with psa.Bfs(psa.pancake(12), index_count=math.factorial(12), segment_bits=seg) as b:
+    with psa.Bfs(psa.pancake(12), index_count=math.factorial(12), segment_bits=seg, cross_marks=cross) as b:
         counts, hashes = b.run(0)
     assert sum(counts) == math.factorial(12)
     assert len(counts) == 15
