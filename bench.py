@@ -160,6 +160,70 @@ def segment_step_bench(psa, steps, ppm=200000, seed=1234):
             "cross_bytes": r["cross_bytes"]}
 
 
+TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256], "grid_blocks": [0, 256, 1024],
+                   "graph_layers": [4, 8, 16], "cross_marks": [0, 1], "tile_bits_log2": [0, 16],
+                   "skip_visited": [0]}
+
+
+def tuned(psa, p, start, index_count=0, candidates=None):
+    from puzzlesearch_amd import autotune as AT
+    t0 = time.perf_counter()
+    res = AT.tune(p, start, index_count=index_count, passes=1, runs=2, candidates=candidates or TUNE_CANDIDATES)
+    log(f"autotune {p}: {res['options']} -> {res['ms']:.2f} ms ({res['tried']} configurations, "
+        f"{time.perf_counter() - t0:.1f} s)")
+    return res
+
+
+def batched_vs_plain(psa, name, p, start, steps, fixture_counts=None):
+    """Batched persistent (tuned) vs plain per-segment launches of one search."""
+    res = tuned(psa, p, start)
+    opts = dict(res["options"])
+    with psa.Bfs(p, graph=1, **opts) as b:
+        dt_b, counts, hashes = timed_runs(psa, b, start, steps, 1)
+    popts = {k: v for k, v in opts.items() if k in ("segment_bits", "block_threads", "tile_bits_log2", "cross_marks")}
+    with psa.Bfs(p, mode=psa.BFS_PLAIN, **popts) as b:
+        dt_p, counts_p, hashes_p = timed_runs(psa, b, start, 1, 1)
+        st_p = b.stats()
+    assert (counts_p, hashes_p) == (counts, hashes)
+    n_states = sum(counts)
+    out = {"workload": name, "states": n_states, "layers": len(counts), "tuned_options": opts,
+           "batched_ms": dt_b * 1e3, "plain_ms": dt_p * 1e3, "plain_expand_launches": st_p.expand_launches,
+           "batched_states_per_s": n_states / dt_b, "batched_vs_plain": dt_p / dt_b}
+    if fixture_counts is not None:
+        out["counts_match_reference"] = counts == fixture_counts
+    log(f"{name}: batched {dt_b * 1e3:.2f} ms, plain {dt_p * 1e3:.2f} ms ({st_p.expand_launches} launches)")
+    return out
+
+
+def serving_bench(psa, rates=(5.0, 20.0, 50.0), n_jobs=30, seed=2024):
+    """Poisson streams of complete-BFS requests over the 16-kind serving mix;
+    every result is checked against the same search run alone."""
+    from puzzlesearch_amd import serve as S
+    ref = {}
+    out = []
+    for rate in rates:
+        res = S.serve(rate=rate, n_jobs=n_jobs, seed=seed)
+        exact = True
+        for r in res["results"]:
+            key = r["kind"]
+            if key not in ref:
+                name, args = key.split("(")
+                nums = [int(x) for x in args.rstrip(")").split(",")]
+                spec = (name, nums[0], nums[1] if len(nums) > 1 else 0)
+                p, n = S.make_puzzle(*spec)
+                with psa.Bfs(p, index_count=n) as b:
+                    ref[key] = b.run(0)
+            exact &= (r["counts"], r["hashes"]) == ref[key]
+        rec = {k: res[k] for k in ("rate", "jobs", "seed", "p50_ms", "p99_ms", "mean_ms", "makespan_s", "states",
+                                   "states_per_s", "max_running", "all_complete_and_summed")}
+        rec["exact_vs_alone"] = exact
+        rec["engine_steps"] = res["engine"]["steps"]
+        out.append(rec)
+        log(f"serving rate {rate}/s: p50 {res['p50_ms']:.1f} ms, p99 {res['p99_ms']:.1f} ms, "
+            f"{res['states_per_s']:.3g} states/s, max running {res['max_running']}, exact={exact}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +234,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-segment-step", action="store_true")
+    ap.add_argument("--no-batched-set", action="store_true", help="skip the 4x3 sliding and Hanoi-16 comparisons")
+    ap.add_argument("--no-tune", action="store_true", help="use the default options instead of the autotuner's")
+    ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving runs")
     ap.add_argument("--profile", action="store_true",
                     help="only the event-timed batched search, --steps times (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -191,9 +258,16 @@ def main():
     p = psa.pancake(n)
     fixture = load_fixture(f"pancake_{n}")
 
+    opts = {"segment_bits": args.segment_bits}
+    tune_res = None
+    if not args.no_tune and not args.profile:
+        tune_res = tuned(psa, p, 0, index_count=N)
+        opts = dict(tune_res["options"])
+    eager_opts = {k: v for k, v in opts.items() if k != "graph_layers"}
+
     def eager_roofline(runs):
         # the batched path without the graph, HIP events around every kernel on its stream
-        with psa.Bfs(p, index_count=N, segment_bits=args.segment_bits, graph=0) as b:
+        with psa.Bfs(p, index_count=N, graph=0, **eager_opts) as b:
             for _ in range(runs):
                 ce, he = b.run(0)
             st = b.stats()
@@ -209,7 +283,7 @@ def main():
         return
 
     # batched persistent path, layer loop in a hipGraph
-    with psa.Bfs(p, index_count=N, segment_bits=args.segment_bits, graph=1) as b:
+    with psa.Bfs(p, index_count=N, graph=1, **opts) as b:
         dt_b, counts, hashes = timed_runs(psa, b, 0, args.steps, args.warmup)
         st_b = b.stats()
     parity = None
@@ -225,14 +299,14 @@ def main():
         f"{exp_bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; finalize {st_e.finalize_ms:.2f} ms total")
 
     # the same batched path with device-scope atomics for children outside the LDS window
-    with psa.Bfs(p, index_count=N, segment_bits=args.segment_bits, graph=1, cross_marks=psa.CROSS_ATOMIC) as b:
+    with psa.Bfs(p, index_count=N, graph=1, **dict(opts, cross_marks=psa.CROSS_ATOMIC)) as b:
         dt_a, counts_a, hashes_a = timed_runs(psa, b, 0, max(1, args.steps // 2), 1)
     assert counts_a == counts and hashes_a == hashes
     log(f"batched, atomic cross marks: {dt_a * 1e3:.2f} ms/search")
 
     plain = None
     if not args.no_plain:
-        with psa.Bfs(p, index_count=N, segment_bits=args.segment_bits, mode=psa.BFS_PLAIN) as b:
+        with psa.Bfs(p, index_count=N, mode=psa.BFS_PLAIN, **eager_opts) as b:
             dt_p, counts_p, hashes_p = timed_runs(psa, b, 0, max(1, args.steps // 2), 1)
             st_p = b.stats()
         assert counts_p == counts and hashes_p == hashes
@@ -240,6 +314,19 @@ def main():
                  "expand_launches_per_search": st_p.expand_launches,
                  "finalize_launches_per_search": st_p.finalize_launches}
         log(f"plain: {dt_p * 1e3:.2f} ms/search ({st_p.expand_launches} expand launches)")
+
+    batched_set = None
+    if not args.no_batched_set:
+        batched_set = [
+            batched_vs_plain(psa, "three-bit BFS of the 4x3 sliding-tile puzzle (12! indexes, 239500800 reachable)",
+                             psa.sliding(4, 3), 0, max(2, args.steps // 2)),
+            batched_vs_plain(psa, "three-bit BFS of 4-peg Hanoi with 16 disks (4^16 states)",
+                             psa.hanoi(16), 0, 2),
+        ]
+
+    serving = None
+    if not args.no_serving:
+        serving = serving_bench(psa)
 
     seg_step = None
     if not args.no_segment_step:
@@ -267,8 +354,9 @@ def main():
         "dtype": "u32",
         "data": "synthetic (the complete 12-pancake state space from the identity)",
         "config": {"workload": f"complete three-bit BFS of the {n}-pancake graph ({N} states, "
-                               f"{len(counts)} layers), segments of 2^{args.segment_bits} states",
-                   "layers": len(counts), "segment_bits": args.segment_bits,
+                               f"{len(counts)} layers)",
+                   "layers": len(counts), "segment_bits": opts.get("segment_bits", args.segment_bits),
+                   "options": opts, "autotune_configurations": tune_res["tried"] if tune_res else 0,
                    "algorithmic_bytes_per_search": st_b.algorithmic_bytes,
                    "children_per_search": st_b.children},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -279,7 +367,9 @@ def main():
         "atomic_cross_marks": {"value": N / dt_a, "ms_per_step": dt_a * 1e3},
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "batched_set": batched_set,
         "segment_step": seg_step,
+        "serving": serving,
         "parity_with_oracle_fixture": parity,
         "cpu_baseline": cpu,
         "device": props["name"],

@@ -232,6 +232,60 @@ int ps_fs_segment_frontier(const ps_fs_segment* w, const uint32_t** d_idx, const
 int ps_fs_segment_buckets(const ps_fs_segment* w, uint32_t* op, uint32_t* segment, uint64_t* values,
                           uint64_t* byte_offsets, uint64_t cap, const uint8_t** d_bytes);
 
+/* --------------------------------------------------- multi-search engine */
+/* Continuous batching of complete three-bit BFS jobs (InMemoryThreeBitBFS,
+ * Common/InMemoryThreeBitBFS.cpp:4-48, for many puzzles at once).  No
+ * counterpart in the reference: it serves a stream of search requests.  Jobs
+ * of the puzzles compiled into the multi-search kernels (sliding tile with 8,
+ * 9, 10 or 12 cells, pancake 8..12, 4-peg Hanoi up to 16 disks) are queued by
+ * ps_batch_submit.  ps_batch_step admits queued jobs first come first served
+ * while their HBM footprint fits the budget, advances every running job by one
+ * layer with three launches shared by all of them, and retires finished jobs,
+ * releasing their memory.  Not thread-safe: one caller drives a batch. */
+typedef struct ps_batch ps_batch;
+
+enum { PS_JOB_QUEUED = 0, PS_JOB_RUNNING = 1, PS_JOB_DONE = 2, PS_JOB_FAILED = 3 };
+
+typedef struct ps_batch_options {
+    int32_t max_jobs;      /* jobs running at once (0: 64) */
+    int32_t segment_bits;  /* segments of jobs larger than one segment (default 22; 0: none) */
+    uint64_t hbm_budget;   /* bytes running jobs may hold (0: half of the free HBM at creation) */
+} ps_batch_options;
+
+typedef struct ps_batch_job_info {
+    int32_t state;      /* PS_JOB_* */
+    int32_t layers;     /* layer counts available (state DONE) */
+    uint64_t footprint; /* HBM bytes while running */
+    uint64_t steps;     /* engine steps the job ran in */
+    uint64_t expanded;  /* parents expanded */
+    uint64_t children;
+    uint64_t submit_ns, admit_ns, done_ns; /* steady-clock times */
+} ps_batch_job_info;
+
+typedef struct ps_batch_stats {
+    uint64_t steps;
+    uint64_t jobs_done;
+    uint64_t states_done;  /* sum of the layer counts of finished jobs */
+    uint64_t expanded;
+    uint64_t hbm_used, hbm_budget;
+    double step_ms;        /* host time spent in steps that ran jobs */
+} ps_batch_stats;
+
+void ps_batch_options_default(ps_batch_options* o);
+int ps_batch_create(const ps_batch_options* opt, ps_stream s, ps_batch** out);
+int ps_batch_destroy(ps_batch* b);
+/* index_count 0: the puzzle's IndexesCount(); fails if the footprint exceeds the budget */
+int ps_batch_submit(ps_batch* b, const ps_puzzle* p, uint64_t start, uint64_t index_count, uint64_t* job_id);
+int ps_batch_step(ps_batch* b, int32_t* running, int32_t* finished, int32_t* queued);
+/* ids of the jobs that finished (or failed) in the last step */
+int ps_batch_finished(const ps_batch* b, uint64_t* ids, int32_t cap, int32_t* n);
+/* a job's state and, once done, its layer counts and frontier hashes (cap entries each) */
+int ps_batch_job(const ps_batch* b, uint64_t job_id, ps_batch_job_info* info, uint64_t* counts, uint64_t* hashes,
+                 int32_t cap);
+/* drop a finished job's results */
+int ps_batch_forget(ps_batch* b, uint64_t job_id);
+int ps_batch_stats_get(const ps_batch* b, ps_batch_stats* out);
+
 /* ----------------------------------------------------------- device BFS */
 typedef struct ps_bfs ps_bfs;
 
@@ -256,7 +310,9 @@ typedef struct ps_bfs_options {
     int32_t graph;          /* 1: capture the batched layer loop in a hipGraph (default on) */
     int32_t graph_layers;   /* layers per captured graph (0: default 4) */
     int32_t cross_marks;    /* PS_CROSS_BYTES (default) or PS_CROSS_ATOMIC, see below */
-    int32_t reserved;
+    int32_t skip_visited;   /* 1: a child outside the LDS window whose bit is set in the
+                               current or previous layer is not marked (it cannot be new:
+                               in an undirected graph children of layer L lie in L-1..L+1) */
 } ps_bfs_options;
 
 typedef struct ps_bfs_stats {

@@ -33,6 +33,7 @@ FLAG_SYMMETRY, FLAG_INV_INDEX = 1, 2
 EXPAND_FULL, EXPAND_IN_SEGMENT, EXPAND_CROSS_SEGMENT = 0, 1, 2
 BFS_BATCHED, BFS_PLAIN = 0, 1
 CROSS_BYTES, CROSS_ATOMIC = 0, 1
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 INVALID_INDEX = (1 << 64) - 1
 
 
@@ -66,7 +67,7 @@ class BfsOptions(ctypes.Structure):
                 ("max_layers", ctypes.c_int32), ("index_count", ctypes.c_uint64),
                 ("tile_bits_log2", ctypes.c_int32), ("block_threads", ctypes.c_int32),
                 ("grid_blocks", ctypes.c_int32), ("hash", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("graph_layers", ctypes.c_int32), ("cross_marks", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("graph_layers", ctypes.c_int32), ("cross_marks", ctypes.c_int32), ("skip_visited", ctypes.c_int32)]
 
 
 class BfsStats(ctypes.Structure):
@@ -111,6 +112,26 @@ class FsResult(ctypes.Structure):
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
+class BatchOptions(ctypes.Structure):
+    """ps_batch_options"""
+    _fields_ = [("max_jobs", _i32), ("segment_bits", _i32), ("hbm_budget", _u64)]
+
+
+class BatchJobInfo(ctypes.Structure):
+    """ps_batch_job_info"""
+    _fields_ = [("state", _i32), ("layers", _i32), ("footprint", _u64), ("steps", _u64), ("expanded", _u64),
+                ("children", _u64), ("submit_ns", _u64), ("admit_ns", _u64), ("done_ns", _u64)]
+
+
+class BatchStats(ctypes.Structure):
+    """ps_batch_stats"""
+    _fields_ = [(n, _u64) for n in ("steps", "jobs_done", "states_done", "expanded", "hbm_used", "hbm_budget")] + \
+        [("step_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _SIGS = {
     "ps_last_error": (ctypes.c_char_p, []),
     "ps_init": (_i32, [_i32]),
@@ -154,6 +175,15 @@ _SIGS = {
     "ps_fs_segment_frontier": (_i32, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
                                       _u64]),
     "ps_fs_segment_buckets": (_i32, [_vp, _vp, _vp, _vp, _vp, _u64, ctypes.POINTER(_vp)]),
+    "ps_batch_options_default": (None, [ctypes.POINTER(BatchOptions)]),
+    "ps_batch_create": (_i32, [ctypes.POINTER(BatchOptions), _vp, ctypes.POINTER(_vp)]),
+    "ps_batch_destroy": (_i32, [_vp]),
+    "ps_batch_submit": (_i32, [_vp, _P, _u64, _u64, ctypes.POINTER(_u64)]),
+    "ps_batch_step": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "ps_batch_finished": (_i32, [_vp, _vp, _i32, ctypes.POINTER(_i32)]),
+    "ps_batch_job": (_i32, [_vp, _u64, ctypes.POINTER(BatchJobInfo), _vp, _vp, _i32]),
+    "ps_batch_forget": (_i32, [_vp, _u64]),
+    "ps_batch_stats_get": (_i32, [_vp, ctypes.POINTER(BatchStats)]),
     "ps_bfs_options_default": (None, [ctypes.POINTER(BfsOptions)]),
     "ps_bfs_create": (_i32, [_P, ctypes.POINTER(BfsOptions), _vp, ctypes.POINTER(_vp)]),
     "ps_bfs_destroy": (_i32, [_vp]),
@@ -374,3 +404,72 @@ def bfs_footprint(p, **opts):
     b = _u64()
     check(lib().ps_bfs_footprint(ctypes.byref(p), ctypes.byref(o), ctypes.byref(b)))
     return b.value
+
+
+class Batch:
+    """Multi-search engine (ps_batch_*): continuous batching of complete
+    three-bit BFS jobs of different puzzles."""
+
+    def __init__(self, max_jobs=64, segment_bits=22, hbm_budget=0, stream=None):
+        o = BatchOptions()
+        lib().ps_batch_options_default(ctypes.byref(o))
+        o.max_jobs, o.segment_bits, o.hbm_budget = max_jobs, segment_bits, hbm_budget
+        h = _vp()
+        check(lib().ps_batch_create(ctypes.byref(o), stream, ctypes.byref(h)))
+        self.h = h.value
+
+    def submit(self, p, start=0, index_count=0):
+        jid = _u64()
+        check(lib().ps_batch_submit(self.h, ctypes.byref(p), start, index_count, ctypes.byref(jid)))
+        return jid.value
+
+    def step(self):
+        r, f, q = _i32(), _i32(), _i32()
+        check(lib().ps_batch_step(self.h, ctypes.byref(r), ctypes.byref(f), ctypes.byref(q)))
+        return r.value, f.value, q.value
+
+    def finished(self):
+        n = _i32()
+        check(lib().ps_batch_finished(self.h, None, 0, ctypes.byref(n)))
+        ids = np.zeros(max(1, n.value), dtype=np.uint64)
+        check(lib().ps_batch_finished(self.h, ids.ctypes.data, n.value, ctypes.byref(n)))
+        return [int(x) for x in ids[:n.value]]
+
+    def job(self, jid, cap=1 << 12):
+        info = BatchJobInfo()
+        counts = np.zeros(cap, dtype=np.uint64)
+        hashes = np.zeros(cap, dtype=np.uint64)
+        rc = lib().ps_batch_job(self.h, jid, ctypes.byref(info), counts.ctypes.data, hashes.ctypes.data, cap)
+        out = {k: getattr(info, k) for k, _ in info._fields_}
+        if rc != PS_OK:
+            out["error"] = lib().ps_last_error().decode(errors="replace")
+        n = min(info.layers, cap)
+        out["counts"] = [int(x) for x in counts[:n]]
+        out["hashes"] = [int(x) for x in hashes[:n]]
+        return out
+
+    def forget(self, jid):
+        check(lib().ps_batch_forget(self.h, jid))
+
+    def stats(self):
+        st = BatchStats()
+        check(lib().ps_batch_stats_get(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            check(lib().ps_batch_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            if self.h and _lib is not None:
+                _lib.ps_batch_destroy(self.h)
+        except Exception:
+            pass

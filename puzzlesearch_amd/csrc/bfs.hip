@@ -24,10 +24,10 @@
 #include <stdexcept>
 #include <vector>
 
+#include "bfs_internal.h"
 #include "bfs_launch.h"
 #include "ps_internal.h"
 
-using psk::BfsDevState;
 
 namespace {
 
@@ -40,80 +40,12 @@ uint64_t host_splitmix64(uint64_t x) {
 
 uint64_t host_word_hash(uint64_t j, uint64_t w) { return w ? host_splitmix64(host_splitmix64(j) ^ w) : 0; }
 
-constexpr int MAX_LAYERS_CAP = 1 << 16;
+constexpr int MAX_LAYERS_CAP = ps::BFS_MAX_LAYERS;
 
 }  // namespace
 
-struct ps_bfs {
-    ps_puzzle puzzle{};
-    ps_bfs_options opt{};
-    ps_puzzle_info info{};
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    int bits = 3;
-    uint64_t nbits = 0, nwords = 0;
-    uint64_t* arr[3] = {nullptr, nullptr, nullptr};
-    uint8_t* xmark = nullptr;  // byte per state, three-bit search with PS_CROSS_BYTES
-    uint32_t nseg = 1;
-    int seg_shift = 63;
-    uint64_t seg_words = 0;
-    unsigned long long* seg_count = nullptr;
-    uint32_t* touched = nullptr;
-    unsigned long long* layer_counts = nullptr;
-    unsigned long long* layer_hashes = nullptr;
-    BfsDevState* st = nullptr;
-    BfsDevState* h_st = nullptr;  // pinned mirror
-    unsigned long long* h_seg = nullptr;  // pinned, 3*nseg
-    uint32_t* h_touched = nullptr;        // pinned
-    int max_layers = MAX_LAYERS_CAP;
-    // expand geometry
-    int use_window = 1;
-    uint64_t tile_words = 0, ntiles = 0;
-    int tpb_expand = 512;
-    uint32_t grid_expand = 0;
-    size_t lds_expand = 0;
-    // finalize geometry
-    int tpb_fin = 256;
-    uint32_t grid_fin = 0;
-    uint64_t chunks_per_seg = 0, nchunks = 0;
-    // two-bit geometry
-    uint32_t grid_bfs2 = 0;
-    // graph
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_layers = 4;
-    // events for timing
-    hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-    std::vector<std::pair<size_t, size_t>> expand_spans, fin_spans;
-    bool collect_kernel_times = true;
-    ps_bfs_stats stats{};
-    int layers_done = 0;
-    uint64_t start_index = 0;
-};
 
 namespace ps {
-
-namespace {
-
-int take_event(ps_bfs* b, hipEvent_t* out, size_t* idx) {
-    if (b->ev_used == b->ev_pool.size()) {
-        hipEvent_t e;
-        PS_HIP_CHECK(hipEventCreate(&e));
-        b->ev_pool.push_back(e);
-    }
-    *idx = b->ev_used;
-    *out = b->ev_pool[b->ev_used++];
-    return PS_OK;
-}
-
-int record(ps_bfs* b, size_t* idx) {
-    if (!b->collect_kernel_times) return PS_OK;
-    hipEvent_t e;
-    if (int r = take_event(b, &e, idx)) return r;
-    PS_HIP_CHECK(hipEventRecord(e, b->stream));
-    return PS_OK;
-}
 
 psk::ExpandArgs expand_args(ps_bfs* b) {
     psk::ExpandArgs a{};
@@ -134,6 +66,7 @@ psk::ExpandArgs expand_args(ps_bfs* b) {
     a.persistent = 1;
     a.use_window = b->use_window;
     a.xmark = b->xmark;
+    a.skip_visited = b->opt.skip_visited;
     return a;
 }
 
@@ -166,6 +99,27 @@ psk::LayerEndArgs layer_end_args(ps_bfs* b) {
     e.st = b->st;
     e.bits = b->bits;
     return e;
+}
+
+namespace {
+
+int take_event(ps_bfs* b, hipEvent_t* out, size_t* idx) {
+    if (b->ev_used == b->ev_pool.size()) {
+        hipEvent_t e;
+        PS_HIP_CHECK(hipEventCreate(&e));
+        b->ev_pool.push_back(e);
+    }
+    *idx = b->ev_used;
+    *out = b->ev_pool[b->ev_used++];
+    return PS_OK;
+}
+
+int record(ps_bfs* b, size_t* idx) {
+    if (!b->collect_kernel_times) return PS_OK;
+    hipEvent_t e;
+    if (int r = take_event(b, &e, idx)) return r;
+    PS_HIP_CHECK(hipEventRecord(e, b->stream));
+    return PS_OK;
 }
 
 int launch_layer_end(ps_bfs* b) {
@@ -381,6 +335,43 @@ uint64_t footprint_bytes(const ps_bfs_options* o, const Geometry& g, int max_lay
 
 using namespace ps;
 
+// Reads the device state into the stats (call after the stream is idle).
+int ps::collect_stats(ps_bfs* b) {
+    PS_HIP_CHECK(hipMemcpy(b->h_st, b->st, sizeof(BfsDevState), hipMemcpyDeviceToHost));
+    const BfsDevState& st = *b->h_st;
+    if (st.error) return fail(PS_ERR_INDEX_RANGE, "a child index exceeded the index space (%llu states)",
+                              (unsigned long long)b->nbits);
+    b->stats.layers = st.layers_recorded;
+    b->stats.expanded = st.expanded;
+    b->stats.children = st.children;
+    if (b->bits == 3) {
+        const uint64_t fin_words_bytes = b->info.has_odd_cycles ? 40 : 32;
+        b->stats.algorithmic_bytes =
+            8ull * st.scan_words + 8ull * st.children + fin_words_bytes * st.fin_full_words + 8ull * st.fin_clear_words;
+        b->stats.expand_bytes = 8ull * st.scan_words + 8ull * st.children;
+        b->stats.scan_words = st.scan_words;
+    } else {
+        // two-bit: every layer reads and writes the whole 2-bit array, 8 bytes per child
+        b->stats.algorithmic_bytes = 16ull * b->nwords * (st.layers_recorded) + 8ull * st.children;
+        b->stats.expand_bytes = b->stats.algorithmic_bytes;
+        b->stats.scan_words = b->nwords * st.layers_recorded;
+    }
+    double ems = 0, fms = 0;
+    for (auto& sp : b->expand_spans) {
+        float ms = 0;
+        PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
+        ems += ms;
+    }
+    for (auto& sp : b->fin_spans) {
+        float ms = 0;
+        PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
+        fms += ms;
+    }
+    b->stats.expand_ms = ems;
+    b->stats.finalize_ms = fms;
+    return PS_OK;
+}
+
 extern "C" {
 
 void ps_bfs_options_default(ps_bfs_options* o) {
@@ -577,42 +568,6 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     PS_API_END
 }
 
-// Reads the device state into the stats (call after the stream is idle).
-static int collect_stats(ps_bfs* b) {
-    PS_HIP_CHECK(hipMemcpy(b->h_st, b->st, sizeof(BfsDevState), hipMemcpyDeviceToHost));
-    const BfsDevState& st = *b->h_st;
-    if (st.error) return fail(PS_ERR_INDEX_RANGE, "a child index exceeded the index space (%llu states)",
-                              (unsigned long long)b->nbits);
-    b->stats.layers = st.layers_recorded;
-    b->stats.expanded = st.expanded;
-    b->stats.children = st.children;
-    if (b->bits == 3) {
-        const uint64_t fin_words_bytes = b->info.has_odd_cycles ? 40 : 32;
-        b->stats.algorithmic_bytes =
-            8ull * st.scan_words + 8ull * st.children + fin_words_bytes * st.fin_full_words + 8ull * st.fin_clear_words;
-        b->stats.expand_bytes = 8ull * st.scan_words + 8ull * st.children;
-        b->stats.scan_words = st.scan_words;
-    } else {
-        // two-bit: every layer reads and writes the whole 2-bit array, 8 bytes per child
-        b->stats.algorithmic_bytes = 16ull * b->nwords * (st.layers_recorded) + 8ull * st.children;
-        b->stats.expand_bytes = b->stats.algorithmic_bytes;
-        b->stats.scan_words = b->nwords * st.layers_recorded;
-    }
-    double ems = 0, fms = 0;
-    for (auto& sp : b->expand_spans) {
-        float ms = 0;
-        PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
-        ems += ms;
-    }
-    for (auto& sp : b->fin_spans) {
-        float ms = 0;
-        PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
-        fms += ms;
-    }
-    b->stats.expand_ms = ems;
-    b->stats.finalize_ms = fms;
-    return PS_OK;
-}
 
 int ps_bfs_step(ps_bfs* b, uint64_t* count, uint64_t* hash) {
     PS_API_BEGIN

@@ -1,0 +1,74 @@
+// The device-resident BFS object shared by the single-search driver
+// (bfs.hip) and the multi-search engine (batch.hip).
+#pragma once
+
+#include <utility>
+#include <vector>
+
+#include "bfs_kernels.h"
+#include "ps_internal.h"
+
+using psk::BfsDevState;
+
+namespace ps {
+constexpr int BFS_MAX_LAYERS = 1 << 16;
+}  // namespace ps
+
+struct ps_bfs {
+    ps_puzzle puzzle{};
+    ps_bfs_options opt{};
+    ps_puzzle_info info{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int bits = 3;
+    uint64_t nbits = 0, nwords = 0;
+    uint64_t* arr[3] = {nullptr, nullptr, nullptr};
+    uint8_t* xmark = nullptr;  // byte per state, three-bit search with PS_CROSS_BYTES
+    uint32_t nseg = 1;
+    int seg_shift = 63;
+    uint64_t seg_words = 0;
+    unsigned long long* seg_count = nullptr;
+    uint32_t* touched = nullptr;
+    unsigned long long* layer_counts = nullptr;
+    unsigned long long* layer_hashes = nullptr;
+    BfsDevState* st = nullptr;
+    BfsDevState* h_st = nullptr;  // pinned mirror
+    unsigned long long* h_seg = nullptr;  // pinned, 3*nseg
+    uint32_t* h_touched = nullptr;        // pinned
+    int max_layers = ps::BFS_MAX_LAYERS;
+    // expand geometry
+    int use_window = 1;
+    uint64_t tile_words = 0, ntiles = 0;
+    int tpb_expand = 512;
+    uint32_t grid_expand = 0;
+    size_t lds_expand = 0;
+    // finalize geometry
+    int tpb_fin = 256;
+    uint32_t grid_fin = 0;
+    uint64_t chunks_per_seg = 0, nchunks = 0;
+    // two-bit geometry
+    uint32_t grid_bfs2 = 0;
+    // graph
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_layers = 4;
+    // events for timing
+    hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<size_t, size_t>> expand_spans, fin_spans;
+    bool collect_kernel_times = true;
+    ps_bfs_stats stats{};
+    int layers_done = 0;
+    uint64_t start_index = 0;
+};
+
+namespace ps {
+
+// kernel arguments of one search in its current geometry
+psk::ExpandArgs expand_args(ps_bfs* b);
+psk::FinalizeArgs finalize_args(ps_bfs* b);
+psk::LayerEndArgs layer_end_args(ps_bfs* b);
+// reads the device state into b->stats (stream idle)
+int collect_stats(ps_bfs* b);
+
+}  // namespace ps

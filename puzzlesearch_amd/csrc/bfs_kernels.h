@@ -62,6 +62,7 @@ struct ExpandArgs {
     int32_t persistent;
     int32_t use_window;
     uint8_t* xmark;  // byte per state for children outside the window; NULL: atomic OR into next
+    int32_t skip_visited;  // skip cross-window children already in cur or old
 };
 
 struct FinalizeArgs {
@@ -204,157 +205,190 @@ inline size_t bfs3_lds_bytes(int tpb, uint64_t tile_words, int use_window, uint3
     return b;
 }
 
+// LDS of the expand kernels: the tile window, one compaction queue per wave,
+// and the segment "touched" bits of the job being expanded.
+struct ExpandLds {
+    uint32_t* win;
+    uint32_t* queues;
+    uint32_t* tl;
+    uint32_t tl_words;  // 0: touched bits go straight to HBM
+};
+
+struct ExpandTally {
+    unsigned long long children = 0, parents = 0, scan = 0;
+};
+
+// Expands every parent of one tile of one search.  All threads of the block
+// call it with the same tile; it starts and ends with a block barrier.
+template <class PZ, int TPB>
+__device__ __forceinline__ void expand_tile(const ExpandArgs& a, uint64_t tile, const ExpandLds& lds,
+                                            uint32_t* s_local, ExpandTally& tally) {
+    BfsDevState* st = a.st;
+    const uint32_t L = st->layer;
+    const uint64_t* __restrict__ cur = a.bits[L % 3];
+    uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
+    const uint64_t* __restrict__ old = a.bits[(L + 2) % 3];
+    const unsigned long long* sc_cur = a.seg_count + (uint64_t)(L % 3) * a.nseg;
+    constexpr int NW = TPB / 64;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    uint32_t* win = lds.win;
+    uint32_t* q = lds.queues + wave * QCAP;
+    const bool track = a.nseg > 1;
+
+    const uint64_t w0 = tile * a.tile_words;
+    const uint64_t w1 = (w0 + a.tile_words < a.nwords) ? w0 + a.tile_words : a.nwords;
+    const uint64_t pw0 = w0 > a.word_lo ? w0 : a.word_lo;
+    const uint64_t pw1 = w1 < a.word_hi ? w1 : a.word_hi;
+    bool live = pw0 < pw1;
+    if (live && track) {
+        const uint64_t s0 = (pw0 * 64) >> a.seg_shift, s1 = (pw1 * 64 - 1) >> a.seg_shift;
+        live = false;
+        for (uint64_t s = s0; s <= s1; ++s) live |= sc_cur[s] != 0;
+    }
+    if (!live) {
+        __syncthreads();
+        return;
+    }
+    if (tid == 0) *s_local = 0;
+    const uint32_t nwin = (uint32_t)(w1 - w0) * 2;
+    if (a.use_window)
+        for (uint32_t i = tid; i < nwin; i += TPB) win[i] = 0;
+    __syncthreads();
+
+    if (tid == 0) tally.scan += pw1 - pw0;
+    const uint64_t win_lo = w0 * 64, win_hi = w1 * 64;
+    bool local_any = false;
+    auto mark = [&](uint64_t child) {
+        if (child >= a.nbits) {
+            atomicOr(&st->error, 1u);
+            return;
+        }
+        ++tally.children;
+        if (a.use_window && child >= win_lo && child < win_hi) {
+            lor(&win[(child - win_lo) >> 5], 1u << (child & 31));
+            local_any = true;
+        } else {
+            if (a.skip_visited && (((cur[child >> 6] | old[child >> 6]) >> (child & 63)) & 1ull)) return;
+            if (a.xmark) a.xmark[child] = 1;
+            else gor(&nxt[child >> 5], 1u << (child & 31));
+            if (track) {
+                const uint64_t s = child >> a.seg_shift;
+                if (lds.tl_words) lor(&lds.tl[s >> 5], 1u << (s & 31));
+                else gor(&a.touched[s >> 5], 1u << (s & 31));
+            }
+        }
+    };
+    auto run_batch = [&](uint32_t n) {
+        for (uint32_t j = lane; j < n; j += 64) {
+            const uint64_t parent = win_lo + q[j];
+            ++tally.parents;
+            PZ::expand(parent, 0u, a.prm, [&](int, uint64_t child, uint32_t) { mark(child); });
+        }
+    };
+
+    int qn = 0;  // wave-uniform queue fill
+    for (uint64_t g = pw0 + (uint64_t)wave * 64; g < pw1; g += (uint64_t)NW * 64) {
+        const uint64_t wi = g + lane;
+        uint64_t bits = (wi < pw1) ? cur[wi] : 0ull;
+        const uint32_t c = (uint32_t)__popcll(bits);
+        const uint32_t incl = wave_incl_scan(c, lane);
+        const int total = (int)__shfl(incl, 63, 64);
+        int off = qn + (int)(incl - c);
+        int end_all = qn + total;
+        const uint32_t base = (uint32_t)((wi - w0) * 64);
+        for (;;) {
+            while (bits && off < QCAP) {
+                q[off++] = base + (uint32_t)(__ffsll((unsigned long long)bits) - 1);
+                bits &= bits - 1;
+            }
+            if (end_all < QCAP) {
+                qn = end_all;
+                break;
+            }
+            wave_sync();
+            run_batch(QCAP);
+            wave_sync();
+            off -= QCAP;
+            end_all -= QCAP;
+            qn = 0;
+            if (end_all == 0) break;
+        }
+    }
+    wave_sync();
+    run_batch((uint32_t)qn);
+    if (local_any) *s_local = 1;
+    __syncthreads();
+    if (a.use_window) {
+        // with byte marks nothing else writes this range of next, so a tile
+        // that owns its whole range (processed once per layer) stores plainly
+        uint32_t* dst = nxt + w0 * 2;
+        const bool own = a.xmark && pw0 == w0 && pw1 == w1;
+        for (uint32_t i = tid; i < nwin; i += TPB) {
+            const uint32_t v = win[i];
+            if (v) {
+                if (own) dst[i] = v;
+                else gor(&dst[i], v);
+            }
+        }
+    }
+    if (tid == 0 && track && *s_local) {
+        const uint64_t s0 = win_lo >> a.seg_shift, s1 = (win_hi - 1) >> a.seg_shift;
+        for (uint64_t s = s0; s <= s1; ++s) {
+            if (lds.tl_words) lor(&lds.tl[s >> 5], 1u << (s & 31));
+            else gor(&a.touched[s >> 5], 1u << (s & 31));
+        }
+    }
+    __syncthreads();
+}
+
 template <class PZ, int TPB>
 __global__ __launch_bounds__(TPB) void k_bfs3_expand(ExpandArgs a) {
     extern __shared__ uint32_t smem[];
     constexpr int NW = TPB / 64;
     BfsDevState* st = a.st;
     if (st->done) return;
-    const uint32_t L = st->layer;
-    const uint64_t* __restrict__ cur = a.bits[L % 3];
-    uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
-    const unsigned long long* sc_cur = a.seg_count + (uint64_t)(L % 3) * a.nseg;
-
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int tid = threadIdx.x;
     const uint32_t win_u32 = a.use_window ? (uint32_t)(a.tile_words * 2) : 0u;
-    uint32_t* win = smem;
-    uint32_t* q = smem + win_u32 + wave * QCAP;
-    const bool track = a.nseg > 1;
-    const uint32_t tl_words = (track && a.nseg <= TOUCH_LDS_MAX) ? (a.nseg + 31) / 32 : 0u;
-    uint32_t* tl = smem + win_u32 + NW * QCAP;
+    ExpandLds lds;
+    lds.win = smem;
+    lds.queues = smem + win_u32;
+    lds.tl = smem + win_u32 + NW * QCAP;
+    lds.tl_words = (a.nseg > 1 && a.nseg <= TOUCH_LDS_MAX) ? (a.nseg + 31) / 32 : 0u;
     __shared__ unsigned long long s_tile;
     __shared__ uint32_t s_local;
+    for (uint32_t i = tid; i < lds.tl_words; i += TPB) lds.tl[i] = 0;
 
-    for (uint32_t i = tid; i < tl_words; i += TPB) tl[i] = 0;
-
-    unsigned long long my_children = 0, my_parents = 0, my_scan = 0;
+    ExpandTally tally;
     uint64_t iter = 0;
     for (;;) {
         if (tid == 0) {
             if (a.persistent) s_tile = a.tile_begin + atomicAdd(&st->tile_counter, 1ull);
             else s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
-            s_local = 0;
         }
         __syncthreads();
         const uint64_t tile = s_tile;
         ++iter;
         if (tile >= a.tile_end) break;
-        const uint64_t w0 = tile * a.tile_words;
-        const uint64_t w1 = (w0 + a.tile_words < a.nwords) ? w0 + a.tile_words : a.nwords;
-        const uint64_t pw0 = w0 > a.word_lo ? w0 : a.word_lo;
-        const uint64_t pw1 = w1 < a.word_hi ? w1 : a.word_hi;
-        bool live = pw0 < pw1;
-        if (live && track) {
-            const uint64_t s0 = (pw0 * 64) >> a.seg_shift, s1 = (pw1 * 64 - 1) >> a.seg_shift;
-            live = false;
-            for (uint64_t s = s0; s <= s1; ++s) live |= sc_cur[s] != 0;
-        }
-        if (!live) {
-            __syncthreads();
-            continue;
-        }
-        const uint32_t nwin = (uint32_t)(w1 - w0) * 2;
-        if (a.use_window)
-            for (uint32_t i = tid; i < nwin; i += TPB) win[i] = 0;
-        __syncthreads();
-
-        if (tid == 0) my_scan += pw1 - pw0;
-        const uint64_t win_lo = w0 * 64, win_hi = w1 * 64;
-        bool local_any = false;
-        auto mark = [&](uint64_t child) {
-            if (child >= a.nbits) {
-                atomicOr(&st->error, 1u);
-                return;
-            }
-            ++my_children;
-            if (a.use_window && child >= win_lo && child < win_hi) {
-                lor(&win[(child - win_lo) >> 5], 1u << (child & 31));
-                local_any = true;
-            } else {
-                if (a.xmark) a.xmark[child] = 1;
-                else gor(&nxt[child >> 5], 1u << (child & 31));
-                if (track) {
-                    const uint64_t s = child >> a.seg_shift;
-                    if (tl_words) lor(&tl[s >> 5], 1u << (s & 31));
-                    else gor(&a.touched[s >> 5], 1u << (s & 31));
-                }
-            }
-        };
-        auto run_batch = [&](uint32_t n) {
-            for (uint32_t j = lane; j < n; j += 64) {
-                const uint64_t parent = win_lo + q[j];
-                ++my_parents;
-                PZ::expand(parent, 0u, a.prm, [&](int, uint64_t child, uint32_t) { mark(child); });
-            }
-        };
-
-        int qn = 0;  // wave-uniform queue fill
-        for (uint64_t g = pw0 + (uint64_t)wave * 64; g < pw1; g += (uint64_t)NW * 64) {
-            const uint64_t wi = g + lane;
-            uint64_t bits = (wi < pw1) ? cur[wi] : 0ull;
-            const uint32_t c = (uint32_t)__popcll(bits);
-            const uint32_t incl = wave_incl_scan(c, lane);
-            const int total = (int)__shfl(incl, 63, 64);
-            int off = qn + (int)(incl - c);
-            int end_all = qn + total;
-            const uint32_t base = (uint32_t)((wi - w0) * 64);
-            for (;;) {
-                while (bits && off < QCAP) {
-                    q[off++] = base + (uint32_t)(__ffsll((unsigned long long)bits) - 1);
-                    bits &= bits - 1;
-                }
-                if (end_all < QCAP) {
-                    qn = end_all;
-                    break;
-                }
-                wave_sync();
-                run_batch(QCAP);
-                wave_sync();
-                off -= QCAP;
-                end_all -= QCAP;
-                qn = 0;
-                if (end_all == 0) break;
-            }
-        }
-        wave_sync();
-        run_batch((uint32_t)qn);
-        if (local_any) s_local = 1;
-        __syncthreads();
-        if (a.use_window) {
-            // with byte marks nothing else writes this range of next, so a tile
-            // that owns its whole range (processed once per layer) stores plainly
-            uint32_t* dst = nxt + w0 * 2;
-            const bool own = a.xmark && pw0 == w0 && pw1 == w1;
-            for (uint32_t i = tid; i < nwin; i += TPB) {
-                const uint32_t v = win[i];
-                if (v) {
-                    if (own) dst[i] = v;
-                    else gor(&dst[i], v);
-                }
-            }
-        }
-        if (tid == 0 && track && s_local) {
-            const uint64_t s0 = win_lo >> a.seg_shift, s1 = (win_hi - 1) >> a.seg_shift;
-            for (uint64_t s = s0; s <= s1; ++s) {
-                if (tl_words) lor(&tl[s >> 5], 1u << (s & 31));
-                else gor(&a.touched[s >> 5], 1u << (s & 31));
-            }
-        }
-        __syncthreads();
+        expand_tile<PZ, TPB>(a, tile, lds, &s_local, tally);
     }
     __syncthreads();
-    for (uint32_t i = tid; i < tl_words; i += TPB) {
-        const uint32_t v = tl[i];
+    for (uint32_t i = tid; i < lds.tl_words; i += TPB) {
+        const uint32_t v = lds.tl[i];
         if (v) gor(&a.touched[i], v);
     }
-    block_add3<TPB>(my_children, my_parents, my_scan, &st->children, &st->expanded, &st->scan_words);
+    block_add3<TPB>(tally.children, tally.parents, tally.scan, &st->children, &st->expanded, &st->scan_words);
 }
 
-// Finalize: new &= ~cur [odd]; new &= ~old; per-segment and layer counts; hash; old = 0.
+// Finalize of one chunk of one search: new &= ~cur [odd]; new &= ~old; per-segment
+// and layer counts; hash; old = 0.  Returns the chunk's (count, hash, full, cleared)
+// through the tally; every thread of the block calls it.
+struct FinTally {
+    unsigned long long cnt = 0, hash = 0, full = 0, clear = 0;
+};
+
 template <int TPB>
-__global__ __launch_bounds__(TPB) void k_bfs3_finalize(FinalizeArgs a) {
+__device__ __forceinline__ void finalize_chunk(const FinalizeArgs& a, uint64_t c, FinTally& tally) {
     BfsDevState* st = a.st;
-    if (st->done) return;
     const uint32_t L = st->layer;
     const uint64_t* __restrict__ cur = a.bits[L % 3];
     uint64_t* nw = a.bits[(L + 1) % 3];
@@ -362,9 +396,73 @@ __global__ __launch_bounds__(TPB) void k_bfs3_finalize(FinalizeArgs a) {
     unsigned long long* sc_new = a.seg_count + (uint64_t)((L + 1) % 3) * a.nseg;
     const unsigned long long* sc_old = a.seg_count + (uint64_t)((L + 2) % 3) * a.nseg;
     const bool track = a.nseg > 1;
-    __shared__ unsigned long long s_chunk;
     __shared__ unsigned long long red[2][TPB / 64];
-    unsigned long long tot_cnt = 0, tot_hash = 0, tot_full = 0, tot_clear = 0;
+    const uint64_t s = c / a.chunks_per_seg;
+    const uint64_t w0 = s * a.seg_words + (c % a.chunks_per_seg) * FIN_CHUNK_WORDS;
+    uint64_t w1 = w0 + FIN_CHUNK_WORDS;
+    const uint64_t seg_end = (s + 1) * a.seg_words;
+    if (w1 > seg_end) w1 = seg_end;
+    if (w1 > a.nwords) w1 = a.nwords;
+    if (w0 >= w1) return;
+    const bool touched = !track || ((a.touched[s >> 5] >> (s & 31)) & 1u);
+    const bool old_live = !track || sc_old[s] != 0;
+    unsigned long long cnt = 0, h = 0;
+    if (touched) {
+        for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) {
+            uint64_t v = nw[j];
+            if (a.xmark) v |= fold_marks(a.xmark + j * 64);
+            v &= ~old[j];
+            if (a.odd) v &= ~cur[j];
+            nw[j] = v;
+            old[j] = 0;
+            cnt += (unsigned long long)__popcll(v);
+            if (a.hash) h += word_hash(j, v);
+        }
+    } else if (old_live) {
+        for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) old[j] = 0;
+    }
+    if (threadIdx.x == 0) {
+        if (touched) tally.full += w1 - w0;
+        else if (old_live) tally.clear += w1 - w0;
+    }
+    if (touched) {
+        cnt = wave_sum(cnt);
+        h = wave_sum(h);
+        const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+        if (lane == 0) {
+            red[0][wave] = cnt;
+            red[1][wave] = h;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long sc = 0, sh = 0;
+            for (int w = 0; w < TPB / 64; ++w) {
+                sc += red[0][w];
+                sh += red[1][w];
+            }
+            if (sc) atomicAdd(&sc_new[s], sc);
+            tally.cnt += sc;
+            tally.hash += sh;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void flush_fin_tally(BfsDevState* st, const FinTally& t) {
+    if (threadIdx.x == 0) {
+        if (t.cnt) atomicAdd(&st->acc_count, t.cnt);
+        if (t.hash) atomicAdd(&st->acc_hash, t.hash);
+        if (t.full) atomicAdd(&st->fin_full_words, t.full);
+        if (t.clear) atomicAdd(&st->fin_clear_words, t.clear);
+    }
+}
+
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_bfs3_finalize(FinalizeArgs a) {
+    BfsDevState* st = a.st;
+    if (st->done) return;
+    __shared__ unsigned long long s_chunk;
+    FinTally tally;
     uint64_t iter = 0;
     for (;;) {
         if (threadIdx.x == 0) {
@@ -376,67 +474,15 @@ __global__ __launch_bounds__(TPB) void k_bfs3_finalize(FinalizeArgs a) {
         ++iter;
         __syncthreads();
         if (c >= a.chunk_end) break;
-        const uint64_t s = c / a.chunks_per_seg;
-        const uint64_t w0 = s * a.seg_words + (c % a.chunks_per_seg) * FIN_CHUNK_WORDS;
-        uint64_t w1 = w0 + FIN_CHUNK_WORDS;
-        const uint64_t seg_end = (s + 1) * a.seg_words;
-        if (w1 > seg_end) w1 = seg_end;
-        if (w1 > a.nwords) w1 = a.nwords;
-        if (w0 >= w1) continue;
-        const bool touched = !track || ((a.touched[s >> 5] >> (s & 31)) & 1u);
-        const bool old_live = !track || sc_old[s] != 0;
-        unsigned long long cnt = 0, h = 0;
-        if (touched) {
-            for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) {
-                uint64_t v = nw[j];
-                if (a.xmark) v |= fold_marks(a.xmark + j * 64);
-                v &= ~old[j];
-                if (a.odd) v &= ~cur[j];
-                nw[j] = v;
-                old[j] = 0;
-                cnt += (unsigned long long)__popcll(v);
-                if (a.hash) h += word_hash(j, v);
-            }
-        } else if (old_live) {
-            for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) old[j] = 0;
-        }
-        if (threadIdx.x == 0) {
-            if (touched) tot_full += w1 - w0;
-            else if (old_live) tot_clear += w1 - w0;
-        }
-        if (touched) {
-            cnt = wave_sum(cnt);
-            h = wave_sum(h);
-            const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-            if (lane == 0) {
-                red[0][wave] = cnt;
-                red[1][wave] = h;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                unsigned long long sc = 0, sh = 0;
-                for (int w = 0; w < TPB / 64; ++w) {
-                    sc += red[0][w];
-                    sh += red[1][w];
-                }
-                if (sc) atomicAdd(&sc_new[s], sc);
-                tot_cnt += sc;
-                tot_hash += sh;
-            }
-            __syncthreads();
-        }
+        finalize_chunk<TPB>(a, c, tally);
     }
-    if (threadIdx.x == 0) {
-        if (tot_cnt) atomicAdd(&st->acc_count, tot_cnt);
-        if (tot_hash) atomicAdd(&st->acc_hash, tot_hash);
-        if (tot_full) atomicAdd(&st->fin_full_words, tot_full);
-        if (tot_clear) atomicAdd(&st->fin_clear_words, tot_clear);
-    }
+    flush_fin_tally(st, tally);
 }
 
-// Single-block bookkeeping at the end of a layer.
+// Bookkeeping at the end of a layer, run by one block: record the layer, rotate
+// the roles, reset the per-layer counters.
 template <int TPB>
-__global__ __launch_bounds__(TPB) void k_bfs_layer_end(LayerEndArgs a) {
+__device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
     BfsDevState* st = a.st;
     if (st->done) return;
     const uint32_t L = st->layer;
@@ -463,6 +509,11 @@ __global__ __launch_bounds__(TPB) void k_bfs_layer_end(LayerEndArgs a) {
         st->tile_counter = 0;
         st->chunk_counter = 0;
     }
+}
+
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_bfs_layer_end(LayerEndArgs a) {
+    layer_end_body<TPB>(a);
 }
 
 // ---------------------------------------------------------------- two-bit
@@ -575,6 +626,111 @@ __global__ __launch_bounds__(TPB) void k_bfs2_hash(const uint64_t* arr, uint64_t
     for (uint64_t j = (uint64_t)blockIdx.x * TPB + threadIdx.x; j < nwords; j += (uint64_t)gridDim.x * TPB)
         h += word_hash(j, arr[j]);
     block_add3<TPB>(h, 0ull, 0ull, &st->acc_hash, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------- multi-search
+// Several searches (jobs) of different puzzles advance one layer per step in
+// the same three launches: every block takes (job, tile) pairs from one global
+// counter over the concatenated tile lists, and (job, chunk) pairs for the
+// finalize; one block per job does the bookkeeping.
+
+constexpr uint32_t MULTI_TL_WORDS = 256;  // segments tracked in LDS per job (8192)
+
+struct MultiJob {
+    ExpandArgs e;
+    FinalizeArgs f;
+    LayerEndArgs l;
+    uint32_t kind;  // index into the multi-search puzzle table
+    uint32_t slot;  // the job's slot in the done-flag array
+    uint64_t tile_base;
+    uint64_t chunk_base;
+};
+
+struct MultiArgs {
+    const MultiJob* jobs;
+    uint32_t njobs;
+    uint64_t total_tiles;
+    uint64_t total_chunks;
+    unsigned long long* counters;  // [0] tiles taken, [1] chunks taken
+    uint32_t* done;                // per slot: the job finished
+};
+
+// the job owning item g (first job whose base is <= g, bases ascending)
+template <bool TILES>
+__device__ __forceinline__ uint32_t multi_find(const MultiArgs& m, uint64_t g) {
+    uint32_t lo = 0, hi = m.njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        const uint64_t base = TILES ? m.jobs[mid].tile_base : m.jobs[mid].chunk_base;
+        if (base <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <class KINDS, int TPB>
+__global__ __launch_bounds__(TPB) void k_multi_expand(MultiArgs m, uint32_t win_u32_max) {
+    extern __shared__ uint32_t smem[];
+    constexpr int NW = TPB / 64;
+    ExpandLds lds;
+    lds.win = smem;
+    lds.queues = smem + win_u32_max;
+    lds.tl = smem + win_u32_max + NW * QCAP;
+    __shared__ unsigned long long s_tile;
+    __shared__ uint32_t s_local;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&m.counters[0], 1ull);
+        __syncthreads();
+        const uint64_t g = s_tile;
+        if (g >= m.total_tiles) break;
+        const MultiJob& J = m.jobs[multi_find<true>(m, g)];
+        if (J.e.st->done) {
+            __syncthreads();
+            continue;
+        }
+        const uint32_t nseg = J.e.nseg;
+        lds.tl_words = (nseg > 1 && (nseg + 31) / 32 <= MULTI_TL_WORDS) ? (nseg + 31) / 32 : 0u;
+        for (uint32_t i = threadIdx.x; i < lds.tl_words; i += TPB) lds.tl[i] = 0;
+        ExpandTally tally;
+        KINDS::template run<TPB>(J.kind, J.e, g - J.tile_base, lds, &s_local, tally);
+        for (uint32_t i = threadIdx.x; i < lds.tl_words; i += TPB) {
+            const uint32_t v = lds.tl[i];
+            if (v) gor(&J.e.touched[i], v);
+        }
+        block_add3<TPB>(tally.children, tally.parents, tally.scan, &J.e.st->children, &J.e.st->expanded,
+                        &J.e.st->scan_words);
+    }
+}
+
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_multi_finalize(MultiArgs m) {
+    __shared__ unsigned long long s_chunk;
+    for (;;) {
+        if (threadIdx.x == 0) s_chunk = atomicAdd(&m.counters[1], 1ull);
+        __syncthreads();
+        const uint64_t g = s_chunk;
+        __syncthreads();
+        if (g >= m.total_chunks) break;
+        const MultiJob& J = m.jobs[multi_find<false>(m, g)];
+        if (J.f.st->done) continue;
+        FinTally tally;
+        finalize_chunk<TPB>(J.f, g - J.chunk_base, tally);
+        flush_fin_tally(J.f.st, tally);
+    }
+}
+
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_multi_layer_end(MultiArgs m) {
+    const MultiJob& J = m.jobs[blockIdx.x];
+    layer_end_body<TPB>(J.l);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m.done[J.slot] = J.l.st->done;
+        if (blockIdx.x == 0) {
+            m.counters[0] = 0;
+            m.counters[1] = 0;
+        }
+    }
 }
 
 }  // namespace psk

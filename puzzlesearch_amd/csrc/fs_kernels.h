@@ -203,28 +203,33 @@ __global__ __launch_bounds__(TPB) void k_fs_write(FsScanArgs a) {
 
 
 // children of the written entries that leave the segment, one slot per move
+struct CrossSink {
+    unsigned long long* out;
+    uint64_t segment, nseg, in_mask;
+    int seg_bits;
+    unsigned long long nx, bad;
+    __device__ __forceinline__ void operator()(int slot, uint64_t child, uint32_t label) {
+        const uint64_t s = child >> seg_bits;
+        if (s == segment) return;
+        if (s >= nseg) {
+            ++bad;
+            return;
+        }
+        out[slot] = (((uint64_t)label * nseg + s) << 32) | (child & in_mask);
+        ++nx;
+    }
+};
+
 template <class PZ, int TPB>
 __global__ __launch_bounds__(TPB) void k_fs_cross(FsCrossArgs a) {
-    const uint64_t in_mask = (1ull << a.seg_bits) - 1ull;
-    unsigned long long nx = 0, bad = 0;
+    CrossSink sink{nullptr, a.segment, a.nseg, (1ull << a.seg_bits) - 1ull, a.seg_bits, 0ull, 0ull};
     for (uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * TPB) {
-        unsigned long long v[PZ::SLOTS];
+        sink.out = a.slots + i * PZ::SLOTS;
 #pragma unroll
-        for (int k = 0; k < PZ::SLOTS; ++k) v[k] = FS_NONE;
-        PZ::expand(a.base | a.idx[i], (uint32_t)a.ops[i], a.prm, [&](int slot, uint64_t child, uint32_t label) {
-            const uint64_t s = child >> a.seg_bits;
-            if (s == a.segment) return;
-            if (s >= a.nseg) {
-                ++bad;
-                return;
-            }
-            v[slot] = (((uint64_t)label * a.nseg + s) << 32) | (child & in_mask);
-            ++nx;
-        });
-#pragma unroll
-        for (int k = 0; k < PZ::SLOTS; ++k) a.slots[i * PZ::SLOTS + k] = v[k];
+        for (int k = 0; k < PZ::SLOTS; ++k) sink.out[k] = FS_NONE;
+        PZ::expand(a.base | a.idx[i], (uint32_t)a.ops[i], a.prm, sink);
     }
-    block_add3<TPB>(nx, bad, 0ull, &a.counters[3], &a.counters[4], nullptr);
+    block_add3<TPB>(sink.nx, sink.bad, 0ull, &a.counters[3], &a.counters[4], nullptr);
 }
 
 
