@@ -162,7 +162,7 @@ def segment_step_bench(psa, steps, ppm=200000, seed=1234):
 
 TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256], "grid_blocks": [0, 256, 1024],
                    "graph_layers": [4, 8, 16], "cross_marks": [0, 1], "tile_bits_log2": [0, 16],
-                   "skip_visited": [0]}
+                   "skip_visited": [0], "direction": [0, 1]}
 
 
 def tuned(psa, p, start, index_count=0, candidates=None):
@@ -236,9 +236,13 @@ def main():
     ap.add_argument("--no-segment-step", action="store_true")
     ap.add_argument("--no-batched-set", action="store_true", help="skip the 4x3 sliding and Hanoi-16 comparisons")
     ap.add_argument("--no-tune", action="store_true", help="use the default options instead of the autotuner's")
-    ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving runs")
+    ap.add_argument("--serving", action="store_true",
+                    help="also run the Poisson serving runs (multi-search engine; off by default while its "
+                         "first GPU run is unexplained)")
     ap.add_argument("--profile", action="store_true",
                     help="only the event-timed batched search, --steps times (for rocprofv3 runs)")
+    ap.add_argument("--profile-segment", action="store_true",
+                    help="only the 15-puzzle segment step, --steps times (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a kernel copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -250,6 +254,9 @@ def main():
     psa.init(0)
     if args.copy_calibration:
         copy_calibration(psa, args.copy_calibration << 20)
+        return
+    if args.profile_segment:
+        print(json.dumps({"profile": "segment_step", **segment_step_bench(psa, args.steps)}))
         return
     props = psa.device_props()
     log("device:", props)
@@ -325,7 +332,7 @@ def main():
         ]
 
     serving = None
-    if not args.no_serving:
+    if args.serving:
         serving = serving_bench(psa)
 
     seg_step = None

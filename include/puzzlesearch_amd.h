@@ -240,16 +240,25 @@ int ps_fs_segment_buckets(const ps_fs_segment* w, uint32_t* op, uint32_t* segmen
  * 9, 10 or 12 cells, pancake 8..12, 4-peg Hanoi up to 16 disks) are queued by
  * ps_batch_submit.  ps_batch_step admits queued jobs first come first served
  * while their HBM footprint fits the budget, advances every running job by one
- * layer with three launches shared by all of them, and retires finished jobs,
- * releasing their memory.  Not thread-safe: one caller drives a batch. */
+ * layer, and retires finished jobs, releasing their memory.  Not thread-safe:
+ * one caller drives a batch. */
 typedef struct ps_batch ps_batch;
 
 enum { PS_JOB_QUEUED = 0, PS_JOB_RUNNING = 1, PS_JOB_DONE = 2, PS_JOB_FAILED = 3 };
+
+/* PS_BATCH_STREAMS: every job runs on its own stream and one layer of every
+ * running job is enqueued before any is waited for, so their kernels overlap.
+ * PS_BATCH_FUSED: all running jobs share one persistent expand, one finalize
+ * and one bookkeeping launch per step; currently disabled (fails with
+ * PS_ERR_UNSUPPORTED). */
+enum { PS_BATCH_STREAMS = 0, PS_BATCH_FUSED = 1 };
 
 typedef struct ps_batch_options {
     int32_t max_jobs;      /* jobs running at once (0: 64) */
     int32_t segment_bits;  /* segments of jobs larger than one segment (default 22; 0: none) */
     uint64_t hbm_budget;   /* bytes running jobs may hold (0: half of the free HBM at creation) */
+    int32_t mode;          /* PS_BATCH_STREAMS (default) or PS_BATCH_FUSED */
+    int32_t reserved;
 } ps_batch_options;
 
 typedef struct ps_batch_job_info {
@@ -296,6 +305,12 @@ enum { PS_BFS_BATCHED = 0, PS_BFS_PLAIN = 1 };
  * one byte of HBM per state), PS_CROSS_ATOMIC ORs the bit into the next-layer
  * array with a device-scope atomic. */
 enum { PS_CROSS_BYTES = 0, PS_CROSS_ATOMIC = 1 };
+/* Layer direction of the three-bit search.  Top-down: every frontier state
+ * marks its children.  Bottom-up: every state not yet visited looks for a
+ * parent in the frontier among its neighbours (all moves here are
+ * reversible), which makes no scattered writes.  Auto goes bottom-up for a
+ * layer whose frontier outnumbers the states not yet visited. */
+enum { PS_DIR_TOP_DOWN = 0, PS_DIR_AUTO = 1, PS_DIR_BOTTOM_UP = 2 };
 
 typedef struct ps_bfs_options {
     int32_t bits;           /* 3: InMemoryThreeBitBFS semantics, 2: InMemoryTwoBitBFS */
@@ -313,6 +328,9 @@ typedef struct ps_bfs_options {
     int32_t skip_visited;   /* 1: a child outside the LDS window whose bit is set in the
                                current or previous layer is not marked (it cannot be new:
                                in an undirected graph children of layer L lie in L-1..L+1) */
+    int32_t direction;      /* PS_DIR_TOP_DOWN (default), PS_DIR_AUTO or PS_DIR_BOTTOM_UP;
+                               batched mode with an LDS window only */
+    int32_t reserved;
 } ps_bfs_options;
 
 typedef struct ps_bfs_stats {
@@ -327,6 +345,8 @@ typedef struct ps_bfs_stats {
     double expand_ms;             /* summed event time of the expand kernels */
     double finalize_ms;
     double total_ms;              /* event time of the whole run */
+    uint64_t pull_words;          /* words scanned by bottom-up layers (old and cur read) */
+    uint64_t pull_probes;         /* frontier bits read outside the tile by bottom-up layers */
 } ps_bfs_stats;
 
 void ps_bfs_options_default(ps_bfs_options* o);

@@ -33,7 +33,9 @@ FLAG_SYMMETRY, FLAG_INV_INDEX = 1, 2
 EXPAND_FULL, EXPAND_IN_SEGMENT, EXPAND_CROSS_SEGMENT = 0, 1, 2
 BFS_BATCHED, BFS_PLAIN = 0, 1
 CROSS_BYTES, CROSS_ATOMIC = 0, 1
+DIR_TOP_DOWN, DIR_AUTO, DIR_BOTTOM_UP = 0, 1, 2
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+BATCH_STREAMS, BATCH_FUSED = 0, 1
 INVALID_INDEX = (1 << 64) - 1
 
 
@@ -67,7 +69,8 @@ class BfsOptions(ctypes.Structure):
                 ("max_layers", ctypes.c_int32), ("index_count", ctypes.c_uint64),
                 ("tile_bits_log2", ctypes.c_int32), ("block_threads", ctypes.c_int32),
                 ("grid_blocks", ctypes.c_int32), ("hash", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("graph_layers", ctypes.c_int32), ("cross_marks", ctypes.c_int32), ("skip_visited", ctypes.c_int32)]
+                ("graph_layers", ctypes.c_int32), ("cross_marks", ctypes.c_int32), ("skip_visited", ctypes.c_int32),
+                ("direction", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class BfsStats(ctypes.Structure):
@@ -75,7 +78,8 @@ class BfsStats(ctypes.Structure):
                 ("algorithmic_bytes", ctypes.c_uint64), ("expand_bytes", ctypes.c_uint64),
                 ("scan_words", ctypes.c_uint64), ("expand_launches", ctypes.c_uint64),
                 ("finalize_launches", ctypes.c_uint64), ("expand_ms", ctypes.c_double),
-                ("finalize_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+                ("finalize_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("pull_words", ctypes.c_uint64), ("pull_probes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -114,7 +118,8 @@ class FsResult(ctypes.Structure):
 
 class BatchOptions(ctypes.Structure):
     """ps_batch_options"""
-    _fields_ = [("max_jobs", _i32), ("segment_bits", _i32), ("hbm_budget", _u64)]
+    _fields_ = [("max_jobs", _i32), ("segment_bits", _i32), ("hbm_budget", _u64), ("mode", _i32),
+                ("reserved", _i32)]
 
 
 class BatchJobInfo(ctypes.Structure):
@@ -410,10 +415,10 @@ class Batch:
     """Multi-search engine (ps_batch_*): continuous batching of complete
     three-bit BFS jobs of different puzzles."""
 
-    def __init__(self, max_jobs=64, segment_bits=22, hbm_budget=0, stream=None):
+    def __init__(self, max_jobs=64, segment_bits=22, hbm_budget=0, stream=None, mode=0):
         o = BatchOptions()
         lib().ps_batch_options_default(ctypes.byref(o))
-        o.max_jobs, o.segment_bits, o.hbm_budget = max_jobs, segment_bits, hbm_budget
+        o.max_jobs, o.segment_bits, o.hbm_budget, o.mode = max_jobs, segment_bits, hbm_budget, mode
         h = _vp()
         check(lib().ps_batch_create(ctypes.byref(o), stream, ctypes.byref(h)))
         self.h = h.value

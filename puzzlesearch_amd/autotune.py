@@ -9,11 +9,12 @@ Searched options (ps_bfs_options):
   cross_marks     PS_CROSS_BYTES / PS_CROSS_ATOMIC
   skip_visited    0 / 1
   graph_layers    layers captured per hipGraph replay
+  direction       0 = top-down layers only, 1 = bottom-up when the frontier outnumbers the unvisited states
 
 The search is coordinate descent from the defaults: one option at a time over
 its candidate values, keeping the fastest, for `passes` rounds.  Every
-candidate's layer counts and frontier hashes must equal the first run's, so a
-configuration that changes the result is rejected, never picked.  Results are
+candidate's layer counts and frontier hashes must equal the first run's: a
+configuration that changes the result raises instead of being picked.  Results are
 stored as JSON keyed by device name and puzzle, e.g.
 
     {"gfx950 ...": {"pancake(12)": {"options": {...}, "ms": 17.3, "tried": 23}}}
@@ -32,6 +33,7 @@ CANDIDATES = {
     "cross_marks": [0, 1],
     "skip_visited": [0, 1],
     "graph_layers": [4, 1, 8, 16],
+    "direction": [0, 1],
 }
 DEFAULTS = {k: v[0] for k, v in CANDIDATES.items()}
 

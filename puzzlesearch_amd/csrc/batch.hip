@@ -10,6 +10,13 @@
 //   3 retires the jobs whose search ended: their layer counts and frontier
 //     hashes are copied out and their device memory is released at once,
 //     so queued jobs can be admitted in the next step.
+// Two ways of running a step:
+//   PS_BATCH_STREAMS (default): every job is a ps_bfs on its own stream; one
+//     layer of every running job is enqueued before any of them is waited for,
+//     so the jobs' kernels overlap on the device;
+//   PS_BATCH_FUSED: the three multi-search launches above.  Disabled: its first
+//     run on the device stopped producing output inside a step and no cause has
+//     been found, so selecting it fails until one is.
 // Each job's arrays, counters and layer index are its own (a ps_bfs), so a job's
 // result does not depend on what else is running.  This is the device form of
 // running InMemoryThreeBitBFS (Common/InMemoryThreeBitBFS.cpp:4-48) for many
@@ -114,7 +121,8 @@ int admit(ps_batch* b) {
         b->queue.pop_front();
         ps_bfs_options o;
         int r = bfs_opts_for(b, j, &o);
-        if (r == PS_OK) r = ps_bfs_create(&j->puzzle, &o, reinterpret_cast<ps_stream>(b->stream), &j->bfs);
+        ps_stream js = b->opt.mode == PS_BATCH_FUSED ? reinterpret_cast<ps_stream>(b->stream) : nullptr;
+        if (r == PS_OK) r = ps_bfs_create(&j->puzzle, &o, js, &j->bfs);
         if (r == PS_OK) {
             b->used += j->footprint;
             r = ps_bfs_reset(j->bfs, j->start);
@@ -196,10 +204,37 @@ int retire(ps_batch* b, Job* j) {
     return PS_OK;
 }
 
+// one layer of every running job, each on its own stream, then the done flags
+int step_streams(ps_batch* b) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (Job* j : b->running) {
+        if (int r = launch_batched_layer(j->bfs, false)) return r;
+        PS_HIP_CHECK(hipMemcpyAsync(j->bfs->h_st, j->bfs->st, sizeof(BfsDevState), hipMemcpyDeviceToHost,
+                                    j->bfs->stream));
+    }
+    std::vector<Job*> still;
+    for (Job* j : b->running) {
+        PS_HIP_CHECK(hipStreamSynchronize(j->bfs->stream));
+        j->steps++;
+        if (j->bfs->h_st->done) {
+            if (int r = retire(b, j)) return r;
+            b->finished_last.push_back(j->id);
+        } else {
+            still.push_back(j);
+        }
+    }
+    b->running.swap(still);
+    b->stats.steps++;
+    b->stats.step_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return PS_OK;
+}
+
 int step(ps_batch* b, int32_t* running, int32_t* finished, int32_t* queued) {
     b->finished_last.clear();
     if (int r = admit(b)) return r;
-    if (!b->running.empty()) {
+    if (!b->running.empty() && b->opt.mode != PS_BATCH_FUSED) {
+        if (int r = step_streams(b)) return r;
+    } else if (!b->running.empty()) {
         const auto t0 = std::chrono::steady_clock::now();
         if (b->table_dirty)
             if (int r = build_table(b)) return r;
@@ -282,6 +317,10 @@ int ps_batch_create(const ps_batch_options* opt, ps_stream s, ps_batch** out) {
     if (o.max_jobs <= 0) o.max_jobs = 64;
     if (o.max_jobs > 4096) return fail(PS_ERR_INVALID_ARGUMENT, "max_jobs must be at most 4096");
     if (o.segment_bits < 0 || o.segment_bits > 40) return fail(PS_ERR_INVALID_ARGUMENT, "segment_bits out of range");
+    if (o.mode == PS_BATCH_FUSED)
+        return fail(PS_ERR_UNSUPPORTED, "the fused multi-search mode is disabled: its first device run stalled "
+                                        "inside a step for a reason not yet found");
+    if (o.mode != PS_BATCH_STREAMS) return fail(PS_ERR_INVALID_ARGUMENT, "unknown batch mode %d", o.mode);
     ps_batch* b = new (std::nothrow) ps_batch();
     if (!b) return fail(PS_ERR_OUT_OF_MEMORY, "host allocation failed");
     auto cleanup = [&](int rc) {

@@ -65,6 +65,7 @@ psk::ExpandArgs expand_args(ps_bfs* b) {
     a.st = b->st;
     a.persistent = 1;
     a.use_window = b->use_window;
+    a.pull_capable = b->pull_capable;
     a.xmark = b->xmark;
     a.skip_visited = b->opt.skip_visited;
     return a;
@@ -91,6 +92,7 @@ psk::FinalizeArgs finalize_args(ps_bfs* b) {
 
 psk::LayerEndArgs layer_end_args(ps_bfs* b) {
     psk::LayerEndArgs e{};
+    e.nbits = b->nbits;
     e.nseg = b->nseg;
     e.seg_count = b->seg_count;
     e.touched = b->touched;
@@ -101,26 +103,12 @@ psk::LayerEndArgs layer_end_args(ps_bfs* b) {
     return e;
 }
 
+}  // namespace ps
+
+namespace ps {
 namespace {
-
-int take_event(ps_bfs* b, hipEvent_t* out, size_t* idx) {
-    if (b->ev_used == b->ev_pool.size()) {
-        hipEvent_t e;
-        PS_HIP_CHECK(hipEventCreate(&e));
-        b->ev_pool.push_back(e);
-    }
-    *idx = b->ev_used;
-    *out = b->ev_pool[b->ev_used++];
-    return PS_OK;
-}
-
-int record(ps_bfs* b, size_t* idx) {
-    if (!b->collect_kernel_times) return PS_OK;
-    hipEvent_t e;
-    if (int r = take_event(b, &e, idx)) return r;
-    PS_HIP_CHECK(hipEventRecord(e, b->stream));
-    return PS_OK;
-}
+int record(ps_bfs* b, size_t* idx);
+}  // namespace
 
 int launch_layer_end(ps_bfs* b) {
     hipLaunchKernelGGL(psk::k_bfs_layer_end<1024>, dim3(1), dim3(1024), 0, b->stream, layer_end_args(b));
@@ -177,6 +165,28 @@ int launch_batched_layer(ps_bfs* b, bool timed) {
     b->stats.expand_launches++;
     b->stats.finalize_launches++;
     return launch_layer_end(b);
+}
+
+
+namespace {
+
+int take_event(ps_bfs* b, hipEvent_t* out, size_t* idx) {
+    if (b->ev_used == b->ev_pool.size()) {
+        hipEvent_t e;
+        PS_HIP_CHECK(hipEventCreate(&e));
+        b->ev_pool.push_back(e);
+    }
+    *idx = b->ev_used;
+    *out = b->ev_pool[b->ev_used++];
+    return PS_OK;
+}
+
+int record(ps_bfs* b, size_t* idx) {
+    if (!b->collect_kernel_times) return PS_OK;
+    hipEvent_t e;
+    if (int r = take_event(b, &e, idx)) return r;
+    PS_HIP_CHECK(hipEventRecord(e, b->stream));
+    return PS_OK;
 }
 
 // One plain layer: host reads the per-segment counts and launches one expand
@@ -348,8 +358,12 @@ int ps::collect_stats(ps_bfs* b) {
         const uint64_t fin_words_bytes = b->info.has_odd_cycles ? 40 : 32;
         b->stats.algorithmic_bytes =
             8ull * st.scan_words + 8ull * st.children + fin_words_bytes * st.fin_full_words + 8ull * st.fin_clear_words;
-        b->stats.expand_bytes = 8ull * st.scan_words + 8ull * st.children;
+        // bottom-up layers: old and cur read per word, one 8-byte read per probe outside the tile
+        b->stats.algorithmic_bytes += 16ull * st.pull_words + 8ull * st.pull_probes;
+        b->stats.expand_bytes = 8ull * st.scan_words + 8ull * st.children + 16ull * st.pull_words + 8ull * st.pull_probes;
         b->stats.scan_words = st.scan_words;
+        b->stats.pull_words = st.pull_words;
+        b->stats.pull_probes = st.pull_probes;
     } else {
         // two-bit: every layer reads and writes the whole 2-bit array, 8 bytes per child
         b->stats.algorithmic_bytes = 16ull * b->nwords * (st.layers_recorded) + 8ull * st.children;
@@ -407,6 +421,8 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
     else ps_bfs_options_default(&opt);
     if (opt.bits != 2 && opt.bits != 3) return fail(PS_ERR_INVALID_ARGUMENT, "bits must be 2 or 3");
     if (opt.mode != PS_BFS_BATCHED && opt.mode != PS_BFS_PLAIN) return fail(PS_ERR_INVALID_ARGUMENT, "bad mode");
+    if (opt.direction < PS_DIR_TOP_DOWN || opt.direction > PS_DIR_BOTTOM_UP)
+        return fail(PS_ERR_INVALID_ARGUMENT, "direction must be PS_DIR_TOP_DOWN, PS_DIR_AUTO or PS_DIR_BOTTOM_UP");
     if (opt.cross_marks != PS_CROSS_BYTES && opt.cross_marks != PS_CROSS_ATOMIC)
         return fail(PS_ERR_INVALID_ARGUMENT, "cross_marks must be PS_CROSS_BYTES or PS_CROSS_ATOMIC");
     if (opt.bits == 2 && opt.mode == PS_BFS_PLAIN) opt.mode = PS_BFS_BATCHED;
@@ -485,6 +501,7 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         if (opt.tile_bits_log2 > 0) block_bits = 1ull << std::min(opt.tile_bits_log2, 40);
         else block_bits = puzzle_locality_block(p);
         b->use_window = opt.tile_bits_log2 >= 0 ? 1 : 0;
+        b->pull_capable = (opt.direction != PS_DIR_TOP_DOWN && b->use_window && opt.mode == PS_BFS_BATCHED) ? 1 : 0;
         if (!b->use_window) block_bits = 1ull << 18;
         b->tile_words = std::max<uint64_t>(1, block_bits / 64);
         b->tpb_expand = opt.block_threads ? opt.block_threads : 512;
@@ -492,11 +509,11 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
             return cleanup(fail(PS_ERR_INVALID_ARGUMENT, "block_threads must be 256 or 512"));
         const size_t lds_limit = hip_lds_limit();
         while (b->use_window && b->tile_words > 1 &&
-               psk::bfs3_lds_bytes(b->tpb_expand, b->tile_words, 1, b->nseg) + 2048 > lds_limit)
+               psk::bfs3_lds_bytes(b->tpb_expand, b->tile_words, 1, b->nseg, b->pull_capable) + 2048 > lds_limit)
             b->tile_words /= 2;
         if (b->tile_words > b->nwords) b->tile_words = b->nwords;
         b->ntiles = (b->nwords + b->tile_words - 1) / b->tile_words;
-        b->lds_expand = psk::bfs3_lds_bytes(b->tpb_expand, b->tile_words, b->use_window, b->nseg);
+        b->lds_expand = psk::bfs3_lds_bytes(b->tpb_expand, b->tile_words, b->use_window, b->nseg, b->pull_capable);
         int occ = 0;
         LaunchCfg c{b->tpb_expand, 0, b->lds_expand, b->stream};
         if (int r = bfs3_expand(*p, psk::ExpandArgs{}, c, &occ)) return cleanup(r);
@@ -555,6 +572,9 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     BfsDevState st{};
     st.max_layers = (uint32_t)b->max_layers;
     st.layers_recorded = 1;
+    st.visited = 1;
+    st.direction = b->pull_capable ? (uint32_t)b->opt.direction : 0u;
+    st.pull = st.direction == PS_DIR_BOTTOM_UP;
     st.done = b->max_layers <= 1 ? 1 : 0;
     PS_HIP_CHECK(hipMemcpy(b->st, &st, sizeof(st), hipMemcpyHostToDevice));
     *b->h_st = st;

@@ -38,6 +38,7 @@ struct ps_bfs {
     int max_layers = ps::BFS_MAX_LAYERS;
     // expand geometry
     int use_window = 1;
+    int pull_capable = 0;  // expand LDS also holds the tile's cur words (bottom-up layers)
     uint64_t tile_words = 0, ntiles = 0;
     int tpb_expand = 512;
     uint32_t grid_expand = 0;
@@ -70,5 +71,7 @@ psk::FinalizeArgs finalize_args(ps_bfs* b);
 psk::LayerEndArgs layer_end_args(ps_bfs* b);
 // reads the device state into b->stats (stream idle)
 int collect_stats(ps_bfs* b);
+// enqueues one complete layer (expand, finalize, bookkeeping) on b->stream
+int launch_batched_layer(ps_bfs* b, bool timed);
 
 }  // namespace ps

@@ -44,6 +44,11 @@ struct BfsDevState {
     unsigned long long scan_words;       // cur words read by the expand kernels
     unsigned long long fin_full_words;   // words finalized (new/cur/old read, new/old written)
     unsigned long long fin_clear_words;  // words of old cleared without a finalize
+    uint32_t pull;       // the current layer is expanded bottom-up (see pull_tile)
+    uint32_t direction;  // 0: always top-down, 1: choose per layer, 2: always bottom-up
+    unsigned long long visited;      // states in the layers recorded so far
+    unsigned long long pull_words;   // words scanned by bottom-up layers (old and cur read)
+    unsigned long long pull_probes;  // cur bits read outside the tile by bottom-up layers
 };
 
 struct ExpandArgs {
@@ -63,6 +68,7 @@ struct ExpandArgs {
     int32_t use_window;
     uint8_t* xmark;  // byte per state for children outside the window; NULL: atomic OR into next
     int32_t skip_visited;  // skip cross-window children already in cur or old
+    int32_t pull_capable;  // LDS holds a copy of the tile's cur words (bottom-up layers possible)
 };
 
 struct FinalizeArgs {
@@ -82,6 +88,7 @@ struct FinalizeArgs {
 };
 
 struct LayerEndArgs {
+    uint64_t nbits;
     uint32_t nseg;
     unsigned long long* seg_count;
     uint32_t* touched;
@@ -198,9 +205,10 @@ __device__ __forceinline__ uint64_t fold_marks(uint8_t* p) {
 }
 
 // LDS bytes the expand kernel needs besides its static shared variables.
-inline size_t bfs3_lds_bytes(int tpb, uint64_t tile_words, int use_window, uint32_t nseg) {
+inline size_t bfs3_lds_bytes(int tpb, uint64_t tile_words, int use_window, uint32_t nseg, int pull_capable = 0) {
     size_t b = (size_t)(tpb / 64) * QCAP * 4;
     if (use_window) b += (size_t)tile_words * 8;
+    if (pull_capable) b += (size_t)tile_words * 8;
     if (nseg > 1 && nseg <= TOUCH_LDS_MAX) b += (size_t)((nseg + 31) / 32) * 4;
     return b;
 }
@@ -209,6 +217,7 @@ inline size_t bfs3_lds_bytes(int tpb, uint64_t tile_words, int use_window, uint3
 // and the segment "touched" bits of the job being expanded.
 struct ExpandLds {
     uint32_t* win;
+    uint64_t* cwin;  // cur words of the tile (bottom-up layers)
     uint32_t* queues;
     uint32_t* tl;
     uint32_t tl_words;  // 0: touched bits go straight to HBM
@@ -341,6 +350,116 @@ __device__ __forceinline__ void expand_tile(const ExpandArgs& a, uint64_t tile, 
     __syncthreads();
 }
 
+// Bottom-up expansion of one tile (direction-optimizing BFS): every unvisited
+// state of the tile (not in old or cur) looks for a parent among its own
+// children, which are its neighbours because every move here is reversible.
+// In-tile neighbours are tested in an LDS copy of the tile's cur words; a
+// neighbour outside the tile costs one read of cur, issued only while no
+// parent has been found.  The tile's new bits are written with plain stores:
+// no other tile writes them.  Requires use_window and pull_capable.
+template <class PZ, int TPB>
+__device__ __forceinline__ void pull_tile(const ExpandArgs& a, uint64_t tile, const ExpandLds& lds,
+                                          uint32_t* s_local, ExpandTally& tally, unsigned long long& probes) {
+    BfsDevState* st = a.st;
+    const uint32_t L = st->layer;
+    const uint64_t* __restrict__ cur = a.bits[L % 3];
+    uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
+    const uint64_t* __restrict__ old = a.bits[(L + 2) % 3];
+    constexpr int NW = TPB / 64;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    uint32_t* win = lds.win;
+    uint64_t* cw = lds.cwin;
+    uint32_t* q = lds.queues + wave * QCAP;
+    const bool track = a.nseg > 1;
+
+    const uint64_t w0 = tile * a.tile_words;
+    const uint64_t w1 = (w0 + a.tile_words < a.nwords) ? w0 + a.tile_words : a.nwords;
+    const uint64_t pw0 = w0 > a.word_lo ? w0 : a.word_lo;
+    const uint64_t pw1 = w1 < a.word_hi ? w1 : a.word_hi;
+    if (pw0 >= pw1) {
+        __syncthreads();
+        return;
+    }
+    if (tid == 0) *s_local = 0;
+    const uint32_t nw = (uint32_t)(w1 - w0);
+    for (uint32_t i = tid; i < nw * 2; i += TPB) win[i] = 0;
+    for (uint32_t i = tid; i < nw; i += TPB) cw[i] = cur[w0 + i];
+    __syncthreads();
+    if (tid == 0) tally.scan += pw1 - pw0;
+    const uint64_t win_lo = w0 * 64, win_hi = w1 * 64;
+    bool local_any = false;
+    auto run_batch = [&](uint32_t n) {
+        for (uint32_t j = lane; j < n; j += 64) {
+            const uint32_t off = q[j];
+            bool hit = false;
+            PZ::expand(win_lo + off, 0u, a.prm, [&](int, uint64_t nb, uint32_t) {
+                if (nb >= win_lo && nb < win_hi) {
+                    const uint64_t o = nb - win_lo;
+                    hit |= ((cw[o >> 6] >> (o & 63)) & 1ull) != 0ull;
+                } else if (!hit && nb < a.nbits) {
+                    ++probes;
+                    hit = ((cur[nb >> 6] >> (nb & 63)) & 1ull) != 0ull;
+                }
+            });
+            if (hit) {
+                lor(&win[off >> 5], 1u << (off & 31));
+                local_any = true;
+            }
+        }
+    };
+    int qn = 0;
+    for (uint64_t g = pw0 + (uint64_t)wave * 64; g < pw1; g += (uint64_t)NW * 64) {
+        const uint64_t wi = g + lane;
+        uint64_t bits = 0;
+        if (wi < pw1) {
+            bits = ~(old[wi] | cw[wi - w0]);
+            const uint64_t lo = wi * 64;
+            if (lo + 64 > a.nbits) bits &= (a.nbits > lo) ? ((1ull << (a.nbits - lo)) - 1ull) : 0ull;
+        }
+        const uint32_t c = (uint32_t)__popcll(bits);
+        const uint32_t incl = wave_incl_scan(c, lane);
+        const int total = (int)__shfl(incl, 63, 64);
+        int off = qn + (int)(incl - c);
+        int end_all = qn + total;
+        const uint32_t base = (uint32_t)((wi - w0) * 64);
+        for (;;) {
+            while (bits && off < QCAP) {
+                q[off++] = base + (uint32_t)(__ffsll((unsigned long long)bits) - 1);
+                bits &= bits - 1;
+            }
+            if (end_all < QCAP) {
+                qn = end_all;
+                break;
+            }
+            wave_sync();
+            run_batch(QCAP);
+            wave_sync();
+            off -= QCAP;
+            end_all -= QCAP;
+            qn = 0;
+            if (end_all == 0) break;
+        }
+    }
+    wave_sync();
+    run_batch((uint32_t)qn);
+    if (local_any) *s_local = 1;
+    __syncthreads();
+    // only words of the tile's own state range [pw0, pw1) can hold new bits
+    uint32_t* dst = nxt + w0 * 2;
+    for (uint32_t i = (uint32_t)(pw0 - w0) * 2 + tid; i < (uint32_t)(pw1 - w0) * 2; i += TPB) {
+        const uint32_t v = win[i];
+        if (v) dst[i] = v;
+    }
+    if (tid == 0 && track && *s_local) {
+        const uint64_t s0 = (pw0 * 64) >> a.seg_shift, s1 = (pw1 * 64 - 1) >> a.seg_shift;
+        for (uint64_t s = s0; s <= s1; ++s) {
+            if (lds.tl_words) lor(&lds.tl[s >> 5], 1u << (s & 31));
+            else gor(&a.touched[s >> 5], 1u << (s & 31));
+        }
+    }
+    __syncthreads();
+}
+
 template <class PZ, int TPB>
 __global__ __launch_bounds__(TPB) void k_bfs3_expand(ExpandArgs a) {
     extern __shared__ uint32_t smem[];
@@ -349,11 +468,15 @@ __global__ __launch_bounds__(TPB) void k_bfs3_expand(ExpandArgs a) {
     if (st->done) return;
     const int tid = threadIdx.x;
     const uint32_t win_u32 = a.use_window ? (uint32_t)(a.tile_words * 2) : 0u;
+    const uint32_t cwin_u32 = a.pull_capable ? (uint32_t)(a.tile_words * 2) : 0u;
     ExpandLds lds;
     lds.win = smem;
-    lds.queues = smem + win_u32;
-    lds.tl = smem + win_u32 + NW * QCAP;
+    lds.cwin = reinterpret_cast<uint64_t*>(smem + win_u32);
+    lds.queues = smem + win_u32 + cwin_u32;
+    lds.tl = lds.queues + NW * QCAP;
     lds.tl_words = (a.nseg > 1 && a.nseg <= TOUCH_LDS_MAX) ? (a.nseg + 31) / 32 : 0u;
+    const bool pull = a.pull_capable && st->pull;
+    unsigned long long probes = 0;
     __shared__ unsigned long long s_tile;
     __shared__ uint32_t s_local;
     for (uint32_t i = tid; i < lds.tl_words; i += TPB) lds.tl[i] = 0;
@@ -369,14 +492,16 @@ __global__ __launch_bounds__(TPB) void k_bfs3_expand(ExpandArgs a) {
         const uint64_t tile = s_tile;
         ++iter;
         if (tile >= a.tile_end) break;
-        expand_tile<PZ, TPB>(a, tile, lds, &s_local, tally);
+        if (pull) pull_tile<PZ, TPB>(a, tile, lds, &s_local, tally, probes);
+        else expand_tile<PZ, TPB>(a, tile, lds, &s_local, tally);
     }
     __syncthreads();
     for (uint32_t i = tid; i < lds.tl_words; i += TPB) {
         const uint32_t v = lds.tl[i];
         if (v) gor(&a.touched[i], v);
     }
-    block_add3<TPB>(tally.children, tally.parents, tally.scan, &st->children, &st->expanded, &st->scan_words);
+    if (pull) block_add3<TPB>(tally.scan, probes, 0ull, &st->pull_words, &st->pull_probes, nullptr);
+    else block_add3<TPB>(tally.children, tally.parents, tally.scan, &st->children, &st->expanded, &st->scan_words);
 }
 
 // Finalize of one chunk of one search: new &= ~cur [odd]; new &= ~old; per-segment
@@ -410,7 +535,7 @@ __device__ __forceinline__ void finalize_chunk(const FinalizeArgs& a, uint64_t c
     if (touched) {
         for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) {
             uint64_t v = nw[j];
-            if (a.xmark) v |= fold_marks(a.xmark + j * 64);
+            if (a.xmark && !st->pull) v |= fold_marks(a.xmark + j * 64);
             v &= ~old[j];
             if (a.odd) v &= ~cur[j];
             nw[j] = v;
@@ -495,6 +620,11 @@ __device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned long long cnt = st->acc_count;
+        st->visited += cnt;
+        // the next layer goes bottom-up when its frontier outnumbers the states
+        // not visited yet (which bound the bottom-up work)
+        const unsigned long long unvisited = a.nbits > st->visited ? a.nbits - st->visited : 0ull;
+        st->pull = st->direction == 2 || (st->direction == 1 && cnt > unvisited);
         if (cnt == 0) {
             st->done = 1;
         } else {
@@ -674,6 +804,7 @@ __global__ __launch_bounds__(TPB) void k_multi_expand(MultiArgs m, uint32_t win_
     constexpr int NW = TPB / 64;
     ExpandLds lds;
     lds.win = smem;
+    lds.cwin = nullptr;  // multi-search layers run top-down
     lds.queues = smem + win_u32_max;
     lds.tl = smem + win_u32_max + NW * QCAP;
     __shared__ unsigned long long s_tile;

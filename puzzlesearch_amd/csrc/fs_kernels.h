@@ -68,6 +68,7 @@ struct FsCrossArgs {
     int seg_bits;
     unsigned long long* slots;     // n * SLOTS: (label * nseg + seg) << 32 | in-segment id, or FS_NONE
     unsigned long long* counters;  // [3] cross children, [4] children outside [0, nseg)
+    unsigned long long* block_count;  // per block of the cross pass: its cross children
 };
 
 // low bit of every nonzero nibble
@@ -130,7 +131,22 @@ __global__ __launch_bounds__(TPB) void k_fs_count(FsScanArgs a) {
     const uint64_t rw = (a.nwords - w0 < FS_REGION_WORDS) ? a.nwords - w0 : FS_REGION_WORDS;
     unsigned long long all = 0, out = 0;
     if (a.dirty[r]) {
-        for (uint64_t j = threadIdx.x; j < rw; j += TPB) {
+        // four words per load; a region is a multiple of 4 words unless it is the
+        // whole (smaller) segment, whose tail is read word by word
+        const uint64_t rw4 = rw & ~3ull;
+        const uint4* f4 = reinterpret_cast<const uint4*>(a.flags_c + w0);
+        for (uint64_t j4 = threadIdx.x; j4 * 4 < rw4; j4 += TPB) {
+            const uint4 v = f4[j4];
+            const uint32_t ws[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (!ws[e]) continue;
+                const uint32_t live = fs_live(ws[e], a.odd, w0 + j4 * 4 + e);
+                all += __popc(live);
+                out += __popc(live & ~nib_full(ws[e], a.ops));
+            }
+        }
+        for (uint64_t j = rw4 + threadIdx.x; j < rw; j += TPB) {
             const uint32_t w = a.flags_c[w0 + j];
             if (!w) continue;
             const uint32_t live = fs_live(w, a.odd, w0 + j);
@@ -171,12 +187,19 @@ __global__ __launch_bounds__(TPB) void k_fs_write(FsScanArgs a) {
     __shared__ uint32_t wave_tot[TPB / 64];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     uint64_t running = a.out_off[r];
-    for (uint64_t it = 0; it * TPB < rw; ++it) {
-        const uint64_t j = it * TPB + threadIdx.x;
-        const uint32_t w = (j < rw) ? a.flags[w0 + j] : 0u;
-        uint32_t keep = 0;
-        if (w) keep = fs_live(w, a.odd, w0 + j) & ~nib_full(w, a.ops);
-        const uint32_t c = __popc(keep);
+    // thread t of iteration `it` owns words it*4*TPB + 4t .. +3: (iteration, thread,
+    // word) order is index order, so one block scan per iteration keeps the output sorted
+    for (uint64_t it = 0; it * 4 * TPB < rw; ++it) {
+        const uint64_t j0 = it * 4 * TPB + (uint64_t)threadIdx.x * 4;
+        uint32_t ws[4] = {0u, 0u, 0u, 0u}, keep[4] = {0u, 0u, 0u, 0u};
+        uint32_t c = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint64_t j = j0 + e;
+            ws[e] = (j < rw) ? a.flags[w0 + j] : 0u;
+            if (ws[e]) keep[e] = fs_live(ws[e], a.odd, w0 + j) & ~nib_full(ws[e], a.ops);
+            c += __popc(keep[e]);
+        }
         const uint32_t incl = wave_incl_scan(c, lane);
         if (lane == 63) wave_tot[wave] = incl;
         __syncthreads();
@@ -187,14 +210,18 @@ __global__ __launch_bounds__(TPB) void k_fs_write(FsScanArgs a) {
             total += t;
         }
         uint64_t pos = running + before + incl - c;
-        while (keep) {
-            const int b = __ffs(keep) - 1;  // bit 4k of nibble k
-            keep &= keep - 1;
-            a.out_idx[pos] = (uint32_t)((w0 + j) * 8 + (uint64_t)(b >> 2));
-            a.out_ops[pos] = (uint8_t)((w >> b) & 15u);
-            ++pos;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            uint32_t kb = keep[e];
+            while (kb) {
+                const int b = __ffs(kb) - 1;  // bit 4k of nibble k
+                kb &= kb - 1;
+                a.out_idx[pos] = (uint32_t)((w0 + j0 + e) * 8 + (uint64_t)(b >> 2));
+                a.out_ops[pos] = (uint8_t)((ws[e] >> b) & 15u);
+                ++pos;
+            }
+            if (ws[e]) a.flags[w0 + j0 + e] = 0u;
         }
-        if (w) a.flags[w0 + j] = 0u;
         running += total;
         __syncthreads();
     }
@@ -220,16 +247,65 @@ struct CrossSink {
     }
 };
 
+constexpr uint32_t FS_CROSS_PER_BLOCK = 4096;  // parents per block of the cross pass
+
+// Block b expands parents [b*4096, (b+1)*4096) and records how many of their
+// slots hold a cross-segment child, so the compaction can place each block's
+// children in (parent, slot) order.
 template <class PZ, int TPB>
 __global__ __launch_bounds__(TPB) void k_fs_cross(FsCrossArgs a) {
     CrossSink sink{nullptr, a.segment, a.nseg, (1ull << a.seg_bits) - 1ull, a.seg_bits, 0ull, 0ull};
-    for (uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * TPB) {
+    const uint64_t p0 = (uint64_t)blockIdx.x * FS_CROSS_PER_BLOCK;
+    const uint64_t p1 = (p0 + FS_CROSS_PER_BLOCK < a.n) ? p0 + FS_CROSS_PER_BLOCK : a.n;
+    for (uint64_t i = p0 + threadIdx.x; i < p1; i += TPB) {
         sink.out = a.slots + i * PZ::SLOTS;
 #pragma unroll
         for (int k = 0; k < PZ::SLOTS; ++k) sink.out[k] = FS_NONE;
         PZ::expand(a.base | a.idx[i], (uint32_t)a.ops[i], a.prm, sink);
     }
-    block_add3<TPB>(sink.nx, sink.bad, 0ull, &a.counters[3], &a.counters[4], nullptr);
+    const unsigned long long nx = sink.nx;
+    block_add3<TPB>(nx, sink.bad, 0ull, &a.counters[3], &a.counters[4], nullptr);
+    // per-block count (block_add3 summed it into counters[3]; sum again for the block slot)
+    __shared__ unsigned long long red[TPB / 64];
+    unsigned long long v = wave_sum(nx);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < TPB / 64; ++w) t += red[w];
+        a.block_count[blockIdx.x] = t;
+    }
+}
+
+// Ordered compaction of the cross slots: block b copies the non-empty slots of
+// its parents to out[block_off[b] ...] in slot order.
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_fs_compact(const unsigned long long* __restrict__ slots, uint64_t nslots,
+                                                    uint32_t slots_per_block,
+                                                    const unsigned long long* __restrict__ block_off,
+                                                    unsigned long long* __restrict__ out) {
+    __shared__ uint32_t wave_tot[TPB / 64];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint64_t s0 = (uint64_t)blockIdx.x * slots_per_block;
+    const uint64_t s1 = (s0 + slots_per_block < nslots) ? s0 + slots_per_block : nslots;
+    uint64_t running = block_off[blockIdx.x];
+    for (uint64_t base = s0; base < s1; base += TPB) {
+        const uint64_t i = base + threadIdx.x;
+        const unsigned long long x = (i < s1) ? slots[i] : FS_NONE;
+        const uint32_t c = x != FS_NONE ? 1u : 0u;
+        const uint32_t incl = wave_incl_scan(c, lane);
+        if (lane == 63) wave_tot[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int k = 0; k < TPB / 64; ++k) {
+            const uint32_t t = wave_tot[k];
+            before += (k < wave) ? t : 0u;
+            total += t;
+        }
+        if (c) out[running + before + incl - 1] = x;
+        running += total;
+        __syncthreads();
+    }
 }
 
 

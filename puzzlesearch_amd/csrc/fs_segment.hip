@@ -132,7 +132,7 @@ struct ps_fs_segment {
     unsigned long long* counters = nullptr;
     unsigned long long* h_counters = nullptr;  // pinned
     GrowBuf out_idx, out_ops, fr_bytes, slot_buf, compact, sorted, keys, vals, uniq, runs, nruns, xs_bytes, tmp,
-        xseg_starts;
+        xseg_starts, block_cnt, block_off;
     uint32_t cus = 256;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // results of the last step (host)
@@ -150,7 +150,8 @@ int destroy_fs(ps_fs_segment* w) {
     if (!w) return PS_OK;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     for (GrowBuf* g : {&w->out_idx, &w->out_ops, &w->fr_bytes, &w->slot_buf, &w->compact, &w->sorted, &w->keys,
-                       &w->vals, &w->uniq, &w->runs, &w->nruns, &w->xs_bytes, &w->tmp, &w->xseg_starts})
+                       &w->vals, &w->uniq, &w->runs, &w->nruns, &w->xs_bytes, &w->tmp, &w->xseg_starts,
+                       &w->block_cnt, &w->block_off})
         g->release(w->stream);
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     if (w->flags) (void)hipFree(w->flags);
@@ -293,27 +294,28 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
         ca.seg_bits = w->seg_bits;
         ca.slots = w->slot_buf.as<unsigned long long>();
         ca.counters = w->counters;
-        const uint32_t grid_out = (uint32_t)std::min<uint64_t>(blocks_for(n_out, FS_TPB), (uint64_t)w->cus * 8);
-        if (int r = fs_cross(w->puzzle, ca, grid_out, s, nullptr)) return r;
-        if (int r = w->compact.ensure(nslots * 8, s)) return r;
-        if (int r = w->nruns.ensure(16, s)) return r;
-        PS_HIP_CHECK(hipcub::DeviceSelect::If(nullptr, tb, w->slot_buf.as<unsigned long long>(),
-                                              w->compact.as<unsigned long long>(), w->nruns.as<int>(), (int)nslots,
-                                              NotNone(), s));
+        const uint64_t nblk = (n_out + psk::FS_CROSS_PER_BLOCK - 1) / psk::FS_CROSS_PER_BLOCK;
+        if (int r = w->block_cnt.ensure(nblk * 8, s)) return r;
+        if (int r = w->block_off.ensure(nblk * 8, s)) return r;
+        ca.block_count = w->block_cnt.as<unsigned long long>();
+        if (int r = fs_cross(w->puzzle, ca, (uint32_t)nblk, s, nullptr)) return r;
+        PS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w->block_cnt.as<unsigned long long>(),
+                                                      w->block_off.as<unsigned long long>(), (int)nblk, s));
         if (int r = tmp_for(w, tb)) return r;
-        PS_HIP_CHECK(hipcub::DeviceSelect::If(w->tmp.p, tb, w->slot_buf.as<unsigned long long>(),
-                                              w->compact.as<unsigned long long>(), w->nruns.as<int>(), (int)nslots,
-                                              NotNone(), s));
+        PS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w->tmp.p, tb, w->block_cnt.as<unsigned long long>(),
+                                                      w->block_off.as<unsigned long long>(), (int)nblk, s));
         if (int r = read_counters(w)) return r;
         if (w->h_counters[4])
             return fail(PS_ERR_INDEX_RANGE, "%llu children fall outside the %u segments",
                         (unsigned long long)w->h_counters[4], w->nseg);
         m = w->h_counters[3];
-        int selected = 0;
-        PS_HIP_CHECK(hipMemcpy(&selected, w->nruns.p, 4, hipMemcpyDeviceToHost));
-        if ((uint64_t)selected != m)
-            return fail(PS_ERR_HIP, "compaction kept %d of %llu cross-segment children", selected,
-                        (unsigned long long)m);
+        if (int r = w->compact.ensure(std::max<uint64_t>(m, 1) * 8, s)) return r;
+        hipLaunchKernelGGL(psk::k_fs_compact<256>, dim3((uint32_t)nblk), dim3(256), 0, s,
+                           (const unsigned long long*)w->slot_buf.as<unsigned long long>(), nslots,
+                           (uint32_t)(psk::FS_CROSS_PER_BLOCK * w->slots),
+                           (const unsigned long long*)w->block_off.as<unsigned long long>(),
+                           w->compact.as<unsigned long long>());
+        PS_HIP_CHECK(hipGetLastError());
     }
     w->bk_op.clear();
     w->bk_seg.clear();

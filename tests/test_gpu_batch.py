@@ -1,5 +1,5 @@
-"""Multi-search engine (ps_batch_*, csrc/batch.hip): searches of different
-puzzles batched in the same launches.  Each job's layer counts and frontier
+"""Multi-search engine (ps_batch_*, csrc/batch.hip), per-job-stream mode:
+searches of different puzzles running side by side.  Each job's layer counts and frontier
 hashes equal the oracle's (InMemoryThreeBitBFS restated,
 Common/InMemoryThreeBitBFS.cpp:4-48) and the same search run alone on a
 ps_bfs, whatever else runs beside it; admission respects the HBM budget; the
@@ -45,6 +45,8 @@ def test_mix_equals_alone_and_oracle(ps):
         while True:
             running, finished, queued = b.step()
             steps += 1
+            if steps % 50 == 0:
+                print(f"step {steps}: running {running}, queued {queued}", flush=True)
             if running == 0 and queued == 0:
                 break
             assert steps < 10000
@@ -78,6 +80,12 @@ def test_admission_respects_budget(ps):
         for jid in ids:
             r = b.job(jid)
             assert (r["counts"], r["hashes"]) == ref
+
+
+def test_fused_mode_is_refused(ps):
+    # its first device run stalled inside a step for a reason not yet found
+    with pytest.raises(ps.PSError):
+        ps.Batch(mode=ps.BATCH_FUSED)
 
 
 def test_rejections(ps):

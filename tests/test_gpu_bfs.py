@@ -32,6 +32,9 @@ MODES = {
     "batched_tile12_256": dict(mode=psa.BFS_BATCHED, tile_bits_log2=12, block_threads=256),
     "batched_atomic": dict(mode=psa.BFS_BATCHED, cross_marks=psa.CROSS_ATOMIC),
     "plain_seg12_atomic": dict(mode=psa.BFS_PLAIN, segment_bits=12, cross_marks=psa.CROSS_ATOMIC),
+    "batched_auto_dir": dict(mode=psa.BFS_BATCHED, direction=psa.DIR_AUTO),
+    "batched_bottom_up": dict(mode=psa.BFS_BATCHED, direction=psa.DIR_BOTTOM_UP),
+    "batched_bottom_up_seg12": dict(mode=psa.BFS_BATCHED, direction=psa.DIR_BOTTOM_UP, segment_bits=12),
 }
 
 
@@ -64,7 +67,10 @@ def test_bfs3_golden(ps, golden, vid, mode):
     assert counts == r["counts"], r["source"]
     _, ohash = oracle_run(vid, golden)
     assert hashes == ohash
-    assert st.expanded == sum(r["counts"])
+    if MODES[mode].get("direction", 0) == 0:
+        assert st.expanded == sum(r["counts"])  # top-down: every state expanded once
+    else:
+        assert st.expanded <= sum(r["counts"])  # bottom-up layers expand no parents
 
 
 @pytest.mark.parametrize("vid", SMALL)
@@ -174,20 +180,37 @@ def test_two_bit_against_oracle_fixture(ps, name):
     assert [str(h) for h in hashes] == rec["hashes2"]
 
 
-@pytest.mark.parametrize("seg,cross", [(0, 0), (20, 0), (26, 0), (22, 1)])
-def test_pancake12_full(ps, seg, cross):
+@pytest.mark.parametrize("seg,cross,direction", [(0, 0, 0), (20, 0, 0), (26, 0, 0), (22, 1, 0), (22, 0, 1), (20, 0, 2)])
+def test_pancake12_full(ps, seg, cross, direction):
     """12-pancake, 12! states, three-bit, identity start: counts and frontier
     hashes equal the oracle; sum = 12!, radius = P(12) = 14; byte and atomic
     cross-window marks."""
     if "pancake_12" not in FIX:
         pytest.skip("oracle fixture pancake_12 not generated")
     rec = FIX["pancake_12"]
-    with psa.Bfs(psa.pancake(12), index_count=math.factorial(12), segment_bits=seg, cross_marks=cross) as b:
+    with psa.Bfs(psa.pancake(12), index_count=math.factorial(12), segment_bits=seg, cross_marks=cross,
+                 direction=direction) as b:
         counts, hashes = b.run(0)
     assert sum(counts) == math.factorial(12)
     assert len(counts) == 15
     assert counts == rec["counts"]
     assert [str(h) for h in hashes] == rec["hashes"]
+
+
+@pytest.mark.parametrize("inv", [False, True])
+def test_pancake13_prefix(ps, golden, inv):
+    """13-pancake in PancakeOptimized's 29-bit segmented index space
+    (13 x 2^29 indexes): the first layers equal the reference's prefix
+    (PuzzlesTests/PuzzleSearchTests.cpp:569) and the oracle's hashes."""
+    r = golden["pancake_13_prefix"]
+    n_layers = len(r["counts"])
+    op = O.pancake(13, inv_index=inv)
+    start = O.start_index(op, r["start"])
+    with psa.Bfs(psa.pancake(13, inv_index=inv), segment_bits=29, max_layers=n_layers) as b:
+        counts, hashes = b.run(start)
+    assert counts == r["counts"]
+    oc, oh = O.bfs3(op, start, max_layers=n_layers)
+    assert (counts, hashes) == (oc, oh)
 
 
 def test_pancake12_reference_sizing(ps):
@@ -207,7 +230,7 @@ def test_sliding_4x3_full(ps, golden, shape):
     SlidingPuzzleTests/TestFrontierSearch.cpp:71 (3x4 by transposition)."""
     w, h = (4, 3) if shape == "sliding_4x3" else (3, 4)
     p = psa.sliding(w, h)
-    with psa.Bfs(p, segment_bits=24) as b:
+    with psa.Bfs(p, segment_bits=24, direction=psa.DIR_AUTO) as b:
         counts, hashes = b.run(0)
     assert counts == golden["sliding_4x3"]["counts"]
     if shape in FIX:
