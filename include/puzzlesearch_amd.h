@@ -223,12 +223,14 @@ int ps_fs_segment_info(const ps_fs_segment* w, uint32_t* segments, int32_t* oper
 int ps_fs_segment_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const uint8_t* d_ops, uint64_t n,
                        const uint32_t* d_xseg_idx, const uint64_t* xseg_op_starts, ps_fs_result* res);
 /* The new frontier: device list (ids ascending, op bits) and its stream;
- * chunk_offsets (host, frontier_chunks + 1 entries) receives chunk byte offsets. */
+ * chunk_offsets (host, frontier_chunks + 1 entries) receives chunk byte offsets.
+ * A device pointer is NULL while the workspace has never held such output. */
 int ps_fs_segment_frontier(const ps_fs_segment* w, const uint32_t** d_idx, const uint8_t** d_ops,
                            const uint8_t** d_bytes, uint64_t* chunk_offsets, uint64_t cap);
 /* The cross-segment buckets in (op, segment) order: bucket b holds values[b]
  * children whose compressed chunks are d_bytes[byte_offsets[b] .. byte_offsets[b+1])
- * (host arrays of cap entries; byte_offsets needs cap + 1). */
+ * (host arrays of cap entries; byte_offsets needs cap + 1).  *d_bytes is NULL
+ * while no step of this workspace has produced a cross-segment child. */
 int ps_fs_segment_buckets(const ps_fs_segment* w, uint32_t* op, uint32_t* segment, uint64_t* values,
                           uint64_t* byte_offsets, uint64_t cap, const uint8_t** d_bytes);
 
@@ -309,7 +311,10 @@ enum { PS_CROSS_BYTES = 0, PS_CROSS_ATOMIC = 1 };
  * marks its children.  Bottom-up: every state not yet visited looks for a
  * parent in the frontier among its neighbours (all moves here are
  * reversible), which makes no scattered writes.  Auto goes bottom-up for a
- * layer whose frontier outnumbers the states not yet visited. */
+ * layer whose frontier outnumbers the states not yet visited.  Bottom-up
+ * layers need every index to be a state: ps_bfs_create refuses
+ * PS_DIR_BOTTOM_UP (PS_ERR_UNSUPPORTED) for symmetry-reduced Hanoi, the packed
+ * sliding index and more than 12 pancakes, where PS_DIR_AUTO stays top-down. */
 enum { PS_DIR_TOP_DOWN = 0, PS_DIR_AUTO = 1, PS_DIR_BOTTOM_UP = 2 };
 
 typedef struct ps_bfs_options {

@@ -296,6 +296,21 @@ int destroy_bfs(ps_bfs* b) {
     return PS_OK;
 }
 
+// A bottom-up layer tests every unvisited index for a neighbour in the
+// frontier, so it is exact only where every index of the space is a state of
+// the searched graph.  That excludes symmetry-reduced Hanoi (non-canonical
+// indexes), the packed sliding index (unused (segment, blank) codes) and the
+// pancake index of more than 12 pancakes (29-bit segments of 12! states).
+bool every_index_a_state(const ps_puzzle& p) {
+    switch (p.kind) {
+    case PS_PANCAKE: return p.size <= 12;
+    case PS_SLIDING: return true;
+    case PS_HANOI4:
+    case PS_HANOI4_OPT: return (p.flags & PS_FLAG_SYMMETRY) == 0;
+    }
+    return false;
+}
+
 struct Geometry {
     uint64_t nbits, nwords, seg_words;
     uint32_t nseg;
@@ -428,6 +443,10 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
     if (opt.bits == 2 && opt.mode == PS_BFS_PLAIN) opt.mode = PS_BFS_BATCHED;
     ps_puzzle_info info;
     if (int r = puzzle_query(p, &info)) return r;
+    if (opt.direction == PS_DIR_BOTTOM_UP && !every_index_a_state(*p))
+        return fail(PS_ERR_UNSUPPORTED,
+                    "bottom-up layers need a puzzle whose every index is a state (not symmetric Hanoi, "
+                    "packed sliding or more than 12 pancakes)");
     Geometry g;
     if (int r = geometry(p, &opt, info, &g)) return r;
     // the puzzle must have a device expander
@@ -501,7 +520,8 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         if (opt.tile_bits_log2 > 0) block_bits = 1ull << std::min(opt.tile_bits_log2, 40);
         else block_bits = puzzle_locality_block(p);
         b->use_window = opt.tile_bits_log2 >= 0 ? 1 : 0;
-        b->pull_capable = (opt.direction != PS_DIR_TOP_DOWN && b->use_window && opt.mode == PS_BFS_BATCHED) ? 1 : 0;
+        b->pull_capable = (opt.direction != PS_DIR_TOP_DOWN && b->use_window && opt.mode == PS_BFS_BATCHED &&
+                           every_index_a_state(*p)) ? 1 : 0;
         if (!b->use_window) block_bits = 1ull << 18;
         b->tile_words = std::max<uint64_t>(1, block_bits / 64);
         b->tpb_expand = opt.block_threads ? opt.block_threads : 512;

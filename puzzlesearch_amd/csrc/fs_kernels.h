@@ -67,7 +67,7 @@ struct FsCrossArgs {
     uint32_t nseg;
     int seg_bits;
     unsigned long long* slots;     // n * SLOTS: (label * nseg + seg) << 32 | in-segment id, or FS_NONE
-    unsigned long long* counters;  // [3] cross children, [4] children outside [0, nseg)
+    unsigned long long* counters;  // [3] cross children, [4] children outside [0, nseg), [7] filled slots
     unsigned long long* block_count;  // per block of the cross pass: its cross children
 };
 
@@ -250,24 +250,28 @@ struct CrossSink {
 constexpr uint32_t FS_CROSS_PER_BLOCK = 4096;  // parents per block of the cross pass
 
 // Block b expands parents [b*4096, (b+1)*4096) and records how many of their
-// slots hold a cross-segment child, so the compaction can place each block's
-// children in (parent, slot) order.
+// slots it filled, so the compaction can place each block's children in
+// (parent, slot) order.  The count is of the slots themselves, read back after
+// the expansion, so the compaction writes exactly sum(block_count) entries;
+// counters[7] receives the same total for the host to compare with the
+// children counted in counters[3].
 template <class PZ, int TPB>
 __global__ __launch_bounds__(TPB) void k_fs_cross(FsCrossArgs a) {
     CrossSink sink{nullptr, a.segment, a.nseg, (1ull << a.seg_bits) - 1ull, a.seg_bits, 0ull, 0ull};
     const uint64_t p0 = (uint64_t)blockIdx.x * FS_CROSS_PER_BLOCK;
     const uint64_t p1 = (p0 + FS_CROSS_PER_BLOCK < a.n) ? p0 + FS_CROSS_PER_BLOCK : a.n;
+    unsigned long long filled = 0;
     for (uint64_t i = p0 + threadIdx.x; i < p1; i += TPB) {
         sink.out = a.slots + i * PZ::SLOTS;
 #pragma unroll
         for (int k = 0; k < PZ::SLOTS; ++k) sink.out[k] = FS_NONE;
         PZ::expand(a.base | a.idx[i], (uint32_t)a.ops[i], a.prm, sink);
+#pragma unroll
+        for (int k = 0; k < PZ::SLOTS; ++k) filled += sink.out[k] != FS_NONE ? 1ull : 0ull;
     }
-    const unsigned long long nx = sink.nx;
-    block_add3<TPB>(nx, sink.bad, 0ull, &a.counters[3], &a.counters[4], nullptr);
-    // per-block count (block_add3 summed it into counters[3]; sum again for the block slot)
+    block_add3<TPB>(sink.nx, sink.bad, filled, &a.counters[3], &a.counters[4], &a.counters[7]);
     __shared__ unsigned long long red[TPB / 64];
-    unsigned long long v = wave_sum(nx);
+    unsigned long long v = wave_sum(filled);
     if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = v;
     __syncthreads();
     if (threadIdx.x == 0) {

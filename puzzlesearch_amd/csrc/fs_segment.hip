@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -60,18 +62,25 @@ struct GrowBuf {
         p = nullptr;
         cap = 0;
     }
+    // a buffer handed to a kernel must have been sized by ensure() first
     template <class T>
     T* as() const {
+        if (!p) {
+            std::fprintf(stderr, "puzzlesearch_amd: segment step used a buffer before sizing it\n");
+            std::abort();
+        }
+        return reinterpret_cast<T*>(p);
+    }
+    // the buffer for reporting to the caller: NULL while it was never sized (nothing in it)
+    template <class T>
+    T* ptr_or_null() const {
         return reinterpret_cast<T*>(p);
     }
 };
 
-struct NotNone {
-    __host__ __device__ bool operator()(const unsigned long long& x) const { return x != psk::FS_NONE; }
-};
-
 // [0] in-segment children [1] count [2] written [3] cross children [4] children
 // outside the segments [5] words scanned [6] frontier ids outside the segment
+// [7] cross-segment slots filled
 constexpr int NCOUNTERS = 8;
 
 uint64_t svi_worst(uint64_t m, bool bounds) {
@@ -309,7 +318,11 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
             return fail(PS_ERR_INDEX_RANGE, "%llu children fall outside the %u segments",
                         (unsigned long long)w->h_counters[4], w->nseg);
         m = w->h_counters[3];
+        if (w->h_counters[7] != m)
+            return fail(PS_ERR_HIP, "%llu cross-segment slots filled for %llu cross-segment children",
+                        (unsigned long long)w->h_counters[7], (unsigned long long)m);
         if (int r = w->compact.ensure(std::max<uint64_t>(m, 1) * 8, s)) return r;
+        if (int r = w->nruns.ensure(16, s)) return r;
         hipLaunchKernelGGL(psk::k_fs_compact<256>, dim3((uint32_t)nblk), dim3(256), 0, s,
                            (const unsigned long long*)w->slot_buf.as<unsigned long long>(), nslots,
                            (uint32_t)(psk::FS_CROSS_PER_BLOCK * w->slots),
@@ -527,9 +540,9 @@ int ps_fs_segment_frontier(const ps_fs_segment* w, const uint32_t** d_idx, const
     PS_API_BEGIN
     if (!w) return fail(PS_ERR_INVALID_ARGUMENT, "NULL segment workspace");
     if (!w->has_result) return fail(PS_ERR_INVALID_ARGUMENT, "no completed step");
-    if (d_idx) *d_idx = w->out_idx.as<uint32_t>();
-    if (d_ops) *d_ops = w->out_ops.as<uint8_t>();
-    if (d_bytes) *d_bytes = w->fr_bytes.as<uint8_t>();
+    if (d_idx) *d_idx = w->out_idx.ptr_or_null<uint32_t>();
+    if (d_ops) *d_ops = w->out_ops.ptr_or_null<uint8_t>();
+    if (d_bytes) *d_bytes = w->fr_bytes.ptr_or_null<uint8_t>();
     if (chunk_offsets) {
         if (cap < w->fr_chunk_off.size())
             return fail(PS_ERR_CAPACITY, "need %zu chunk offsets", w->fr_chunk_off.size());
@@ -552,7 +565,7 @@ int ps_fs_segment_buckets(const ps_fs_segment* w, uint32_t* op, uint32_t* segmen
         if (values) values[b] = w->bk_values[b];
     }
     if (byte_offsets) std::memcpy(byte_offsets, w->bk_byte_off.data(), (nb + 1) * 8);
-    if (d_bytes) *d_bytes = w->xs_bytes.as<uint8_t>();
+    if (d_bytes) *d_bytes = w->xs_bytes.ptr_or_null<uint8_t>();
     return PS_OK;
     PS_API_END
 }
