@@ -45,6 +45,7 @@ extern "C" {
 #define PS_ERR_CAPACITY 6
 #define PS_ERR_INDEX_RANGE 7
 #define PS_ERR_CANCELLED 8
+#define PS_ERR_INTERNAL 9 /* a device-side consistency check failed */
 
 typedef void* ps_stream;
 
@@ -305,8 +306,12 @@ enum { PS_BFS_BATCHED = 0, PS_BFS_PLAIN = 1 };
  * window: PS_CROSS_BYTES stores 1 into a byte-per-state array that the
  * finalize pass folds into the next layer (no read-modify-write traffic; costs
  * one byte of HBM per state), PS_CROSS_ATOMIC ORs the bit into the next-layer
- * array with a device-scope atomic. */
-enum { PS_CROSS_BYTES = 0, PS_CROSS_ATOMIC = 1 };
+ * array with a device-scope atomic.  PS_CROSS_LISTS (10..12 pancakes, batched
+ * or plain mode, top-down layers, the default 9!-rank tiles, index_count >=
+ * n!) appends the child to a list owned by the parent's tile and the child's
+ * tile; a per-tile pass ORs every list into the next layer.  Each of the n!/9!
+ * tiles owns (n-9)*9! four-byte entries: 5.75 GB for 12 pancakes. */
+enum { PS_CROSS_BYTES = 0, PS_CROSS_ATOMIC = 1, PS_CROSS_LISTS = 2 };
 /* Layer direction of the three-bit search.  Top-down: every frontier state
  * marks its children.  Bottom-up: every state not yet visited looks for a
  * parent in the frontier among its neighbours (all moves here are
@@ -329,7 +334,7 @@ typedef struct ps_bfs_options {
     int32_t hash;           /* 1: per-layer frontier hashes */
     int32_t graph;          /* 1: capture the batched layer loop in a hipGraph (default on) */
     int32_t graph_layers;   /* layers per captured graph (0: default 4) */
-    int32_t cross_marks;    /* PS_CROSS_BYTES (default) or PS_CROSS_ATOMIC, see below */
+    int32_t cross_marks;    /* PS_CROSS_BYTES (default), PS_CROSS_ATOMIC or PS_CROSS_LISTS, see above */
     int32_t skip_visited;   /* 1: a child outside the LDS window whose bit is set in the
                                current or previous layer is not marked (it cannot be new:
                                in an undirected graph children of layer L lie in L-1..L+1) */
@@ -352,6 +357,9 @@ typedef struct ps_bfs_stats {
     double total_ms;              /* event time of the whole run */
     uint64_t pull_words;          /* words scanned by bottom-up layers (old and cur read) */
     uint64_t pull_probes;         /* frontier bits read outside the tile by bottom-up layers */
+    double apply_ms;              /* summed event time of the cross-list apply kernels */
+    uint64_t apply_launches;
+    uint64_t list_entries;        /* cross-tile list entries written (PS_CROSS_LISTS), all layers */
 } ps_bfs_stats;
 
 void ps_bfs_options_default(ps_bfs_options* o);

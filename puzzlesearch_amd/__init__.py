@@ -26,13 +26,14 @@ PS_ERR_UNSUPPORTED = 5
 PS_ERR_CAPACITY = 6
 PS_ERR_INDEX_RANGE = 7
 PS_ERR_CANCELLED = 8
+PS_ERR_INTERNAL = 9
 
 # puzzle kinds / flags / expand modes
 PANCAKE, SLIDING, SLIDING_PACKED, HANOI4, HANOI4_OPT = 1, 2, 3, 4, 5
 FLAG_SYMMETRY, FLAG_INV_INDEX = 1, 2
 EXPAND_FULL, EXPAND_IN_SEGMENT, EXPAND_CROSS_SEGMENT = 0, 1, 2
 BFS_BATCHED, BFS_PLAIN = 0, 1
-CROSS_BYTES, CROSS_ATOMIC = 0, 1
+CROSS_BYTES, CROSS_ATOMIC, CROSS_LISTS = 0, 1, 2
 DIR_TOP_DOWN, DIR_AUTO, DIR_BOTTOM_UP = 0, 1, 2
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 BATCH_STREAMS, BATCH_FUSED = 0, 1
@@ -79,7 +80,9 @@ class BfsStats(ctypes.Structure):
                 ("scan_words", ctypes.c_uint64), ("expand_launches", ctypes.c_uint64),
                 ("finalize_launches", ctypes.c_uint64), ("expand_ms", ctypes.c_double),
                 ("finalize_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("pull_words", ctypes.c_uint64), ("pull_probes", ctypes.c_uint64)]
+                ("pull_words", ctypes.c_uint64), ("pull_probes", ctypes.c_uint64),
+                ("apply_ms", ctypes.c_double), ("apply_launches", ctypes.c_uint64),
+                ("list_entries", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -90,6 +90,29 @@ psk::FinalizeArgs finalize_args(ps_bfs* b) {
     return f;
 }
 
+psk::ListArgs list_args(ps_bfs* b) {
+    psk::ListArgs x{};
+    x.lists = b->xlists;
+    x.counts = b->xcounts;
+    x.flags = b->xflags;
+    x.tiles = b->xtiles;
+    return x;
+}
+
+psk::ApplyArgs apply_args(ps_bfs* b) {
+    psk::ApplyArgs a{};
+    a.x = list_args(b);
+    for (int i = 0; i < 3; ++i) a.bits[i] = b->arr[i];
+    a.seg_shift = b->seg_shift;
+    a.nseg = b->nseg;
+    a.touched = b->touched;
+    a.st = b->st;
+    a.persistent = 1;
+    a.tile_begin = 0;
+    a.tile_end = b->xtiles;
+    return a;
+}
+
 psk::LayerEndArgs layer_end_args(ps_bfs* b) {
     psk::LayerEndArgs e{};
     e.nbits = b->nbits;
@@ -146,20 +169,37 @@ int launch_batched_layer(ps_bfs* b, bool timed) {
         }
         return launch_layer_end(b);
     }
-    size_t e0 = 0, e1 = 0, e2 = 0;
+    size_t e0 = 0, e1 = 0, e2 = 0, ea = 0;
     if (timed)
         if (int r = record(b, &e0)) return r;
     LaunchCfg ce{b->tpb_expand, b->grid_expand, b->lds_expand, b->stream};
-    if (int r = bfs3_expand(b->puzzle, expand_args(b), ce, nullptr)) return r;
+    if (b->lists) {
+        psk::ExpandArgs a = expand_args(b);
+        a.tile_end = std::min<uint64_t>(a.tile_end, b->xtiles);
+        if (int r = pkl_expand(b->puzzle, a, list_args(b), ce, nullptr)) return r;
+    } else {
+        if (int r = bfs3_expand(b->puzzle, expand_args(b), ce, nullptr)) return r;
+    }
     if (timed)
         if (int r = record(b, &e1)) return r;
+    size_t efin = e1;  // start of the finalize span
+    if (b->lists) {
+        LaunchCfg ca{512, b->grid_apply, b->lds_apply, b->stream};
+        if (int r = pkl_apply(b->puzzle, apply_args(b), ca, nullptr)) return r;
+        b->stats.apply_launches++;
+        if (timed) {
+            if (int r = record(b, &ea)) return r;
+            if (b->collect_kernel_times) b->apply_spans.push_back({e1, ea});
+            efin = ea;
+        }
+    }
     hipLaunchKernelGGL(psk::k_bfs3_finalize<256>, dim3(b->grid_fin), dim3(256), 0, b->stream, finalize_args(b));
     PS_HIP_CHECK(hipGetLastError());
     if (timed) {
         if (int r = record(b, &e2)) return r;
         if (b->collect_kernel_times) {
             b->expand_spans.push_back({e0, e1});
-            b->fin_spans.push_back({e1, e2});
+            b->fin_spans.push_back({efin, e2});
         }
     }
     b->stats.expand_launches++;
@@ -216,11 +256,26 @@ int run_plain_layer(ps_bfs* b, bool* done) {
         a.word_hi = w1;
         a.tile_begin = w0 / b->tile_words;
         a.tile_end = (w1 + b->tile_words - 1) / b->tile_words;
+        if (b->lists) a.tile_end = std::min<uint64_t>(a.tile_end, b->xtiles);
+        if (a.tile_end <= a.tile_begin) continue;
         LaunchCfg c{b->tpb_expand, (uint32_t)(a.tile_end - a.tile_begin), b->lds_expand, b->stream};
-        if (int r = bfs3_expand(b->puzzle, a, c, nullptr)) return r;
+        if (b->lists) {
+            if (int r = pkl_expand(b->puzzle, a, list_args(b), c, nullptr)) return r;
+        } else {
+            if (int r = bfs3_expand(b->puzzle, a, c, nullptr)) return r;
+        }
         b->stats.expand_launches++;
     }
     if (int r = record(b, &e1)) return r;
+    size_t efin = e1;  // start of the finalize span
+    if (b->lists) {
+        // every list of this layer is complete: one pass over all tiles
+        LaunchCfg ca{512, b->grid_apply, b->lds_apply, b->stream};
+        if (int r = pkl_apply(b->puzzle, apply_args(b), ca, nullptr)) return r;
+        b->stats.apply_launches++;
+        if (int r = record(b, &efin)) return r;
+        if (b->collect_kernel_times) b->apply_spans.push_back({e1, efin});
+    }
     if (b->nseg > 1) {
         PS_HIP_CHECK(hipMemcpyAsync(b->h_touched, b->touched, (size_t)((b->nseg + 31) / 32) * 4,
                                     hipMemcpyDeviceToHost, b->stream));
@@ -265,6 +320,7 @@ int build_graph(ps_bfs* b) {
     // the launches above were only captured; do not count them
     b->stats.expand_launches = 0;
     b->stats.finalize_launches = 0;
+    b->stats.apply_launches = 0;
     return PS_OK;
 }
 
@@ -281,6 +337,9 @@ int destroy_bfs(ps_bfs* b) {
         if (b->arr[i] && !alias) (void)hipFree(b->arr[i]);
     }
     if (b->xmark) (void)hipFree(b->xmark);
+    if (b->xlists) (void)hipFree(b->xlists);
+    if (b->xcounts) (void)hipFree(b->xcounts);
+    if (b->xflags) (void)hipFree(b->xflags);
     if (b->seg_count) (void)hipFree(b->seg_count);
     if (b->touched) (void)hipFree(b->touched);
     if (b->layer_counts) (void)hipFree(b->layer_counts);
@@ -347,9 +406,25 @@ int geometry(const ps_puzzle* p, const ps_bfs_options* o, const ps_puzzle_info& 
     return PS_OK;
 }
 
-uint64_t footprint_bytes(const ps_bfs_options* o, const Geometry& g, int max_layers) {
+// Cross-tile lists of n pancakes: n!/9! tiles, each (n-9)*9! entries and
+// one fill count per list.
+void list_geometry(int n, uint32_t* tiles, uint32_t* slots, uint64_t* entries) {
+    const uint32_t t = (uint32_t)(factorial(n) / factorial(9));
+    const uint32_t sl[4] = {0u, 9u, 81u, 585u};
+    *tiles = t;
+    *slots = sl[n - 9];
+    *entries = (uint64_t)t * (uint64_t)(n - 9) * factorial(9);
+}
+
+uint64_t footprint_bytes(const ps_puzzle* p, const ps_bfs_options* o, const Geometry& g, int max_layers) {
     uint64_t arrays = (o->bits == 2 ? 1 : 3) * g.nwords * 8;
     if (o->bits == 3 && o->cross_marks == PS_CROSS_BYTES) arrays += g.nwords * 64;
+    if (o->bits == 3 && o->cross_marks == PS_CROSS_LISTS && p->kind == PS_PANCAKE && p->size >= 10 && p->size <= 12) {
+        uint32_t tiles, slots;
+        uint64_t entries;
+        list_geometry(p->size, &tiles, &slots, &entries);
+        arrays += entries * 4 + (uint64_t)tiles * slots * 4 + tiles;
+    }
     const uint64_t meta = 3ull * g.nseg * 8 + ((g.nseg + 31) / 32) * 4 + 2ull * max_layers * 8 + sizeof(BfsDevState);
     return arrays + meta;
 }
@@ -364,8 +439,9 @@ using namespace ps;
 int ps::collect_stats(ps_bfs* b) {
     PS_HIP_CHECK(hipMemcpy(b->h_st, b->st, sizeof(BfsDevState), hipMemcpyDeviceToHost));
     const BfsDevState& st = *b->h_st;
-    if (st.error) return fail(PS_ERR_INDEX_RANGE, "a child index exceeded the index space (%llu states)",
-                              (unsigned long long)b->nbits);
+    if (st.error & 1u) return fail(PS_ERR_INDEX_RANGE, "a child index exceeded the index space (%llu states)",
+                                   (unsigned long long)b->nbits);
+    if (st.error) return fail(PS_ERR_INTERNAL, "cross-tile list out of range (device error bits %u)", st.error);
     b->stats.layers = st.layers_recorded;
     b->stats.expanded = st.expanded;
     b->stats.children = st.children;
@@ -396,8 +472,16 @@ int ps::collect_stats(ps_bfs* b) {
         PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
         fms += ms;
     }
+    double ams = 0;
+    for (auto& sp : b->apply_spans) {
+        float ms = 0;
+        PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
+        ams += ms;
+    }
     b->stats.expand_ms = ems;
     b->stats.finalize_ms = fms;
+    b->stats.apply_ms = ams;
+    b->stats.list_entries = b->lists ? st.expanded * (uint64_t)(b->puzzle.size - 9) : 0;
     return PS_OK;
 }
 
@@ -421,7 +505,7 @@ int ps_bfs_footprint(const ps_puzzle* p, const ps_bfs_options* opt, uint64_t* by
     Geometry g;
     if (int r = geometry(p, opt, info, &g)) return r;
     const int ml = opt->max_layers > 0 ? std::min(opt->max_layers, MAX_LAYERS_CAP) : MAX_LAYERS_CAP;
-    *bytes = footprint_bytes(opt, g, ml);
+    *bytes = footprint_bytes(p, opt, g, ml);
     return PS_OK;
     PS_API_END
 }
@@ -438,8 +522,8 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
     if (opt.mode != PS_BFS_BATCHED && opt.mode != PS_BFS_PLAIN) return fail(PS_ERR_INVALID_ARGUMENT, "bad mode");
     if (opt.direction < PS_DIR_TOP_DOWN || opt.direction > PS_DIR_BOTTOM_UP)
         return fail(PS_ERR_INVALID_ARGUMENT, "direction must be PS_DIR_TOP_DOWN, PS_DIR_AUTO or PS_DIR_BOTTOM_UP");
-    if (opt.cross_marks != PS_CROSS_BYTES && opt.cross_marks != PS_CROSS_ATOMIC)
-        return fail(PS_ERR_INVALID_ARGUMENT, "cross_marks must be PS_CROSS_BYTES or PS_CROSS_ATOMIC");
+    if (opt.cross_marks != PS_CROSS_BYTES && opt.cross_marks != PS_CROSS_ATOMIC && opt.cross_marks != PS_CROSS_LISTS)
+        return fail(PS_ERR_INVALID_ARGUMENT, "cross_marks must be PS_CROSS_BYTES, PS_CROSS_ATOMIC or PS_CROSS_LISTS");
     if (opt.bits == 2 && opt.mode == PS_BFS_PLAIN) opt.mode = PS_BFS_BATCHED;
     ps_puzzle_info info;
     if (int r = puzzle_query(p, &info)) return r;
@@ -449,6 +533,16 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
                     "packed sliding or more than 12 pancakes)");
     Geometry g;
     if (int r = geometry(p, &opt, info, &g)) return r;
+    if (opt.cross_marks == PS_CROSS_LISTS) {
+        if (opt.bits != 3 || p->kind != PS_PANCAKE || p->size < 10 || p->size > 12)
+            return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS covers the three-bit search of 10..12 pancakes");
+        if (g.nbits < factorial(p->size))
+            return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS needs index_count >= %d!", p->size);
+        if (opt.direction != PS_DIR_TOP_DOWN)
+            return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS runs top-down layers only");
+        if (opt.tile_bits_log2 != 0)
+            return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS uses its own tiles of 9! states (tile_bits_log2 = 0)");
+    }
     // the puzzle must have a device expander
     if (int r = dispatch<FAM_ALL>(*p, PS_EXPAND_FULL, [](auto) { return PS_OK; })) return r;
 
@@ -490,6 +584,14 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
     if (b->bits == 2) b->arr[1] = b->arr[2] = b->arr[0];
     if (b->bits == 3 && opt.cross_marks == PS_CROSS_BYTES)
         if (int r = dalloc((void**)&b->xmark, b->nwords * 64)) return cleanup(r);
+    if (b->bits == 3 && opt.cross_marks == PS_CROSS_LISTS) {
+        uint64_t entries;
+        list_geometry(p->size, &b->xtiles, &b->xslots, &entries);
+        b->lists = 1;
+        if (int r = dalloc((void**)&b->xlists, entries * 4)) return cleanup(r);
+        if (int r = dalloc((void**)&b->xcounts, (uint64_t)b->xtiles * b->xslots * 4)) return cleanup(r);
+        if (int r = dalloc((void**)&b->xflags, b->xtiles)) return cleanup(r);
+    }
     if (int r = dalloc((void**)&b->seg_count, (uint64_t)3 * b->nseg * 8)) return cleanup(r);
     if (int r = dalloc((void**)&b->touched, (uint64_t)((b->nseg + 31) / 32) * 4)) return cleanup(r);
     if (int r = dalloc((void**)&b->layer_counts, (uint64_t)b->max_layers * 8)) return cleanup(r);
@@ -521,8 +623,9 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         else block_bits = puzzle_locality_block(p);
         b->use_window = opt.tile_bits_log2 >= 0 ? 1 : 0;
         b->pull_capable = (opt.direction != PS_DIR_TOP_DOWN && b->use_window && opt.mode == PS_BFS_BATCHED &&
-                           every_index_a_state(*p)) ? 1 : 0;
+                           every_index_a_state(*p) && !b->lists) ? 1 : 0;
         if (!b->use_window) block_bits = 1ull << 18;
+        if (b->lists) block_bits = factorial(9);
         b->tile_words = std::max<uint64_t>(1, block_bits / 64);
         b->tpb_expand = opt.block_threads ? opt.block_threads : 512;
         if (b->tpb_expand != 256 && b->tpb_expand != 512)
@@ -535,11 +638,26 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         b->ntiles = (b->nwords + b->tile_words - 1) / b->tile_words;
         b->lds_expand = psk::bfs3_lds_bytes(b->tpb_expand, b->tile_words, b->use_window, b->nseg, b->pull_capable);
         int occ = 0;
-        LaunchCfg c{b->tpb_expand, 0, b->lds_expand, b->stream};
-        if (int r = bfs3_expand(*p, psk::ExpandArgs{}, c, &occ)) return cleanup(r);
+        if (b->lists) {
+            b->tile_words = factorial(9) / 64;
+            b->ntiles = (b->nwords + b->tile_words - 1) / b->tile_words;
+            b->lds_expand = pkl_expand_lds_bytes(*p, b->tpb_expand, b->nseg);
+            LaunchCfg c{b->tpb_expand, 0, b->lds_expand, b->stream};
+            if (int r = pkl_expand(*p, psk::ExpandArgs{}, psk::ListArgs{}, c, &occ)) return cleanup(r);
+            b->lds_apply = pkl_apply_lds_bytes(*p);
+            int occ_a = 0;
+            LaunchCfg ca{512, 0, b->lds_apply, b->stream};
+            if (int r = pkl_apply(*p, psk::ApplyArgs{}, ca, &occ_a)) return cleanup(r);
+            if (occ_a < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "list apply kernel does not fit (LDS %zu bytes)", b->lds_apply));
+            b->grid_apply = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)occ_a * cus, (int64_t)b->xtiles));
+        } else {
+            LaunchCfg c{b->tpb_expand, 0, b->lds_expand, b->stream};
+            if (int r = bfs3_expand(*p, psk::ExpandArgs{}, c, &occ)) return cleanup(r);
+        }
         if (occ < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "expand kernel does not fit (LDS %zu bytes)", b->lds_expand));
         const int64_t want = (int64_t)occ * cus;
-        b->grid_expand = (uint32_t)std::max<int64_t>(1, opt.grid_blocks > 0 ? opt.grid_blocks : std::min<int64_t>(want, (int64_t)b->ntiles));
+        const int64_t tiles_live = b->lists ? (int64_t)b->xtiles : (int64_t)b->ntiles;
+        b->grid_expand = (uint32_t)std::max<int64_t>(1, opt.grid_blocks > 0 ? opt.grid_blocks : std::min<int64_t>(want, tiles_live));
         b->chunks_per_seg = (b->seg_words + psk::FIN_CHUNK_WORDS - 1) / psk::FIN_CHUNK_WORDS;
         b->nchunks = (uint64_t)b->nseg * b->chunks_per_seg;
         b->grid_fin = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * 8, b->nchunks));
@@ -568,6 +686,10 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
         if (int r = launch_fill(b->arr[i], 0, b->nwords * 8, b->stream)) return r;
     if (b->xmark)
         if (int r = launch_fill(b->xmark, 0, b->nwords * 64, b->stream)) return r;
+    if (b->xcounts)
+        if (int r = launch_fill(b->xcounts, 0, (uint64_t)b->xtiles * b->xslots * 4, b->stream)) return r;
+    if (b->xflags)
+        if (int r = launch_fill(b->xflags, 0, b->xtiles, b->stream)) return r;
     if (int r = launch_fill(b->seg_count, 0, (uint64_t)3 * b->nseg * 8, b->stream)) return r;
     if (int r = launch_fill(b->touched, 0, (uint64_t)((b->nseg + 31) / 32) * 4, b->stream)) return r;
     PS_HIP_CHECK(hipStreamSynchronize(b->stream));
@@ -602,6 +724,7 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     b->ev_used = 0;
     b->expand_spans.clear();
     b->fin_spans.clear();
+    b->apply_spans.clear();
     b->layers_done = 1;
     b->start_index = start;
     return PS_OK;
@@ -657,6 +780,7 @@ int ps_bfs_run(ps_bfs* b, uint64_t start, uint64_t* counts, uint64_t* hashes, in
             PS_HIP_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
             b->stats.expand_launches += b->graph_layers;
             b->stats.finalize_launches += b->bits == 3 ? b->graph_layers : 0;
+            b->stats.apply_launches += b->lists ? b->graph_layers : 0;
         } else {
             if (int r = launch_batched_layer(b, true)) return r;
         }

@@ -49,6 +49,15 @@ struct ps_bfs {
     uint64_t chunks_per_seg = 0, nchunks = 0;
     // two-bit geometry
     uint32_t grid_bfs2 = 0;
+    // cross-tile child lists (PS_CROSS_LISTS, 10..12 pancakes)
+    int lists = 0;
+    uint32_t* xlists = nullptr;   // [tile][flip - 10][9!] child ranks
+    uint32_t* xcounts = nullptr;  // [tile][slot] fill counts
+    uint8_t* xflags = nullptr;    // [tile] a list reaches the tile this layer
+    uint32_t xtiles = 0;          // n!/9!
+    uint32_t xslots = 0;          // lists per tile
+    uint32_t grid_apply = 0;
+    size_t lds_apply = 0;
     // graph
     hipGraphExec_t graph_exec = nullptr;
     int graph_layers = 4;
@@ -56,7 +65,7 @@ struct ps_bfs {
     hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    std::vector<std::pair<size_t, size_t>> expand_spans, fin_spans;
+    std::vector<std::pair<size_t, size_t>> expand_spans, fin_spans, apply_spans;
     bool collect_kernel_times = true;
     ps_bfs_stats stats{};
     int layers_done = 0;
@@ -69,6 +78,8 @@ namespace ps {
 psk::ExpandArgs expand_args(ps_bfs* b);
 psk::FinalizeArgs finalize_args(ps_bfs* b);
 psk::LayerEndArgs layer_end_args(ps_bfs* b);
+psk::ListArgs list_args(ps_bfs* b);
+psk::ApplyArgs apply_args(ps_bfs* b);
 // reads the device state into b->stats (stream idle)
 int collect_stats(ps_bfs* b);
 // enqueues one complete layer (expand, finalize, bookkeeping) on b->stream

@@ -32,7 +32,8 @@ constexpr uint64_t FIN_CHUNK_WORDS = 8192;  // words per finalize work item
 struct BfsDevState {
     uint32_t layer;      // index of the current frontier layer
     uint32_t done;       // set when a layer came out empty or max_layers reached
-    uint32_t error;      // bit0: a child index >= nbits was generated
+    uint32_t error;      // bit0: a child index >= nbits was generated; bit1: a cross-tile list
+                         // slot or fill count out of range; bit2: a list entry outside its tile
     uint32_t max_layers;
     unsigned long long tile_counter;
     unsigned long long chunk_counter;
@@ -49,6 +50,7 @@ struct BfsDevState {
     unsigned long long visited;      // states in the layers recorded so far
     unsigned long long pull_words;   // words scanned by bottom-up layers (old and cur read)
     unsigned long long pull_probes;  // cur bits read outside the tile by bottom-up layers
+    unsigned long long apply_counter;  // next destination tile of the cross-list apply pass
 };
 
 struct ExpandArgs {
@@ -69,6 +71,25 @@ struct ExpandArgs {
     uint8_t* xmark;  // byte per state for children outside the window; NULL: atomic OR into next
     int32_t skip_visited;  // skip cross-window children already in cur or old
     int32_t pull_capable;  // LDS holds a copy of the tile's cur words (bottom-up layers possible)
+};
+
+// Cross-tile child lists of the 10..12-pancake search (bfs_pancake_lists.h).
+struct ListArgs {
+    uint32_t* lists;   // child ranks, [source tile][flip - 10][9!]
+    uint32_t* counts;  // list fill counts, [source tile][slot of pancake_lists.h]
+    uint8_t* flags;    // per tile: 1 when a list of this layer reaches it
+    uint32_t tiles;    // tiles holding states: N!/9!
+};
+
+struct ApplyArgs {
+    ListArgs x;
+    uint64_t* bits[3];
+    int32_t seg_shift;
+    uint32_t nseg;
+    uint32_t* touched;
+    BfsDevState* st;
+    int32_t persistent;
+    uint64_t tile_begin, tile_end;
 };
 
 struct FinalizeArgs {
@@ -511,11 +532,81 @@ struct FinTally {
     unsigned long long cnt = 0, hash = 0, full = 0, clear = 0;
 };
 
+// One word of the finalize: new &= ~old [& ~cur]; returns the new word.
+__device__ __forceinline__ uint64_t fin_word(uint64_t v, uint64_t o, uint64_t c, bool odd) {
+    v &= ~o;
+    if (odd) v &= ~c;
+    return v;
+}
+
+// Words [w0, w1) of a touched segment, two words per thread and step, the
+// loads of FIN_UNROLL steps issued before any store (the arrays are distinct
+// allocations).  XM: fold the cross-window mark bytes into new.
+constexpr int FIN_UNROLL = 4;
+
+template <int TPB, bool XM>
+__device__ __forceinline__ void finalize_words(const uint64_t* __restrict__ cur, uint64_t* __restrict__ nw,
+                                               uint64_t* __restrict__ old, uint8_t* __restrict__ xmark, uint64_t w0,
+                                               uint64_t w1, bool odd, bool hash, unsigned long long& cnt,
+                                               unsigned long long& h) {
+    constexpr int U = XM ? 2 : FIN_UNROLL;
+    const uint64_t npair = (w1 - w0) / 2;  // w0 is even: 16-byte accesses are aligned
+    for (uint64_t p0 = threadIdx.x; p0 < npair; p0 += (uint64_t)TPB * U) {
+        ulonglong2 vn[U], vo[U], vc[U];
+        uint64_t xb[U][2];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t p = p0 + (uint64_t)u * TPB;
+            if (p < npair) {
+                const uint64_t j = w0 + 2 * p;
+                vn[u] = *reinterpret_cast<const ulonglong2*>(nw + j);
+                vo[u] = *reinterpret_cast<const ulonglong2*>(old + j);
+                vc[u] = odd ? *reinterpret_cast<const ulonglong2*>(cur + j) : make_ulonglong2(0ull, 0ull);
+                if (XM) {
+                    xb[u][0] = fold_marks(xmark + j * 64);
+                    xb[u][1] = fold_marks(xmark + (j + 1) * 64);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t p = p0 + (uint64_t)u * TPB;
+            if (p < npair) {
+                const uint64_t j = w0 + 2 * p;
+                uint64_t a = vn[u].x, b = vn[u].y;
+                if (XM) {
+                    a |= xb[u][0];
+                    b |= xb[u][1];
+                }
+                a = fin_word(a, vo[u].x, vc[u].x, odd);
+                b = fin_word(b, vo[u].y, vc[u].y, odd);
+                *reinterpret_cast<ulonglong2*>(nw + j) = make_ulonglong2(a, b);
+                *reinterpret_cast<ulonglong2*>(old + j) = make_ulonglong2(0ull, 0ull);
+                cnt += (unsigned long long)(__popcll(a) + __popcll(b));
+                if (hash) h += word_hash(j, a) + word_hash(j + 1, b);
+            }
+        }
+    }
+    // an odd word count leaves one word
+    if (((w1 - w0) & 1) && threadIdx.x == 0) {
+        const uint64_t j = w1 - 1;
+        uint64_t v = nw[j];
+        if (XM) v |= fold_marks(xmark + j * 64);
+        v = fin_word(v, old[j], odd ? cur[j] : 0ull, odd);
+        nw[j] = v;
+        old[j] = 0;
+        cnt += (unsigned long long)__popcll(v);
+        if (hash) h += word_hash(j, v);
+    }
+}
+
+// Finalize of one chunk of one search: new &= ~cur [odd]; new &= ~old; per-segment
+// and layer counts; hash; old = 0.  Every thread of the block calls it.
 template <int TPB>
 __device__ __forceinline__ void finalize_chunk(const FinalizeArgs& a, uint64_t c, FinTally& tally) {
     BfsDevState* st = a.st;
     const uint32_t L = st->layer;
-    const uint64_t* __restrict__ cur = a.bits[L % 3];
+    const uint64_t* cur = a.bits[L % 3];
     uint64_t* nw = a.bits[(L + 1) % 3];
     uint64_t* old = a.bits[(L + 2) % 3];
     unsigned long long* sc_new = a.seg_count + (uint64_t)((L + 1) % 3) * a.nseg;
@@ -533,16 +624,9 @@ __device__ __forceinline__ void finalize_chunk(const FinalizeArgs& a, uint64_t c
     const bool old_live = !track || sc_old[s] != 0;
     unsigned long long cnt = 0, h = 0;
     if (touched) {
-        for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) {
-            uint64_t v = nw[j];
-            if (a.xmark && !st->pull) v |= fold_marks(a.xmark + j * 64);
-            v &= ~old[j];
-            if (a.odd) v &= ~cur[j];
-            nw[j] = v;
-            old[j] = 0;
-            cnt += (unsigned long long)__popcll(v);
-            if (a.hash) h += word_hash(j, v);
-        }
+        const bool odd = a.odd != 0, hash = a.hash != 0;
+        if (a.xmark && !st->pull) finalize_words<TPB, true>(cur, nw, old, a.xmark, w0, w1, odd, hash, cnt, h);
+        else finalize_words<TPB, false>(cur, nw, old, nullptr, w0, w1, odd, hash, cnt, h);
     } else if (old_live) {
         for (uint64_t j = w0 + threadIdx.x; j < w1; j += TPB) old[j] = 0;
     }
@@ -638,6 +722,7 @@ __device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
         st->acc_hash = 0;
         st->tile_counter = 0;
         st->chunk_counter = 0;
+        st->apply_counter = 0;
     }
 }
 

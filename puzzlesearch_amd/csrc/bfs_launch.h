@@ -25,6 +25,14 @@ PS_DECLARE_BFS_FAMILY(sliding)
 PS_DECLARE_BFS_FAMILY(packed)
 PS_DECLARE_BFS_FAMILY(hanoi)
 
+// Cross-tile list kernels of 10..12 pancakes (bfs_pancake_lists.h), defined in
+// bfs_pancake.hip.  occ != NULL: report resident blocks per CU only.
+int pkl_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::ListArgs& x, const LaunchCfg& c, int* occ);
+int pkl_apply(const ps_puzzle& p, const psk::ApplyArgs& a, const LaunchCfg& c, int* occ);
+// LDS bytes of the two kernels; 0 when the puzzle has no list kernels
+size_t pkl_expand_lds_bytes(const ps_puzzle& p, int tpb, uint32_t nseg);
+size_t pkl_apply_lds_bytes(const ps_puzzle& p);
+
 inline int bfs3_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const LaunchCfg& c, int* occ) {
     switch (family_of(p)) {
     case FAM_PANCAKE: return bfs3_expand_pancake(p, a, c, occ);

@@ -1,8 +1,71 @@
-// Instantiates the device BFS and segment-step kernels for the pancake puzzle family.
+// Instantiates the device BFS and segment-step kernels for the pancake puzzle
+// family, and the cross-tile list kernels of 10..12 pancakes.
 #include "bfs_launch.h"
+#include "bfs_pancake_lists.h"
 #include "fs_launch.h"
 
 namespace ps {
 PS_DEFINE_BFS_FAMILY(pancake, FAM_PANCAKE)
 PS_DEFINE_FS_FAMILY(pancake, FAM_PANCAKE)
+
+namespace {
+
+template <class K, class... Args>
+int launch_pkl(K kern, const LaunchCfg& c, int* occ, const Args&... args) {
+    if (occ) return launch_or_query(kern, c, occ, nullptr);
+    hipLaunchKernelGGL(kern, dim3(c.grid), dim3(c.tpb), c.lds, c.stream, args...);
+    PS_HIP_CHECK(hipGetLastError());
+    return PS_OK;
+}
+
+bool pkl_size(const ps_puzzle& p) {
+    return p.kind == PS_PANCAKE && p.size >= 10 && p.size <= 12;
+}
+
+}  // namespace
+
+int pkl_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::ListArgs& x, const LaunchCfg& c, int* occ) {
+    if (!pkl_size(p)) return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
+    switch (p.size * 1000 + c.tpb) {
+    case 10256: return launch_pkl(psk::k_pkl_expand<10, 256>, c, occ, a, x);
+    case 10512: return launch_pkl(psk::k_pkl_expand<10, 512>, c, occ, a, x);
+    case 11256: return launch_pkl(psk::k_pkl_expand<11, 256>, c, occ, a, x);
+    case 11512: return launch_pkl(psk::k_pkl_expand<11, 512>, c, occ, a, x);
+    case 12256: return launch_pkl(psk::k_pkl_expand<12, 256>, c, occ, a, x);
+    case 12512: return launch_pkl(psk::k_pkl_expand<12, 512>, c, occ, a, x);
+    }
+    return fail(PS_ERR_INVALID_ARGUMENT, "block_threads %d not in {256, 512}", c.tpb);
+}
+
+int pkl_apply(const ps_puzzle& p, const psk::ApplyArgs& a, const LaunchCfg& c, int* occ) {
+    if (!pkl_size(p)) return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
+    if (c.tpb != 512) return fail(PS_ERR_INVALID_ARGUMENT, "the list apply kernel runs 512 threads per block");
+    switch (p.size) {
+    case 10: return launch_pkl(psk::k_pkl_apply<10, 512>, c, occ, a);
+    case 11: return launch_pkl(psk::k_pkl_apply<11, 512>, c, occ, a);
+    case 12: return launch_pkl(psk::k_pkl_apply<12, 512>, c, occ, a);
+    }
+    return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
+}
+
+size_t pkl_expand_lds_bytes(const ps_puzzle& p, int tpb, uint32_t nseg) {
+    if (!pkl_size(p)) return 0;
+    switch (p.size) {
+    case 10: return psk::pkl_expand_lds<10>(tpb, nseg);
+    case 11: return psk::pkl_expand_lds<11>(tpb, nseg);
+    case 12: return psk::pkl_expand_lds<12>(tpb, nseg);
+    }
+    return 0;
+}
+
+size_t pkl_apply_lds_bytes(const ps_puzzle& p) {
+    if (!pkl_size(p)) return 0;
+    switch (p.size) {
+    case 10: return psk::pkl_apply_lds<10>();
+    case 11: return psk::pkl_apply_lds<11>();
+    case 12: return psk::pkl_apply_lds<12>();
+    }
+    return 0;
+}
+
 }  // namespace ps

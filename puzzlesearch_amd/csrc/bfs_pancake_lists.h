@@ -1,0 +1,317 @@
+// Three-bit BFS layer kernels of 10..12 pancakes with cross-tile child lists
+// (PS_CROSS_LISTS); the list geometry is in pancake_lists.h.
+//
+//   k_pkl_expand  scan the frontier of each 9!-rank tile; flips of <= 9
+//                 elements are OR-ed into the tile's LDS bitmap (stored into
+//                 next once per tile), flips of >= 10 elements are appended
+//                 to the source tile's list for the child's tile (fill count
+//                 in LDS: no global atomics, entries of a list are written
+//                 in order so L2 merges them into whole lines)
+//   k_pkl_apply   for each destination tile, gather every list that reaches
+//                 it into an LDS bitmap and OR that into next
+// The finalize and bookkeeping kernels of the three-bit search follow
+// unchanged (there is no byte-mark array to fold).
+//
+// Compared with one scattered byte store per cross child (a random HBM
+// write), a list entry costs 4 bytes written in whole lines and 4 bytes read
+// back in order.
+#pragma once
+
+#include "bfs_kernels.h"
+#include "pancake_lists.h"
+
+namespace psk {
+
+// LDS of k_pkl_expand: tile bitmap, compaction queues, list fill counts, touched bits
+template <int N>
+inline size_t pkl_expand_lds(int tpb, uint32_t nseg) {
+    size_t b = (size_t)2 * pkl::kTileWords * 4 + (size_t)(tpb / 64) * QCAP * 4 + (size_t)pkl::Geo<N>::NSLOT * 4;
+    if (nseg > 1 && nseg <= TOUCH_LDS_MAX) b += (size_t)((nseg + 31) / 32) * 4;
+    return b;
+}
+
+// LDS of k_pkl_apply: tile bitmap, per incoming list its start, count and work-item offset
+template <int N>
+inline size_t pkl_apply_lds() {
+    constexpr uint32_t NS = pkl::Geo<N>::NSLOT;
+    return (size_t)2 * pkl::kTileWords * 4 + (size_t)NS * 8 + (size_t)NS * 4 + (size_t)(NS + 1) * 4;
+}
+
+template <int N, int TPB>
+__global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
+    using G = pkl::Geo<N>;
+    constexpr int NW = TPB / 64;
+    constexpr uint32_t WIN = 2 * pkl::kTileWords;  // u32 words of a tile bitmap
+    extern __shared__ uint32_t smem[];
+    BfsDevState* st = a.st;
+    if (st->done) return;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    uint32_t* win = smem;
+    uint32_t* q = smem + WIN + wave * QCAP;
+    uint32_t* cnt = smem + WIN + NW * QCAP;
+    uint32_t* tl = cnt + G::NSLOT;
+    const uint32_t tl_words = (a.nseg > 1 && a.nseg <= TOUCH_LDS_MAX) ? (a.nseg + 31) / 32 : 0u;
+    const bool track = a.nseg > 1;
+    const uint32_t L = st->layer;
+    const uint64_t* __restrict__ cur = a.bits[L % 3];
+    uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
+    const unsigned long long* sc_cur = a.seg_count + (uint64_t)(L % 3) * a.nseg;
+    __shared__ unsigned long long s_tile;
+    __shared__ uint32_t s_local;
+    for (uint32_t i = tid; i < tl_words; i += TPB) tl[i] = 0;
+
+    unsigned long long parents = 0, scan = 0;
+    uint32_t err = 0;
+    uint64_t iter = 0;
+    for (;;) {
+        if (tid == 0) {
+            if (a.persistent) s_tile = a.tile_begin + atomicAdd(&st->tile_counter, 1ull);
+            else s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
+        }
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        ++iter;
+        if (tile >= a.tile_end) break;
+        const uint64_t w0 = tile * pkl::kTileWords, w1 = w0 + pkl::kTileWords;
+        const uint64_t pw0 = w0 > a.word_lo ? w0 : a.word_lo;
+        const uint64_t pw1 = w1 < a.word_hi ? w1 : a.word_hi;
+        bool live = pw0 < pw1;
+        if (live && track) {
+            const uint64_t s0 = (pw0 * 64) >> a.seg_shift, s1 = (pw1 * 64 - 1) >> a.seg_shift;
+            live = false;
+            for (uint64_t s = s0; s <= s1; ++s) live |= sc_cur[s] != 0;
+        }
+        if (!live) {
+            __syncthreads();
+            continue;
+        }
+        const uint32_t T = (uint32_t)tile;
+        if (tid == 0) s_local = 0;
+        for (uint32_t i = tid; i < WIN; i += TPB) win[i] = 0;
+        // fill counts start where an earlier piece of this tile (plain mode)
+        // left them; the apply pass of the previous layer cleared the rest
+        for (uint32_t i = tid; i < G::NSLOT; i += TPB) cnt[i] = x.counts[(uint64_t)T * G::NSLOT + i];
+        uint32_t dp[3];
+        pkl::tile_digits<N>(T, dp);
+        uint32_t* lists_t = x.lists + (uint64_t)T * G::CROSS * pkl::kTile;
+        __syncthreads();
+        if (tid == 0) scan += pw1 - pw0;
+
+        bool any = false;
+        auto run_batch = [&](uint32_t n) {
+            for (uint32_t j = lane; j < n; j += 64) {
+                const uint32_t off = q[j];
+                ++parents;
+                any = true;
+                pkl::children<N>(
+                    T, dp, off, [&](uint32_t c) { lor(&win[c >> 5], 1u << (c & 31)); },
+                    [&](int k, uint32_t slot, uint32_t child) {
+                        const int f = k - 10;
+                        if (slot >= pkl::kSlots[f]) {
+                            err |= 2u;
+                            return;
+                        }
+                        const uint32_t pos = __hip_atomic_fetch_add(&cnt[pkl::kSlotBase[f] + slot], 1u,
+                                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (pos >= pkl::kCap[f]) {
+                            err |= 2u;
+                            return;
+                        }
+                        lists_t[(uint32_t)f * pkl::kTile + slot * pkl::kCap[f] + pos] = child;
+                    });
+            }
+        };
+
+        int qn = 0;  // wave-uniform queue fill
+        for (uint64_t g = pw0 + (uint64_t)wave * 64; g < pw1; g += (uint64_t)NW * 64) {
+            const uint64_t wi = g + lane;
+            uint64_t bits = (wi < pw1) ? cur[wi] : 0ull;
+            const uint32_t c = (uint32_t)__popcll(bits);
+            const uint32_t incl = wave_incl_scan(c, lane);
+            const int total = (int)__shfl(incl, 63, 64);
+            int off = qn + (int)(incl - c);
+            int end_all = qn + total;
+            const uint32_t base = (uint32_t)((wi - w0) * 64);
+            for (;;) {
+                while (bits && off < QCAP) {
+                    q[off++] = base + (uint32_t)(__ffsll((unsigned long long)bits) - 1);
+                    bits &= bits - 1;
+                }
+                if (end_all < QCAP) {
+                    qn = end_all;
+                    break;
+                }
+                wave_sync();
+                run_batch(QCAP);
+                wave_sync();
+                off -= QCAP;
+                end_all -= QCAP;
+                qn = 0;
+                if (end_all == 0) break;
+            }
+        }
+        wave_sync();
+        run_batch((uint32_t)qn);
+        if (any) s_local = 1;
+        __syncthreads();
+        // a tile expanded whole in one visit (batched mode) owns its range of
+        // next until the apply pass; a piece of a tile (plain mode) ORs
+        uint32_t* dst = nxt + w0 * 2;
+        const bool own = pw0 == w0 && pw1 == w1;
+        for (uint32_t i = tid; i < WIN; i += TPB) {
+            const uint32_t v = win[i];
+            if (v) {
+                if (own) dst[i] = v;
+                else gor(&dst[i], v);
+            }
+        }
+        for (uint32_t i = tid; i < G::NSLOT; i += TPB) {
+            const uint32_t c = cnt[i];
+            x.counts[(uint64_t)T * G::NSLOT + i] = c;
+            if (c) {
+                // the apply pass visits only tiles some list reaches
+                const int k = i < pkl::kSlotBase[1] ? 10 : (i < pkl::kSlotBase[2] ? 11 : 12);
+                x.flags[pkl::dest_of<N>(T, k, i - pkl::kSlotBase[k - 10])] = 1;
+            }
+        }
+        if (tid == 0 && track && s_local) {
+            const uint64_t s0 = (w0 * 64) >> a.seg_shift, s1 = (w1 * 64 - 1) >> a.seg_shift;
+            for (uint64_t s = s0; s <= s1; ++s) {
+                if (tl_words) lor(&tl[s >> 5], 1u << (s & 31));
+                else gor(&a.touched[s >> 5], 1u << (s & 31));
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < tl_words; i += TPB) {
+        const uint32_t v = tl[i];
+        if (v) gor(&a.touched[i], v);
+    }
+    if (err) atomicOr(&st->error, err);
+    block_add3<TPB>(parents * (unsigned long long)(N - 1), parents, scan, &st->children, &st->expanded,
+                    &st->scan_words);
+}
+
+// Work items of k_pkl_apply: 1024 list entries, 16 per lane read as four
+// 16-byte vectors (list starts are 16-byte aligned and every capacity is a
+// multiple of 16, so a vector that starts below the fill count stays inside
+// its list).
+constexpr uint32_t PKL_ITEM = 1024;
+
+template <int N, int TPB>
+__global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
+    using G = pkl::Geo<N>;
+    constexpr int NW = TPB / 64;
+    constexpr uint32_t WIN = 2 * pkl::kTileWords;
+    extern __shared__ uint32_t smem[];
+    BfsDevState* st = a.st;
+    if (st->done) return;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    uint32_t* win = smem;
+    unsigned long long* lbase = reinterpret_cast<unsigned long long*>(smem + WIN);  // WIN is even: 8-byte aligned
+    uint32_t* lcnt = reinterpret_cast<uint32_t*>(lbase + G::NSLOT);
+    uint32_t* lstart = lcnt + G::NSLOT;  // NSLOT + 1 entries
+    const uint32_t L = st->layer;
+    uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
+    __shared__ unsigned long long s_tile;
+    __shared__ uint32_t s_flag;
+    uint32_t err = 0;
+    uint64_t iter = 0;
+    for (;;) {
+        if (tid == 0) {
+            if (a.persistent) s_tile = a.tile_begin + atomicAdd(&st->apply_counter, 1ull);
+            else s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
+            s_flag = s_tile < a.tile_end ? a.x.flags[s_tile] : 0u;
+        }
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        ++iter;
+        if (tile >= a.tile_end) break;
+        if (!s_flag) {
+            __syncthreads();
+            continue;
+        }
+        const uint32_t t = (uint32_t)tile;
+        if (tid == 0) a.x.flags[t] = 0;
+        for (uint32_t i = tid; i < G::NSLOT; i += TPB) {
+            int k = 10;
+            uint32_t src = 0, slot = 0;
+            pkl::incoming<N>(t, i, &k, &src, &slot);
+            const uint64_t ci = (uint64_t)src * G::NSLOT + pkl::kSlotBase[k - 10] + slot;
+            uint32_t c = a.x.counts[ci];
+            if (c) {
+                a.x.counts[ci] = 0;
+                if (c > pkl::kCap[k - 10]) {
+                    err |= 2u;
+                    c = 0;
+                }
+            }
+            lcnt[i] = c;
+            lbase[i] = pkl::list_base(src, G::CROSS, k, slot);
+        }
+        for (uint32_t i = tid; i < WIN; i += TPB) win[i] = 0;
+        __syncthreads();
+        // work items: exclusive scan over the lists (wave 0)
+        if (wave == 0) {
+            uint32_t run = 0;
+            for (uint32_t b = 0; b < G::NSLOT; b += 64) {
+                const uint32_t i = b + lane;
+                const uint32_t c = i < G::NSLOT ? (lcnt[i] + PKL_ITEM - 1) / PKL_ITEM : 0u;
+                const uint32_t incl = wave_incl_scan(c, lane);
+                if (i < G::NSLOT) lstart[i] = run + incl - c;
+                run += __shfl(incl, 63, 64);
+            }
+            if (lane == 0) lstart[G::NSLOT] = run;
+        }
+        __syncthreads();
+        const uint32_t total = lstart[G::NSLOT];
+        const uint32_t tbase = t * pkl::kTile;
+        for (uint32_t w = wave; w < total; w += NW) {
+            // the list of work item w: lstart[lo] <= w < lstart[lo + 1]
+            uint32_t lo = 0, hi = G::NSLOT;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (lstart[mid] <= w) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t e0 = (w - lstart[lo]) * PKL_ITEM;
+            const uint32_t n = lcnt[lo];
+            const uint4* src = reinterpret_cast<const uint4*>(a.x.lists + lbase[lo] + e0);
+            uint4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t e = e0 + 4u * (lane + 64u * u);
+                v[u] = (e < n) ? src[lane + 64 * u] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t e = e0 + 4u * (lane + 64u * u);
+                const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    if (e + m < n) {
+                        const uint32_t o = ev[m] - tbase;
+                        if (o < pkl::kTile) lor(&win[o >> 5], 1u << (o & 31));
+                        else err |= 4u;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t* dst = nxt + (uint64_t)t * WIN;
+        for (uint32_t i = tid; i < WIN; i += TPB) {
+            const uint32_t vv = win[i];
+            if (vv) dst[i] |= vv;
+        }
+        if (tid == 0 && a.nseg > 1) {
+            const uint64_t s0 = ((uint64_t)t * pkl::kTile) >> a.seg_shift;
+            const uint64_t s1 = ((uint64_t)(t + 1) * pkl::kTile - 1) >> a.seg_shift;
+            for (uint64_t sg = s0; sg <= s1; ++sg) gor(&a.touched[sg >> 5], 1u << (sg & 31));
+        }
+        __syncthreads();
+    }
+    if (err) atomicOr(&st->error, err);
+}
+
+}  // namespace psk

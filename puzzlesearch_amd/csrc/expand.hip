@@ -11,12 +11,6 @@ namespace ps {
 
 namespace {
 
-uint64_t factorial(int n) {
-    uint64_t r = 1;
-    for (int i = 2; i <= n; ++i) r *= (uint64_t)i;
-    return r;
-}
-
 bool packed_shape_supported(int w, int h) {
     static const int shapes[][2] = {{2, 2}, {3, 2}, {4, 2}, {5, 2}, {6, 2}, {7, 2}, {8, 2}, {3, 3}, {4, 3}, {5, 3}, {4, 4}};
     for (auto& s : shapes)

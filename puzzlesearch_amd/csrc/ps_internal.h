@@ -28,6 +28,12 @@ uint64_t puzzle_locality_block(const ps_puzzle* p);
 
 inline hipStream_t stream_of(ps_stream s) { return reinterpret_cast<hipStream_t>(s); }
 
+inline uint64_t factorial(int n) {
+    uint64_t r = 1;
+    for (int i = 2; i <= n; ++i) r *= (uint64_t)i;
+    return r;
+}
+
 int launch_fill(void* dst, uint8_t value, uint64_t bytes, hipStream_t s);
 
 }  // namespace ps

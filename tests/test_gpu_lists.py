@@ -1,0 +1,135 @@
+"""GPU parity of the cross-tile list mode of the three-bit search of 10..12
+pancakes (PS_CROSS_LISTS): per-layer counts and frontier hashes equal the
+oracle's in batched (graph and eager) and plain (per-segment) execution, with
+several segment sizes and block shapes, from the identity and from other
+start states; the list bookkeeping is left clean between runs; unsupported
+configurations are refused."""
+import json
+import math
+import os
+
+import pytest
+
+import oracle_lib as O
+import puzzlesearch_amd as psa
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "oracle_fixtures.json")) as f:
+    FIX = json.load(f)
+
+LISTS = dict(cross_marks=psa.CROSS_LISTS)
+MODES = {
+    "batched_graph": dict(mode=psa.BFS_BATCHED, graph=1),
+    "batched_eager": dict(mode=psa.BFS_BATCHED, graph=0),
+    "batched_seg20_256": dict(mode=psa.BFS_BATCHED, segment_bits=20, block_threads=256),
+    "batched_seg12": dict(mode=psa.BFS_BATCHED, segment_bits=12),
+    "plain": dict(mode=psa.BFS_PLAIN),
+    "plain_seg20": dict(mode=psa.BFS_PLAIN, segment_bits=20),
+    "plain_seg16_256": dict(mode=psa.BFS_PLAIN, segment_bits=16, block_threads=256),
+}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+@pytest.mark.parametrize("name", ["pancake_10", "pancake_11"])
+def test_lists_against_oracle_fixture(ps, name, mode):
+    rec = FIX[name]
+    n = rec["puzzle"]["size"]
+    print(f"{name} {mode}", flush=True)
+    with psa.Bfs(psa.pancake(n), index_count=rec["index_count"], **LISTS, **MODES[mode]) as b:
+        counts, hashes = b.run(rec["start"])
+        st = b.stats()
+    assert counts == rec["counts"]
+    assert [str(h) for h in hashes] == rec["hashes"]
+    assert st.expanded == math.factorial(n)
+    assert st.children == math.factorial(n) * (n - 1)
+    assert st.list_entries == math.factorial(n) * (n - 9)
+    assert st.apply_launches >= len(counts) - 1
+
+
+@pytest.mark.parametrize("start", [1, 1234567, math.factorial(10) - 1])
+def test_lists_other_starts(ps, start):
+    """Starts other than the identity: the oracle's counts and hashes."""
+    op = O.pancake(10)
+    oc, oh = O.bfs3(op, start)
+    with psa.Bfs(psa.pancake(10), **LISTS) as b:
+        assert b.run(start) == (oc, oh)
+
+
+def test_lists_graph_replays_and_reruns(ps):
+    """Graph replays, eager runs and a run cut short by max_layers leave the
+    lists clean: every later run equals the first."""
+    p = psa.pancake(11)
+    with psa.Bfs(p, graph=1, **LISTS) as g, psa.Bfs(p, graph=0, **LISTS) as e, \
+            psa.Bfs(p, max_layers=5, **LISTS) as m:
+        ref = e.run(0)
+        for _ in range(3):
+            assert g.run(0) == ref
+            cut = m.run(0)
+            assert cut == (ref[0][:5], ref[1][:5])
+            assert e.run(0) == ref
+            psa.sync()
+    assert ref[0] == FIX["pancake_11"]["counts"]
+
+
+def test_lists_step_api(ps):
+    rec = FIX["pancake_10"]
+    with psa.Bfs(psa.pancake(10), **LISTS) as b:
+        b.reset(0)
+        counts, hashes = [1], []
+        while True:
+            c, h = b.step()
+            if c == 0:
+                break
+            counts.append(c)
+            hashes.append(h)
+    assert counts == rec["counts"]
+    assert [str(h) for h in hashes] == rec["hashes"][1:]
+
+
+@pytest.mark.parametrize("seg,mode", [(0, psa.BFS_BATCHED), (22, psa.BFS_BATCHED), (20, psa.BFS_PLAIN)])
+def test_pancake12_lists_full(ps, seg, mode):
+    """12-pancake, 12! states: counts and hashes equal the oracle fixture;
+    sum = 12!, radius = P(12) = 14."""
+    rec = FIX["pancake_12"]
+    print(f"pancake_12 lists seg={seg} mode={mode}", flush=True)
+    with psa.Bfs(psa.pancake(12), index_count=math.factorial(12), segment_bits=seg, mode=mode, **LISTS) as b:
+        counts, hashes = b.run(0)
+    assert sum(counts) == math.factorial(12)
+    assert len(counts) == 15
+    assert counts == rec["counts"]
+    assert [str(h) for h in hashes] == rec["hashes"]
+
+
+def test_pancake12_lists_reference_sizing(ps):
+    """PancakeOptimized's 2^29 index space (tiles past 12! hold no states)."""
+    rec = FIX["pancake_12"]
+    with psa.Bfs(psa.pancake(12), segment_bits=24, **LISTS) as b:
+        counts, hashes = b.run(0)
+    assert counts == rec["counts"]
+    assert [str(h) for h in hashes] == rec["hashes"]
+
+
+@pytest.mark.parametrize("puzzle,opts", [
+    (psa.pancake(9), {}),
+    (psa.pancake(13), dict(segment_bits=29)),
+    (psa.sliding(4, 3), {}),
+    (psa.hanoi(10), {}),
+    (psa.pancake(10), dict(direction=psa.DIR_AUTO)),
+    (psa.pancake(10), dict(tile_bits_log2=16)),
+    (psa.pancake(10), dict(index_count=1000)),
+    (psa.pancake(10), dict(bits=2)),
+])
+def test_lists_refused(ps, puzzle, opts):
+    with pytest.raises(psa.PSError) as e:
+        psa.Bfs(puzzle, **LISTS, **opts)
+    assert e.value.code == psa.PS_ERR_UNSUPPORTED
+
+
+def test_lists_footprint(ps):
+    n = 12
+    base = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), cross_marks=psa.CROSS_ATOMIC)
+    lists = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), **LISTS)
+    tiles = math.factorial(n) // math.factorial(9)
+    assert lists - base == tiles * (n - 9) * math.factorial(9) * 4 + tiles * 585 * 4 + tiles
