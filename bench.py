@@ -3,10 +3,12 @@
 A step is one complete search of the workload (12! = 479,001,600 states,
 11 prefix-reversal moves, identity start, 15 layers), device-resident from the
 start state to the empty layer.  The headline value is expanded states per
-second of the batched persistent path (one persistent expand kernel, one
-persistent finalize kernel and one bookkeeping kernel per layer, the layer loop
-captured in a hipGraph).  The plain per-segment path (one launch per live
-segment per layer, host-driven) is timed beside it on the same input.
+second of the batched persistent path (per layer one persistent expand kernel,
+the cross-tile list apply kernel when the tuner picks cross-tile lists, one
+persistent finalize kernel and one bookkeeping kernel, the layer loop captured
+in a hipGraph), with the options the autotuner picks.  The plain per-segment
+path (one launch per live segment per layer, host-driven) is timed beside it on
+the same input, and so are the other ways of marking children outside a tile.
 
 The roofline object covers the dominant kernel (the fused scan-expand-mark
 kernel): algorithmic bytes per launch (8 B per frontier word scanned + 8 B per
@@ -15,7 +17,9 @@ the stream the kernel runs on.  cpu_baseline is the CPU oracle (a one-thread
 restatement of the reference's InMemoryThreeBitBFS step) timed on a bounded
 slice of the densest layer's frontier.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
+The Poisson serving runs (multi-search engine) report p50/p99 job latency.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-serving]
 """
 import argparse
 import json
@@ -163,9 +167,12 @@ def segment_step_bench(psa, steps, ppm=200000, seed=1234):
 TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256], "grid_blocks": [0, 256, 1024],
                    "graph_layers": [4, 8, 16], "cross_marks": [0, 1], "tile_bits_log2": [0, 16],
                    "skip_visited": [0], "direction": [0, 1]}
+# 10..12 pancakes: cross-tile lists first (bottom-up and other windows do not combine with them)
+PANCAKE_CANDIDATES = dict(TUNE_CANDIDATES, cross_marks=[2, 0, 1])
 
 
 def tuned(psa, p, start, index_count=0, candidates=None):
+    """Autotuned options (coordinate descent) for one search."""
     from puzzlesearch_amd import autotune as AT
     t0 = time.perf_counter()
     res = AT.tune(p, start, index_count=index_count, passes=1, runs=2, candidates=candidates or TUNE_CANDIDATES)
@@ -231,14 +238,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=12, help="pancake size")
     ap.add_argument("--segment-bits", type=int, default=22)
+    ap.add_argument("--cross-marks", choices=["auto", "bytes", "atomic", "lists"], default="auto",
+                    help="children outside the tile: auto = the tuner's choice (with --profile or --no-tune: "
+                         "lists for 10..12 pancakes, else bytes)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-segment-step", action="store_true")
     ap.add_argument("--no-batched-set", action="store_true", help="skip the 4x3 sliding and Hanoi-16 comparisons")
     ap.add_argument("--no-tune", action="store_true", help="use the default options instead of the autotuner's")
-    ap.add_argument("--serving", action="store_true",
-                    help="also run the Poisson serving runs (multi-search engine; off by default while its "
-                         "first GPU run is unexplained)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving runs")
     ap.add_argument("--profile", action="store_true",
                     help="only the event-timed batched search, --steps times (for rocprofv3 runs)")
     ap.add_argument("--profile-segment", action="store_true",
@@ -265,10 +273,13 @@ def main():
     p = psa.pancake(n)
     fixture = load_fixture(f"pancake_{n}")
 
-    opts = {"segment_bits": args.segment_bits}
+    cm = {"bytes": psa.CROSS_BYTES, "atomic": psa.CROSS_ATOMIC, "lists": psa.CROSS_LISTS}.get(args.cross_marks)
+    if cm is None:
+        cm = psa.CROSS_LISTS if 10 <= n <= 12 else psa.CROSS_BYTES
+    opts = {"segment_bits": args.segment_bits, "cross_marks": cm}
     tune_res = None
-    if not args.no_tune and not args.profile:
-        tune_res = tuned(psa, p, 0, index_count=N)
+    if not args.no_tune and not args.profile and args.cross_marks == "auto":
+        tune_res = tuned(psa, p, 0, index_count=N, candidates=PANCAKE_CANDIDATES if 10 <= n <= 12 else None)
         opts = dict(tune_res["options"])
     eager_opts = {k: v for k, v in opts.items() if k != "graph_layers"}
 
@@ -284,7 +295,8 @@ def main():
 
     if args.profile:
         ce, he, st_e, ms, by = eager_roofline(args.steps)
-        print(json.dumps({"profile": "k_bfs3_expand", "launches_last_run": st_e.expand_launches,
+        kname = "k_pkl_expand" if opts.get("cross_marks") == psa.CROSS_LISTS else "k_bfs3_expand"
+        print(json.dumps({"profile": kname, "launches_last_run": st_e.expand_launches,
                           "avg_launch_ms": ms, "algorithmic_bytes_per_launch": by,
                           "achieved_GBps": by / (ms * 1e-3) / 1e9, "counts": ce}))
         return
@@ -303,13 +315,19 @@ def main():
     assert counts_e == counts and hashes_e == hashes
     achieved = exp_bytes_launch / (exp_launch_ms * 1e-3) / 1e9
     log(f"expand kernel: {st_e.expand_launches} launches, avg {exp_launch_ms:.3f} ms, "
-        f"{exp_bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; finalize {st_e.finalize_ms:.2f} ms total")
+        f"{exp_bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; apply {st_e.apply_ms:.2f} ms, "
+        f"finalize {st_e.finalize_ms:.2f} ms total")
 
-    # the same batched path with device-scope atomics for children outside the LDS window
-    with psa.Bfs(p, index_count=N, graph=1, **dict(opts, cross_marks=psa.CROSS_ATOMIC)) as b:
-        dt_a, counts_a, hashes_a = timed_runs(psa, b, 0, max(1, args.steps // 2), 1)
-    assert counts_a == counts and hashes_a == hashes
-    log(f"batched, atomic cross marks: {dt_a * 1e3:.2f} ms/search")
+    # the same batched path with the other ways of marking children outside the tile
+    other_marks = {}
+    for name, mode in (("byte_cross_marks", psa.CROSS_BYTES), ("atomic_cross_marks", psa.CROSS_ATOMIC)):
+        if opts.get("cross_marks") == mode:
+            continue
+        with psa.Bfs(p, index_count=N, graph=1, **dict(opts, cross_marks=mode)) as b:
+            dt_x, counts_x, hashes_x = timed_runs(psa, b, 0, max(1, args.steps // 2), 1)
+        assert counts_x == counts and hashes_x == hashes
+        other_marks[name] = {"value": N / dt_x, "ms_per_step": dt_x * 1e3}
+        log(f"batched, {name.replace('_', ' ')}: {dt_x * 1e3:.2f} ms/search")
 
     plain = None
     if not args.no_plain:
@@ -332,7 +350,7 @@ def main():
         ]
 
     serving = None
-    if args.serving:
+    if not args.no_serving:
         serving = serving_bench(psa)
 
     seg_step = None
@@ -346,7 +364,8 @@ def main():
         log("cpu baseline:", cpu)
 
     value = N / dt_b
-    traffic = traffic_from_profile(f"k_bfs3_expand_pancake{n}_traffic.txt")
+    kernel = "k_pkl_expand" if opts.get("cross_marks") == psa.CROSS_LISTS else "k_bfs3_expand"
+    traffic = traffic_from_profile(f"{kernel}_pancake{n}_traffic.txt")
     out = {
         "metric": "expanded states/s (batched persistent three-bit BFS, 12-pancake)",
         "value": value,
@@ -368,10 +387,10 @@ def main():
                    "children_per_search": st_b.children},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_bfs3_expand", "avg_launch_ms": exp_launch_ms,
+                     "kernel": kernel, "avg_launch_ms": exp_launch_ms,
                      "algorithmic_bytes_per_launch": exp_bytes_launch},
         "whole_search_hbm_frac": st_b.algorithmic_bytes / dt_b / 1e9 / HBM_PEAK_GBS,
-        "atomic_cross_marks": {"value": N / dt_a, "ms_per_step": dt_a * 1e3},
+        **other_marks,
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "batched_set": batched_set,

@@ -63,7 +63,8 @@ def test_mix_equals_alone_and_oracle(ps):
 def test_admission_respects_budget(ps):
     specs = [("pancake", 10, 0)] * 6
     p, _, n = _puzzles(ps, *specs[0])
-    fp = ps.bfs_footprint(p, index_count=n, segment_bits=0, cross_marks=0)
+    # the engine runs 10..12 pancakes with cross-tile lists
+    fp = ps.bfs_footprint(p, index_count=n, segment_bits=0, cross_marks=ps.CROSS_LISTS)
     with ps.Batch(max_jobs=8, hbm_budget=2 * fp + fp // 2) as b:
         ids = [b.submit(p, 0, n) for _ in specs]
         saw_queue = False
