@@ -628,8 +628,9 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         if (b->lists) block_bits = factorial(9);
         b->tile_words = std::max<uint64_t>(1, block_bits / 64);
         b->tpb_expand = opt.block_threads ? opt.block_threads : 512;
-        if (b->tpb_expand != 256 && b->tpb_expand != 512)
-            return cleanup(fail(PS_ERR_INVALID_ARGUMENT, "block_threads must be 256 or 512"));
+        if (b->tpb_expand != 256 && b->tpb_expand != 512 && !(b->lists && b->tpb_expand == 1024))
+            return cleanup(fail(PS_ERR_INVALID_ARGUMENT,
+                                "block_threads must be 256 or 512 (or 1024 with PS_CROSS_LISTS)"));
         const size_t lds_limit = hip_lds_limit();
         while (b->use_window && b->tile_words > 1 &&
                psk::bfs3_lds_bytes(b->tpb_expand, b->tile_words, 1, b->nseg, b->pull_capable) + 2048 > lds_limit)

@@ -26,15 +26,18 @@ bool pkl_size(const ps_puzzle& p) {
 
 int pkl_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::ListArgs& x, const LaunchCfg& c, int* occ) {
     if (!pkl_size(p)) return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
-    switch (p.size * 1000 + c.tpb) {
+    switch (p.size * (c.tpb >= 1000 ? 10000 : 1000) + c.tpb) {
     case 10256: return launch_pkl(psk::k_pkl_expand<10, 256>, c, occ, a, x);
     case 10512: return launch_pkl(psk::k_pkl_expand<10, 512>, c, occ, a, x);
     case 11256: return launch_pkl(psk::k_pkl_expand<11, 256>, c, occ, a, x);
     case 11512: return launch_pkl(psk::k_pkl_expand<11, 512>, c, occ, a, x);
     case 12256: return launch_pkl(psk::k_pkl_expand<12, 256>, c, occ, a, x);
     case 12512: return launch_pkl(psk::k_pkl_expand<12, 512>, c, occ, a, x);
+    case 101024: return launch_pkl(psk::k_pkl_expand<10, 1024>, c, occ, a, x);
+    case 111024: return launch_pkl(psk::k_pkl_expand<11, 1024>, c, occ, a, x);
+    case 121024: return launch_pkl(psk::k_pkl_expand<12, 1024>, c, occ, a, x);
     }
-    return fail(PS_ERR_INVALID_ARGUMENT, "block_threads %d not in {256, 512}", c.tpb);
+    return fail(PS_ERR_INVALID_ARGUMENT, "block_threads %d not in {256, 512, 1024}", c.tpb);
 }
 
 int pkl_apply(const ps_puzzle& p, const psk::ApplyArgs& a, const LaunchCfg& c, int* occ) {

@@ -22,10 +22,19 @@
 
 namespace psk {
 
-// LDS of k_pkl_expand: tile bitmap, compaction queues, list fill counts, touched bits
+// LDS of k_pkl_expand: tile bitmap, child tables (pancake_lists.h), compaction
+// queues, list fill counts, touched bits
+constexpr size_t PKL_TAB_BYTES = 720 * sizeof(pkl::Tab6) + 720 * 4 + 504 * 4;
+// Per-wave parent queue.  A full queue is expanded with lane j taking entries
+// j*S .. j*S+S-1 (S = PKL_QCAP / 64): the queue holds consecutive frontier
+// states, and short flips keep a child next to its parent, so lanes S
+// parents apart OR into different LDS words instead of serializing on one.
+constexpr int PKL_QCAP = 384;
+
 template <int N>
 inline size_t pkl_expand_lds(int tpb, uint32_t nseg) {
-    size_t b = (size_t)2 * pkl::kTileWords * 4 + (size_t)(tpb / 64) * QCAP * 4 + (size_t)pkl::Geo<N>::NSLOT * 4;
+    size_t b = (size_t)2 * pkl::kTileWords * 4 + PKL_TAB_BYTES + (size_t)(tpb / 64) * PKL_QCAP * 4 +
+               (size_t)pkl::Geo<N>::NSLOT * 4;
     if (nseg > 1 && nseg <= TOUCH_LDS_MAX) b += (size_t)((nseg + 31) / 32) * 4;
     return b;
 }
@@ -47,8 +56,11 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
     if (st->done) return;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     uint32_t* win = smem;
-    uint32_t* q = smem + WIN + wave * QCAP;
-    uint32_t* cnt = smem + WIN + NW * QCAP;
+    pkl::Tab6* tab6 = reinterpret_cast<pkl::Tab6*>(smem + WIN);  // WIN * 4 bytes is a multiple of 16
+    uint32_t* tab6lo = reinterpret_cast<uint32_t*>(tab6 + 720);
+    uint32_t* tab3 = tab6lo + 720;
+    uint32_t* q = tab3 + 504 + wave * PKL_QCAP;
+    uint32_t* cnt = tab3 + 504 + NW * PKL_QCAP;
     uint32_t* tl = cnt + G::NSLOT;
     const uint32_t tl_words = (a.nseg > 1 && a.nseg <= TOUCH_LDS_MAX) ? (a.nseg + 31) / 32 : 0u;
     const bool track = a.nseg > 1;
@@ -59,6 +71,8 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
     __shared__ unsigned long long s_tile;
     __shared__ uint32_t s_local;
     for (uint32_t i = tid; i < tl_words; i += TPB) tl[i] = 0;
+    for (uint32_t l = tid; l < 720; l += TPB) tab6[l] = pkl::tab6_entry(l, &tab6lo[l]);
+    for (uint32_t h = tid; h < 504; h += TPB) tab3[h] = pkl::tab3_entry(h);
 
     unsigned long long parents = 0, scan = 0;
     uint32_t err = 0;
@@ -93,18 +107,17 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
         for (uint32_t i = tid; i < G::NSLOT; i += TPB) cnt[i] = x.counts[(uint64_t)T * G::NSLOT + i];
         uint32_t dp[3];
         pkl::tile_digits<N>(T, dp);
+        const pkl::TileK tk = pkl::tile_consts<N>(dp);
         uint32_t* lists_t = x.lists + (uint64_t)T * G::CROSS * pkl::kTile;
         __syncthreads();
         if (tid == 0) scan += pw1 - pw0;
 
         bool any = false;
-        auto run_batch = [&](uint32_t n) {
-            for (uint32_t j = lane; j < n; j += 64) {
-                const uint32_t off = q[j];
+        auto expand_one = [&](uint32_t off) {
                 ++parents;
                 any = true;
-                pkl::children<N>(
-                    T, dp, off, [&](uint32_t c) { lor(&win[c >> 5], 1u << (c & 31)); },
+                pkl::children_tab<N>(
+                    tab6, tab6lo, tab3, tk, off, [&](uint32_t c) { lor(&win[c >> 5], 1u << (c & 31)); },
                     [&](int k, uint32_t slot, uint32_t child) {
                         const int f = k - 10;
                         if (slot >= pkl::kSlots[f]) {
@@ -119,6 +132,14 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
                         }
                         lists_t[(uint32_t)f * pkl::kTile + slot * pkl::kCap[f] + pos] = child;
                     });
+        };
+        auto run_batch = [&](uint32_t n) {
+            if (n == (uint32_t)PKL_QCAP) {
+                constexpr uint32_t S = PKL_QCAP / 64;
+#pragma unroll 1
+                for (uint32_t r = 0; r < S; ++r) expand_one(q[lane * S + r]);
+            } else {
+                for (uint32_t j = lane; j < n; j += 64) expand_one(q[j]);
             }
         };
 
@@ -133,19 +154,19 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
             int end_all = qn + total;
             const uint32_t base = (uint32_t)((wi - w0) * 64);
             for (;;) {
-                while (bits && off < QCAP) {
+                while (bits && off < PKL_QCAP) {
                     q[off++] = base + (uint32_t)(__ffsll((unsigned long long)bits) - 1);
                     bits &= bits - 1;
                 }
-                if (end_all < QCAP) {
+                if (end_all < PKL_QCAP) {
                     qn = end_all;
                     break;
                 }
                 wave_sync();
-                run_batch(QCAP);
+                run_batch(PKL_QCAP);
                 wave_sync();
-                off -= QCAP;
-                end_all -= QCAP;
+                off -= PKL_QCAP;
+                end_all -= PKL_QCAP;
                 qn = 0;
                 if (end_all == 0) break;
             }

@@ -166,6 +166,245 @@ PKL_HD inline uint32_t rank_in(uint32_t mask, uint32_t e) {
 #endif
 }
 
+// ---------------------------------------------------------------------------
+// Table-driven form of children() (same visits, same order).
+//
+// Positions are numbered from the end here: slot j is position N-1-j, and the
+// per-position values (digits d, running counts cc) are kept four to a 32-bit
+// word, one byte per slot (word w holds slots 4w..4w+3).
+//
+//  * The last six positions: a 720-entry table indexed by l = off mod 6!
+//    gives their digits, the tile offsets of the flips of 2..6 elements
+//    (relative to off - l) and the counts cc after the sweep reaches position
+//    N-6 (the flips of <= 6 elements involve only these six elements).
+//  * The next three positions: a 504-entry table indexed by off / 6! gives
+//    their digits.
+//  * The sweep over positions s = N-7..0 updates all active slots at once:
+//    cc[q] += (cc[q] + d[q] >= d[s]) is a byte-wise compare done by adding
+//    0x80 - d[s] to each byte of cc + d and taking bit 7.  The rank of the
+//    reversed suffix, sum_q cc[q] (q-s)!, is a byte dot product (v_dot4) of
+//    the count word with factorial weights; weights above 255 are factored
+//    as (common factorial) x (ratio <= 255).
+
+struct alignas(16) Tab6 {
+    uint32_t d0;   // digits of slots 0..3
+    uint32_t cc0;  // counts of slots 0..3 after position N-6
+    uint32_t c234; // offsets (< 720) of the flips of 2, 3, 4 elements, 10 bits each
+    uint32_t c56;  // flips of 5 and 6 elements (10 bits each), bit 20: count of slot 4
+};
+
+// Entry l of the six-position table (pattern of the last six elements with
+// Lehmer digits e0..e5, radices 6..1); *lo receives the digits of slots 4, 5.
+PKL_HD inline Tab6 tab6_entry(uint32_t l, uint32_t* lo) {
+    uint32_t e[6], r = l;
+    for (int i = 5; i >= 0; --i) {
+        e[i] = r % (uint32_t)(6 - i);
+        r /= (uint32_t)(6 - i);
+    }
+    uint32_t a[6], avail = 0x3Fu;
+    for (int i = 0; i < 6; ++i) {
+        a[i] = select_elem(avail, e[i]);
+        avail &= ~(1u << a[i]);
+    }
+    auto rank6 = [](const uint32_t(&b)[6]) {
+        uint32_t seen = 0, rr = 0;
+        for (int i = 0; i < 6; ++i) {
+            rr = rr * (uint32_t)(6 - i) + (b[i] - rank_in(seen, b[i]) - 0u);
+            seen |= 1u << b[i];
+        }
+        return rr;
+    };
+    uint32_t c[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int k = 2; k <= 6; ++k) {
+        uint32_t b[6];
+        for (int i = 0; i < 6; ++i) b[i] = a[i];
+        for (int i = 0; i < k / 2; ++i) {
+            const uint32_t t = b[6 - k + i];
+            b[6 - k + i] = b[5 - i];
+            b[5 - i] = t;
+        }
+        c[k] = rank6(b);
+    }
+    // counts c(0, q) = #{p in [0, q) : a[p] < a[q]} of the six-pattern; slot j is pattern index 5-j
+    uint32_t cq[6];
+    for (int q = 0; q < 6; ++q) {
+        cq[q] = 0;
+        for (int p2 = 0; p2 < q; ++p2) cq[q] += a[p2] < a[q] ? 1u : 0u;
+    }
+    Tab6 t;
+    t.d0 = e[5] | (e[4] << 8) | (e[3] << 16) | (e[2] << 24);
+    t.cc0 = cq[5] | (cq[4] << 8) | (cq[3] << 16) | (cq[2] << 24);
+    t.c234 = c[2] | (c[3] << 10) | (c[4] << 20);
+    t.c56 = c[5] | (c[6] << 10) | (cq[1] << 20);
+    *lo = e[1] | (e[0] << 8);
+    return t;
+}
+
+// Entry h of the three-position table: the digits of positions N-7, N-8, N-9
+// (radices 7, 8, 9) of h = off / 6!, one per byte.
+PKL_HD inline uint32_t tab3_entry(uint32_t h) {
+    const uint32_t d7 = h % 7u, t = h / 7u;
+    return d7 | ((t % 8u) << 8) | ((t / 8u) << 16);
+}
+
+// Per-tile constants of children_tab().
+struct TileK {
+    uint32_t td2;     // digits of slots 9..11 (the tile prefix) in bytes 1..3
+    uint32_t pre[3];  // sum_{i < s} d_i (N-1-i)! for s = 0, 1, 2
+};
+
+template <int N>
+PKL_HD inline TileK tile_consts(const uint32_t (&dp)[3]) {
+    constexpr int P = N - 9;
+    TileK k;
+    k.td2 = 0;
+    for (int i = 0; i < P; ++i) k.td2 |= dp[i] << (8 * (N - 1 - i - 8));  // position i is slot N-1-i
+    k.pre[0] = 0;
+    k.pre[1] = dp[0] * kFact[N - 1];
+    k.pre[2] = k.pre[1] + (P > 1 ? dp[1] * kFact[N - 2] : 0u);
+    return k;
+}
+
+PKL_HD inline uint32_t udot4(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_udot4(a, b, c, false);
+#else
+    for (int i = 0; i < 4; ++i) c += ((a >> (8 * i)) & 255u) * ((b >> (8 * i)) & 255u);
+    return c;
+#endif
+}
+
+// byte b of w in all four bytes
+PKL_HD inline uint32_t splat_byte(uint32_t w, int b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_perm(w, w, 0x01010101u * (uint32_t)b);
+#else
+    return ((w >> (8 * b)) & 255u) * 0x01010101u;
+#endif
+}
+
+// Weight groups of one count word at sweep step m (= N-1-s active slots):
+// slot j has weight (m-j)! (0 when j >= m); a group is a byte-weight word
+// (ratios <= 255) and the common factor of its members.
+struct WGroups {
+    uint32_t pack[4];
+    uint32_t factor[4];
+    int n;
+};
+
+constexpr uint32_t cfact(int n) { return n <= 1 ? 1u : (uint32_t)n * cfact(n - 1); }
+
+constexpr WGroups weight_groups(int m, int w) {
+    WGroups g{{0, 0, 0, 0}, {0, 0, 0, 0}, 0};
+    int b = 0;
+    while (b < 4 && 4 * w + b < m) {
+        // extend the group from byte b while the ratio to its smallest weight stays <= 255
+        int e = b;
+        while (e + 1 < 4 && 4 * w + e + 1 < m &&
+               cfact(m - 4 * w - b) / cfact(m - 4 * w - (e + 1)) <= 255u)
+            ++e;
+        const uint32_t f = cfact(m - 4 * w - e);
+        uint32_t pack = 0;
+        for (int x = b; x <= e; ++x) pack |= (cfact(m - 4 * w - x) / f) << (8 * x);
+        g.pack[g.n] = pack;
+        g.factor[g.n] = f;
+        ++g.n;
+        b = e + 1;
+    }
+    return g;
+}
+
+// sum over the active slots of word w of cc[slot] * weight, added to acc
+template <int M, int W>
+PKL_HD inline uint32_t rho_word(uint32_t ccw, uint32_t acc) {
+    constexpr WGroups g = weight_groups(M, W);
+    PKL_UNROLL
+    for (int i = 0; i < g.n; ++i) {
+        if (g.factor[i] == 1u) acc = udot4(ccw, g.pack[i], acc);
+        else acc += mul_small(udot4(ccw, g.pack[i], 0u), g.factor[i]);
+    }
+    return acc;
+}
+
+// mask of the low bit of every active byte of word w (slots < m)
+constexpr uint32_t active_lsb(int m, int w) {
+    uint32_t r = 0;
+    for (int b = 0; b < 4; ++b)
+        if (4 * w + b < m) r |= 1u << (8 * b);
+    return r;
+}
+
+template <int N, int S>
+struct Sweep {
+    // one step of the sweep at position S (S = N-7 .. 0) and the steps below it
+    template <class FI, class FX>
+    PKL_HD static inline void run(uint32_t (&cc)[3], const uint32_t (&d)[3], const uint32_t (&e)[3],
+                                  const uint32_t (&base)[3], const TileK& tk, uint32_t& r1, uint32_t& r2,
+                                  uint32_t& r3, FI& in, FX& cross) {
+        constexpr int P = N - 9;
+        constexpr int M = N - 1 - S;  // active slots 0..M-1; position S is slot M
+        const uint32_t ns = 0u - splat_byte(d[M >> 2], M & 3);
+        PKL_UNROLL
+        for (int w = 0; w < 3; ++w) {
+            if (4 * w < M) {
+                const uint32_t y = cc[w] + e[w] + ns;  // byte b: cc + d + 0x80 - d[S], no borrow
+                cc[w] += (y >> 7) & active_lsb(M, w);
+            }
+        }
+        uint32_t rho = rho_word<M, 0>(cc[0], 0u);
+        rho = rho_word<M, 1>(cc[1], rho);
+        if constexpr (M > 8) rho = rho_word<M, 2>(cc[2], rho);
+        if constexpr (S == P) {
+            const uint32_t x = cc[0] + d[0];
+            r1 = x & 255u;
+            r2 = (x >> 8) & 255u;
+            r3 = (x >> 16) & 255u;
+        }
+        constexpr int K = N - S;
+        if constexpr (K <= 9) in(base[S - P] + rho);
+        else cross(K, slot_of(K, r1, r2, r3), tk.pre[S] + rho);
+        if constexpr (S > 0) Sweep<N, S - 1>::run(cc, d, e, base, tk, r1, r2, r3, in, cross);
+    }
+};
+
+// children() with the tables: tab6[l] (and tab6lo[l] = digits of slots 4, 5)
+// for l < 720, tab3[h] for h < 504.
+template <int N, class TA, class TB, class TC, class FI, class FX>
+PKL_HD inline void children_tab(const TA& tab6, const TB& tab6lo, const TC& tab3, const TileK& tk, uint32_t off,
+                                FI&& in, FX&& cross) {
+    constexpr int P = N - 9;
+    const uint32_t h = off / 720u;
+    const uint32_t l = off - h * 720u;
+    const Tab6 t = tab6[l];
+    const uint32_t lo = tab6lo[l];
+    const uint32_t hd = tab3[h];
+    uint32_t d[3], cc[3], e[3];
+    d[0] = t.d0;
+    d[1] = lo | (hd << 16);             // slots 4, 5 | slots 6, 7 (positions N-7, N-8)
+    d[2] = ((hd >> 16) & 255u) | tk.td2;  // slot 8 (position N-9) | the tile prefix
+    cc[0] = t.cc0;
+    cc[1] = (t.c56 >> 20) & 1u;
+    cc[2] = 0;
+    PKL_UNROLL
+    for (int w = 0; w < 3; ++w) e[w] = d[w] + 0x80808080u;
+    // flips of 2..6 elements: from the table
+    const uint32_t b6 = off - l;
+    in(b6 + (t.c234 & 1023u));
+    in(b6 + ((t.c234 >> 10) & 1023u));
+    in(b6 + ((t.c234 >> 20) & 1023u));
+    in(b6 + (t.c56 & 1023u));
+    in(b6 + ((t.c56 >> 10) & 1023u));
+    // in-tile base of the flips of 9, 8, 7 elements: the tile offset less the
+    // reversed suffix (index - suffix), by position s = P, P+1, P+2
+    uint32_t base[3];
+    base[0] = 0;
+    base[1] = ((hd >> 16) & 255u) * 40320u;
+    base[2] = (h - (hd & 255u)) * 720u;
+    uint32_t r1 = 0, r2 = 0, r3 = 0;
+    Sweep<N, N - 7>::run(cc, d, e, base, tk, r1, r2, r3, in, cross);
+    (void)P;
+}
+
 // Prefix elements of tile t.
 template <int N>
 PKL_HD inline void tile_prefix(uint32_t t, uint32_t (&x)[3]) {

@@ -36,11 +36,36 @@ int g_fail = 0;
         }                                   \
     } while (0)
 
+// the table-driven routine (children_tab) visits exactly what children() visits
+template <int N>
+void check_tab(uint32_t T, uint32_t step) {
+    std::vector<pkl::Tab6> tab6(720);
+    std::vector<uint32_t> tab6lo(720), tab3(504);
+    for (uint32_t l = 0; l < 720; ++l) tab6[l] = pkl::tab6_entry(l, &tab6lo[l]);
+    for (uint32_t h = 0; h < 504; ++h) tab3[h] = pkl::tab3_entry(h);
+    uint32_t dp[3];
+    pkl::tile_digits<N>(T, dp);
+    const pkl::TileK tk = pkl::tile_consts<N>(dp);
+    for (uint32_t off = 0; off < pkl::kTile; off += step) {
+        std::vector<uint64_t> a, b;
+        pkl::children<N>(T, dp, off, [&](uint32_t c) { a.push_back(c); },
+                         [&](int k, uint32_t slot, uint32_t c) { a.push_back(((uint64_t)k << 48) | ((uint64_t)slot << 32) | c); });
+        pkl::children_tab<N>(tab6, tab6lo, tab3, tk, off, [&](uint32_t c) { b.push_back(c); },
+                             [&](int k, uint32_t slot, uint32_t c) { b.push_back(((uint64_t)k << 48) | ((uint64_t)slot << 32) | c); });
+        CHECK(a == b, "tile %u off %u: table-driven children differ (%zu vs %zu visits)", T, off, a.size(), b.size());
+        if (a != b && g_fail < 5) {
+            for (size_t i = 0; i < a.size() && i < b.size(); ++i)
+                if (a[i] != b[i]) std::printf("  visit %zu: %llx vs %llx\n", i, (unsigned long long)a[i], (unsigned long long)b[i]);
+        }
+    }
+}
+
 template <int N>
 void check_tile(uint32_t T) {
     using G = pkl::Geo<N>;
     uint32_t dp[3];
     pkl::tile_digits<N>(T, dp);
+    check_tab<N>(T, 1);
     std::vector<uint32_t> fill(G::NSLOT, 0);
     for (uint32_t off = 0; off < pkl::kTile; ++off) {
         const uint64_t parent = (uint64_t)T * pkl::kTile + off;
