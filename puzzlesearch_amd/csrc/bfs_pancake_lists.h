@@ -39,11 +39,23 @@ inline size_t pkl_expand_lds(int tpb, uint32_t nseg) {
     return b;
 }
 
-// LDS of k_pkl_apply: tile bitmap, per incoming list its start, count and work-item offset
+// Work items of k_pkl_apply: 1024 list entries, 16 per lane read as four
+// 16-byte vectors (list starts are 16-byte aligned and every capacity is a
+// multiple of 16, so a vector that starts below the fill count stays inside
+// its list).  A tile has at most 3*9!/1024 + NSLOT work items.
+constexpr uint32_t PKL_ITEM = 1024;
+template <int N>
+constexpr uint32_t pkl_max_items() {
+    return (uint32_t)(pkl::Geo<N>::CROSS * pkl::kTile / PKL_ITEM) + pkl::Geo<N>::NSLOT + 1;
+}
+
+// LDS of k_pkl_apply: tile bitmap, per incoming list its start, count and
+// first work item, and the list of every work item
 template <int N>
 inline size_t pkl_apply_lds() {
     constexpr uint32_t NS = pkl::Geo<N>::NSLOT;
-    return (size_t)2 * pkl::kTileWords * 4 + (size_t)NS * 8 + (size_t)NS * 4 + (size_t)(NS + 1) * 4;
+    return (size_t)2 * pkl::kTileWords * 4 + (size_t)NS * 8 + (size_t)NS * 4 + (size_t)(NS + 1) * 4 +
+           (size_t)pkl_max_items<N>() * 2;
 }
 
 template <int N, int TPB>
@@ -214,12 +226,6 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
                     &st->scan_words);
 }
 
-// Work items of k_pkl_apply: 1024 list entries, 16 per lane read as four
-// 16-byte vectors (list starts are 16-byte aligned and every capacity is a
-// multiple of 16, so a vector that starts below the fill count stays inside
-// its list).
-constexpr uint32_t PKL_ITEM = 1024;
-
 template <int N, int TPB>
 __global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
     using G = pkl::Geo<N>;
@@ -233,6 +239,7 @@ __global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
     unsigned long long* lbase = reinterpret_cast<unsigned long long*>(smem + WIN);  // WIN is even: 8-byte aligned
     uint32_t* lcnt = reinterpret_cast<uint32_t*>(lbase + G::NSLOT);
     uint32_t* lstart = lcnt + G::NSLOT;  // NSLOT + 1 entries
+    uint16_t* item_list = reinterpret_cast<uint16_t*>(lstart + G::NSLOT + 1);
     const uint32_t L = st->layer;
     uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
     __shared__ unsigned long long s_tile;
@@ -287,34 +294,47 @@ __global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
         }
         __syncthreads();
         const uint32_t total = lstart[G::NSLOT];
+        // the list of every work item
+        for (uint32_t i = tid; i < G::NSLOT; i += TPB)
+            for (uint32_t w = lstart[i]; w < lstart[i + 1]; ++w) item_list[w] = (uint16_t)i;
+        __syncthreads();
         const uint32_t tbase = t * pkl::kTile;
         for (uint32_t w = wave; w < total; w += NW) {
-            // the list of work item w: lstart[lo] <= w < lstart[lo + 1]
-            uint32_t lo = 0, hi = G::NSLOT;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (lstart[mid] <= w) lo = mid;
-                else hi = mid;
-            }
-            const uint32_t e0 = (w - lstart[lo]) * PKL_ITEM;
-            const uint32_t n = lcnt[lo];
-            const uint4* src = reinterpret_cast<const uint4*>(a.x.lists + lbase[lo] + e0);
+            const uint32_t li = __builtin_amdgcn_readfirstlane(item_list[w]);
+            const uint32_t e0 = (w - lstart[li]) * PKL_ITEM;
+            const uint32_t n = lcnt[li];
+            const uint4* src = reinterpret_cast<const uint4*>(a.x.lists + lbase[li] + e0);
             uint4 v[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const uint32_t e = e0 + 4u * (lane + 64u * u);
                 v[u] = (e < n) ? src[lane + 64 * u] : make_uint4(0u, 0u, 0u, 0u);
             }
+            if (e0 + PKL_ITEM <= n) {
+                // a full work item, branch-free: an entry outside the tile ORs no bit
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t e = e0 + 4u * (lane + 64u * u);
-                const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    if (e + m < n) {
+                    for (int m = 0; m < 4; ++m) {
                         const uint32_t o = ev[m] - tbase;
-                        if (o < pkl::kTile) lor(&win[o >> 5], 1u << (o & 31));
-                        else err |= 4u;
+                        const bool ok = o < pkl::kTile;
+                        err |= ok ? 0u : 4u;
+                        lor(&win[ok ? (o >> 5) : (uint32_t)lane], ok ? (1u << (o & 31)) : 0u);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t e = e0 + 4u * (lane + 64u * u);
+                    const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        if (e + m < n) {
+                            const uint32_t o = ev[m] - tbase;
+                            if (o < pkl::kTile) lor(&win[o >> 5], 1u << (o & 31));
+                            else err |= 4u;
+                        }
                     }
                 }
             }
