@@ -238,6 +238,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=12, help="pancake size")
     ap.add_argument("--segment-bits", type=int, default=22)
+    ap.add_argument("--grid-blocks", type=int, default=0, help="persistent expand blocks (0: resident blocks x CUs)")
+    ap.add_argument("--block-threads", type=int, default=0, help="expand block size (0: default)")
     ap.add_argument("--cross-marks", choices=["auto", "bytes", "atomic", "lists"], default="auto",
                     help="children outside the tile: auto = the tuner's choice (with --profile or --no-tune: "
                          "lists for 10..12 pancakes, else bytes)")
@@ -276,7 +278,8 @@ def main():
     cm = {"bytes": psa.CROSS_BYTES, "atomic": psa.CROSS_ATOMIC, "lists": psa.CROSS_LISTS}.get(args.cross_marks)
     if cm is None:
         cm = psa.CROSS_LISTS if 10 <= n <= 12 else psa.CROSS_BYTES
-    opts = {"segment_bits": args.segment_bits, "cross_marks": cm}
+    opts = {"segment_bits": args.segment_bits, "cross_marks": cm, "grid_blocks": args.grid_blocks,
+            "block_threads": args.block_threads}
     tune_res = None
     if not args.no_tune and not args.profile and args.cross_marks == "auto":
         tune_res = tuned(psa, p, 0, index_count=N, candidates=PANCAKE_CANDIDATES if 10 <= n <= 12 else None)
