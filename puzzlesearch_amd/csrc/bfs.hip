@@ -184,7 +184,7 @@ int launch_batched_layer(ps_bfs* b, bool timed) {
         if (int r = record(b, &e1)) return r;
     size_t efin = e1;  // start of the finalize span
     if (b->lists) {
-        LaunchCfg ca{512, b->grid_apply, b->lds_apply, b->stream};
+        LaunchCfg ca{b->tpb_apply, b->grid_apply, b->lds_apply, b->stream};
         if (int r = pkl_apply(b->puzzle, apply_args(b), ca, nullptr)) return r;
         b->stats.apply_launches++;
         if (timed) {
@@ -270,7 +270,7 @@ int run_plain_layer(ps_bfs* b, bool* done) {
     size_t efin = e1;  // start of the finalize span
     if (b->lists) {
         // every list of this layer is complete: one pass over all tiles
-        LaunchCfg ca{512, b->grid_apply, b->lds_apply, b->stream};
+        LaunchCfg ca{b->tpb_apply, b->grid_apply, b->lds_apply, b->stream};
         if (int r = pkl_apply(b->puzzle, apply_args(b), ca, nullptr)) return r;
         b->stats.apply_launches++;
         if (int r = record(b, &efin)) return r;
@@ -647,7 +647,7 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
             if (int r = pkl_expand(*p, psk::ExpandArgs{}, psk::ListArgs{}, c, &occ)) return cleanup(r);
             b->lds_apply = pkl_apply_lds_bytes(*p);
             int occ_a = 0;
-            LaunchCfg ca{512, 0, b->lds_apply, b->stream};
+            LaunchCfg ca{b->tpb_apply, 0, b->lds_apply, b->stream};
             if (int r = pkl_apply(*p, psk::ApplyArgs{}, ca, &occ_a)) return cleanup(r);
             if (occ_a < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "list apply kernel does not fit (LDS %zu bytes)", b->lds_apply));
             b->grid_apply = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)occ_a * cus, (int64_t)b->xtiles));

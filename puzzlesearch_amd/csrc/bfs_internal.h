@@ -57,6 +57,7 @@ struct ps_bfs {
     uint32_t xtiles = 0;          // n!/9!
     uint32_t xslots = 0;          // lists per tile
     uint32_t grid_apply = 0;
+    int tpb_apply = 1024;
     size_t lds_apply = 0;
     // graph
     hipGraphExec_t graph_exec = nullptr;

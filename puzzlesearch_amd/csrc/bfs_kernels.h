@@ -50,7 +50,6 @@ struct BfsDevState {
     unsigned long long visited;      // states in the layers recorded so far
     unsigned long long pull_words;   // words scanned by bottom-up layers (old and cur read)
     unsigned long long pull_probes;  // cur bits read outside the tile by bottom-up layers
-    unsigned long long apply_counter;  // next destination tile of the cross-list apply pass
 };
 
 struct ExpandArgs {
@@ -675,7 +674,8 @@ __global__ __launch_bounds__(TPB) void k_bfs3_finalize(FinalizeArgs a) {
     uint64_t iter = 0;
     for (;;) {
         if (threadIdx.x == 0) {
-            if (a.persistent) s_chunk = a.chunk_begin + atomicAdd(&st->chunk_counter, 1ull);
+            // chunks carry similar work: a static stride, no shared counter
+            if (a.persistent) s_chunk = a.chunk_begin + blockIdx.x + iter * gridDim.x;
             else s_chunk = (iter == 0) ? a.chunk_begin + blockIdx.x : ~0ull;
         }
         __syncthreads();
@@ -722,7 +722,6 @@ __device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
         st->acc_hash = 0;
         st->tile_counter = 0;
         st->chunk_counter = 0;
-        st->apply_counter = 0;
     }
 }
 

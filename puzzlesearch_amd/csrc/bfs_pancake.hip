@@ -42,12 +42,15 @@ int pkl_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::ListArgs
 
 int pkl_apply(const ps_puzzle& p, const psk::ApplyArgs& a, const LaunchCfg& c, int* occ) {
     if (!pkl_size(p)) return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
-    if (c.tpb != 512) return fail(PS_ERR_INVALID_ARGUMENT, "the list apply kernel runs 512 threads per block");
-    switch (p.size) {
-    case 10: return launch_pkl(psk::k_pkl_apply<10, 512>, c, occ, a);
-    case 11: return launch_pkl(psk::k_pkl_apply<11, 512>, c, occ, a);
-    case 12: return launch_pkl(psk::k_pkl_apply<12, 512>, c, occ, a);
+    switch (p.size * 10000 + c.tpb) {
+    case 100512: return launch_pkl(psk::k_pkl_apply<10, 512>, c, occ, a);
+    case 110512: return launch_pkl(psk::k_pkl_apply<11, 512>, c, occ, a);
+    case 120512: return launch_pkl(psk::k_pkl_apply<12, 512>, c, occ, a);
+    case 101024: return launch_pkl(psk::k_pkl_apply<10, 1024>, c, occ, a);
+    case 111024: return launch_pkl(psk::k_pkl_apply<11, 1024>, c, occ, a);
+    case 121024: return launch_pkl(psk::k_pkl_apply<12, 1024>, c, occ, a);
     }
+    if (c.tpb != 512 && c.tpb != 1024) return fail(PS_ERR_INVALID_ARGUMENT, "the list apply kernel runs 512 or 1024 threads");
     return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
 }
 

@@ -83,16 +83,22 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
     __shared__ unsigned long long s_tile;
     __shared__ uint32_t s_local;
     for (uint32_t i = tid; i < tl_words; i += TPB) tl[i] = 0;
-    for (uint32_t l = tid; l < 720; l += TPB) tab6[l] = pkl::tab6_entry(l, &tab6lo[l]);
-    for (uint32_t h = tid; h < 504; h += TPB) tab3[h] = pkl::tab3_entry(h);
+    bool tables = false;  // built before the block's first live tile
 
     unsigned long long parents = 0, scan = 0;
     uint32_t err = 0;
     uint64_t iter = 0;
     for (;;) {
         if (tid == 0) {
-            if (a.persistent) s_tile = a.tile_begin + atomicAdd(&st->tile_counter, 1ull);
-            else s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
+            if (a.persistent) {
+                // a plain read first: blocks arriving after the last tile make no atomic
+                const unsigned long long taken = __hip_atomic_load(&st->tile_counter, __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_AGENT);
+                s_tile = taken >= a.tile_end - a.tile_begin ? ~0ull
+                                                            : a.tile_begin + atomicAdd(&st->tile_counter, 1ull);
+            } else {
+                s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
+            }
         }
         __syncthreads();
         const uint64_t tile = s_tile;
@@ -113,6 +119,11 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
         }
         const uint32_t T = (uint32_t)tile;
         if (tid == 0) s_local = 0;
+        if (!tables) {
+            for (uint32_t l = tid; l < 720; l += TPB) tab6[l] = pkl::tab6_entry(l, &tab6lo[l]);
+            for (uint32_t h = tid; h < 504; h += TPB) tab3[h] = pkl::tab3_entry(h);
+            tables = true;
+        }
         for (uint32_t i = tid; i < WIN; i += TPB) win[i] = 0;
         // fill counts start where an earlier piece of this tile (plain mode)
         // left them; the apply pass of the previous layer cleared the rest
@@ -248,7 +259,8 @@ __global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
     uint64_t iter = 0;
     for (;;) {
         if (tid == 0) {
-            if (a.persistent) s_tile = a.tile_begin + atomicAdd(&st->apply_counter, 1ull);
+            // destination tiles carry similar work: a static stride, no shared counter
+            if (a.persistent) s_tile = a.tile_begin + blockIdx.x + iter * gridDim.x;
             else s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
             s_flag = s_tile < a.tile_end ? a.x.flags[s_tile] : 0u;
         }
