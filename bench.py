@@ -3,17 +3,20 @@
 A step is one complete search of the workload (12! = 479,001,600 states,
 11 prefix-reversal moves, identity start, 15 layers), device-resident from the
 start state to the empty layer.  The headline value is expanded states per
-second of the batched persistent path (per layer one persistent expand kernel,
-the cross-tile list apply kernel when the tuner picks cross-tile lists, one
-persistent finalize kernel and one bookkeeping kernel, the layer loop captured
-in a hipGraph), with the options the autotuner picks.  The plain per-segment
+second of the batched persistent path, with the options the autotuner picks:
+per layer one persistent kernel that finalizes and expands every tile
+(k_pkl_step) when the tuner picks cross-tile lists, else one persistent expand
+and one persistent finalize kernel, then one bookkeeping kernel; the layer loop
+is captured in a hipGraph.  The plain per-segment
 path (one launch per live segment per layer, host-driven) is timed beside it on
 the same input, and so are the other ways of marking children outside a tile.
 
-The roofline object covers the dominant kernel (the fused scan-expand-mark
-kernel): algorithmic bytes per launch (8 B per frontier word scanned + 8 B per
-child generated) over its average launch duration, timed with HIP events on
-the stream the kernel runs on.  cpu_baseline is the CPU oracle (a one-thread
+The roofline object covers the dominant kernel (k_pkl_step with cross-tile
+lists, else the scan-expand-mark kernel k_bfs3_expand): algorithmic bytes per
+launch (8 B per frontier word scanned + 8 B per child generated, and for
+k_pkl_step also 40 B per finalized word and 8 B per cleared word) over its
+average launch duration, timed with HIP events on the stream the kernel runs
+on.  cpu_baseline is the CPU oracle (a one-thread
 restatement of the reference's InMemoryThreeBitBFS step) timed on a bounded
 slice of the densest layer's frontier.
 
@@ -171,6 +174,12 @@ TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256]
 PANCAKE_CANDIDATES = dict(TUNE_CANDIDATES, cross_marks=[2, 0, 1])
 
 
+def dominant_kernel(psa, opts):
+    """The kernel the roofline covers: batched searches with cross-tile lists
+    run one fused gather/finalize/expand kernel per layer."""
+    return "k_pkl_step" if opts.get("cross_marks") == psa.CROSS_LISTS else "k_bfs3_expand"
+
+
 def tuned(psa, p, start, index_count=0, candidates=None):
     """Autotuned options (coordinate descent) for one search."""
     from puzzlesearch_amd import autotune as AT
@@ -298,8 +307,7 @@ def main():
 
     if args.profile:
         ce, he, st_e, ms, by = eager_roofline(args.steps)
-        kname = "k_pkl_expand" if opts.get("cross_marks") == psa.CROSS_LISTS else "k_bfs3_expand"
-        print(json.dumps({"profile": kname, "launches_last_run": st_e.expand_launches,
+        print(json.dumps({"profile": dominant_kernel(psa, opts), "launches_last_run": st_e.expand_launches,
                           "avg_launch_ms": ms, "algorithmic_bytes_per_launch": by,
                           "achieved_GBps": by / (ms * 1e-3) / 1e9, "counts": ce}))
         return
@@ -317,7 +325,7 @@ def main():
     counts_e, hashes_e, st_e, exp_launch_ms, exp_bytes_launch = eager_roofline(2)
     assert counts_e == counts and hashes_e == hashes
     achieved = exp_bytes_launch / (exp_launch_ms * 1e-3) / 1e9
-    log(f"expand kernel: {st_e.expand_launches} launches, avg {exp_launch_ms:.3f} ms, "
+    log(f"{dominant_kernel(psa, opts)}: {st_e.expand_launches} launches, avg {exp_launch_ms:.3f} ms, "
         f"{exp_bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; apply {st_e.apply_ms:.2f} ms, "
         f"finalize {st_e.finalize_ms:.2f} ms total")
 
@@ -367,7 +375,7 @@ def main():
         log("cpu baseline:", cpu)
 
     value = N / dt_b
-    kernel = "k_pkl_expand" if opts.get("cross_marks") == psa.CROSS_LISTS else "k_bfs3_expand"
+    kernel = dominant_kernel(psa, opts)
     traffic = traffic_from_profile(f"{kernel}_pancake{n}_traffic.txt")
     out = {
         "metric": "expanded states/s (batched persistent three-bit BFS, 12-pancake)",

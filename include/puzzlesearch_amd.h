@@ -358,7 +358,8 @@ typedef struct ps_bfs_stats {
     uint64_t pull_words;          /* words scanned by bottom-up layers (old and cur read) */
     uint64_t pull_probes;         /* frontier bits read outside the tile by bottom-up layers */
     double apply_ms;              /* summed event time of the cross-list apply kernels */
-    uint64_t apply_launches;
+    uint64_t apply_launches;      /* batched PS_CROSS_LISTS runs one fused kernel per layer, counted
+                                     in expand_launches (apply and finalize launches stay 0) */
     uint64_t list_entries;        /* cross-tile list entries written (PS_CROSS_LISTS), all layers */
 } ps_bfs_stats;
 

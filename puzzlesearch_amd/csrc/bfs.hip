@@ -113,6 +113,26 @@ psk::ApplyArgs apply_args(ps_bfs* b) {
     return a;
 }
 
+psk::StepArgs step_args(ps_bfs* b, bool first) {
+    psk::StepArgs a{};
+    for (int i = 0; i < 3; ++i) a.bits[i] = b->arr[i];
+    a.st = b->st;
+    a.lists[0] = b->xlists;
+    a.lists[1] = b->xlists1;
+    a.counts[0] = b->xcounts;
+    a.counts[1] = b->xcounts1;
+    a.xflags[0] = b->xflags;
+    a.xflags[1] = b->xflags1;
+    a.wflags[0] = b->wflags;
+    a.wflags[1] = b->wflags + b->xtiles;
+    a.tcount = b->tcount;
+    a.tiles = b->xtiles;
+    a.odd = b->info.has_odd_cycles;
+    a.hash = b->opt.hash;
+    a.first = first ? 1 : 0;
+    return a;
+}
+
 psk::LayerEndArgs layer_end_args(ps_bfs* b) {
     psk::LayerEndArgs e{};
     e.nbits = b->nbits;
@@ -167,6 +187,21 @@ int launch_batched_layer(ps_bfs* b, bool timed) {
                                b->nwords, b->st);
             PS_HIP_CHECK(hipGetLastError());
         }
+        return launch_layer_end(b);
+    }
+    if (b->fused) {
+        if (!b->primed)
+            if (int r = prime_fused(b)) return r;
+        size_t f0 = 0, f1 = 0;
+        if (timed)
+            if (int r = record(b, &f0)) return r;
+        LaunchCfg cs{b->tpb_expand, b->grid_step, b->lds_step, b->stream};
+        if (int r = pkl_step(b->puzzle, step_args(b, false), cs, nullptr)) return r;
+        if (timed) {
+            if (int r = record(b, &f1)) return r;
+            if (b->collect_kernel_times) b->expand_spans.push_back({f0, f1});
+        }
+        b->stats.expand_launches++;
         return launch_layer_end(b);
     }
     size_t e0 = 0, e1 = 0, e2 = 0, ea = 0;
@@ -228,6 +263,22 @@ int record(ps_bfs* b, size_t* idx) {
     PS_HIP_CHECK(hipEventRecord(e, b->stream));
     return PS_OK;
 }
+
+}  // namespace
+
+int prime_fused(ps_bfs* b) {
+    size_t f0 = 0, f1 = 0;
+    if (int r = record(b, &f0)) return r;
+    LaunchCfg cs{b->tpb_expand, b->grid_step, b->lds_step, b->stream};
+    if (int r = pkl_step(b->puzzle, step_args(b, true), cs, nullptr)) return r;
+    if (int r = record(b, &f1)) return r;
+    if (b->collect_kernel_times) b->expand_spans.push_back({f0, f1});
+    b->stats.expand_launches++;
+    b->primed = true;
+    return PS_OK;
+}
+
+namespace {
 
 // One plain layer: host reads the per-segment counts and launches one expand
 // and one finalize kernel per segment that needs it.
@@ -340,6 +391,11 @@ int destroy_bfs(ps_bfs* b) {
     if (b->xlists) (void)hipFree(b->xlists);
     if (b->xcounts) (void)hipFree(b->xcounts);
     if (b->xflags) (void)hipFree(b->xflags);
+    if (b->xlists1) (void)hipFree(b->xlists1);
+    if (b->xcounts1) (void)hipFree(b->xcounts1);
+    if (b->xflags1) (void)hipFree(b->xflags1);
+    if (b->wflags) (void)hipFree(b->wflags);
+    if (b->tcount) (void)hipFree(b->tcount);
     if (b->seg_count) (void)hipFree(b->seg_count);
     if (b->touched) (void)hipFree(b->touched);
     if (b->layer_counts) (void)hipFree(b->layer_counts);
@@ -424,6 +480,8 @@ uint64_t footprint_bytes(const ps_puzzle* p, const ps_bfs_options* o, const Geom
         uint64_t entries;
         list_geometry(p->size, &tiles, &slots, &entries);
         arrays += entries * 4 + (uint64_t)tiles * slots * 4 + tiles;
+        // batched mode: a second parity of lists, counts and flags, tile flags and counts
+        if (o->mode == PS_BFS_BATCHED) arrays += entries * 4 + (uint64_t)tiles * slots * 4 + tiles + 2ull * tiles + 12ull * tiles;
     }
     const uint64_t meta = 3ull * g.nseg * 8 + ((g.nseg + 31) / 32) * 4 + 2ull * max_layers * 8 + sizeof(BfsDevState);
     return arrays + meta;
@@ -482,6 +540,8 @@ int ps::collect_stats(ps_bfs* b) {
     b->stats.finalize_ms = fms;
     b->stats.apply_ms = ams;
     b->stats.list_entries = b->lists ? st.expanded * (uint64_t)(b->puzzle.size - 9) : 0;
+    // the fused kernel also finalizes: its bytes are the whole search's
+    if (b->fused) b->stats.expand_bytes = b->stats.algorithmic_bytes;
     return PS_OK;
 }
 
@@ -591,6 +651,14 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         if (int r = dalloc((void**)&b->xlists, entries * 4)) return cleanup(r);
         if (int r = dalloc((void**)&b->xcounts, (uint64_t)b->xtiles * b->xslots * 4)) return cleanup(r);
         if (int r = dalloc((void**)&b->xflags, b->xtiles)) return cleanup(r);
+        if (opt.mode == PS_BFS_BATCHED) {
+            b->fused = 1;
+            if (int r = dalloc((void**)&b->xlists1, entries * 4)) return cleanup(r);
+            if (int r = dalloc((void**)&b->xcounts1, (uint64_t)b->xtiles * b->xslots * 4)) return cleanup(r);
+            if (int r = dalloc((void**)&b->xflags1, b->xtiles)) return cleanup(r);
+            if (int r = dalloc((void**)&b->wflags, 2ull * b->xtiles)) return cleanup(r);
+            if (int r = dalloc((void**)&b->tcount, 12ull * b->xtiles)) return cleanup(r);
+        }
     }
     if (int r = dalloc((void**)&b->seg_count, (uint64_t)3 * b->nseg * 8)) return cleanup(r);
     if (int r = dalloc((void**)&b->touched, (uint64_t)((b->nseg + 31) / 32) * 4)) return cleanup(r);
@@ -651,6 +719,15 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
             if (int r = pkl_apply(*p, psk::ApplyArgs{}, ca, &occ_a)) return cleanup(r);
             if (occ_a < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "list apply kernel does not fit (LDS %zu bytes)", b->lds_apply));
             b->grid_apply = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)occ_a * cus, (int64_t)b->xtiles));
+            if (b->fused) {
+                b->lds_step = pkl_step_lds_bytes(*p, b->tpb_expand);
+                int occ_s = 0;
+                LaunchCfg cs{b->tpb_expand, 0, b->lds_step, b->stream};
+                if (int r = pkl_step(*p, psk::StepArgs{}, cs, &occ_s)) return cleanup(r);
+                if (occ_s < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "list step kernel does not fit (LDS %zu bytes)", b->lds_step));
+                b->grid_step = (uint32_t)std::max<int64_t>(1, opt.grid_blocks > 0 ? opt.grid_blocks
+                                                                                : std::min<int64_t>((int64_t)occ_s * cus, (int64_t)b->xtiles));
+            }
         } else {
             LaunchCfg c{b->tpb_expand, 0, b->lds_expand, b->stream};
             if (int r = bfs3_expand(*p, psk::ExpandArgs{}, c, &occ)) return cleanup(r);
@@ -691,6 +768,13 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
         if (int r = launch_fill(b->xcounts, 0, (uint64_t)b->xtiles * b->xslots * 4, b->stream)) return r;
     if (b->xflags)
         if (int r = launch_fill(b->xflags, 0, b->xtiles, b->stream)) return r;
+    if (b->fused) {
+        if (int r = launch_fill(b->xcounts1, 0, (uint64_t)b->xtiles * b->xslots * 4, b->stream)) return r;
+        if (int r = launch_fill(b->xflags1, 0, b->xtiles, b->stream)) return r;
+        if (int r = launch_fill(b->wflags, 0, 2ull * b->xtiles, b->stream)) return r;
+        if (int r = launch_fill(b->tcount, 0, 12ull * b->xtiles, b->stream)) return r;
+    }
+    b->primed = false;
     if (int r = launch_fill(b->seg_count, 0, (uint64_t)3 * b->nseg * 8, b->stream)) return r;
     if (int r = launch_fill(b->touched, 0, (uint64_t)((b->nseg + 31) / 32) * 4, b->stream)) return r;
     PS_HIP_CHECK(hipStreamSynchronize(b->stream));
@@ -768,6 +852,9 @@ int ps_bfs_run(ps_bfs* b, uint64_t start, uint64_t* counts, uint64_t* hashes, in
     if (int r = ps_bfs_reset(b, start)) return r;
     const bool use_graph = b->opt.mode == PS_BFS_BATCHED && b->opt.graph;
     b->collect_kernel_times = !use_graph;
+    // a fused search expands its start layer before the (captured) layer loop
+    if (b->fused && !b->primed)
+        if (int r = prime_fused(b)) return r;
     if (use_graph)
         if (int r = build_graph(b)) return r;
     PS_HIP_CHECK(hipEventRecord(b->ev_run0, b->stream));
@@ -780,8 +867,8 @@ int ps_bfs_run(ps_bfs* b, uint64_t start, uint64_t* counts, uint64_t* hashes, in
         if (use_graph) {
             PS_HIP_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
             b->stats.expand_launches += b->graph_layers;
-            b->stats.finalize_launches += b->bits == 3 ? b->graph_layers : 0;
-            b->stats.apply_launches += b->lists ? b->graph_layers : 0;
+            b->stats.finalize_launches += b->bits == 3 && !b->fused ? b->graph_layers : 0;
+            b->stats.apply_launches += b->lists && !b->fused ? b->graph_layers : 0;
         } else {
             if (int r = launch_batched_layer(b, true)) return r;
         }

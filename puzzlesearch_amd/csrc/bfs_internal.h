@@ -58,6 +58,17 @@ struct ps_bfs {
     uint32_t xslots = 0;          // lists per tile
     uint32_t grid_apply = 0;
     int tpb_apply = 1024;
+    // batched mode with lists: one fused kernel per layer (k_pkl_step) over
+    // lists of two parities; the start layer is expanded by a first launch
+    int fused = 0;
+    bool primed = false;
+    uint32_t* xlists1 = nullptr;  // parity 1 (parity 0: xlists, xcounts, xflags)
+    uint32_t* xcounts1 = nullptr;
+    uint8_t* xflags1 = nullptr;
+    uint8_t* wflags = nullptr;    // [2][tile]
+    uint32_t* tcount = nullptr;   // [3][tile]
+    uint32_t grid_step = 0;
+    size_t lds_step = 0;
     size_t lds_apply = 0;
     // graph
     hipGraphExec_t graph_exec = nullptr;
@@ -81,6 +92,9 @@ psk::FinalizeArgs finalize_args(ps_bfs* b);
 psk::LayerEndArgs layer_end_args(ps_bfs* b);
 psk::ListArgs list_args(ps_bfs* b);
 psk::ApplyArgs apply_args(ps_bfs* b);
+psk::StepArgs step_args(ps_bfs* b, bool first);
+// expands the start layer of a fused search (once after each reset)
+int prime_fused(ps_bfs* b);
 // reads the device state into b->stats (stream idle)
 int collect_stats(ps_bfs* b);
 // enqueues one complete layer (expand, finalize, bookkeeping) on b->stream

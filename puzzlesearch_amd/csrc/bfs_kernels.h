@@ -36,7 +36,7 @@ struct BfsDevState {
                          // slot or fill count out of range; bit2: a list entry outside its tile
     uint32_t max_layers;
     unsigned long long tile_counter;
-    unsigned long long chunk_counter;
+    unsigned long long chunk_counter;  // tiles taken by the first launch of a fused list search
     unsigned long long children;
     unsigned long long expanded;
     unsigned long long acc_count;
@@ -89,6 +89,21 @@ struct ApplyArgs {
     BfsDevState* st;
     int32_t persistent;
     uint64_t tile_begin, tile_end;
+};
+
+// One fused layer of the batched search with cross-tile lists (k_pkl_step).
+struct StepArgs {
+    uint64_t* bits[3];
+    BfsDevState* st;
+    uint32_t* lists[2];   // by parity of the launch that filled them
+    uint32_t* counts[2];
+    uint8_t* xflags[2];   // a list of this parity reaches the tile
+    uint8_t* wflags[2];   // the tile's own expansion marked its next words
+    uint32_t* tcount;     // [3][tiles] frontier states per tile, by layer mod 3
+    uint32_t tiles;
+    int32_t odd;
+    int32_t hash;
+    int32_t first;        // 1: expand the start layer only
 };
 
 struct FinalizeArgs {

@@ -29,9 +29,11 @@ PS_DECLARE_BFS_FAMILY(hanoi)
 // bfs_pancake.hip.  occ != NULL: report resident blocks per CU only.
 int pkl_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::ListArgs& x, const LaunchCfg& c, int* occ);
 int pkl_apply(const ps_puzzle& p, const psk::ApplyArgs& a, const LaunchCfg& c, int* occ);
-// LDS bytes of the two kernels; 0 when the puzzle has no list kernels
+int pkl_step(const ps_puzzle& p, const psk::StepArgs& a, const LaunchCfg& c, int* occ);
+// LDS bytes of the kernels; 0 when the puzzle has no list kernels
 size_t pkl_expand_lds_bytes(const ps_puzzle& p, int tpb, uint32_t nseg);
 size_t pkl_apply_lds_bytes(const ps_puzzle& p);
+size_t pkl_step_lds_bytes(const ps_puzzle& p, int tpb);
 
 inline int bfs3_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const LaunchCfg& c, int* occ) {
     switch (family_of(p)) {

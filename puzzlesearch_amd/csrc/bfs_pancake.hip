@@ -54,6 +54,32 @@ int pkl_apply(const ps_puzzle& p, const psk::ApplyArgs& a, const LaunchCfg& c, i
     return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
 }
 
+int pkl_step(const ps_puzzle& p, const psk::StepArgs& a, const LaunchCfg& c, int* occ) {
+    if (!pkl_size(p)) return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
+    switch (p.size * 10000 + c.tpb) {
+    case 100256: return launch_pkl(psk::k_pkl_step<10, 256>, c, occ, a);
+    case 100512: return launch_pkl(psk::k_pkl_step<10, 512>, c, occ, a);
+    case 101024: return launch_pkl(psk::k_pkl_step<10, 1024>, c, occ, a);
+    case 110256: return launch_pkl(psk::k_pkl_step<11, 256>, c, occ, a);
+    case 110512: return launch_pkl(psk::k_pkl_step<11, 512>, c, occ, a);
+    case 111024: return launch_pkl(psk::k_pkl_step<11, 1024>, c, occ, a);
+    case 120256: return launch_pkl(psk::k_pkl_step<12, 256>, c, occ, a);
+    case 120512: return launch_pkl(psk::k_pkl_step<12, 512>, c, occ, a);
+    case 121024: return launch_pkl(psk::k_pkl_step<12, 1024>, c, occ, a);
+    }
+    return fail(PS_ERR_INVALID_ARGUMENT, "block_threads %d not in {256, 512, 1024}", c.tpb);
+}
+
+size_t pkl_step_lds_bytes(const ps_puzzle& p, int tpb) {
+    if (!pkl_size(p)) return 0;
+    switch (p.size) {
+    case 10: return psk::pkl_step_lds<10>(tpb);
+    case 11: return psk::pkl_step_lds<11>(tpb);
+    case 12: return psk::pkl_step_lds<12>(tpb);
+    }
+    return 0;
+}
+
 size_t pkl_expand_lds_bytes(const ps_puzzle& p, int tpb, uint32_t nseg) {
     if (!pkl_size(p)) return 0;
     switch (p.size) {

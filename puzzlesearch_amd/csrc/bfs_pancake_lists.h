@@ -58,6 +58,225 @@ inline size_t pkl_apply_lds() {
            (size_t)pkl_max_items<N>() * 2;
 }
 
+// Bytes of the per-list metadata of a gather: start, count, first work item,
+// and the list of every work item.
+template <int N>
+constexpr size_t pkl_gather_meta_bytes() {
+    constexpr uint32_t NS = pkl::Geo<N>::NSLOT;
+    return (size_t)NS * 8 + (size_t)NS * 4 + (size_t)(NS + 1) * 4 + (size_t)pkl_max_items<N>() * 2;
+}
+
+// LDS of k_pkl_step: tile bitmap, child tables, the parent queues (expansion)
+// sharing their space with the gather metadata (gather), list fill counts.
+template <int N>
+inline size_t pkl_step_lds(int tpb) {
+    const size_t queues = (size_t)(tpb / 64) * PKL_QCAP * 4;
+    const size_t meta = pkl_gather_meta_bytes<N>();
+    return (size_t)2 * pkl::kTileWords * 4 + PKL_TAB_BYTES + (queues > meta ? queues : meta) +
+           (size_t)pkl::Geo<N>::NSLOT * 4;
+}
+
+// LDS pointers of the tile expansion.
+struct PklExpLds {
+    uint32_t* win;  // tile bitmap (next)
+    const pkl::Tab6* tab6;
+    const uint32_t* tab6lo;
+    const uint32_t* tab3;
+    uint32_t* q;    // this wave's parent queue
+    uint32_t* cnt;  // list fill counts of the tile
+};
+
+// Expands the parents of tile T whose bits are in frontier words [pw0, pw1)
+// (tile words start at w0): flips of <= 9 elements into lds.win, flips of
+// >= 10 elements appended to the tile's lists.  Wave-synchronous only; the
+// caller brackets it with block barriers.
+template <int N, int TPB>
+__device__ __forceinline__ void pkl_expand_range(const uint64_t* __restrict__ front, uint64_t w0, uint64_t pw0,
+                                                 uint64_t pw1, const PklExpLds& lds, const pkl::TileK& tk,
+                                                 uint32_t* lists_t, unsigned long long& parents, uint32_t& err,
+                                                 bool& any) {
+    constexpr int NW = TPB / 64;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    uint32_t* win = lds.win;
+    uint32_t* q = lds.q;
+    uint32_t* cnt = lds.cnt;
+    auto expand_one = [&](uint32_t off) {
+        ++parents;
+        any = true;
+        pkl::children_tab<N>(
+            lds.tab6, lds.tab6lo, lds.tab3, tk, off, [&](uint32_t c) { lor(&win[c >> 5], 1u << (c & 31)); },
+            [&](int k, uint32_t slot, uint32_t child) {
+                const int f = k - 10;
+                if (slot >= pkl::kSlots[f]) {
+                    err |= 2u;
+                    return;
+                }
+                const uint32_t pos = __hip_atomic_fetch_add(&cnt[pkl::kSlotBase[f] + slot], 1u, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (pos >= pkl::kCap[f]) {
+                    err |= 2u;
+                    return;
+                }
+                lists_t[(uint32_t)f * pkl::kTile + slot * pkl::kCap[f] + pos] = child;
+            });
+    };
+    auto run_batch = [&](uint32_t n) {
+        if (n == (uint32_t)PKL_QCAP) {
+            constexpr uint32_t S = PKL_QCAP / 64;
+#pragma unroll 1
+            for (uint32_t r = 0; r < S; ++r) expand_one(q[lane * S + r]);
+        } else {
+            for (uint32_t j = lane; j < n; j += 64) expand_one(q[j]);
+        }
+    };
+    int qn = 0;  // wave-uniform queue fill
+    for (uint64_t g = pw0 + (uint64_t)wave * 64; g < pw1; g += (uint64_t)NW * 64) {
+        const uint64_t wi = g + lane;
+        uint64_t bits = (wi < pw1) ? front[wi] : 0ull;
+        const uint32_t c = (uint32_t)__popcll(bits);
+        const uint32_t incl = wave_incl_scan(c, lane);
+        const int total = (int)__shfl(incl, 63, 64);
+        int off = qn + (int)(incl - c);
+        int end_all = qn + total;
+        const uint32_t base = (uint32_t)((wi - w0) * 64);
+        for (;;) {
+            while (bits && off < PKL_QCAP) {
+                q[off++] = base + (uint32_t)(__ffsll((unsigned long long)bits) - 1);
+                bits &= bits - 1;
+            }
+            if (end_all < PKL_QCAP) {
+                qn = end_all;
+                break;
+            }
+            wave_sync();
+            run_batch(PKL_QCAP);
+            wave_sync();
+            off -= PKL_QCAP;
+            end_all -= PKL_QCAP;
+            qn = 0;
+            if (end_all == 0) break;
+        }
+    }
+    wave_sync();
+    run_batch((uint32_t)qn);
+}
+
+// Stores the tile's list fill counts and flags every destination tile of a
+// non-empty list.  all_counts: also store zero counts.
+template <int N, int TPB>
+__device__ __forceinline__ void pkl_store_counts(uint32_t T, const uint32_t* cnt, uint32_t* counts, uint8_t* flags,
+                                                 bool all_counts) {
+    using G = pkl::Geo<N>;
+    for (uint32_t i = threadIdx.x; i < G::NSLOT; i += TPB) {
+        const uint32_t c = cnt[i];
+        if (c || all_counts) counts[(uint64_t)T * G::NSLOT + i] = c;
+        if (c) {
+            const int k = i < pkl::kSlotBase[1] ? 10 : (i < pkl::kSlotBase[2] ? 11 : 12);
+            flags[pkl::dest_of<N>(T, k, i - pkl::kSlotBase[k - 10])] = 1;
+        }
+    }
+}
+
+// Gathers every list that reaches destination tile t into win (which the
+// caller has cleared), clearing the lists' fill counts.  Every thread of the
+// block calls it; it ends with a block barrier.  meta: LDS of
+// pkl_gather_meta_bytes<N>() bytes, 8-byte aligned.
+template <int N, int TPB>
+__device__ __forceinline__ void pkl_gather(uint32_t t, uint32_t* counts, const uint32_t* lists, uint32_t* win,
+                                           uint8_t* meta, uint32_t& err) {
+    using G = pkl::Geo<N>;
+    constexpr int NW = TPB / 64;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    unsigned long long* lbase = reinterpret_cast<unsigned long long*>(meta);
+    uint32_t* lcnt = reinterpret_cast<uint32_t*>(lbase + G::NSLOT);
+    uint32_t* lstart = lcnt + G::NSLOT;  // NSLOT + 1 entries
+    uint16_t* item_list = reinterpret_cast<uint16_t*>(lstart + G::NSLOT + 1);
+    for (uint32_t i = tid; i < G::NSLOT; i += TPB) {
+        int k = 10;
+        uint32_t src = 0, slot = 0;
+        pkl::incoming<N>(t, i, &k, &src, &slot);
+        const uint64_t ci = (uint64_t)src * G::NSLOT + pkl::kSlotBase[k - 10] + slot;
+        uint32_t c = counts[ci];
+        if (c) {
+            counts[ci] = 0;
+            if (c > pkl::kCap[k - 10]) {
+                err |= 2u;
+                c = 0;
+            }
+        }
+        lcnt[i] = c;
+        lbase[i] = pkl::list_base(src, G::CROSS, k, slot);
+    }
+    __syncthreads();
+    // work items: exclusive scan over the lists (wave 0)
+    if (wave == 0) {
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < G::NSLOT; b += 64) {
+            const uint32_t i = b + lane;
+            const uint32_t c = i < G::NSLOT ? (lcnt[i] + PKL_ITEM - 1) / PKL_ITEM : 0u;
+            const uint32_t incl = wave_incl_scan(c, lane);
+            if (i < G::NSLOT) lstart[i] = run + incl - c;
+            run += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) lstart[G::NSLOT] = run;
+    }
+    __syncthreads();
+    const uint32_t total = lstart[G::NSLOT];
+    // the list of every work item
+    for (uint32_t i = tid; i < G::NSLOT; i += TPB)
+        for (uint32_t w = lstart[i]; w < lstart[i + 1]; ++w) item_list[w] = (uint16_t)i;
+    __syncthreads();
+    const uint32_t tbase = t * pkl::kTile;
+    for (uint32_t w = wave; w < total; w += NW) {
+        const uint32_t li = __builtin_amdgcn_readfirstlane(item_list[w]);
+        const uint32_t e0 = (w - lstart[li]) * PKL_ITEM;
+        const uint32_t n = lcnt[li];
+        const uint4* src = reinterpret_cast<const uint4*>(lists + lbase[li] + e0);
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t e = e0 + 4u * (lane + 64u * u);
+            v[u] = (e < n) ? src[lane + 64 * u] : make_uint4(0u, 0u, 0u, 0u);
+        }
+        if (e0 + PKL_ITEM <= n) {
+            // a full work item, branch-free: an entry outside the tile ORs no bit
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const uint32_t o = ev[m] - tbase;
+                    const bool ok = o < pkl::kTile;
+                    err |= ok ? 0u : 4u;
+                    lor(&win[ok ? (o >> 5) : (uint32_t)lane], ok ? (1u << (o & 31)) : 0u);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t e = e0 + 4u * (lane + 64u * u);
+                const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    if (e + m < n) {
+                        const uint32_t o = ev[m] - tbase;
+                        if (o < pkl::kTile) lor(&win[o >> 5], 1u << (o & 31));
+                        else err |= 4u;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Takes the next tile: a plain read first, so blocks arriving after the last
+// tile make no atomic.  Thread 0 only.
+__device__ __forceinline__ unsigned long long pkl_take_tile(unsigned long long* counter, unsigned long long n) {
+    const unsigned long long taken = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return taken >= n ? ~0ull : atomicAdd(counter, 1ull);
+}
+
 template <int N, int TPB>
 __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
     using G = pkl::Geo<N>;
@@ -66,14 +285,18 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
     extern __shared__ uint32_t smem[];
     BfsDevState* st = a.st;
     if (st->done) return;
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    uint32_t* win = smem;
+    const int tid = threadIdx.x, wave = tid / 64;
+    PklExpLds lds;
+    lds.win = smem;
     pkl::Tab6* tab6 = reinterpret_cast<pkl::Tab6*>(smem + WIN);  // WIN * 4 bytes is a multiple of 16
     uint32_t* tab6lo = reinterpret_cast<uint32_t*>(tab6 + 720);
     uint32_t* tab3 = tab6lo + 720;
-    uint32_t* q = tab3 + 504 + wave * PKL_QCAP;
-    uint32_t* cnt = tab3 + 504 + NW * PKL_QCAP;
-    uint32_t* tl = cnt + G::NSLOT;
+    lds.tab6 = tab6;
+    lds.tab6lo = tab6lo;
+    lds.tab3 = tab3;
+    lds.q = tab3 + 504 + wave * PKL_QCAP;
+    lds.cnt = tab3 + 504 + NW * PKL_QCAP;
+    uint32_t* tl = lds.cnt + G::NSLOT;
     const uint32_t tl_words = (a.nseg > 1 && a.nseg <= TOUCH_LDS_MAX) ? (a.nseg + 31) / 32 : 0u;
     const bool track = a.nseg > 1;
     const uint32_t L = st->layer;
@@ -91,11 +314,8 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
     for (;;) {
         if (tid == 0) {
             if (a.persistent) {
-                // a plain read first: blocks arriving after the last tile make no atomic
-                const unsigned long long taken = __hip_atomic_load(&st->tile_counter, __ATOMIC_RELAXED,
-                                                                   __HIP_MEMORY_SCOPE_AGENT);
-                s_tile = taken >= a.tile_end - a.tile_begin ? ~0ull
-                                                            : a.tile_begin + atomicAdd(&st->tile_counter, 1ull);
+                const unsigned long long k = pkl_take_tile(&st->tile_counter, a.tile_end - a.tile_begin);
+                s_tile = k == ~0ull ? ~0ull : a.tile_begin + k;
             } else {
                 s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
             }
@@ -124,78 +344,18 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
             for (uint32_t h = tid; h < 504; h += TPB) tab3[h] = pkl::tab3_entry(h);
             tables = true;
         }
-        for (uint32_t i = tid; i < WIN; i += TPB) win[i] = 0;
+        for (uint32_t i = tid; i < WIN; i += TPB) lds.win[i] = 0;
         // fill counts start where an earlier piece of this tile (plain mode)
         // left them; the apply pass of the previous layer cleared the rest
-        for (uint32_t i = tid; i < G::NSLOT; i += TPB) cnt[i] = x.counts[(uint64_t)T * G::NSLOT + i];
+        for (uint32_t i = tid; i < G::NSLOT; i += TPB) lds.cnt[i] = x.counts[(uint64_t)T * G::NSLOT + i];
         uint32_t dp[3];
         pkl::tile_digits<N>(T, dp);
         const pkl::TileK tk = pkl::tile_consts<N>(dp);
         uint32_t* lists_t = x.lists + (uint64_t)T * G::CROSS * pkl::kTile;
         __syncthreads();
         if (tid == 0) scan += pw1 - pw0;
-
         bool any = false;
-        auto expand_one = [&](uint32_t off) {
-                ++parents;
-                any = true;
-                pkl::children_tab<N>(
-                    tab6, tab6lo, tab3, tk, off, [&](uint32_t c) { lor(&win[c >> 5], 1u << (c & 31)); },
-                    [&](int k, uint32_t slot, uint32_t child) {
-                        const int f = k - 10;
-                        if (slot >= pkl::kSlots[f]) {
-                            err |= 2u;
-                            return;
-                        }
-                        const uint32_t pos = __hip_atomic_fetch_add(&cnt[pkl::kSlotBase[f] + slot], 1u,
-                                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (pos >= pkl::kCap[f]) {
-                            err |= 2u;
-                            return;
-                        }
-                        lists_t[(uint32_t)f * pkl::kTile + slot * pkl::kCap[f] + pos] = child;
-                    });
-        };
-        auto run_batch = [&](uint32_t n) {
-            if (n == (uint32_t)PKL_QCAP) {
-                constexpr uint32_t S = PKL_QCAP / 64;
-#pragma unroll 1
-                for (uint32_t r = 0; r < S; ++r) expand_one(q[lane * S + r]);
-            } else {
-                for (uint32_t j = lane; j < n; j += 64) expand_one(q[j]);
-            }
-        };
-
-        int qn = 0;  // wave-uniform queue fill
-        for (uint64_t g = pw0 + (uint64_t)wave * 64; g < pw1; g += (uint64_t)NW * 64) {
-            const uint64_t wi = g + lane;
-            uint64_t bits = (wi < pw1) ? cur[wi] : 0ull;
-            const uint32_t c = (uint32_t)__popcll(bits);
-            const uint32_t incl = wave_incl_scan(c, lane);
-            const int total = (int)__shfl(incl, 63, 64);
-            int off = qn + (int)(incl - c);
-            int end_all = qn + total;
-            const uint32_t base = (uint32_t)((wi - w0) * 64);
-            for (;;) {
-                while (bits && off < PKL_QCAP) {
-                    q[off++] = base + (uint32_t)(__ffsll((unsigned long long)bits) - 1);
-                    bits &= bits - 1;
-                }
-                if (end_all < PKL_QCAP) {
-                    qn = end_all;
-                    break;
-                }
-                wave_sync();
-                run_batch(PKL_QCAP);
-                wave_sync();
-                off -= PKL_QCAP;
-                end_all -= PKL_QCAP;
-                qn = 0;
-                if (end_all == 0) break;
-            }
-        }
-        wave_sync();
-        run_batch((uint32_t)qn);
+        pkl_expand_range<N, TPB>(cur, w0, pw0, pw1, lds, tk, lists_t, parents, err, any);
         if (any) s_local = 1;
         __syncthreads();
         // a tile expanded whole in one visit (batched mode) owns its range of
@@ -203,21 +363,13 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
         uint32_t* dst = nxt + w0 * 2;
         const bool own = pw0 == w0 && pw1 == w1;
         for (uint32_t i = tid; i < WIN; i += TPB) {
-            const uint32_t v = win[i];
+            const uint32_t v = lds.win[i];
             if (v) {
                 if (own) dst[i] = v;
                 else gor(&dst[i], v);
             }
         }
-        for (uint32_t i = tid; i < G::NSLOT; i += TPB) {
-            const uint32_t c = cnt[i];
-            x.counts[(uint64_t)T * G::NSLOT + i] = c;
-            if (c) {
-                // the apply pass visits only tiles some list reaches
-                const int k = i < pkl::kSlotBase[1] ? 10 : (i < pkl::kSlotBase[2] ? 11 : 12);
-                x.flags[pkl::dest_of<N>(T, k, i - pkl::kSlotBase[k - 10])] = 1;
-            }
-        }
+        pkl_store_counts<N, TPB>(T, lds.cnt, x.counts, x.flags, true);
         if (tid == 0 && track && s_local) {
             const uint64_t s0 = (w0 * 64) >> a.seg_shift, s1 = (w1 * 64 - 1) >> a.seg_shift;
             for (uint64_t s = s0; s <= s1; ++s) {
@@ -239,18 +391,13 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
 
 template <int N, int TPB>
 __global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
-    using G = pkl::Geo<N>;
-    constexpr int NW = TPB / 64;
     constexpr uint32_t WIN = 2 * pkl::kTileWords;
     extern __shared__ uint32_t smem[];
     BfsDevState* st = a.st;
     if (st->done) return;
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int tid = threadIdx.x;
     uint32_t* win = smem;
-    unsigned long long* lbase = reinterpret_cast<unsigned long long*>(smem + WIN);  // WIN is even: 8-byte aligned
-    uint32_t* lcnt = reinterpret_cast<uint32_t*>(lbase + G::NSLOT);
-    uint32_t* lstart = lcnt + G::NSLOT;  // NSLOT + 1 entries
-    uint16_t* item_list = reinterpret_cast<uint16_t*>(lstart + G::NSLOT + 1);
+    uint8_t* meta = reinterpret_cast<uint8_t*>(smem + WIN);  // WIN is even: 8-byte aligned
     const uint32_t L = st->layer;
     uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
     __shared__ unsigned long long s_tile;
@@ -274,84 +421,8 @@ __global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
         }
         const uint32_t t = (uint32_t)tile;
         if (tid == 0) a.x.flags[t] = 0;
-        for (uint32_t i = tid; i < G::NSLOT; i += TPB) {
-            int k = 10;
-            uint32_t src = 0, slot = 0;
-            pkl::incoming<N>(t, i, &k, &src, &slot);
-            const uint64_t ci = (uint64_t)src * G::NSLOT + pkl::kSlotBase[k - 10] + slot;
-            uint32_t c = a.x.counts[ci];
-            if (c) {
-                a.x.counts[ci] = 0;
-                if (c > pkl::kCap[k - 10]) {
-                    err |= 2u;
-                    c = 0;
-                }
-            }
-            lcnt[i] = c;
-            lbase[i] = pkl::list_base(src, G::CROSS, k, slot);
-        }
         for (uint32_t i = tid; i < WIN; i += TPB) win[i] = 0;
-        __syncthreads();
-        // work items: exclusive scan over the lists (wave 0)
-        if (wave == 0) {
-            uint32_t run = 0;
-            for (uint32_t b = 0; b < G::NSLOT; b += 64) {
-                const uint32_t i = b + lane;
-                const uint32_t c = i < G::NSLOT ? (lcnt[i] + PKL_ITEM - 1) / PKL_ITEM : 0u;
-                const uint32_t incl = wave_incl_scan(c, lane);
-                if (i < G::NSLOT) lstart[i] = run + incl - c;
-                run += __shfl(incl, 63, 64);
-            }
-            if (lane == 0) lstart[G::NSLOT] = run;
-        }
-        __syncthreads();
-        const uint32_t total = lstart[G::NSLOT];
-        // the list of every work item
-        for (uint32_t i = tid; i < G::NSLOT; i += TPB)
-            for (uint32_t w = lstart[i]; w < lstart[i + 1]; ++w) item_list[w] = (uint16_t)i;
-        __syncthreads();
-        const uint32_t tbase = t * pkl::kTile;
-        for (uint32_t w = wave; w < total; w += NW) {
-            const uint32_t li = __builtin_amdgcn_readfirstlane(item_list[w]);
-            const uint32_t e0 = (w - lstart[li]) * PKL_ITEM;
-            const uint32_t n = lcnt[li];
-            const uint4* src = reinterpret_cast<const uint4*>(a.x.lists + lbase[li] + e0);
-            uint4 v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t e = e0 + 4u * (lane + 64u * u);
-                v[u] = (e < n) ? src[lane + 64 * u] : make_uint4(0u, 0u, 0u, 0u);
-            }
-            if (e0 + PKL_ITEM <= n) {
-                // a full work item, branch-free: an entry outside the tile ORs no bit
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) {
-                        const uint32_t o = ev[m] - tbase;
-                        const bool ok = o < pkl::kTile;
-                        err |= ok ? 0u : 4u;
-                        lor(&win[ok ? (o >> 5) : (uint32_t)lane], ok ? (1u << (o & 31)) : 0u);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const uint32_t e = e0 + 4u * (lane + 64u * u);
-                    const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) {
-                        if (e + m < n) {
-                            const uint32_t o = ev[m] - tbase;
-                            if (o < pkl::kTile) lor(&win[o >> 5], 1u << (o & 31));
-                            else err |= 4u;
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
+        pkl_gather<N, TPB>(t, a.x.counts, a.x.lists, win, meta, err);
         uint32_t* dst = nxt + (uint64_t)t * WIN;
         for (uint32_t i = tid; i < WIN; i += TPB) {
             const uint32_t vv = win[i];
@@ -365,6 +436,195 @@ __global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
         __syncthreads();
     }
     if (err) atomicOr(&st->error, err);
+}
+
+// One whole layer of the batched search with cross-tile lists, per tile:
+//  1. (not in the first launch) finalize the next layer of the tile: gather
+//     the lists that reach it (produced by the previous launch), then
+//     new = (next | gathered) & ~old & ~cur, count, hash, old = 0 -- or only
+//     clear old where nothing reached the tile;
+//  2. expand the tile's new frontier: flips of <= 9 elements into the tile's
+//     bitmap, stored into the array cleared in step 1 (the following layer's
+//     next), flips of >= 10 elements into the lists of the other parity.
+// Every dependency between the two steps is inside the tile; across tiles they
+// meet only through lists completed by the previous launch.  The first launch
+// (first = 1) expands the start layer.  Layer L (st->layer) is the last
+// finalized layer; launch m finalizes and expands layer m = L + 1.
+// Four waves per SIMD (16 per CU): two 512-thread blocks share a CU, as the
+// LDS allows (at most 128 VGPRs; unbounded, the compiler takes more and
+// leaves one block per CU).
+template <int N, int TPB>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_pkl_step(StepArgs a) {
+    using G = pkl::Geo<N>;
+    constexpr int NW = TPB / 64;
+    constexpr uint32_t WIN = 2 * pkl::kTileWords;
+    extern __shared__ uint32_t smem[];
+    BfsDevState* st = a.st;
+    if (st->done) return;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    PklExpLds lds;
+    lds.win = smem;
+    pkl::Tab6* tab6 = reinterpret_cast<pkl::Tab6*>(smem + WIN);
+    uint32_t* tab6lo = reinterpret_cast<uint32_t*>(tab6 + 720);
+    uint32_t* tab3 = tab6lo + 720;
+    lds.tab6 = tab6;
+    lds.tab6lo = tab6lo;
+    lds.tab3 = tab3;
+    uint32_t* shared_region = tab3 + 504;  // queues (expansion) or gather metadata
+    lds.q = shared_region + wave * PKL_QCAP;
+    {
+        const size_t qbytes = (size_t)NW * PKL_QCAP * 4, mbytes = pkl_gather_meta_bytes<N>();
+        lds.cnt = shared_region + (qbytes > mbytes ? qbytes : mbytes) / 4;
+    }
+    uint8_t* meta = reinterpret_cast<uint8_t*>(shared_region);  // tab3 + 504 words: 8-byte aligned
+    const bool first = a.first != 0;
+    const uint32_t L = st->layer;
+    const uint64_t* cur = a.bits[L % 3];
+    uint64_t* nw = a.bits[(L + 1) % 3];
+    uint64_t* old = a.bits[(L + 2) % 3];
+    // the expansion: frontier, its next array, list parity
+    const uint64_t* front = first ? a.bits[0] : nw;
+    uint32_t* enext = reinterpret_cast<uint32_t*>(first ? a.bits[1] : old);
+    const int gp = (int)(L & 1u), xp = first ? 0 : (int)((L + 1u) & 1u);
+    uint32_t* tc_new = a.tcount + (uint64_t)(first ? 0 : (L + 1) % 3) * a.tiles;
+    const uint32_t* tc_old = a.tcount + (uint64_t)((L + 2) % 3) * a.tiles;
+    __shared__ unsigned long long s_tile;
+    __shared__ uint32_t s_xf, s_wf, s_oldc, s_local;
+    __shared__ unsigned long long s_red[2][NW];
+    __shared__ unsigned long long s_cnt;
+    bool tables = false;
+    unsigned long long parents = 0, scan = 0, fin_full = 0, fin_clear = 0, lcount = 0, lhash = 0;
+    uint32_t err = 0;
+    // the first launch has no bookkeeping launch after it to reset tile_counter
+    unsigned long long* counter = first ? &st->chunk_counter : &st->tile_counter;
+    for (;;) {
+        if (tid == 0) {
+            s_tile = pkl_take_tile(counter, a.tiles);
+            if (s_tile != ~0ull) {
+                s_xf = first ? 0u : a.xflags[gp][s_tile];
+                s_wf = first ? 0u : a.wflags[gp][s_tile];
+                s_oldc = first ? 0u : tc_old[s_tile];
+            }
+        }
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        if (tile >= a.tiles) break;
+        const uint32_t T = (uint32_t)tile;
+        const uint64_t w0 = tile * pkl::kTileWords, w1 = w0 + pkl::kTileWords;
+        bool expand = true;
+        if (!first) {
+            // 1. finalize the tile's next layer
+            unsigned long long c = 0, h = 0;
+            if (s_xf | s_wf) {
+                for (uint32_t i = tid; i < WIN; i += TPB) lds.win[i] = 0;
+                __syncthreads();
+                if (s_xf) pkl_gather<N, TPB>(T, a.counts[gp], a.lists[gp], lds.win, meta, err);
+                const uint64_t* gw = reinterpret_cast<const uint64_t*>(lds.win);
+                // two words per thread in flight before any store (the arrays are
+                // distinct); four push the kernel past 128 VGPRs into spills
+                constexpr int U = 2;
+                for (uint32_t j0 = tid; j0 < pkl::kTileWords; j0 += TPB * U) {
+                    uint64_t vn[U], vo[U], vc[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const uint32_t j = j0 + u * TPB;
+                        if (j < pkl::kTileWords) {
+                            vn[u] = nw[w0 + j];
+                            vo[u] = old[w0 + j];
+                            vc[u] = a.odd ? cur[w0 + j] : 0ull;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const uint32_t j = j0 + u * TPB;
+                        if (j < pkl::kTileWords) {
+                            const uint64_t v = (vn[u] | gw[j]) & ~vo[u] & ~vc[u];
+                            nw[w0 + j] = v;
+                            old[w0 + j] = 0;
+                            c += (unsigned long long)__popcll(v);
+                            if (a.hash) h += word_hash(w0 + j, v);
+                        }
+                    }
+                }
+                if (tid == 0) {
+                    a.xflags[gp][T] = 0;
+                    a.wflags[gp][T] = 0;
+                    fin_full += pkl::kTileWords;
+                }
+            } else if (s_oldc) {
+                for (uint64_t j = tid; j < pkl::kTileWords; j += TPB) old[w0 + j] = 0;
+                if (tid == 0) fin_clear += pkl::kTileWords;
+            }
+            c = wave_sum(c);
+            h = wave_sum(h);
+            if (lane == 0) {
+                s_red[0][wave] = c;
+                s_red[1][wave] = h;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                unsigned long long sc = 0, sh = 0;
+                for (int w = 0; w < NW; ++w) {
+                    sc += s_red[0][w];
+                    sh += s_red[1][w];
+                }
+                tc_new[T] = (uint32_t)sc;
+                lcount += sc;
+                lhash += sh;
+                s_cnt = sc;
+            }
+            __syncthreads();
+            expand = s_cnt != 0;
+        }
+        if (expand) {
+            // 2. expand the tile's frontier
+            if (tid == 0) s_local = 0;
+            if (!tables) {
+                for (uint32_t l = tid; l < 720; l += TPB) tab6[l] = pkl::tab6_entry(l, &tab6lo[l]);
+                for (uint32_t hh = tid; hh < 504; hh += TPB) tab3[hh] = pkl::tab3_entry(hh);
+                tables = true;
+            }
+            for (uint32_t i = tid; i < WIN; i += TPB) lds.win[i] = 0;
+            // the lists of this parity were consumed (counts cleared) by the previous launch
+            for (uint32_t i = tid; i < G::NSLOT; i += TPB) lds.cnt[i] = 0;
+            uint32_t dp[3];
+            pkl::tile_digits<N>(T, dp);
+            const pkl::TileK tk = pkl::tile_consts<N>(dp);
+            uint32_t* lists_t = a.lists[xp] + (uint64_t)T * G::CROSS * pkl::kTile;
+            __syncthreads();
+            if (tid == 0) scan += pkl::kTileWords;
+            bool any = false;
+            unsigned long long tile_parents = 0;
+            pkl_expand_range<N, TPB>(front, w0, w0, w1, lds, tk, lists_t, tile_parents, err, any);
+            parents += tile_parents;
+            if (any) s_local = 1;
+            __syncthreads();
+            uint32_t* dst = enext + w0 * 2;
+            for (uint32_t i = tid; i < WIN; i += TPB) {
+                const uint32_t v = lds.win[i];
+                if (v) dst[i] = v;
+            }
+            pkl_store_counts<N, TPB>(T, lds.cnt, a.counts[xp], a.xflags[xp], false);
+            if (tid == 0 && s_local) a.wflags[xp][T] = 1;
+            if (first) {
+                // the start layer's count per tile (the finalize two launches on clears old by it)
+                tile_parents = wave_sum(tile_parents);
+                if (lane == 0) s_red[0][wave] = tile_parents;
+                __syncthreads();
+                if (tid == 0) {
+                    unsigned long long sc = 0;
+                    for (int w = 0; w < NW; ++w) sc += s_red[0][w];
+                    a.tcount[T] = (uint32_t)sc;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (err) atomicOr(&st->error, err);
+    block_add3<TPB>(parents * (unsigned long long)(N - 1), parents, scan, &st->children, &st->expanded,
+                    &st->scan_words);
+    block_add3<TPB>(lcount, lhash, 0ull, &st->acc_count, &st->acc_hash, nullptr);
+    block_add3<TPB>(fin_full, fin_clear, 0ull, &st->fin_full_words, &st->fin_clear_words, nullptr);
 }
 
 }  // namespace psk

@@ -45,7 +45,11 @@ def test_lists_against_oracle_fixture(ps, name, mode):
     assert st.expanded == math.factorial(n)
     assert st.children == math.factorial(n) * (n - 1)
     assert st.list_entries == math.factorial(n) * (n - 9)
-    assert st.apply_launches >= len(counts) - 1
+    if MODES[mode]["mode"] == psa.BFS_PLAIN:
+        assert st.apply_launches >= len(counts) - 1
+    else:  # one fused gather/finalize/expand kernel per layer
+        assert st.apply_launches == 0 and st.finalize_launches == 0
+        assert st.expand_launches >= len(counts)
 
 
 @pytest.mark.parametrize("start", [1, 1234567, math.factorial(10) - 1])
@@ -131,5 +135,9 @@ def test_lists_footprint(ps):
     n = 12
     base = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), cross_marks=psa.CROSS_ATOMIC)
     lists = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), **LISTS)
+    plain = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), mode=psa.BFS_PLAIN, **LISTS)
     tiles = math.factorial(n) // math.factorial(9)
-    assert lists - base == tiles * (n - 9) * math.factorial(9) * 4 + tiles * 585 * 4 + tiles
+    one = tiles * (n - 9) * math.factorial(9) * 4 + tiles * 585 * 4 + tiles  # lists, counts, flags
+    assert plain - base == one
+    # batched: lists of two layer parities, per-tile window flags and frontier counts
+    assert lists - base == 2 * one + 2 * tiles + 12 * tiles
