@@ -171,7 +171,7 @@ TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256]
                    "graph_layers": [4, 8, 16], "cross_marks": [0, 1], "tile_bits_log2": [0, 16],
                    "skip_visited": [0], "direction": [0, 1]}
 # 10..12 pancakes: cross-tile lists first (bottom-up and other windows do not combine with them)
-PANCAKE_CANDIDATES = dict(TUNE_CANDIDATES, cross_marks=[2, 0, 1])
+PANCAKE_CANDIDATES = dict(TUNE_CANDIDATES, cross_marks=[2, 0, 1], block_threads=[0, 512, 256])
 
 
 def dominant_kernel(psa, opts):

@@ -329,7 +329,8 @@ typedef struct ps_bfs_options {
     int32_t max_layers;     /* 0: until a layer is empty */
     uint64_t index_count;   /* 0: Puzzle::IndexesCount() */
     int32_t tile_bits_log2; /* 0: tuned default; else LDS window of 2^k states (-1: no window) */
-    int32_t block_threads;  /* 0: tuned default; 256 or 512 (1024 with PS_CROSS_LISTS) */
+    int32_t block_threads;  /* 0: tuned default (1024 for batched PS_CROSS_LISTS, else 512);
+                               256 or 512 (1024 with PS_CROSS_LISTS) */
     int32_t grid_blocks;    /* 0: tuned default (persistent grid) */
     int32_t hash;           /* 1: per-layer frontier hashes */
     int32_t graph;          /* 1: capture the batched layer loop in a hipGraph (default on) */

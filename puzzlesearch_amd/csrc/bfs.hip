@@ -126,7 +126,11 @@ psk::StepArgs step_args(ps_bfs* b, bool first) {
     a.wflags[0] = b->wflags;
     a.wflags[1] = b->wflags + b->xtiles;
     a.tcount = b->tcount;
+    a.active = b->active;
     a.tiles = b->xtiles;
+    // a start outside the tiles (an index space past n!) leaves nothing to expand
+    const uint64_t t0 = b->start_index / factorial(9);
+    a.first_tile = (uint32_t)std::min<uint64_t>(t0, b->xtiles);
     a.odd = b->info.has_odd_cycles;
     a.hash = b->opt.hash;
     a.first = first ? 1 : 0;
@@ -154,6 +158,7 @@ int record(ps_bfs* b, size_t* idx);
 }  // namespace
 
 int launch_layer_end(ps_bfs* b) {
+    if (b->fused) return pkl_layer_end(layer_end_args(b), step_args(b, false), 1, b->stream);
     hipLaunchKernelGGL(psk::k_bfs_layer_end<1024>, dim3(1), dim3(1024), 0, b->stream, layer_end_args(b));
     PS_HIP_CHECK(hipGetLastError());
     return PS_OK;
@@ -274,6 +279,7 @@ int prime_fused(ps_bfs* b) {
     if (int r = record(b, &f1)) return r;
     if (b->collect_kernel_times) b->expand_spans.push_back({f0, f1});
     b->stats.expand_launches++;
+    if (int r = pkl_layer_end(layer_end_args(b), step_args(b, false), 0, b->stream)) return r;
     b->primed = true;
     return PS_OK;
 }
@@ -396,6 +402,7 @@ int destroy_bfs(ps_bfs* b) {
     if (b->xflags1) (void)hipFree(b->xflags1);
     if (b->wflags) (void)hipFree(b->wflags);
     if (b->tcount) (void)hipFree(b->tcount);
+    if (b->active) (void)hipFree(b->active);
     if (b->seg_count) (void)hipFree(b->seg_count);
     if (b->touched) (void)hipFree(b->touched);
     if (b->layer_counts) (void)hipFree(b->layer_counts);
@@ -480,8 +487,10 @@ uint64_t footprint_bytes(const ps_puzzle* p, const ps_bfs_options* o, const Geom
         uint64_t entries;
         list_geometry(p->size, &tiles, &slots, &entries);
         arrays += entries * 4 + (uint64_t)tiles * slots * 4 + tiles;
-        // batched mode: a second parity of lists, counts and flags, tile flags and counts
-        if (o->mode == PS_BFS_BATCHED) arrays += entries * 4 + (uint64_t)tiles * slots * 4 + tiles + 2ull * tiles + 12ull * tiles;
+        // batched mode: a second parity of lists, counts and flags, tile flags,
+        // counts and the list of tiles a launch visits
+        if (o->mode == PS_BFS_BATCHED)
+            arrays += entries * 4 + (uint64_t)tiles * slots * 4 + tiles + 2ull * tiles + 12ull * tiles + 4ull * tiles;
     }
     const uint64_t meta = 3ull * g.nseg * 8 + ((g.nseg + 31) / 32) * 4 + 2ull * max_layers * 8 + sizeof(BfsDevState);
     return arrays + meta;
@@ -658,6 +667,7 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
             if (int r = dalloc((void**)&b->xflags1, b->xtiles)) return cleanup(r);
             if (int r = dalloc((void**)&b->wflags, 2ull * b->xtiles)) return cleanup(r);
             if (int r = dalloc((void**)&b->tcount, 12ull * b->xtiles)) return cleanup(r);
+            if (int r = dalloc((void**)&b->active, 4ull * b->xtiles)) return cleanup(r);
         }
     }
     if (int r = dalloc((void**)&b->seg_count, (uint64_t)3 * b->nseg * 8)) return cleanup(r);
@@ -695,7 +705,10 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         if (!b->use_window) block_bits = 1ull << 18;
         if (b->lists) block_bits = factorial(9);
         b->tile_words = std::max<uint64_t>(1, block_bits / 64);
-        b->tpb_expand = opt.block_threads ? opt.block_threads : 512;
+        // the fused list kernel fills a CU with one 1024-thread block: half as many
+        // tiles in flight as two 512-thread blocks, so fewer partly written list
+        // lines are evicted from L2
+        b->tpb_expand = opt.block_threads ? opt.block_threads : (b->lists && opt.mode == PS_BFS_BATCHED ? 1024 : 512);
         if (b->tpb_expand != 256 && b->tpb_expand != 512 && !(b->lists && b->tpb_expand == 1024))
             return cleanup(fail(PS_ERR_INVALID_ARGUMENT,
                                 "block_threads must be 256 or 512 (or 1024 with PS_CROSS_LISTS)"));

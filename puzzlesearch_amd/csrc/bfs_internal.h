@@ -67,6 +67,7 @@ struct ps_bfs {
     uint8_t* xflags1 = nullptr;
     uint8_t* wflags = nullptr;    // [2][tile]
     uint32_t* tcount = nullptr;   // [3][tile]
+    uint32_t* active = nullptr;   // [tile] the tiles the next step launch visits
     uint32_t grid_step = 0;
     size_t lds_step = 0;
     size_t lds_apply = 0;

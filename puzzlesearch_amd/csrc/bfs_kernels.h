@@ -47,6 +47,8 @@ struct BfsDevState {
     unsigned long long fin_clear_words;  // words of old cleared without a finalize
     uint32_t pull;       // the current layer is expanded bottom-up (see pull_tile)
     uint32_t direction;  // 0: always top-down, 1: choose per layer, 2: always bottom-up
+    uint32_t active_tiles;  // tiles the next fused list launch visits (StepArgs::active)
+    uint32_t reserved;
     unsigned long long visited;      // states in the layers recorded so far
     unsigned long long pull_words;   // words scanned by bottom-up layers (old and cur read)
     unsigned long long pull_probes;  // cur bits read outside the tile by bottom-up layers
@@ -100,7 +102,9 @@ struct StepArgs {
     uint8_t* xflags[2];   // a list of this parity reaches the tile
     uint8_t* wflags[2];   // the tile's own expansion marked its next words
     uint32_t* tcount;     // [3][tiles] frontier states per tile, by layer mod 3
+    uint32_t* active;     // [tiles] the tiles a launch visits (st->active_tiles of them)
     uint32_t tiles;
+    uint32_t first_tile;  // the start state's tile (first launch)
     int32_t odd;
     int32_t hash;
     int32_t first;        // 1: expand the start layer only

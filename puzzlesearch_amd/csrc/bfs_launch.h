@@ -30,6 +30,8 @@ PS_DECLARE_BFS_FAMILY(hanoi)
 int pkl_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::ListArgs& x, const LaunchCfg& c, int* occ);
 int pkl_apply(const ps_puzzle& p, const psk::ApplyArgs& a, const LaunchCfg& c, int* occ);
 int pkl_step(const ps_puzzle& p, const psk::StepArgs& a, const LaunchCfg& c, int* occ);
+// bookkeeping between two k_pkl_step launches (advance: also the layer end)
+int pkl_layer_end(const psk::LayerEndArgs& a, const psk::StepArgs& s, int advance, hipStream_t stream);
 // LDS bytes of the kernels; 0 when the puzzle has no list kernels
 size_t pkl_expand_lds_bytes(const ps_puzzle& p, int tpb, uint32_t nseg);
 size_t pkl_apply_lds_bytes(const ps_puzzle& p);

@@ -70,6 +70,12 @@ int pkl_step(const ps_puzzle& p, const psk::StepArgs& a, const LaunchCfg& c, int
     return fail(PS_ERR_INVALID_ARGUMENT, "block_threads %d not in {256, 512, 1024}", c.tpb);
 }
 
+int pkl_layer_end(const psk::LayerEndArgs& a, const psk::StepArgs& s, int advance, hipStream_t stream) {
+    hipLaunchKernelGGL(psk::k_pkl_layer_end<1024>, dim3(1), dim3(1024), 0, stream, a, s, advance);
+    PS_HIP_CHECK(hipGetLastError());
+    return PS_OK;
+}
+
 size_t pkl_step_lds_bytes(const ps_puzzle& p, int tpb) {
     if (!pkl_size(p)) return 0;
     switch (p.size) {

@@ -66,13 +66,15 @@ constexpr size_t pkl_gather_meta_bytes() {
     return (size_t)NS * 8 + (size_t)NS * 4 + (size_t)(NS + 1) * 4 + (size_t)pkl_max_items<N>() * 2;
 }
 
-// LDS of k_pkl_step: tile bitmap, child tables, the parent queues (expansion)
-// sharing their space with the gather metadata (gather), list fill counts.
+// LDS of k_pkl_step: tile bitmap, (1024-thread blocks) the tile's new
+// frontier words, child tables, the parent queues (expansion) sharing their
+// space with the gather metadata (gather), list fill counts.
 template <int N>
 inline size_t pkl_step_lds(int tpb) {
     const size_t queues = (size_t)(tpb / 64) * PKL_QCAP * 4;
     const size_t meta = pkl_gather_meta_bytes<N>();
-    return (size_t)2 * pkl::kTileWords * 4 + PKL_TAB_BYTES + (queues > meta ? queues : meta) +
+    const size_t front = tpb >= 1024 ? (size_t)pkl::kTileWords * 8 : 0;
+    return (size_t)2 * pkl::kTileWords * 4 + front + PKL_TAB_BYTES + (queues > meta ? queues : meta) +
            (size_t)pkl::Geo<N>::NSLOT * 4;
 }
 
@@ -87,11 +89,11 @@ struct PklExpLds {
 };
 
 // Expands the parents of tile T whose bits are in frontier words [pw0, pw1)
-// (tile words start at w0): flips of <= 9 elements into lds.win, flips of
-// >= 10 elements appended to the tile's lists.  Wave-synchronous only; the
-// caller brackets it with block barriers.
-template <int N, int TPB>
-__device__ __forceinline__ void pkl_expand_range(const uint64_t* __restrict__ front, uint64_t w0, uint64_t pw0,
+// (tile words start at w0; front(i) reads frontier word i): flips of <= 9
+// elements into lds.win, flips of >= 10 elements appended to the tile's lists.
+// Wave-synchronous only; the caller brackets it with block barriers.
+template <int N, int TPB, class Front>
+__device__ __forceinline__ void pkl_expand_range(Front front, uint64_t w0, uint64_t pw0,
                                                  uint64_t pw1, const PklExpLds& lds, const pkl::TileK& tk,
                                                  uint32_t* lists_t, unsigned long long& parents, uint32_t& err,
                                                  bool& any) {
@@ -132,7 +134,7 @@ __device__ __forceinline__ void pkl_expand_range(const uint64_t* __restrict__ fr
     int qn = 0;  // wave-uniform queue fill
     for (uint64_t g = pw0 + (uint64_t)wave * 64; g < pw1; g += (uint64_t)NW * 64) {
         const uint64_t wi = g + lane;
-        uint64_t bits = (wi < pw1) ? front[wi] : 0ull;
+        uint64_t bits = (wi < pw1) ? front(wi) : 0ull;
         const uint32_t c = (uint32_t)__popcll(bits);
         const uint32_t incl = wave_incl_scan(c, lane);
         const int total = (int)__shfl(incl, 63, 64);
@@ -270,11 +272,13 @@ __device__ __forceinline__ void pkl_gather(uint32_t t, uint32_t* counts, const u
     __syncthreads();
 }
 
-// Takes the next tile: a plain read first, so blocks arriving after the last
-// tile make no atomic.  Thread 0 only.
+// Takes the next of n tiles (~0 when none is left): a plain read first, so
+// blocks arriving after the last tile make no atomic.  Thread 0 only.
 __device__ __forceinline__ unsigned long long pkl_take_tile(unsigned long long* counter, unsigned long long n) {
     const unsigned long long taken = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return taken >= n ? ~0ull : atomicAdd(counter, 1ull);
+    if (taken >= n) return ~0ull;
+    const unsigned long long k = atomicAdd(counter, 1ull);  // other blocks may have taken the rest
+    return k < n ? k : ~0ull;
 }
 
 template <int N, int TPB>
@@ -355,7 +359,8 @@ __global__ __launch_bounds__(TPB) void k_pkl_expand(ExpandArgs a, ListArgs x) {
         __syncthreads();
         if (tid == 0) scan += pw1 - pw0;
         bool any = false;
-        pkl_expand_range<N, TPB>(cur, w0, pw0, pw1, lds, tk, lists_t, parents, err, any);
+        pkl_expand_range<N, TPB>([&](uint64_t i) { return cur[i]; }, w0, pw0, pw1, lds, tk, lists_t, parents, err,
+                                 any);
         if (any) s_local = 1;
         __syncthreads();
         // a tile expanded whole in one visit (batched mode) owns its range of
@@ -447,24 +452,31 @@ __global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
 //     bitmap, stored into the array cleared in step 1 (the following layer's
 //     next), flips of >= 10 elements into the lists of the other parity.
 // Every dependency between the two steps is inside the tile; across tiles they
-// meet only through lists completed by the previous launch.  The first launch
-// (first = 1) expands the start layer.  Layer L (st->layer) is the last
+// meet only through lists completed by the previous launch.  The launch visits
+// the st->active_tiles tiles of a.active (k_pkl_layer_end lists the tiles
+// that have anything to finalize or clear); the first launch (first = 1)
+// expands the start layer, in a.first_tile.  Layer L (st->layer) is the last
 // finalized layer; launch m finalizes and expands layer m = L + 1.
-// Four waves per SIMD (16 per CU): two 512-thread blocks share a CU, as the
-// LDS allows (at most 128 VGPRs; unbounded, the compiler takes more and
-// leaves one block per CU).
+// Four waves per SIMD (16 per CU): one 1024-thread block or two 512-thread
+// blocks per CU, as the LDS allows (at most 128 VGPRs; unbounded, the compiler
+// takes more and leaves one 512-thread block per CU).  1024-thread blocks keep
+// the finalized frontier words in LDS and expand from there.
 template <int N, int TPB>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_pkl_step(StepArgs a) {
     using G = pkl::Geo<N>;
     constexpr int NW = TPB / 64;
     constexpr uint32_t WIN = 2 * pkl::kTileWords;
+    constexpr bool FRONT_LDS = TPB >= 1024;
+    static_assert(pkl::kTileWords % 2 == 0, "tile words are finalized in pairs");
     extern __shared__ uint32_t smem[];
     BfsDevState* st = a.st;
     if (st->done) return;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     PklExpLds lds;
     lds.win = smem;
-    pkl::Tab6* tab6 = reinterpret_cast<pkl::Tab6*>(smem + WIN);
+    // WIN * 4 bytes is a multiple of 16: every region below is 16-byte aligned
+    uint64_t* flds = FRONT_LDS ? reinterpret_cast<uint64_t*>(smem + WIN) : nullptr;
+    pkl::Tab6* tab6 = reinterpret_cast<pkl::Tab6*>(smem + WIN + (FRONT_LDS ? WIN : 0));
     uint32_t* tab6lo = reinterpret_cast<uint32_t*>(tab6 + 720);
     uint32_t* tab3 = tab6lo + 720;
     lds.tab6 = tab6;
@@ -479,11 +491,13 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     uint8_t* meta = reinterpret_cast<uint8_t*>(shared_region);  // tab3 + 504 words: 8-byte aligned
     const bool first = a.first != 0;
     const uint32_t L = st->layer;
+    const uint32_t nvisit = first ? (a.first_tile < a.tiles ? 1u : 0u) : st->active_tiles;
     const uint64_t* cur = a.bits[L % 3];
     uint64_t* nw = a.bits[(L + 1) % 3];
     uint64_t* old = a.bits[(L + 2) % 3];
     // the expansion: frontier, its next array, list parity
     const uint64_t* front = first ? a.bits[0] : nw;
+    const bool front_lds = FRONT_LDS && !first;
     uint32_t* enext = reinterpret_cast<uint32_t*>(first ? a.bits[1] : old);
     const int gp = (int)(L & 1u), xp = first ? 0 : (int)((L + 1u) & 1u);
     uint32_t* tc_new = a.tcount + (uint64_t)(first ? 0 : (L + 1) % 3) * a.tiles;
@@ -492,14 +506,19 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ uint32_t s_xf, s_wf, s_oldc, s_local;
     __shared__ unsigned long long s_red[2][NW];
     __shared__ unsigned long long s_cnt;
+    // per-block tallies kept by thread 0 (scan, fin_full, fin_clear, count, hash):
+    // in LDS rather than in registers every thread would carry across the tile loop
+    __shared__ unsigned long long s_tally[5];
+    if (tid < 5) s_tally[tid] = 0;
     bool tables = false;
-    unsigned long long parents = 0, scan = 0, fin_full = 0, fin_clear = 0, lcount = 0, lhash = 0;
+    unsigned long long parents = 0;
     uint32_t err = 0;
     // the first launch has no bookkeeping launch after it to reset tile_counter
     unsigned long long* counter = first ? &st->chunk_counter : &st->tile_counter;
     for (;;) {
         if (tid == 0) {
-            s_tile = pkl_take_tile(counter, a.tiles);
+            const unsigned long long k = pkl_take_tile(counter, nvisit);
+            s_tile = k == ~0ull ? ~0ull : (first ? (unsigned long long)a.first_tile : (unsigned long long)a.active[k]);
             if (s_tile != ~0ull) {
                 s_xf = first ? 0u : a.xflags[gp][s_tile];
                 s_wf = first ? 0u : a.wflags[gp][s_tile];
@@ -519,41 +538,33 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
                 for (uint32_t i = tid; i < WIN; i += TPB) lds.win[i] = 0;
                 __syncthreads();
                 if (s_xf) pkl_gather<N, TPB>(T, a.counts[gp], a.lists[gp], lds.win, meta, err);
-                const uint64_t* gw = reinterpret_cast<const uint64_t*>(lds.win);
-                // two words per thread in flight before any store (the arrays are
-                // distinct); four push the kernel past 128 VGPRs into spills
-                constexpr int U = 2;
-                for (uint32_t j0 = tid; j0 < pkl::kTileWords; j0 += TPB * U) {
-                    uint64_t vn[U], vo[U], vc[U];
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const uint32_t j = j0 + u * TPB;
-                        if (j < pkl::kTileWords) {
-                            vn[u] = nw[w0 + j];
-                            vo[u] = old[w0 + j];
-                            vc[u] = a.odd ? cur[w0 + j] : 0ull;
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const uint32_t j = j0 + u * TPB;
-                        if (j < pkl::kTileWords) {
-                            const uint64_t v = (vn[u] | gw[j]) & ~vo[u] & ~vc[u];
-                            nw[w0 + j] = v;
-                            old[w0 + j] = 0;
-                            c += (unsigned long long)__popcll(v);
-                            if (a.hash) h += word_hash(w0 + j, v);
-                        }
-                    }
+                // pairs of words (w0 is even: 16-byte aligned), one pair per
+                // thread and step; more in flight push the kernel past 128 VGPRs
+                const ulonglong2* gw = reinterpret_cast<const ulonglong2*>(lds.win);
+                ulonglong2* nw2 = reinterpret_cast<ulonglong2*>(nw + w0);
+                ulonglong2* old2 = reinterpret_cast<ulonglong2*>(old + w0);
+                const ulonglong2* cur2 = reinterpret_cast<const ulonglong2*>(cur + w0);
+                for (uint32_t q = tid; q < pkl::kTileWords / 2; q += TPB) {
+                    const ulonglong2 vn = nw2[q], vo = old2[q];
+                    const ulonglong2 vc = a.odd ? cur2[q] : make_ulonglong2(0ull, 0ull);
+                    const ulonglong2 g = gw[q];
+                    ulonglong2 v;
+                    v.x = (vn.x | g.x) & ~vo.x & ~vc.x;
+                    v.y = (vn.y | g.y) & ~vo.y & ~vc.y;
+                    nw2[q] = v;
+                    old2[q] = make_ulonglong2(0ull, 0ull);
+                    if (FRONT_LDS) reinterpret_cast<ulonglong2*>(flds)[q] = v;
+                    c += (unsigned long long)(__popcll(v.x) + __popcll(v.y));
+                    if (a.hash) h += word_hash(w0 + 2 * q, v.x) + word_hash(w0 + 2 * q + 1, v.y);
                 }
                 if (tid == 0) {
                     a.xflags[gp][T] = 0;
                     a.wflags[gp][T] = 0;
-                    fin_full += pkl::kTileWords;
+                    s_tally[1] += pkl::kTileWords;
                 }
             } else if (s_oldc) {
                 for (uint64_t j = tid; j < pkl::kTileWords; j += TPB) old[w0 + j] = 0;
-                if (tid == 0) fin_clear += pkl::kTileWords;
+                if (tid == 0) s_tally[2] += pkl::kTileWords;
             }
             c = wave_sum(c);
             h = wave_sum(h);
@@ -569,8 +580,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
                     sh += s_red[1][w];
                 }
                 tc_new[T] = (uint32_t)sc;
-                lcount += sc;
-                lhash += sh;
+                s_tally[3] += sc;
+                s_tally[4] += sh;
                 s_cnt = sc;
             }
             __syncthreads();
@@ -592,10 +603,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
             const pkl::TileK tk = pkl::tile_consts<N>(dp);
             uint32_t* lists_t = a.lists[xp] + (uint64_t)T * G::CROSS * pkl::kTile;
             __syncthreads();
-            if (tid == 0) scan += pkl::kTileWords;
+            if (tid == 0) s_tally[0] += pkl::kTileWords;
             bool any = false;
             unsigned long long tile_parents = 0;
-            pkl_expand_range<N, TPB>(front, w0, w0, w1, lds, tk, lists_t, tile_parents, err, any);
+            pkl_expand_range<N, TPB>(
+                [&](uint64_t i) { return front_lds ? flds[i - w0] : front[i]; }, w0, w0, w1, lds, tk, lists_t,
+                tile_parents, err, any);
             parents += tile_parents;
             if (any) s_local = 1;
             __syncthreads();
@@ -621,10 +634,58 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
         __syncthreads();
     }
     if (err) atomicOr(&st->error, err);
-    block_add3<TPB>(parents * (unsigned long long)(N - 1), parents, scan, &st->children, &st->expanded,
-                    &st->scan_words);
-    block_add3<TPB>(lcount, lhash, 0ull, &st->acc_count, &st->acc_hash, nullptr);
-    block_add3<TPB>(fin_full, fin_clear, 0ull, &st->fin_full_words, &st->fin_clear_words, nullptr);
+    if (tid == 0) {
+        if (s_tally[1]) atomicAdd(&st->fin_full_words, s_tally[1]);
+        if (s_tally[2]) atomicAdd(&st->fin_clear_words, s_tally[2]);
+        if (s_tally[3]) atomicAdd(&st->acc_count, s_tally[3]);
+        if (s_tally[4]) atomicAdd(&st->acc_hash, s_tally[4]);
+    }
+    block_add3<TPB>(parents * (unsigned long long)(N - 1), parents, tid == 0 ? s_tally[0] : 0ull, &st->children,
+                    &st->expanded, &st->scan_words);
+}
+
+// Bookkeeping between two launches of k_pkl_step (one block): with advance,
+// the layer end of every batched search (layer_end_body); then, unless the
+// search is done, the list of tiles the next launch visits -- those that a
+// list or their own expansion reached, or whose old words hold states to
+// clear -- in tile order, and a zero frontier count for every other tile.
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_pkl_layer_end(LayerEndArgs a, StepArgs s, int advance) {
+    constexpr int NW = TPB / 64;
+    BfsDevState* st = a.st;
+    if (advance) layer_end_body<TPB>(a);
+    __syncthreads();
+    if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const uint32_t L = __hip_atomic_load(&st->layer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int gp = (int)(L & 1u);
+    const uint8_t* xf = s.xflags[gp];
+    const uint8_t* wf = s.wflags[gp];
+    const uint32_t* tc_old = s.tcount + (uint64_t)((L + 2) % 3) * s.tiles;
+    uint32_t* tc_new = s.tcount + (uint64_t)((L + 1) % 3) * s.tiles;
+    __shared__ uint32_t s_wsum[NW];
+    uint32_t base = 0;  // active tiles listed so far (uniform)
+    for (uint32_t t0 = 0; t0 < s.tiles; t0 += TPB) {
+        const uint32_t T = t0 + (uint32_t)tid;
+        bool act = false;
+        if (T < s.tiles) {
+            act = (xf[T] | wf[T]) != 0 || tc_old[T] != 0;
+            if (!act) tc_new[T] = 0;
+        }
+        const unsigned long long m = __ballot(act);
+        if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t off = base, total = 0;
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = s_wsum[w];
+            if (w < wave) off += c;
+            total += c;
+        }
+        if (act) s.active[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = T;
+        base += total;
+        __syncthreads();
+    }
+    if (tid == 0) st->active_tiles = base;
 }
 
 }  // namespace psk

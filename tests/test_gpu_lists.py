@@ -139,5 +139,6 @@ def test_lists_footprint(ps):
     tiles = math.factorial(n) // math.factorial(9)
     one = tiles * (n - 9) * math.factorial(9) * 4 + tiles * 585 * 4 + tiles  # lists, counts, flags
     assert plain - base == one
-    # batched: lists of two layer parities, per-tile window flags and frontier counts
-    assert lists - base == 2 * one + 2 * tiles + 12 * tiles
+    # batched: lists of two layer parities, per-tile window flags, frontier
+    # counts and the list of tiles a launch visits
+    assert lists - base == 2 * one + 2 * tiles + 12 * tiles + 4 * tiles
