@@ -726,7 +726,7 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
             b->lds_expand = pkl_expand_lds_bytes(*p, b->tpb_expand, b->nseg);
             LaunchCfg c{b->tpb_expand, 0, b->lds_expand, b->stream};
             if (int r = pkl_expand(*p, psk::ExpandArgs{}, psk::ListArgs{}, c, &occ)) return cleanup(r);
-            b->lds_apply = pkl_apply_lds_bytes(*p);
+            b->lds_apply = pkl_apply_lds_bytes(*p, b->tpb_apply);
             int occ_a = 0;
             LaunchCfg ca{b->tpb_apply, 0, b->lds_apply, b->stream};
             if (int r = pkl_apply(*p, psk::ApplyArgs{}, ca, &occ_a)) return cleanup(r);

@@ -34,7 +34,7 @@ int pkl_step(const ps_puzzle& p, const psk::StepArgs& a, const LaunchCfg& c, int
 int pkl_layer_end(const psk::LayerEndArgs& a, const psk::StepArgs& s, int advance, hipStream_t stream);
 // LDS bytes of the kernels; 0 when the puzzle has no list kernels
 size_t pkl_expand_lds_bytes(const ps_puzzle& p, int tpb, uint32_t nseg);
-size_t pkl_apply_lds_bytes(const ps_puzzle& p);
+size_t pkl_apply_lds_bytes(const ps_puzzle& p, int tpb);
 size_t pkl_step_lds_bytes(const ps_puzzle& p, int tpb);
 
 inline int bfs3_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const LaunchCfg& c, int* occ) {

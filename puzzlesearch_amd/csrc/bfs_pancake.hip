@@ -96,12 +96,12 @@ size_t pkl_expand_lds_bytes(const ps_puzzle& p, int tpb, uint32_t nseg) {
     return 0;
 }
 
-size_t pkl_apply_lds_bytes(const ps_puzzle& p) {
+size_t pkl_apply_lds_bytes(const ps_puzzle& p, int tpb) {
     if (!pkl_size(p)) return 0;
     switch (p.size) {
-    case 10: return psk::pkl_apply_lds<10>();
-    case 11: return psk::pkl_apply_lds<11>();
-    case 12: return psk::pkl_apply_lds<12>();
+    case 10: return psk::pkl_apply_lds<10>(tpb);
+    case 11: return psk::pkl_apply_lds<11>(tpb);
+    case 12: return psk::pkl_apply_lds<12>(tpb);
     }
     return 0;
 }

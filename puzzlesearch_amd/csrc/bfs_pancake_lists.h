@@ -39,31 +39,39 @@ inline size_t pkl_expand_lds(int tpb, uint32_t nseg) {
     return b;
 }
 
-// Work items of k_pkl_apply: 1024 list entries, 16 per lane read as four
-// 16-byte vectors (list starts are 16-byte aligned and every capacity is a
-// multiple of 16, so a vector that starts below the fill count stays inside
-// its list).  A tile has at most 3*9!/1024 + NSLOT work items.
-constexpr uint32_t PKL_ITEM = 1024;
-template <int N>
-constexpr uint32_t pkl_max_items() {
-    return (uint32_t)(pkl::Geo<N>::CROSS * pkl::kTile / PKL_ITEM) + pkl::Geo<N>::NSLOT + 1;
+// Work items of a gather: ITEM list entries of one list, read as 16-byte
+// vectors (list starts are 16-byte aligned and every capacity is a multiple
+// of 16, so a vector that starts below the fill count stays inside its list);
+// a wave reads four vectors per lane at a time: one item of 1024 entries, or
+// (1024-thread blocks, whose LDS holds the longer item table) four items of
+// 256 entries, mostly of four different lists -- a flip-12 list holds at most
+// 6! entries, and more lists in flight hide more of the memory latency.  A
+// tile has at most 3*9!/ITEM + NSLOT work items.
+template <int TPB>
+constexpr uint32_t pkl_item() {
+    return TPB >= 1024 ? 256u : 1024u;
 }
-
-// LDS of k_pkl_apply: tile bitmap, per incoming list its start, count and
-// first work item, and the list of every work item
-template <int N>
-inline size_t pkl_apply_lds() {
-    constexpr uint32_t NS = pkl::Geo<N>::NSLOT;
-    return (size_t)2 * pkl::kTileWords * 4 + (size_t)NS * 8 + (size_t)NS * 4 + (size_t)(NS + 1) * 4 +
-           (size_t)pkl_max_items<N>() * 2;
+template <int N, uint32_t ITEM>
+constexpr uint32_t pkl_max_items() {
+    return (uint32_t)(pkl::Geo<N>::CROSS * pkl::kTile / ITEM) + pkl::Geo<N>::NSLOT + 1;
 }
 
 // Bytes of the per-list metadata of a gather: start, count, first work item,
 // and the list of every work item.
-template <int N>
+template <int N, uint32_t ITEM>
 constexpr size_t pkl_gather_meta_bytes() {
     constexpr uint32_t NS = pkl::Geo<N>::NSLOT;
-    return (size_t)NS * 8 + (size_t)NS * 4 + (size_t)(NS + 1) * 4 + (size_t)pkl_max_items<N>() * 2;
+    return (size_t)NS * 8 + (size_t)NS * 4 + (size_t)(NS + 1) * 4 + (size_t)pkl_max_items<N, ITEM>() * 2;
+}
+template <int N>
+inline size_t pkl_gather_meta_bytes(int tpb) {
+    return tpb >= 1024 ? pkl_gather_meta_bytes<N, pkl_item<1024>()>() : pkl_gather_meta_bytes<N, pkl_item<512>()>();
+}
+
+// LDS of k_pkl_apply: tile bitmap and the gather metadata
+template <int N>
+inline size_t pkl_apply_lds(int tpb) {
+    return (size_t)2 * pkl::kTileWords * 4 + pkl_gather_meta_bytes<N>(tpb);
 }
 
 // LDS of k_pkl_step: tile bitmap, (1024-thread blocks) the tile's new
@@ -72,7 +80,7 @@ constexpr size_t pkl_gather_meta_bytes() {
 template <int N>
 inline size_t pkl_step_lds(int tpb) {
     const size_t queues = (size_t)(tpb / 64) * PKL_QCAP * 4;
-    const size_t meta = pkl_gather_meta_bytes<N>();
+    const size_t meta = pkl_gather_meta_bytes<N>(tpb);
     const size_t front = tpb >= 1024 ? (size_t)pkl::kTileWords * 8 : 0;
     return (size_t)2 * pkl::kTileWords * 4 + front + PKL_TAB_BYTES + (queues > meta ? queues : meta) +
            (size_t)pkl::Geo<N>::NSLOT * 4;
@@ -102,33 +110,53 @@ __device__ __forceinline__ void pkl_expand_range(Front front, uint64_t w0, uint6
     uint32_t* win = lds.win;
     uint32_t* q = lds.q;
     uint32_t* cnt = lds.cnt;
-    auto expand_one = [&](uint32_t off) {
+    constexpr int CROSS = pkl::Geo<N>::CROSS;
+    auto expand_one = [&](uint32_t off, const pkl::TabRow& row) {
         ++parents;
         any = true;
-        pkl::children_tab<N>(
-            lds.tab6, lds.tab6lo, lds.tab3, tk, off, [&](uint32_t c) { lor(&win[c >> 5], 1u << (c & 31)); },
+        uint32_t xs[CROSS], xc[CROSS];
+        pkl::children_row<N>(
+            row, tk, off, [&](uint32_t c) { lor(&win[c >> 5], 1u << (c & 31)); },
             [&](int k, uint32_t slot, uint32_t child) {
-                const int f = k - 10;
-                if (slot >= pkl::kSlots[f]) {
-                    err |= 2u;
-                    return;
-                }
-                const uint32_t pos = __hip_atomic_fetch_add(&cnt[pkl::kSlotBase[f] + slot], 1u, __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (pos >= pkl::kCap[f]) {
-                    err |= 2u;
-                    return;
-                }
-                lists_t[(uint32_t)f * pkl::kTile + slot * pkl::kCap[f] + pos] = child;
+                xs[k - 10] = slot;
+                xc[k - 10] = child;
             });
+        // the fill-count atomics of the cross flips back to back, under one wait
+        // (a slot out of range, which slot_of never returns, counts on the last
+        // slot and is reported, its child not stored)
+        uint32_t pos[CROSS];
+#pragma unroll
+        for (int f = 0; f < CROSS; ++f) {
+            const uint32_t sl = xs[f] < pkl::kSlots[f] ? xs[f] : pkl::kSlots[f] - 1u;
+            pos[f] = __hip_atomic_fetch_add(&cnt[pkl::kSlotBase[f] + sl], 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+#pragma unroll
+        for (int f = 0; f < CROSS; ++f) {
+            if (xs[f] < pkl::kSlots[f] && pos[f] < pkl::kCap[f])
+                lists_t[(uint32_t)f * pkl::kTile + xs[f] * pkl::kCap[f] + pos[f]] = xc[f];
+            else
+                err |= 2u;
+        }
     };
+    auto row_of = [&](uint32_t off) { return pkl::tab_row(lds.tab6, lds.tab6lo, lds.tab3, off); };
     auto run_batch = [&](uint32_t n) {
         if (n == (uint32_t)PKL_QCAP) {
             constexpr uint32_t S = PKL_QCAP / 64;
+            uint32_t off = q[lane * S];
+            pkl::TabRow row = row_of(off);
 #pragma unroll 1
-            for (uint32_t r = 0; r < S; ++r) expand_one(q[lane * S + r]);
+            for (uint32_t r = 0; r < S; ++r) {
+                // the next parent's table row is read before this parent is
+                // expanded (the last step reads its own row again)
+                const uint32_t off_n = q[lane * S + (r + 1 < S ? r + 1 : r)];
+                const pkl::TabRow row_n = row_of(off_n);
+                expand_one(off, row);
+                off = off_n;
+                row = row_n;
+            }
         } else {
-            for (uint32_t j = lane; j < n; j += 64) expand_one(q[j]);
+            for (uint32_t j = lane; j < n; j += 64) expand_one(q[j], row_of(q[j]));
         }
     };
     int qn = 0;  // wave-uniform queue fill
@@ -182,12 +210,13 @@ __device__ __forceinline__ void pkl_store_counts(uint32_t T, const uint32_t* cnt
 // Gathers every list that reaches destination tile t into win (which the
 // caller has cleared), clearing the lists' fill counts.  Every thread of the
 // block calls it; it ends with a block barrier.  meta: LDS of
-// pkl_gather_meta_bytes<N>() bytes, 8-byte aligned.
+// pkl_gather_meta_bytes<N, pkl_item<TPB>()>() bytes, 8-byte aligned.
 template <int N, int TPB>
 __device__ __forceinline__ void pkl_gather(uint32_t t, uint32_t* counts, const uint32_t* lists, uint32_t* win,
                                            uint8_t* meta, uint32_t& err) {
     using G = pkl::Geo<N>;
     constexpr int NW = TPB / 64;
+    constexpr uint32_t ITEM = pkl_item<TPB>();
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     unsigned long long* lbase = reinterpret_cast<unsigned long long*>(meta);
     uint32_t* lcnt = reinterpret_cast<uint32_t*>(lbase + G::NSLOT);
@@ -215,7 +244,7 @@ __device__ __forceinline__ void pkl_gather(uint32_t t, uint32_t* counts, const u
         uint32_t run = 0;
         for (uint32_t b = 0; b < G::NSLOT; b += 64) {
             const uint32_t i = b + lane;
-            const uint32_t c = i < G::NSLOT ? (lcnt[i] + PKL_ITEM - 1) / PKL_ITEM : 0u;
+            const uint32_t c = i < G::NSLOT ? (lcnt[i] + ITEM - 1) / ITEM : 0u;
             const uint32_t incl = wave_incl_scan(c, lane);
             if (i < G::NSLOT) lstart[i] = run + incl - c;
             run += __shfl(incl, 63, 64);
@@ -229,21 +258,14 @@ __device__ __forceinline__ void pkl_gather(uint32_t t, uint32_t* counts, const u
         for (uint32_t w = lstart[i]; w < lstart[i + 1]; ++w) item_list[w] = (uint16_t)i;
     __syncthreads();
     const uint32_t tbase = t * pkl::kTile;
-    for (uint32_t w = wave; w < total; w += NW) {
-        const uint32_t li = __builtin_amdgcn_readfirstlane(item_list[w]);
-        const uint32_t e0 = (w - lstart[li]) * PKL_ITEM;
-        const uint32_t n = lcnt[li];
-        const uint4* src = reinterpret_cast<const uint4*>(lists + lbase[li] + e0);
-        uint4 v[4];
+    constexpr int UPI = (int)(ITEM / 256);  // vectors per lane per item
+    constexpr int GRP = 4 / UPI;            // items per wave and step
+    // applies vectors v[0..UPI) of the item that starts at entry e0 of a list of n
+    auto apply = [&](const uint4* v, uint32_t e0, uint32_t n) {
+        if (e0 + ITEM <= n) {
+            // a full item, branch-free: an entry outside the tile ORs no bit
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t e = e0 + 4u * (lane + 64u * u);
-            v[u] = (e < n) ? src[lane + 64 * u] : make_uint4(0u, 0u, 0u, 0u);
-        }
-        if (e0 + PKL_ITEM <= n) {
-            // a full work item, branch-free: an entry outside the tile ORs no bit
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < UPI; ++u) {
                 const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
@@ -254,20 +276,57 @@ __device__ __forceinline__ void pkl_gather(uint32_t t, uint32_t* counts, const u
                 }
             }
         } else {
+            // the last item of a list: only the groups of 256 entries that hold
+            // any, each entry tested without a branch (an entry past the end
+            // ORs no bit)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < UPI; ++u) {
+                if (e0 + 256u * u >= n) break;  // wave-uniform
                 const uint32_t e = e0 + 4u * (lane + 64u * u);
                 const uint32_t ev[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    if (e + m < n) {
-                        const uint32_t o = ev[m] - tbase;
-                        if (o < pkl::kTile) lor(&win[o >> 5], 1u << (o & 31));
-                        else err |= 4u;
-                    }
+                    const uint32_t o = ev[m] - tbase;
+                    const bool live = e + m < n;
+                    const bool ok = live && o < pkl::kTile;
+                    err |= (live && !ok) ? 4u : 0u;
+                    lor(&win[ok ? (o >> 5) : (uint32_t)lane], ok ? (1u << (o & 31)) : 0u);
                 }
             }
         }
+    };
+    // wave w takes items w, w + NW, .. w + (GRP-1)*NW, then moves on by GRP*NW;
+    // every load of a step is issued before its first item is applied
+    for (uint32_t w = wave; w < total; w += (uint32_t)(GRP * NW)) {
+        uint4 v[4];
+        uint32_t e0[GRP], n[GRP];
+#pragma unroll
+        for (int g = 0; g < GRP; ++g) {
+            const uint32_t wi = w + (uint32_t)(g * NW);
+            e0[g] = 0;
+            n[g] = 0;
+            if (wi < total) {  // wave-uniform
+                // the item's list, start and length are wave-uniform: scalar registers
+                const uint32_t li = __builtin_amdgcn_readfirstlane(item_list[wi]);
+                e0[g] = __builtin_amdgcn_readfirstlane((wi - lstart[li]) * ITEM);
+                n[g] = __builtin_amdgcn_readfirstlane(lcnt[li]);
+                const unsigned long long lb = lbase[li];
+                const uint64_t first = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(lb >> 32)) << 32) |
+                                       __builtin_amdgcn_readfirstlane((uint32_t)lb);
+                const uint4* src = reinterpret_cast<const uint4*>(lists + first + e0[g]);
+#pragma unroll
+                for (int u = 0; u < UPI; ++u) {
+                    const uint32_t e = e0[g] + 4u * (lane + 64u * u);
+                    v[g * UPI + u] = (e < n[g]) ? src[lane + 64 * u] : make_uint4(0u, 0u, 0u, 0u);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < UPI; ++u) v[g * UPI + u] = make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < GRP; ++g)
+            if (n[g]) apply(v + g * UPI, e0[g], n[g]);
     }
     __syncthreads();
 }
@@ -485,7 +544,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     uint32_t* shared_region = tab3 + 504;  // queues (expansion) or gather metadata
     lds.q = shared_region + wave * PKL_QCAP;
     {
-        const size_t qbytes = (size_t)NW * PKL_QCAP * 4, mbytes = pkl_gather_meta_bytes<N>();
+        const size_t qbytes = (size_t)NW * PKL_QCAP * 4, mbytes = pkl_gather_meta_bytes<N, pkl_item<TPB>()>();
         lds.cnt = shared_region + (qbytes > mbytes ? qbytes : mbytes) / 4;
     }
     uint8_t* meta = reinterpret_cast<uint8_t*>(shared_region);  // tab3 + 504 words: 8-byte aligned

@@ -367,17 +367,28 @@ struct Sweep {
     }
 };
 
-// children() with the tables: tab6[l] (and tab6lo[l] = digits of slots 4, 5)
-// for l < 720, tab3[h] for h < 504.
-template <int N, class TA, class TB, class TC, class FI, class FX>
-PKL_HD inline void children_tab(const TA& tab6, const TB& tab6lo, const TC& tab3, const TileK& tk, uint32_t off,
-                                FI&& in, FX&& cross) {
+// The table entries of one parent: tab6[l], tab6lo[l] and tab3[h] for the
+// tile offset off = h*720 + l.
+struct TabRow {
+    Tab6 t;
+    uint32_t lo, hd;
+};
+template <class TA, class TB, class TC>
+PKL_HD inline TabRow tab_row(const TA& tab6, const TB& tab6lo, const TC& tab3, uint32_t off) {
+    const uint32_t h = off / 720u;
+    const uint32_t l = off - h * 720u;
+    return TabRow{tab6[l], tab6lo[l], tab3[h]};
+}
+
+// children() from the parent's table row (tab_row of the same offset).
+template <int N, class FI, class FX>
+PKL_HD inline void children_row(const TabRow& row, const TileK& tk, uint32_t off, FI&& in, FX&& cross) {
     constexpr int P = N - 9;
     const uint32_t h = off / 720u;
     const uint32_t l = off - h * 720u;
-    const Tab6 t = tab6[l];
-    const uint32_t lo = tab6lo[l];
-    const uint32_t hd = tab3[h];
+    const Tab6& t = row.t;
+    const uint32_t lo = row.lo;
+    const uint32_t hd = row.hd;
     uint32_t d[3], cc[3], e[3];
     d[0] = t.d0;
     d[1] = lo | (hd << 16);             // slots 4, 5 | slots 6, 7 (positions N-7, N-8)
@@ -403,6 +414,14 @@ PKL_HD inline void children_tab(const TA& tab6, const TB& tab6lo, const TC& tab3
     uint32_t r1 = 0, r2 = 0, r3 = 0;
     Sweep<N, N - 7>::run(cc, d, e, base, tk, r1, r2, r3, in, cross);
     (void)P;
+}
+
+// children() with the tables: tab6[l] (and tab6lo[l] = digits of slots 4, 5)
+// for l < 720, tab3[h] for h < 504.
+template <int N, class TA, class TB, class TC, class FI, class FX>
+PKL_HD inline void children_tab(const TA& tab6, const TB& tab6lo, const TC& tab3, const TileK& tk, uint32_t off,
+                                FI&& in, FX&& cross) {
+    children_row<N>(tab_row(tab6, tab6lo, tab3, off), tk, off, in, cross);
 }
 
 // Prefix elements of tile t.
