@@ -707,7 +707,13 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
 // the layer end of every batched search (layer_end_body); then, unless the
 // search is done, the list of tiles the next launch visits -- those that a
 // list or their own expansion reached, or whose old words hold states to
-// clear -- in tile order, and a zero frontier count for every other tile.
+// clear -- and a zero frontier count for every other tile.  The list is
+// ordered by the tile's count in the layer just finalized, largest first (a
+// counting sort on 256 buckets of the largest count): a tile's work in the
+// next launch follows that count, and taking the longest tiles first
+// shortens the launch's tail.
+constexpr uint32_t PKL_MAX_TILES = 1320;  // 12!/9!
+static_assert(pkl::Geo<12>::TILES == PKL_MAX_TILES, "lists cover at most 12 pancakes");
 template <int TPB>
 __global__ __launch_bounds__(TPB) void k_pkl_layer_end(LayerEndArgs a, StepArgs s, int advance) {
     constexpr int NW = TPB / 64;
@@ -720,31 +726,53 @@ __global__ __launch_bounds__(TPB) void k_pkl_layer_end(LayerEndArgs a, StepArgs 
     const int gp = (int)(L & 1u);
     const uint8_t* xf = s.xflags[gp];
     const uint8_t* wf = s.wflags[gp];
+    const uint32_t* tc_cur = s.tcount + (uint64_t)(L % 3) * s.tiles;
     const uint32_t* tc_old = s.tcount + (uint64_t)((L + 2) % 3) * s.tiles;
     uint32_t* tc_new = s.tcount + (uint64_t)((L + 1) % 3) * s.tiles;
+    const uint32_t tiles = s.tiles < PKL_MAX_TILES ? s.tiles : PKL_MAX_TILES;  // the host checks tiles <= 1320
+    __shared__ uint16_t s_key[PKL_MAX_TILES];  // 0xffff: not visited, else 255 - bucket
+    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_max;
     __shared__ uint32_t s_wsum[NW];
-    uint32_t base = 0;  // active tiles listed so far (uniform)
-    for (uint32_t t0 = 0; t0 < s.tiles; t0 += TPB) {
-        const uint32_t T = t0 + (uint32_t)tid;
-        bool act = false;
-        if (T < s.tiles) {
-            act = (xf[T] | wf[T]) != 0 || tc_old[T] != 0;
-            if (!act) tc_new[T] = 0;
+    for (uint32_t i = tid; i < 256; i += TPB) s_start[i] = 0;
+    if (tid == 0) s_max = 0;
+    __syncthreads();
+    for (uint32_t T = tid; T < tiles; T += TPB) {
+        const bool act = (xf[T] | wf[T]) != 0 || tc_old[T] != 0;
+        if (act) {
+            atomicMax(&s_max, tc_cur[T]);
+        } else {
+            tc_new[T] = 0;
         }
-        const unsigned long long m = __ballot(act);
-        if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t off = base, total = 0;
-        for (int w = 0; w < NW; ++w) {
-            const uint32_t c = s_wsum[w];
-            if (w < wave) off += c;
-            total += c;
-        }
-        if (act) s.active[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = T;
-        base += total;
-        __syncthreads();
+        s_key[T] = act ? 0u : 0xffffu;
     }
-    if (tid == 0) st->active_tiles = base;
+    __syncthreads();
+    const uint32_t mx = s_max;
+    for (uint32_t T = tid; T < tiles; T += TPB) {
+        if (s_key[T] == 0xffffu) continue;
+        const uint32_t b = (uint32_t)(((uint64_t)tc_cur[T] * 256u) / ((uint64_t)mx + 1u));  // 0..255
+        s_key[T] = (uint16_t)(255u - b);
+        atomicAdd(&s_start[255u - b], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the bucket sizes (wave 0), largest counts first
+    if (wave == 0) {
+        uint32_t run = 0;
+        for (uint32_t b0 = 0; b0 < 256; b0 += 64) {
+            const uint32_t c = s_start[b0 + lane];
+            const uint32_t incl = wave_incl_scan(c, lane);
+            s_start[b0 + lane] = run + incl - c;
+            run += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) s_wsum[0] = run;
+    }
+    __syncthreads();
+    for (uint32_t T = tid; T < tiles; T += TPB) {
+        const uint32_t k = s_key[T];
+        if (k == 0xffffu) continue;
+        s.active[atomicAdd(&s_start[k], 1u)] = T;
+    }
+    if (tid == 0) st->active_tiles = s_wsum[0];
 }
 
 }  // namespace psk
