@@ -99,10 +99,44 @@ __device__ __forceinline__ uint32_t fs_live(uint32_t w, const unsigned long long
     return nz;
 }
 
+// One OR per flag word: consecutive children of a parent that share a word
+// (the horizontal moves of the sliding puzzles: index -+ 1) are merged before
+// the atomic.
+struct FsMarkSink {
+    uint32_t* flags;
+    uint8_t* dirty;
+    uint64_t segment, in_mask;
+    int seg_bits;
+    uint64_t pend_w;
+    uint32_t pend_m;
+    unsigned long long marks;
+    __device__ __forceinline__ void flush() {
+        if (!pend_m) return;
+        gor(&flags[pend_w], pend_m);
+        uint8_t* d = &dirty[pend_w >> (FS_REGION_BITS - 3)];
+        if (!*d) *d = 1;
+        pend_m = 0;
+    }
+    __device__ __forceinline__ void operator()(int, uint64_t child, uint32_t label) {
+        if ((child >> seg_bits) != segment) return;
+        const uint64_t c = child & in_mask;
+        const uint32_t m = (1u << label) << ((c & 7) * 4);
+        ++marks;
+        if ((c >> 3) == pend_w) {
+            pend_m |= m;
+            return;
+        }
+        flush();
+        pend_w = c >> 3;
+        pend_m = m;
+    }
+};
+
 template <class PZ, int TPB>
 __global__ __launch_bounds__(TPB) void k_fs_mark_in(FsMarkArgs a) {
     const uint64_t in_mask = (1ull << a.seg_bits) - 1ull;
-    unsigned long long marks = 0, bad = 0;
+    FsMarkSink sink{a.flags, a.dirty, a.segment, in_mask, a.seg_bits, ~0ull, 0u, 0ull};
+    unsigned long long bad = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * TPB) {
         const uint32_t id = a.idx[i];
         if ((uint64_t)id > in_mask) {  // not an id of this segment: reported after the step
@@ -110,131 +144,199 @@ __global__ __launch_bounds__(TPB) void k_fs_mark_in(FsMarkArgs a) {
             continue;
         }
         if (a.odd) atomicOr(&a.odd[id >> 6], 1ull << (id & 63));
-        PZ::expand(a.base | id, (uint32_t)a.ops[i], a.prm, [&](int, uint64_t child, uint32_t label) {
-            if ((child >> a.seg_bits) != a.segment) return;
-            const uint64_t c = child & in_mask;
-            gor(&a.flags[c >> 3], (1u << label) << ((c & 7) * 4));
-            uint8_t* d = &a.dirty[c >> FS_REGION_BITS];
-            if (!*d) *d = 1;
-            ++marks;
-        });
+        PZ::expand(a.base | id, (uint32_t)a.ops[i], a.prm, sink);
+        sink.flush();
+        sink.pend_w = ~0ull;
     }
-    block_add3<TPB>(marks, bad, 0ull, &a.counters[0], &a.counters[6], nullptr);
+    block_add3<TPB>(sink.marks, bad, 0ull, &a.counters[0], &a.counters[6], nullptr);
 }
 
 
-// pass 1 of ScanBitsAndClear: per region, entries counted and entries written
+// pass 1 of ScanBitsAndClear: per region, entries counted and entries written.
+// Persistent blocks take regions round-robin; a full region is read as
+// FS_REGION_WORDS/4/TPB 16-byte loads per thread, all issued before any is
+// used.  The totals go to the counters once per block.
 template <int TPB>
 __global__ __launch_bounds__(TPB) void k_fs_count(FsScanArgs a) {
-    const uint64_t r = blockIdx.x;
-    const uint64_t w0 = r * FS_REGION_WORDS;
-    const uint64_t rw = (a.nwords - w0 < FS_REGION_WORDS) ? a.nwords - w0 : FS_REGION_WORDS;
-    unsigned long long all = 0, out = 0;
-    if (a.dirty[r]) {
-        // four words per load; a region is a multiple of 4 words unless it is the
-        // whole (smaller) segment, whose tail is read word by word
-        const uint64_t rw4 = rw & ~3ull;
-        const uint4* f4 = reinterpret_cast<const uint4*>(a.flags_c + w0);
-        for (uint64_t j4 = threadIdx.x; j4 * 4 < rw4; j4 += TPB) {
-            const uint4 v = f4[j4];
-            const uint32_t ws[4] = {v.x, v.y, v.z, v.w};
+    constexpr uint32_t PER = FS_REGION_WORDS / 4 / TPB;
+    static_assert(PER * 4 * TPB == FS_REGION_WORDS, "a region is a whole number of block loads");
+    const uint64_t nregions = (a.nwords + FS_REGION_WORDS - 1) / FS_REGION_WORDS;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __shared__ uint32_t red[2][TPB / 64];
+    unsigned long long all = 0;       // per thread, over every region of the block
+    unsigned long long out_blk = 0, scanned = 0;  // thread 0
+    uint32_t par = 0;
+    for (uint64_t r = blockIdx.x; r < nregions; r += gridDim.x, par ^= 1u) {
+        const uint64_t w0 = r * FS_REGION_WORDS;
+        const uint64_t rw = (a.nwords - w0 < FS_REGION_WORDS) ? a.nwords - w0 : FS_REGION_WORDS;
+        uint32_t out = 0;
+        const bool dirty = a.dirty[r] != 0;
+        if (dirty && rw == FS_REGION_WORDS) {
+            const uint4* f4 = reinterpret_cast<const uint4*>(a.flags_c + w0);
+            uint4 v[PER];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                if (!ws[e]) continue;
-                const uint32_t live = fs_live(ws[e], a.odd, w0 + j4 * 4 + e);
+            for (uint32_t k = 0; k < PER; ++k) v[k] = f4[k * TPB + threadIdx.x];
+#pragma unroll
+            for (uint32_t k = 0; k < PER; ++k) {
+                const uint32_t ws[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (!ws[e]) continue;
+                    const uint32_t live = fs_live(ws[e], a.odd, w0 + ((uint64_t)k * TPB + threadIdx.x) * 4 + e);
+                    all += __popc(live);
+                    out += __popc(live & ~nib_full(ws[e], a.ops));
+                }
+            }
+        } else if (dirty) {  // the whole segment is smaller than a region
+            for (uint64_t j = threadIdx.x; j < rw; j += TPB) {
+                const uint32_t w = a.flags_c[w0 + j];
+                if (!w) continue;
+                const uint32_t live = fs_live(w, a.odd, w0 + j);
                 all += __popc(live);
-                out += __popc(live & ~nib_full(ws[e], a.ops));
+                out += __popc(live & ~nib_full(w, a.ops));
             }
         }
-        for (uint64_t j = rw4 + threadIdx.x; j < rw; j += TPB) {
-            const uint32_t w = a.flags_c[w0 + j];
-            if (!w) continue;
-            const uint32_t live = fs_live(w, a.odd, w0 + j);
-            all += __popc(live);
-            out += __popc(live & ~nib_full(w, a.ops));
+        out = wave_sum(out);
+        if (lane == 0) red[par][wave] = out;
+        __syncthreads();
+        // red[par] is written again two regions later, after the next barrier,
+        // which thread 0 reaches only once it has read it here
+        if (threadIdx.x == 0) {
+            uint32_t so = 0;
+#pragma unroll
+            for (int w = 0; w < TPB / 64; ++w) so += red[par][w];
+            a.cnt_out[r] = so;
+            out_blk += so;
+            if (dirty) scanned += rw;
         }
     }
-    all = wave_sum(all);
-    out = wave_sum(out);
-    __shared__ unsigned long long red[2][TPB / 64];
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    if (lane == 0) {
-        red[0][wave] = all;
-        red[1][wave] = out;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long sa = 0, so = 0;
-        for (int w = 0; w < TPB / 64; ++w) {
-            sa += red[0][w];
-            so += red[1][w];
-        }
-        a.cnt_out[r] = so;
-        if (a.dirty[r]) atomicAdd(&a.counters[5], (unsigned long long)rw);
-        if (sa) atomicAdd(&a.counters[1], sa);
-        if (so) atomicAdd(&a.counters[2], so);
-    }
+    block_add3<TPB>(all, threadIdx.x == 0 ? out_blk : 0ull, threadIdx.x == 0 ? scanned : 0ull, &a.counters[1],
+                    &a.counters[2], &a.counters[5]);
 }
 
 // pass 2: ordered write of (id, op bits) for entries with an unused move, and
-// the clear of the region (flags words and dirty byte)
+// the clear of the region (flags words and dirty byte).  Persistent blocks
+// take regions round-robin.  In a full region thread t of load k owns words
+// 4*(k*TPB + t) .. +3, so (load, thread, word) order is index order: the
+// block scans the per-load counts of CH loads at a time after one barrier
+// and writes every entry at its sorted position.
 template <int TPB>
 __global__ __launch_bounds__(TPB) void k_fs_write(FsScanArgs a) {
-    const uint64_t r = blockIdx.x;
-    if (!a.dirty[r]) return;
-    const uint64_t w0 = r * FS_REGION_WORDS;
-    const uint64_t rw = (a.nwords - w0 < FS_REGION_WORDS) ? a.nwords - w0 : FS_REGION_WORDS;
-    __shared__ uint32_t wave_tot[TPB / 64];
+    constexpr uint32_t PER = FS_REGION_WORDS / 4 / TPB;
+    constexpr uint32_t CH = PER < 4 ? PER : 4;
+    static_assert(PER % CH == 0, "a region is a whole number of chunks");
+    constexpr int NW = TPB / 64;
+    const uint64_t nregions = (a.nwords + FS_REGION_WORDS - 1) / FS_REGION_WORDS;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    uint64_t running = a.out_off[r];
-    // thread t of iteration `it` owns words it*4*TPB + 4t .. +3: (iteration, thread,
-    // word) order is index order, so one block scan per iteration keeps the output sorted
-    for (uint64_t it = 0; it * 4 * TPB < rw; ++it) {
-        const uint64_t j0 = it * 4 * TPB + (uint64_t)threadIdx.x * 4;
-        uint32_t ws[4] = {0u, 0u, 0u, 0u}, keep[4] = {0u, 0u, 0u, 0u};
-        uint32_t c = 0;
+    __shared__ uint32_t tot[2][CH][NW];
+    uint32_t par = 0;
+    for (uint64_t r = blockIdx.x; r < nregions; r += gridDim.x) {
+        if (!a.dirty[r]) continue;  // the same value for the whole block: it is cleared after the barrier below
+        const uint64_t w0 = r * FS_REGION_WORDS;
+        const uint64_t rw = (a.nwords - w0 < FS_REGION_WORDS) ? a.nwords - w0 : FS_REGION_WORDS;
+        uint64_t running = a.out_off[r];
+        if (rw == FS_REGION_WORDS) {
+            uint4* f4 = reinterpret_cast<uint4*>(a.flags + w0);
+            // the region in PER/CH chunks of CH loads per thread, one barrier each
+            for (uint32_t k0 = 0; k0 < PER; k0 += CH, par ^= 1u) {
+                uint4 v[CH];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const uint64_t j = j0 + e;
-            ws[e] = (j < rw) ? a.flags[w0 + j] : 0u;
-            if (ws[e]) keep[e] = fs_live(ws[e], a.odd, w0 + j) & ~nib_full(ws[e], a.ops);
-            c += __popc(keep[e]);
-        }
-        const uint32_t incl = wave_incl_scan(c, lane);
-        if (lane == 63) wave_tot[wave] = incl;
-        __syncthreads();
-        uint32_t before = 0, total = 0;
-        for (int k = 0; k < TPB / 64; ++k) {
-            const uint32_t t = wave_tot[k];
-            before += (k < wave) ? t : 0u;
-            total += t;
-        }
-        uint64_t pos = running + before + incl - c;
+                for (uint32_t k = 0; k < CH; ++k) v[k] = f4[(k0 + k) * TPB + threadIdx.x];
+                uint32_t c[CH], incl[CH];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            uint32_t kb = keep[e];
-            while (kb) {
-                const int b = __ffs(kb) - 1;  // bit 4k of nibble k
-                kb &= kb - 1;
-                a.out_idx[pos] = (uint32_t)((w0 + j0 + e) * 8 + (uint64_t)(b >> 2));
-                a.out_ops[pos] = (uint8_t)((ws[e] >> b) & 15u);
-                ++pos;
+                for (uint32_t k = 0; k < CH; ++k) {
+                    const uint32_t ws[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+                    const uint64_t j0 = w0 + ((uint64_t)(k0 + k) * TPB + threadIdx.x) * 4;
+                    c[k] = 0;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (ws[e]) c[k] += __popc(fs_live(ws[e], a.odd, j0 + e) & ~nib_full(ws[e], a.ops));
+                    incl[k] = wave_incl_scan(c[k], lane);
+                    if (lane == 63) tot[par][k][wave] = incl[k];
+                }
+                __syncthreads();
+                // tot[par] is written again two chunks later, after the next
+                // chunk's barrier, which every thread reaches only once done here
+#pragma unroll
+                for (uint32_t k = 0; k < CH; ++k) {
+                    uint32_t before = 0, total = 0;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) {
+                        const uint32_t t = tot[par][k][w];
+                        before += (w < wave) ? t : 0u;
+                        total += t;
+                    }
+                    const uint32_t ws[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+                    const uint64_t j0 = w0 + ((uint64_t)(k0 + k) * TPB + threadIdx.x) * 4;
+                    uint64_t pos = running + before + incl[k] - c[k];
+                    for (int e = 0; e < 4; ++e) {
+                        if (!ws[e]) continue;
+                        uint32_t kb = fs_live(ws[e], a.odd, j0 + e) & ~nib_full(ws[e], a.ops);
+                        while (kb) {
+                            const int b = __ffs(kb) - 1;  // bit 4k of nibble k
+                            kb &= kb - 1;
+                            a.out_idx[pos] = (uint32_t)((j0 + e) * 8 + (uint64_t)(b >> 2));
+                            a.out_ops[pos] = (uint8_t)((ws[e] >> b) & 15u);
+                            ++pos;
+                        }
+                    }
+                    if (ws[0] | ws[1] | ws[2] | ws[3]) f4[(k0 + k) * TPB + threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+                    running += total;
+                }
             }
-            if (ws[e]) a.flags[w0 + j0 + e] = 0u;
+            if (threadIdx.x == 0) a.dirty[r] = 0;  // every thread read it before the first barrier above
+            continue;
         }
-        running += total;
-        __syncthreads();
+        // the whole segment is smaller than a region (nregions == 1): thread t
+        // of iteration `it` owns words it*4*TPB + 4t .. +3, one block scan per iteration
+        for (uint64_t it = 0; it * 4 * TPB < rw; ++it) {
+            const uint64_t j0 = it * 4 * TPB + (uint64_t)threadIdx.x * 4;
+            uint32_t ws[4] = {0u, 0u, 0u, 0u}, keep[4] = {0u, 0u, 0u, 0u};
+            uint32_t c = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint64_t j = j0 + e;
+                ws[e] = (j < rw) ? a.flags[w0 + j] : 0u;
+                if (ws[e]) keep[e] = fs_live(ws[e], a.odd, w0 + j) & ~nib_full(ws[e], a.ops);
+                c += __popc(keep[e]);
+            }
+            const uint32_t incl = wave_incl_scan(c, lane);
+            if (lane == 63) tot[0][0][wave] = incl;
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+            for (int k = 0; k < NW; ++k) {
+                const uint32_t t = tot[0][0][k];
+                before += (k < wave) ? t : 0u;
+                total += t;
+            }
+            uint64_t pos = running + before + incl - c;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                uint32_t kb = keep[e];
+                while (kb) {
+                    const int b = __ffs(kb) - 1;
+                    kb &= kb - 1;
+                    a.out_idx[pos] = (uint32_t)((w0 + j0 + e) * 8 + (uint64_t)(b >> 2));
+                    a.out_ops[pos] = (uint8_t)((ws[e] >> b) & 15u);
+                    ++pos;
+                }
+                if (ws[e]) a.flags[w0 + j0 + e] = 0u;
+            }
+            running += total;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.dirty[r] = 0;
     }
-    if (threadIdx.x == 0) a.dirty[r] = 0;
 }
 
 
-// children of the written entries that leave the segment, one slot per move
+// children of the written entries that leave the segment, in slot order
 struct CrossSink {
-    unsigned long long* out;
     uint64_t segment, nseg, in_mask;
     int seg_bits;
-    unsigned long long nx, bad;
+    unsigned long long bad;
+    uint32_t valid;            // bit k: slot k holds a cross child
+    unsigned long long v[8];   // by slot; PZ::SLOTS <= 8
     __device__ __forceinline__ void operator()(int slot, uint64_t child, uint32_t label) {
         const uint64_t s = child >> seg_bits;
         if (s == segment) return;
@@ -242,74 +344,70 @@ struct CrossSink {
             ++bad;
             return;
         }
-        out[slot] = (((uint64_t)label * nseg + s) << 32) | (child & in_mask);
-        ++nx;
+        v[slot] = (((uint64_t)label * nseg + s) << 32) | (child & in_mask);
+        valid |= 1u << slot;
     }
 };
 
 constexpr uint32_t FS_CROSS_PER_BLOCK = 4096;  // parents per block of the cross pass
 
-// Block b expands parents [b*4096, (b+1)*4096) and records how many of their
-// slots it filled, so the compaction can place each block's children in
-// (parent, slot) order.  The count is of the slots themselves, read back after
-// the expansion, so the compaction writes exactly sum(block_count) entries;
-// counters[7] receives the same total for the host to compare with the
-// children counted in counters[3].
+// Block b expands parents [b*4096, (b+1)*4096) and writes their cross
+// children, compacted in (parent, slot) order, to the front of its own area
+// slots[b*4096*SLOTS ...]; block_count[b] is how many it wrote, so the
+// compaction copies exactly sum(block_count) entries.  counters[7] receives
+// the same total for the host to compare with the children counted in
+// counters[3].
 template <class PZ, int TPB>
 __global__ __launch_bounds__(TPB) void k_fs_cross(FsCrossArgs a) {
-    CrossSink sink{nullptr, a.segment, a.nseg, (1ull << a.seg_bits) - 1ull, a.seg_bits, 0ull, 0ull};
+    static_assert(PZ::SLOTS <= 8, "cross sink holds 8 slots");
+    constexpr int NW = TPB / 64;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    CrossSink sink{a.segment, a.nseg, (1ull << a.seg_bits) - 1ull, a.seg_bits, 0ull, 0u, {}};
     const uint64_t p0 = (uint64_t)blockIdx.x * FS_CROSS_PER_BLOCK;
     const uint64_t p1 = (p0 + FS_CROSS_PER_BLOCK < a.n) ? p0 + FS_CROSS_PER_BLOCK : a.n;
-    unsigned long long filled = 0;
-    for (uint64_t i = p0 + threadIdx.x; i < p1; i += TPB) {
-        sink.out = a.slots + i * PZ::SLOTS;
-#pragma unroll
-        for (int k = 0; k < PZ::SLOTS; ++k) sink.out[k] = FS_NONE;
-        PZ::expand(a.base | a.idx[i], (uint32_t)a.ops[i], a.prm, sink);
-#pragma unroll
-        for (int k = 0; k < PZ::SLOTS; ++k) filled += sink.out[k] != FS_NONE ? 1ull : 0ull;
-    }
-    block_add3<TPB>(sink.nx, sink.bad, filled, &a.counters[3], &a.counters[4], &a.counters[7]);
-    __shared__ unsigned long long red[TPB / 64];
-    unsigned long long v = wave_sum(filled);
-    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int w = 0; w < TPB / 64; ++w) t += red[w];
-        a.block_count[blockIdx.x] = t;
-    }
-}
-
-// Ordered compaction of the cross slots: block b copies the non-empty slots of
-// its parents to out[block_off[b] ...] in slot order.
-template <int TPB>
-__global__ __launch_bounds__(TPB) void k_fs_compact(const unsigned long long* __restrict__ slots, uint64_t nslots,
-                                                    uint32_t slots_per_block,
-                                                    const unsigned long long* __restrict__ block_off,
-                                                    unsigned long long* __restrict__ out) {
-    __shared__ uint32_t wave_tot[TPB / 64];
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const uint64_t s0 = (uint64_t)blockIdx.x * slots_per_block;
-    const uint64_t s1 = (s0 + slots_per_block < nslots) ? s0 + slots_per_block : nslots;
-    uint64_t running = block_off[blockIdx.x];
-    for (uint64_t base = s0; base < s1; base += TPB) {
-        const uint64_t i = base + threadIdx.x;
-        const unsigned long long x = (i < s1) ? slots[i] : FS_NONE;
-        const uint32_t c = x != FS_NONE ? 1u : 0u;
+    unsigned long long* out = a.slots + p0 * PZ::SLOTS;
+    __shared__ uint32_t tot[2][NW];
+    uint32_t running = 0, par = 0;
+    unsigned long long nx = 0;
+    for (uint64_t b0 = p0; b0 < p1; b0 += TPB, par ^= 1u) {
+        const uint64_t i = b0 + threadIdx.x;
+        sink.valid = 0;
+        if (i < p1) PZ::expand(a.base | a.idx[i], (uint32_t)a.ops[i], a.prm, sink);
+        const uint32_t c = (uint32_t)__popc(sink.valid);
+        nx += c;
         const uint32_t incl = wave_incl_scan(c, lane);
-        if (lane == 63) wave_tot[wave] = incl;
+        if (lane == 63) tot[par][wave] = incl;
         __syncthreads();
         uint32_t before = 0, total = 0;
-        for (int k = 0; k < TPB / 64; ++k) {
-            const uint32_t t = wave_tot[k];
-            before += (k < wave) ? t : 0u;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t t = tot[par][w];
+            before += (w < wave) ? t : 0u;
             total += t;
         }
-        if (c) out[running + before + incl - 1] = x;
+        uint32_t pos = running + before + incl - c;
+#pragma unroll
+        for (int k = 0; k < PZ::SLOTS; ++k)
+            if (sink.valid & (1u << k)) out[pos++] = sink.v[k];
         running += total;
-        __syncthreads();
     }
+    block_add3<TPB>(nx, sink.bad, nx, &a.counters[3], &a.counters[4], &a.counters[7]);
+    if (threadIdx.x == 0) a.block_count[blockIdx.x] = running;
+}
+
+// Ordered compaction of the cross children: block b copies the block_count[b]
+// entries at the front of its area (slots_per_block entries from
+// slots[b*slots_per_block]) to out[block_off[b] ...].
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_fs_compact(const unsigned long long* __restrict__ slots,
+                                                    uint32_t slots_per_block,
+                                                    const unsigned long long* __restrict__ block_count,
+                                                    const unsigned long long* __restrict__ block_off,
+                                                    unsigned long long* __restrict__ out) {
+    const unsigned long long* src = slots + (uint64_t)blockIdx.x * slots_per_block;
+    unsigned long long* dst = out + block_off[blockIdx.x];
+    const uint64_t n = block_count[blockIdx.x];
+    for (uint64_t i = threadIdx.x; i < n; i += TPB) dst[i] = src[i];
 }
 
 

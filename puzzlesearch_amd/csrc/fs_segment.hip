@@ -241,7 +241,8 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
     sa.cnt_out = w->cnt_out;
     sa.counters = w->counters;
     sa.out_off = w->out_off;
-    hipLaunchKernelGGL(psk::k_fs_count<FS_TPB>, dim3((uint32_t)w->nregions), dim3(FS_TPB), 0, s, sa);
+    const uint32_t grid_scan = (uint32_t)std::min<uint64_t>(w->nregions, (uint64_t)w->cus * 8);
+    hipLaunchKernelGGL(psk::k_fs_count<FS_TPB>, dim3(grid_scan), dim3(FS_TPB), 0, s, sa);
     PS_HIP_CHECK(hipGetLastError());
     size_t tb = 0;
     PS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w->cnt_out, w->out_off, (int)w->nregions, s));
@@ -260,7 +261,7 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
     if (int r = w->out_ops.ensure(std::max<uint64_t>(n_out, 1), s)) return r;
     sa.out_idx = w->out_idx.as<uint32_t>();
     sa.out_ops = w->out_ops.as<uint8_t>();
-    hipLaunchKernelGGL(psk::k_fs_write<FS_TPB>, dim3((uint32_t)w->nregions), dim3(FS_TPB), 0, s, sa);
+    hipLaunchKernelGGL(psk::k_fs_write<FS_TPB>, dim3(grid_scan), dim3(FS_TPB), 0, s, sa);
     PS_HIP_CHECK(hipGetLastError());
     if (w->odd && n) {
         hipLaunchKernelGGL(psk::k_fs_clear_odd, dim3(grid_par), dim3(256), 0, s, d_idx, n, seg_mask, w->odd_bits);
@@ -324,8 +325,9 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
         if (int r = w->compact.ensure(std::max<uint64_t>(m, 1) * 8, s)) return r;
         if (int r = w->nruns.ensure(16, s)) return r;
         hipLaunchKernelGGL(psk::k_fs_compact<256>, dim3((uint32_t)nblk), dim3(256), 0, s,
-                           (const unsigned long long*)w->slot_buf.as<unsigned long long>(), nslots,
+                           (const unsigned long long*)w->slot_buf.as<unsigned long long>(),
                            (uint32_t)(psk::FS_CROSS_PER_BLOCK * w->slots),
+                           (const unsigned long long*)w->block_cnt.as<unsigned long long>(),
                            (const unsigned long long*)w->block_off.as<unsigned long long>(),
                            w->compact.as<unsigned long long>());
         PS_HIP_CHECK(hipGetLastError());
