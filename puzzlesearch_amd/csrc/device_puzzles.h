@@ -378,30 +378,66 @@ struct Packed {
         a[15] = (uint32_t)(z + W);
     }
 
+    // Slots whose child may lie outside the parent's segment of 2^seg_bits
+    // indexes.  A horizontal move changes only the blank cell, index -+ 1
+    // without carry, so it stays in any segment of 16 or more indexes.  With
+    // the 15-puzzle's own segments (seg_bits 32: the segment is the packed
+    // values of tile-list positions 12..14) a vertical move changes the
+    // segment only if it reorders a list position >= 12: the packed value of
+    // a position depends only on the set of tiles left of it.  Up from blank
+    // cell z reorders positions z-W .. z-1, down reorders z .. z+W-1.
+    __device__ __forceinline__ static uint32_t cross_slots(uint64_t index, int seg_bits) {
+        const int z = (int)(index & 15u);
+        uint32_t m = seg_bits >= 4 ? 0x3u : 0xFu;
+        if (S == 16 && seg_bits == 32) {
+            if (z - 1 < 12) m &= ~0x1u;
+            if (z + W - 1 < 12) m &= ~0x2u;
+        }
+        return m;
+    }
+
+    // slots that have a child: the move exists and its op bit is not set
+    __device__ __forceinline__ static uint32_t live_slots(uint64_t index, uint32_t op_bits) {
+        const int z = (int)(index & 15u);
+        uint32_t m = 0;
+        if (z >= W && !(op_bits & (1u << OP_UP))) m |= 1u;
+        if (z < S - W && !(op_bits & (1u << OP_DOWN))) m |= 2u;
+        if (z % W > 0 && !(op_bits & (1u << OP_LEFT))) m |= 4u;
+        if (z % W < W - 1 && !(op_bits & (1u << OP_RIGHT))) m |= 8u;
+        return m;
+    }
+
+    // `want`: the slots to compute (bit k: slot k); the others are skipped,
+    // and the permutation is unpacked only when a vertical move is wanted.
     template <class F>
-    __device__ __forceinline__ static void expand(uint64_t index, uint32_t op_bits, const PuzzleParams&, F&& f) {
-        uint32_t a[16];
-        from_index(index, a);
-        unpack(a);
-        const int blank = (int)a[15];
-        if (blank >= W && !(op_bits & (1u << OP_UP))) {
-            uint32_t b[16];
+    __device__ __forceinline__ static void expand(uint64_t index, uint32_t op_bits, const PuzzleParams&, F&& f,
+                                                  uint32_t want = 0xFu) {
+        const int blank = (int)(index & 15u);
+        const bool up = (want & 1u) && blank >= W && !(op_bits & (1u << OP_UP));
+        const bool dn = (want & 2u) && blank < S - W && !(op_bits & (1u << OP_DOWN));
+        if (up || dn) {
+            uint32_t a[16];
+            from_index(index, a);
+            unpack(a);
+            if (up) {
+                uint32_t b[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) b[i] = a[i];
-            rotate_up(b);
-            pack(b);
-            f(0, to_index(b), OP_DOWN);
-        }
-        if (blank < S - W && !(op_bits & (1u << OP_DOWN))) {
-            uint32_t b[16];
+                for (int i = 0; i < 16; ++i) b[i] = a[i];
+                rotate_up(b);
+                pack(b);
+                f(0, to_index(b), OP_DOWN);
+            }
+            if (dn) {
+                uint32_t b[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) b[i] = a[i];
-            rotate_dn(b);
-            pack(b);
-            f(1, to_index(b), OP_UP);
+                for (int i = 0; i < 16; ++i) b[i] = a[i];
+                rotate_dn(b);
+                pack(b);
+                f(1, to_index(b), OP_UP);
+            }
         }
-        if (blank % W > 0 && !(op_bits & (1u << OP_LEFT))) f(2, index - 1, OP_RIGHT);
-        if (blank % W < W - 1 && !(op_bits & (1u << OP_RIGHT))) f(3, index + 1, OP_LEFT);
+        if ((want & 4u) && blank % W > 0 && !(op_bits & (1u << OP_LEFT))) f(2, index - 1, OP_RIGHT);
+        if ((want & 8u) && blank % W < W - 1 && !(op_bits & (1u << OP_RIGHT))) f(3, index + 1, OP_LEFT);
     }
 };
 

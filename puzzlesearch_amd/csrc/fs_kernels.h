@@ -351,33 +351,80 @@ struct CrossSink {
 
 constexpr uint32_t FS_CROSS_PER_BLOCK = 4096;  // parents per block of the cross pass
 
+// puzzles that can name the slots whose child may leave the segment
+// (PZ::cross_slots(index, seg_bits)); the others expand every slot
+template <class PZ, class = void>
+struct HasCrossSlots : std::false_type {};
+template <class PZ>
+struct HasCrossSlots<PZ, std::void_t<decltype(PZ::cross_slots(0ull, 0))>> : std::true_type {};
+
 // Block b expands parents [b*4096, (b+1)*4096) and writes their cross
 // children, compacted in (parent, slot) order, to the front of its own area
 // slots[b*4096*SLOTS ...]; block_count[b] is how many it wrote, so the
 // compaction copies exactly sum(block_count) entries.  counters[7] receives
 // the same total for the host to compare with the children counted in
-// counters[3].
+// counters[3].  For puzzles that name the slots that may leave the segment
+// the block first queues, in parent order, only the parents with such a
+// slot, so the lanes of a wave expand parents that all have work.
 template <class PZ, int TPB>
 __global__ __launch_bounds__(TPB) void k_fs_cross(FsCrossArgs a) {
     static_assert(PZ::SLOTS <= 8, "cross sink holds 8 slots");
     constexpr int NW = TPB / 64;
+    constexpr bool QUEUE = HasCrossSlots<PZ>::value;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     CrossSink sink{a.segment, a.nseg, (1ull << a.seg_bits) - 1ull, a.seg_bits, 0ull, 0u, {}};
     const uint64_t p0 = (uint64_t)blockIdx.x * FS_CROSS_PER_BLOCK;
     const uint64_t p1 = (p0 + FS_CROSS_PER_BLOCK < a.n) ? p0 + FS_CROSS_PER_BLOCK : a.n;
     unsigned long long* out = a.slots + p0 * PZ::SLOTS;
     __shared__ uint32_t tot[2][NW];
-    uint32_t running = 0, par = 0;
+    __shared__ uint16_t q[QUEUE ? FS_CROSS_PER_BLOCK : 1];
+    uint32_t par = 0;
+    uint32_t nq = (uint32_t)(p1 - p0);
+    if constexpr (QUEUE) {
+        uint32_t qn = 0;
+        for (uint64_t b0 = p0; b0 < p1; b0 += TPB, par ^= 1u) {
+            const uint64_t i = b0 + threadIdx.x;
+            bool need = false;
+            if (i < p1) {
+                const uint64_t parent = a.base | a.idx[i];
+                need = (PZ::cross_slots(parent, a.seg_bits) & PZ::live_slots(parent, (uint32_t)a.ops[i])) != 0u;
+            }
+            const unsigned long long m = __ballot(need);
+            if (lane == 0) tot[par][wave] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint32_t t = tot[par][w];
+                before += (w < wave) ? t : 0u;
+                total += t;
+            }
+            if (need) q[qn + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)(i - p0);
+            qn += total;
+        }
+        __syncthreads();  // the queue is complete
+        nq = qn;
+    }
+    uint32_t running = 0;
     unsigned long long nx = 0;
-    for (uint64_t b0 = p0; b0 < p1; b0 += TPB, par ^= 1u) {
-        const uint64_t i = b0 + threadIdx.x;
+    for (uint32_t k0 = 0; k0 < nq; k0 += TPB, par ^= 1u) {
+        const uint32_t k = k0 + threadIdx.x;
         sink.valid = 0;
-        if (i < p1) PZ::expand(a.base | a.idx[i], (uint32_t)a.ops[i], a.prm, sink);
+        if (k < nq) {
+            const uint64_t i = p0 + (QUEUE ? (uint64_t)q[k] : (uint64_t)k);
+            const uint64_t parent = a.base | a.idx[i];
+            if constexpr (QUEUE)
+                PZ::expand(parent, (uint32_t)a.ops[i], a.prm, sink, PZ::cross_slots(parent, a.seg_bits));
+            else
+                PZ::expand(parent, (uint32_t)a.ops[i], a.prm, sink);
+        }
         const uint32_t c = (uint32_t)__popc(sink.valid);
         nx += c;
         const uint32_t incl = wave_incl_scan(c, lane);
         if (lane == 63) tot[par][wave] = incl;
         __syncthreads();
+        // tot[par] is written again two iterations later, after the next
+        // iteration's barrier, which every thread reaches only once done here
         uint32_t before = 0, total = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
@@ -387,8 +434,8 @@ __global__ __launch_bounds__(TPB) void k_fs_cross(FsCrossArgs a) {
         }
         uint32_t pos = running + before + incl - c;
 #pragma unroll
-        for (int k = 0; k < PZ::SLOTS; ++k)
-            if (sink.valid & (1u << k)) out[pos++] = sink.v[k];
+        for (int kk = 0; kk < PZ::SLOTS; ++kk)
+            if (sink.valid & (1u << kk)) out[pos++] = sink.v[kk];
         running += total;
     }
     block_add3<TPB>(nx, sink.bad, nx, &a.counters[3], &a.counters[4], &a.counters[7]);

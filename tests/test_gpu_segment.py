@@ -80,6 +80,25 @@ def test_fifteen_puzzle_segment(ps, ppm):
     assert st["children_in"] == ref["children_in"] and st["children_cross"] == ref["children_cross"]
 
 
+def test_fifteen_puzzle_every_blank_cell(ps):
+    # parents with the blank in every cell, so each of the cross pass's slot
+    # skips (horizontal moves, vertical moves left of list position 12) is
+    # checked against the oracle's full expansion
+    from puzzlesearch_amd.segment import FsSegment
+    op = O.sliding_packed(4, 4)
+    rng = np.random.default_rng(7)
+    seg = 0x0A5  # t12 = 5, t13 = 10, t14 = 0
+    m = np.unique(rng.integers(0, 3832012800 // 16, size=60000, dtype=np.uint64))
+    idx = (m * np.uint64(16) + rng.integers(0, 16, size=m.size, dtype=np.uint64)).astype(np.uint32)
+    ops = rng.integers(0, 16, size=idx.size, dtype=np.uint64).astype(np.uint8)
+    assert set((idx & 15).tolist()) == set(range(16))
+    with FsSegment(_ps_puzzle(ps, op), 32) as w:
+        dev = w.step(seg, idx, ops)
+    ref = O.fs_segment_step(op, 32, seg, 3805, idx, ops)
+    assert_same(dev, ref, "every blank cell")
+    assert dev["stats"]["children_cross"] == ref["children_cross"] > 0
+
+
 def test_fifteen_puzzle_chunking_and_reuse(ps):
     from puzzlesearch_amd.segment import FsSegment
     op = O.sliding_packed(4, 4)
