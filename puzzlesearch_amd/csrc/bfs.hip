@@ -147,6 +147,10 @@ psk::LayerEndArgs layer_end_args(ps_bfs* b) {
     e.layer_hashes = b->layer_hashes;
     e.st = b->st;
     e.bits = b->bits;
+    e.active = b->tile_list;
+    e.ntiles = b->ntiles;
+    e.tile_bits = b->tile_words * 64;
+    e.seg_shift = b->seg_shift;
     return e;
 }
 
@@ -218,7 +222,9 @@ int launch_batched_layer(ps_bfs* b, bool timed) {
         a.tile_end = std::min<uint64_t>(a.tile_end, b->xtiles);
         if (int r = pkl_expand(b->puzzle, a, list_args(b), ce, nullptr)) return r;
     } else {
-        if (int r = bfs3_expand(b->puzzle, expand_args(b), ce, nullptr)) return r;
+        psk::ExpandArgs a = expand_args(b);
+        a.active = b->tile_list;
+        if (int r = bfs3_expand(b->puzzle, a, ce, nullptr)) return r;
     }
     if (timed)
         if (int r = record(b, &e1)) return r;
@@ -394,6 +400,7 @@ int destroy_bfs(ps_bfs* b) {
         if (b->arr[i] && !alias) (void)hipFree(b->arr[i]);
     }
     if (b->xmark) (void)hipFree(b->xmark);
+    if (b->tile_list) (void)hipFree(b->tile_list);
     if (b->xlists) (void)hipFree(b->xlists);
     if (b->xcounts) (void)hipFree(b->xcounts);
     if (b->xflags) (void)hipFree(b->xflags);
@@ -491,6 +498,11 @@ uint64_t footprint_bytes(const ps_puzzle* p, const ps_bfs_options* o, const Geom
         // counts and the list of tiles a launch visits
         if (o->mode == PS_BFS_BATCHED)
             arrays += entries * 4 + (uint64_t)tiles * slots * 4 + tiles + 2ull * tiles + 12ull * tiles + 4ull * tiles;
+    } else if (o->bits == 3 && o->mode == PS_BFS_BATCHED) {
+        // the list of tiles a top-down expand visits: 4 bytes per tile, tiles
+        // of at least 64 words unless tile_bits_log2 asks for smaller ones
+        const uint64_t tw = o->tile_bits_log2 > 0 && o->tile_bits_log2 < 12 ? 1ull << std::max(o->tile_bits_log2 - 6, 0) : 64;
+        arrays += (g.nwords + tw - 1) / tw * 4;
     }
     const uint64_t meta = 3ull * g.nseg * 8 + ((g.nseg + 31) / 32) * 4 + 2ull * max_layers * 8 + sizeof(BfsDevState);
     return arrays + meta;
@@ -752,6 +764,8 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         b->chunks_per_seg = (b->seg_words + psk::FIN_CHUNK_WORDS - 1) / psk::FIN_CHUNK_WORDS;
         b->nchunks = (uint64_t)b->nseg * b->chunks_per_seg;
         b->grid_fin = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * 8, b->nchunks));
+        if (opt.mode == PS_BFS_BATCHED && !b->lists)
+            if (int r = dalloc((void**)&b->tile_list, b->ntiles * 4)) return cleanup(r);
     }
     *out = b;
     return PS_OK;
@@ -810,6 +824,11 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     PS_HIP_CHECK(hipMemcpy(b->layer_counts, &c0, 8, hipMemcpyHostToDevice));
     PS_HIP_CHECK(hipMemcpy(b->layer_hashes, &h0, 8, hipMemcpyHostToDevice));
     BfsDevState st{};
+    if (b->tile_list) {  // layer 0 visits the start state's tile
+        const uint32_t t0 = (uint32_t)(start / (b->tile_words * 64));
+        PS_HIP_CHECK(hipMemcpy(b->tile_list, &t0, 4, hipMemcpyHostToDevice));
+        st.active_tiles = 1;
+    }
     st.max_layers = (uint32_t)b->max_layers;
     st.layers_recorded = 1;
     st.visited = 1;

@@ -40,6 +40,7 @@ struct ps_bfs {
     int use_window = 1;
     int pull_capable = 0;  // expand LDS also holds the tile's cur words (bottom-up layers)
     uint64_t tile_words = 0, ntiles = 0;
+    uint32_t* tile_list = nullptr;  // batched top-down layers: the tiles the next expand visits
     int tpb_expand = 512;
     uint32_t grid_expand = 0;
     size_t lds_expand = 0;

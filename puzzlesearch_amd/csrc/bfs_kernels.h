@@ -72,6 +72,7 @@ struct ExpandArgs {
     uint8_t* xmark;  // byte per state for children outside the window; NULL: atomic OR into next
     int32_t skip_visited;  // skip cross-window children already in cur or old
     int32_t pull_capable;  // LDS holds a copy of the tile's cur words (bottom-up layers possible)
+    const uint32_t* active;  // top-down layers visit st->active_tiles tiles listed here; NULL: every tile
 };
 
 // Cross-tile child lists of the 10..12-pancake search (bfs_pancake_lists.h).
@@ -135,6 +136,10 @@ struct LayerEndArgs {
     unsigned long long* layer_hashes;
     BfsDevState* st;
     int32_t bits;  // 3 or 2
+    uint32_t* active;    // NULL, or the list of tiles the next top-down expand visits (ExpandArgs::active)
+    uint64_t ntiles;
+    uint64_t tile_bits;  // states per tile
+    int32_t seg_shift;
 };
 
 struct Bfs2Args {
@@ -522,10 +527,20 @@ __global__ __launch_bounds__(TPB) void k_bfs3_expand(ExpandArgs a) {
 
     ExpandTally tally;
     uint64_t iter = 0;
+    // a top-down layer with a tile list visits only the tiles holding frontier
+    // states: a sparse layer costs no pass over every tile and no contended
+    // counter per empty tile
+    const bool listed = a.active != nullptr && !pull;
     for (;;) {
         if (tid == 0) {
-            if (a.persistent) s_tile = a.tile_begin + atomicAdd(&st->tile_counter, 1ull);
-            else s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
+            if (listed) {
+                const unsigned long long k = atomicAdd(&st->tile_counter, 1ull);
+                s_tile = k < st->active_tiles ? (uint64_t)a.active[k] : a.tile_end;
+            } else if (a.persistent) {
+                s_tile = a.tile_begin + atomicAdd(&st->tile_counter, 1ull);
+            } else {
+                s_tile = (iter == 0) ? a.tile_begin + blockIdx.x : ~0ull;
+            }
         }
         __syncthreads();
         const uint64_t tile = s_tile;
@@ -714,6 +729,7 @@ __device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
     BfsDevState* st = a.st;
     if (st->done) return;
     const uint32_t L = st->layer;
+    __shared__ uint32_t s_list;
     if (a.bits == 3) {
         unsigned long long* sc_next_new = a.seg_count + (uint64_t)((L + 2) % 3) * a.nseg;
         for (uint32_t i = threadIdx.x; i < a.nseg; i += TPB) sc_next_new[i] = 0;
@@ -741,7 +757,47 @@ __device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
         st->acc_hash = 0;
         st->tile_counter = 0;
         st->chunk_counter = 0;
+        s_list = (a.active && !st->done && !st->pull) ? 1u : 0u;
     }
+    __syncthreads();
+    if (!s_list) return;
+    // the tiles the next (top-down) expand visits: those overlapping a segment
+    // with states in the layer just finalized, in tile order
+    constexpr int NW = TPB / 64;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const unsigned long long* sc = a.seg_count + (uint64_t)((L + 1) % 3) * a.nseg;
+    __shared__ uint32_t s_wsum[NW];
+    uint32_t base = 0;
+    for (uint64_t t0 = 0; t0 < a.ntiles; t0 += TPB) {
+        const uint64_t T = t0 + threadIdx.x;
+        bool live = false;
+        if (T < a.ntiles) {
+            if (a.nseg <= 1) {
+                live = true;
+            } else {
+                const uint64_t b0 = T * a.tile_bits;
+                const uint64_t b1 = (b0 + a.tile_bits < a.nbits ? b0 + a.tile_bits : a.nbits) - 1;
+                const uint64_t s0 = b0 >> a.seg_shift, s1 = b1 >> a.seg_shift;
+                if (s1 - s0 >= 64) live = true;  // many small segments: visit the tile
+                else
+                    for (uint64_t sg = s0; sg <= s1 && !live; ++sg) live = sc[sg] != 0;
+            }
+        }
+        const unsigned long long m = __ballot(live);
+        if (lane == 0) s_wsum[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t off = base, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = s_wsum[w];
+            off += (w < wave) ? c : 0u;
+            total += c;
+        }
+        if (live) a.active[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)T;
+        base += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) st->active_tiles = base;
 }
 
 template <int TPB>

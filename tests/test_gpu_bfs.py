@@ -229,13 +229,14 @@ def test_pancake12_reference_sizing(ps):
     assert [str(h) for h in hashes] == rec["hashes"]
 
 
-@pytest.mark.parametrize("shape", ["sliding_4x3", "sliding_3x4"])
-def test_sliding_4x3_full(ps, golden, shape):
+@pytest.mark.parametrize("shape,cross", [("sliding_4x3", 0), ("sliding_3x4", 0), ("sliding_4x3", 1)])
+def test_sliding_4x3_full(ps, golden, shape, cross):
     """4x3 sliding tile, 12! indexes (half reachable): counts equal
-    SlidingPuzzleTests/TestFrontierSearch.cpp:71 (3x4 by transposition)."""
+    SlidingPuzzleTests/TestFrontierSearch.cpp:71 (3x4 by transposition);
+    byte and atomic cross-window marks."""
     w, h = (4, 3) if shape == "sliding_4x3" else (3, 4)
     p = psa.sliding(w, h)
-    with psa.Bfs(p, segment_bits=24, direction=psa.DIR_AUTO) as b:
+    with psa.Bfs(p, segment_bits=24, direction=psa.DIR_AUTO, cross_marks=cross) as b:
         counts, hashes = b.run(0)
     assert counts == golden["sliding_4x3"]["counts"]
     if shape in FIX:
@@ -243,10 +244,15 @@ def test_sliding_4x3_full(ps, golden, shape):
 
 
 def test_hanoi16_full(ps):
-    """4-peg Hanoi, 16 disks, 4^16 states: every state is reached once."""
+    """4-peg Hanoi, 16 disks, 4^16 states: every state is reached once; the
+    search with byte cross-window marks and the batched search with atomic
+    marks (2^20-state segments, the tuned shape) agree in every layer count
+    and frontier hash."""
     with psa.Bfs(psa.hanoi(16), segment_bits=24) as b:
         counts, hashes = b.run(0)
         st = b.stats()
     assert sum(counts) == 4 ** 16
     assert st.expanded == 4 ** 16
     assert counts[:11] == [1, 3, 6, 12, 30, 30, 66, 96, 126, 210, 330]  # shared prefix with the 21-disk vector
+    with psa.Bfs(psa.hanoi(16), segment_bits=20, block_threads=256, cross_marks=psa.CROSS_ATOMIC, graph=1) as b:
+        assert b.run(0) == (counts, hashes)
