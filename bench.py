@@ -342,14 +342,24 @@ def main():
 
     plain = None
     if not args.no_plain:
-        with psa.Bfs(p, index_count=N, mode=psa.BFS_PLAIN, **eager_opts) as b:
-            dt_p, counts_p, hashes_p = timed_runs(psa, b, 0, max(1, args.steps // 2), 1)
-            st_p = b.stats()
-        assert counts_p == counts and hashes_p == hashes
-        plain = {"value": N / dt_p, "ms_per_step": dt_p * 1e3,
-                 "expand_launches_per_search": st_p.expand_launches,
-                 "finalize_launches_per_search": st_p.finalize_launches}
-        log(f"plain: {dt_p * 1e3:.2f} ms/search ({st_p.expand_launches} expand launches)")
+        # the plain per-segment path at --segment-bits (the batched list path
+        # does not depend on the segment size), and at 2^26, the largest
+        # segment of the 2^20..2^26 range and the plain path's fastest
+        plain = {}
+        for key, sb in (("", args.segment_bits), ("seg26_", 26)):
+            popts = dict(eager_opts, segment_bits=sb)
+            with psa.Bfs(p, index_count=N, mode=psa.BFS_PLAIN, **popts) as b:
+                dt_p, counts_p, hashes_p = timed_runs(psa, b, 0, max(1, args.steps // 2), 1)
+                st_p = b.stats()
+            assert counts_p == counts and hashes_p == hashes
+            if not key:
+                plain.update({"value": N / dt_p, "ms_per_step": dt_p * 1e3, "segment_bits": sb,
+                              "expand_launches_per_search": st_p.expand_launches,
+                              "finalize_launches_per_search": st_p.finalize_launches})
+            else:
+                plain.update({key + "value": N / dt_p, key + "ms_per_step": dt_p * 1e3,
+                              key + "expand_launches_per_search": st_p.expand_launches})
+            log(f"plain, 2^{sb}-state segments: {dt_p * 1e3:.2f} ms/search ({st_p.expand_launches} expand launches)")
 
     batched_set = None
     if not args.no_batched_set:
@@ -404,6 +414,7 @@ def main():
         **other_marks,
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "batched_vs_plain_seg26": (value / plain["seg26_value"]) if plain else None,
         "batched_set": batched_set,
         "segment_step": seg_step,
         "serving": serving,

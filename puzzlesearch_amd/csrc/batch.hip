@@ -101,11 +101,15 @@ int bfs_opts_for(const ps_batch* b, const Job* j, ps_bfs_options* o) {
     ps_puzzle_info info{};
     if (int r = puzzle_query(&j->puzzle, &info)) return r;
     const uint64_t n = j->index_count ? j->index_count : info.indexes_count;
-    // per-job streams: 10..12 pancakes run with cross-tile lists (the fused
-    // kernel has byte marks only)
-    if (b->opt.mode == PS_BATCH_STREAMS && j->puzzle.kind == PS_PANCAKE && j->puzzle.size >= 10 &&
-        j->puzzle.size <= 12 && n >= factorial(j->puzzle.size))
-        o->cross_marks = PS_CROSS_LISTS;
+    // per-job streams: 10..12 pancakes run with cross-tile lists, the other
+    // searches OR their cross-window children in with atomics, which needs no
+    // byte per state (the fused kernel has byte marks only)
+    if (b->opt.mode == PS_BATCH_STREAMS) {
+        o->cross_marks = PS_CROSS_ATOMIC;
+        if (j->puzzle.kind == PS_PANCAKE && j->puzzle.size >= 10 && j->puzzle.size <= 12 &&
+            n >= factorial(j->puzzle.size))
+            o->cross_marks = PS_CROSS_LISTS;
+    }
     // segments pay off once a search is large; small ones run as one segment
     o->segment_bits = (b->opt.segment_bits > 0 && n > (1ull << b->opt.segment_bits)) ? b->opt.segment_bits : 0;
     return PS_OK;

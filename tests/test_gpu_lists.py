@@ -133,12 +133,18 @@ def test_lists_refused(ps, puzzle, opts):
 
 def test_lists_footprint(ps):
     n = 12
-    base = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), cross_marks=psa.CROSS_ATOMIC)
+    base = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), mode=psa.BFS_PLAIN,
+                             cross_marks=psa.CROSS_ATOMIC)
+    batched = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), cross_marks=psa.CROSS_ATOMIC)
     lists = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), **LISTS)
     plain = psa.bfs_footprint(psa.pancake(n), index_count=math.factorial(n), mode=psa.BFS_PLAIN, **LISTS)
     tiles = math.factorial(n) // math.factorial(9)
     one = tiles * (n - 9) * math.factorial(9) * 4 + tiles * 585 * 4 + tiles  # lists, counts, flags
     assert plain - base == one
+    # a batched search without lists holds the list of tiles its expand
+    # visits: 4 bytes per tile of at least 64 words
+    nwords = (math.factorial(n) + 63) // 64
+    assert batched - base == (nwords + 63) // 64 * 4
     # batched: lists of two layer parities, per-tile window flags, frontier
     # counts and the list of tiles a launch visits
     assert lists - base == 2 * one + 2 * tiles + 12 * tiles + 4 * tiles
