@@ -60,6 +60,14 @@ SPECS = [
          puzzle=dict(kind="hanoi", disks=21, symmetry=False), start="index:0", partial=True),
     dict(id="hanoi_21_sym_prefix", file="PuzzlesTests/PuzzleSearchTests.cpp", test="Opt3BitBFS_21_Sym",
          puzzle=dict(kind="hanoi", disks=21, symmetry=True), start="index:0", partial=True),
+    # HanoiTowersTests/HanoiSearchTests.cpp: the first 121 layers (maxSteps 120)
+    # of the 4-peg search with 17..21 disks
+    dict(id="hanoi_17_120", file="HanoiTowersTests/HanoiSearchTests.cpp", test="Search17",
+         puzzle=dict(kind="hanoi", disks=17, symmetry=False), start="index:0", partial=True),
+    dict(id="hanoi_18_120", file="HanoiTowersTests/HanoiSearchTests.cpp", test="Search18",
+         puzzle=dict(kind="hanoi", disks=18, symmetry=False), start="index:0", partial=True),
+    dict(id="hanoi_19_120", file="HanoiTowersTests/HanoiSearchTests.cpp", test="Search19",
+         puzzle=dict(kind="hanoi", disks=19, symmetry=False), start="index:0", partial=True),
     dict(id="sliding_5x3_prefix", file="PuzzlesTests/PuzzleSearchTests.cpp", test="OptFrontierSearch_5x3_1th",
          puzzle=dict(kind="sliding", width=5, height=3), start="0 1 2 3 4 5 6 7 8 9 10 11 12 13 14",
          partial=True),

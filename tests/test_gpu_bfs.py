@@ -256,3 +256,20 @@ def test_hanoi16_full(ps):
     assert counts[:11] == [1, 3, 6, 12, 30, 30, 66, 96, 126, 210, 330]  # shared prefix with the 21-disk vector
     with psa.Bfs(psa.hanoi(16), segment_bits=20, block_threads=256, cross_marks=psa.CROSS_ATOMIC, graph=1) as b:
         assert b.run(0) == (counts, hashes)
+
+
+@pytest.mark.parametrize("vid,disks", [("hanoi_17_120", 17), ("hanoi_18_120", 18), ("hanoi_19_120", 19)])
+def test_hanoi_deep_prefix(ps, golden, vid, disks):
+    """4-peg Hanoi with 17, 18 and 19 disks, the whole 4^n index space in HBM
+    (three bit arrays of 2 GiB, 8 GiB and 32 GiB): the first 121 layers equal
+    the reference's vertical Hanoi search (HanoiTowersTests/HanoiSearchTests.cpp
+    Search17..Search19, maxSteps 120).  The 17-disk vector leaves the others
+    where the largest disk first moves, so moves of every disk are exercised."""
+    r = golden[vid]
+    print(f"hanoi {disks} disks, {len(r['counts'])} layers", flush=True)
+    with psa.Bfs(psa.hanoi(disks), segment_bits=24, cross_marks=psa.CROSS_ATOMIC,
+                 max_layers=len(r["counts"])) as b:
+        counts, _ = b.run(0)
+        st = b.stats()
+    assert counts == r["counts"], r["source"]
+    assert st.expanded == sum(r["counts"][:-1])  # every layer but the last is expanded

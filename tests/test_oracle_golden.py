@@ -93,6 +93,14 @@ def test_sparse_prefix_hanoi21(oracle, golden, vid):
     assert sparse_bfs(p, 0, len(r["counts"])) == r["counts"], r["source"]
 
 
+@pytest.mark.parametrize("vid", ["hanoi_17_120", "hanoi_18_120", "hanoi_19_120"])
+def test_sparse_prefix_hanoi_vertical(oracle, golden, vid):
+    # the first 24 of the 121 layers of HanoiTowersTests/HanoiSearchTests.cpp
+    r = golden[vid]
+    p = O.from_spec(r["puzzle"])
+    assert sparse_bfs(p, 0, 24) == r["counts"][:24], r["source"]
+
+
 def test_sparse_prefix_pancake13(oracle, golden):
     r = golden["pancake_13_prefix"]
     assert sparse_bfs(O.pancake(13), 0, 5) == r["counts"][:5]
