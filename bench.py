@@ -190,6 +190,30 @@ def tuned(psa, p, start, index_count=0, candidates=None):
     return res
 
 
+# options the autotuner picks for the config-3 searches on MI355X, fixed for
+# rocprofv3 runs (--profile-set) so that a profile does not depend on a tuning pass
+PROFILE_SETS = {
+    "sliding4x3": (lambda psa: psa.sliding(4, 3),
+                   dict(block_threads=512, segment_bits=20, cross_marks=1, direction=1)),
+    "hanoi16": (lambda psa: psa.hanoi(16), dict(block_threads=512, segment_bits=20, cross_marks=1)),
+}
+
+
+def profile_set(psa, name, runs):
+    """Eager batched searches of one config-3 workload (every kernel on the
+    search's stream, no graph), for kernel traces and counters."""
+    make, opts = PROFILE_SETS[name]
+    with psa.Bfs(make(psa), graph=0, **opts) as b:
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            counts, _ = b.run(0)
+            dt = time.perf_counter() - t0
+        st = b.stats()
+    return {"profile": name, "options": opts, "layers": len(counts), "states": sum(counts), "ms_last_run": dt * 1e3,
+            "expand_launches": st.expand_launches, "expand_ms": st.expand_ms, "finalize_ms": st.finalize_ms,
+            "children": st.children, "algorithmic_bytes": st.algorithmic_bytes}
+
+
 def batched_vs_plain(psa, name, p, start, steps, fixture_counts=None):
     """Batched persistent (tuned) vs plain per-segment launches of one search."""
     res = tuned(psa, p, start)
@@ -260,6 +284,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving runs")
     ap.add_argument("--profile", action="store_true",
                     help="only the event-timed batched search, --steps times (for rocprofv3 runs)")
+    ap.add_argument("--profile-set", choices=sorted(PROFILE_SETS),
+                    help="only eager batched searches of one config-3 workload, --steps times (for rocprofv3 runs)")
     ap.add_argument("--profile-segment", action="store_true",
                     help="only the 15-puzzle segment step, --steps times (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -273,6 +299,9 @@ def main():
     psa.init(0)
     if args.copy_calibration:
         copy_calibration(psa, args.copy_calibration << 20)
+        return
+    if args.profile_set:
+        print(json.dumps(profile_set(psa, args.profile_set, args.steps)))
         return
     if args.profile_segment:
         print(json.dumps({"profile": "segment_step", **segment_step_bench(psa, args.steps)}))
