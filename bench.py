@@ -31,6 +31,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -214,6 +216,90 @@ def profile_set(psa, name, runs):
             "children": st.children, "algorithmic_bytes": st.algorithmic_bytes}
 
 
+def codec_bench(psa, n_chunks=256, reps=5, seed=7):
+    """Device-resident codec throughput on frontier-like chunks of 2^18 sorted
+    ids (the reference's StreamVInt chunk size): StreamVInt with 4-bit bounds,
+    FrontierCompression (gaps of mean 16: StreamVInt chunks; mean 3: bitmap
+    chunks) and OpBits4.  Each figure is bytes in + bytes out over the call's
+    wall time (the encoders read chunk sizes back to the host once per call);
+    every round trip is checked."""
+    lib = psa.lib()
+    rng = np.random.default_rng(seed)
+    per = 1 << 18
+    n = n_chunks * per
+    out = {"values": n, "chunks": n_chunks}
+
+    def frontier(mean_gap):
+        gaps = rng.geometric(1.0 / mean_gap, size=(n_chunks, per)).astype(np.uint64)
+        v = np.cumsum(gaps, axis=1) + (np.arange(n_chunks, dtype=np.uint64) << np.uint64(32 - 8))[:, None]
+        return v.astype(np.uint32).reshape(-1)
+
+    def timed(fn):
+        fn()
+        psa.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        psa.sync()
+        return (time.perf_counter() - t0) / reps
+
+    starts = (np.arange(n_chunks + 1, dtype=np.uint64) * np.uint64(per))
+    d_st = psa.DeviceBuffer.from_array(starts)
+    cap = 4 * n_chunks + 17 * n // 4 + n // 2 + 4096
+    d_out = psa.DeviceBuffer(cap)
+    d_off = psa.DeviceBuffer(8 * (n_chunks + 1))
+    d_dec = psa.DeviceBuffer(4 * n + 64)
+    d_decb = psa.DeviceBuffer(n + 64)
+    d_vs = psa.DeviceBuffer(8 * (n_chunks + 1))
+
+    # StreamVInt with bounds
+    vals = frontier(16)
+    bounds = rng.integers(0, 16, size=n, dtype=np.uint8)
+    d_v = psa.DeviceBuffer.from_array(vals)
+    d_b = psa.DeviceBuffer.from_array(bounds)
+    enc = lambda: psa.check(lib.ps_svint_encode(d_v.ptr, d_b.ptr, d_st.ptr, n_chunks, d_out.ptr, cap, d_off.ptr, None))
+    t_enc = timed(enc)
+    nbytes = int(d_off.download(np.uint64, n_chunks + 1)[-1])
+    dec = lambda: psa.check(lib.ps_svint_decode(d_out.ptr, d_off.ptr, n_chunks, 1, d_dec.ptr, d_decb.ptr, n,
+                                                d_vs.ptr, None))
+    t_dec = timed(dec)
+    assert np.array_equal(d_dec.download(np.uint32, n), vals) and np.array_equal(d_decb.download(np.uint8, n), bounds)
+    moved = 5 * n + nbytes
+    out["svint"] = {"stream_bytes": nbytes, "encode_ms": t_enc * 1e3, "decode_ms": t_dec * 1e3,
+                    "encode_GBps": moved / t_enc / 1e9, "decode_GBps": moved / t_dec / 1e9,
+                    "encode_hbm_frac": moved / t_enc / 1e9 / HBM_PEAK_GBS,
+                    "decode_hbm_frac": moved / t_dec / 1e9 / HBM_PEAK_GBS}
+
+    # FrontierCompression, StreamVInt-coded (mean gap 16) and bitmap-coded (mean gap 3) chunks
+    for name, gap in (("fc_svint", 16), ("fc_bitmap", 3)):
+        vals = frontier(gap)
+        d_v.upload(vals)
+        enc = lambda: psa.check(lib.ps_fc_encode(d_v.ptr, d_st.ptr, n_chunks, 1, d_out.ptr, cap, d_off.ptr, None))
+        t_enc = timed(enc)
+        nbytes = int(d_off.download(np.uint64, n_chunks + 1)[-1])
+        dec = lambda: psa.check(lib.ps_fc_decode(d_out.ptr, d_off.ptr, n_chunks, d_dec.ptr, n, d_vs.ptr, None))
+        t_dec = timed(dec)
+        assert np.array_equal(d_dec.download(np.uint32, n), vals)
+        moved = 4 * n + nbytes
+        out[name] = {"stream_bytes": nbytes, "encode_ms": t_enc * 1e3, "decode_ms": t_dec * 1e3,
+                     "encode_GBps": moved / t_enc / 1e9, "decode_GBps": moved / t_dec / 1e9}
+
+    # OpBits4: 8 op-flag bytes <-> 4 bytes
+    ops = rng.integers(0, 16, size=n, dtype=np.uint8)
+    d_ops = psa.DeviceBuffer.from_array(ops)
+    enc = lambda: psa.check(lib.ps_opbits_encode(4, d_ops.ptr, n, d_out.ptr, None))
+    dec = lambda: psa.check(lib.ps_opbits_decode(4, d_out.ptr, n, d_decb.ptr, None))
+    t_enc = timed(enc)
+    t_dec = timed(dec)
+    assert np.array_equal(d_decb.download(np.uint8, n), ops)
+    moved = n + n // 2
+    out["opbits4"] = {"encode_ms": t_enc * 1e3, "decode_ms": t_dec * 1e3, "encode_GBps": moved / t_enc / 1e9,
+                      "decode_GBps": moved / t_dec / 1e9}
+    for buf in (d_st, d_out, d_off, d_dec, d_decb, d_vs, d_v, d_b, d_ops):
+        buf.free()
+    return out
+
+
 def batched_vs_plain(psa, name, p, start, steps, fixture_counts=None):
     """Batched persistent (tuned) vs plain per-segment launches of one search."""
     res = tuned(psa, p, start)
@@ -279,6 +365,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-segment-step", action="store_true")
+    ap.add_argument("--no-codecs", action="store_true", help="skip the device codec throughput runs")
+    ap.add_argument("--profile-codecs", action="store_true", help="only the codec runs (for rocprofv3 runs)")
     ap.add_argument("--no-batched-set", action="store_true", help="skip the 4x3 sliding and Hanoi-16 comparisons")
     ap.add_argument("--no-tune", action="store_true", help="use the default options instead of the autotuner's")
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving runs")
@@ -302,6 +390,9 @@ def main():
         return
     if args.profile_set:
         print(json.dumps(profile_set(psa, args.profile_set, args.steps)))
+        return
+    if args.profile_codecs:
+        print(json.dumps({"profile": "codecs", **codec_bench(psa)}))
         return
     if args.profile_segment:
         print(json.dumps({"profile": "segment_step", **segment_step_bench(psa, args.steps)}))
@@ -403,6 +494,11 @@ def main():
     if not args.no_serving:
         serving = serving_bench(psa)
 
+    codecs = None
+    if not args.no_codecs:
+        codecs = codec_bench(psa)
+        log("codecs:", codecs)
+
     seg_step = None
     if not args.no_segment_step:
         seg_step = segment_step_bench(psa, max(2, args.steps))
@@ -446,6 +542,7 @@ def main():
         "batched_vs_plain_seg26": (value / plain["seg26_value"]) if plain else None,
         "batched_set": batched_set,
         "segment_step": seg_step,
+        "codecs": codecs,
         "serving": serving,
         "parity_with_oracle_fixture": parity,
         "cpu_baseline": cpu,

@@ -43,6 +43,16 @@ __device__ __forceinline__ uint32_t chunk_of(const uint64_t* tuple_base, uint32_
     return lo;
 }
 
+// inclusive sum over the 64 lanes of a wave
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
 struct SviPlan {
     // per chunk
     const uint64_t* starts;      // value starts, nchunks+1
@@ -168,21 +178,88 @@ __global__ void k_read_counts(const uint8_t* in, const uint64_t* in_off, uint32_
     }
 }
 
-// sequential descriptor walk of one chunk per thread
-__global__ void k_svi_walk(const uint8_t* in, const uint64_t* in_off, const uint64_t* tuple_base,
-                           const uint32_t* select, uint32_t nsel, uint32_t* tuple_off, uint32_t* payload_end) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nsel) return;
+// Descriptor walk of one chunk per block, in parallel: a chunk's tuples
+// (descriptor byte + 1..4 value bytes each, 5..17 bytes) tile its payload
+// [4, E), E = chunk bytes - bound bytes.  Thread t owns payload bytes
+// [s0, s1); the first tuple starting at or after s0 starts at s0 + o,
+// o in [0, 16].  Each thread walks its range from all 17 entry offsets at once
+// (independent loads), recording where each walk leaves the range and how many
+// tuples it passed; thread 0 chains the ranges from entry 0; then every thread
+// walks its range once more from its true entry, writing the tuple offsets.
+// A chunk whose tuples do not end exactly at E counts in *err.
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_svi_walk_par(const uint8_t* in, const uint64_t* in_off,
+                                                      const uint64_t* tuple_base, const uint32_t* select,
+                                                      int with_bounds, uint32_t* tuple_off, uint32_t* payload_end,
+                                                      uint32_t* err) {
+    constexpr int NE = 17;
+    const uint32_t k = blockIdx.x;
     const uint32_t c = select ? select[k] : k;
     const uint8_t* p = in + in_off[c];
-    const uint64_t b = tuple_base[k], e = tuple_base[k + 1];
-    uint32_t pos = 4;
-    for (uint64_t t = b; t < e; ++t) {
-        tuple_off[t] = pos;
+    const uint64_t clen = in_off[c + 1] - in_off[c];
+    const uint64_t T = tuple_base[k + 1] - tuple_base[k];
+    const uint64_t bnd = with_bounds ? T * 2 : 0;  // (T*4 padded values) / 2
+    const uint64_t E = clen >= 4 + bnd ? clen - bnd : 4;
+    const uint64_t L = (E - 4 + TPB - 1) / TPB;
+    const uint64_t s0 = 4 + (uint64_t)threadIdx.x * L;
+    const uint64_t s1 = s0 + L < E ? s0 + L : (s0 < E ? E : s0);
+    __shared__ uint8_t s_ex[TPB][NE];
+    __shared__ uint16_t s_n[TPB][NE];  // tuples of a range: at most its bytes / 5
+    __shared__ uint32_t s_entry[TPB], s_base[TPB];
+    {
+        uint64_t pos[NE];
+        uint32_t n[NE];
+#pragma unroll
+        for (int o = 0; o < NE; ++o) {
+            pos[o] = s0 + (uint64_t)o;
+            n[o] = 0;
+        }
+        // all 17 descriptor loads of a step are issued before any is used; a
+        // walk that has left the range reads byte 0 of the chunk instead
+        // (a selected chunk is not empty) and does not move
+        bool any = s0 < s1;
+        while (any) {
+            uint32_t d[NE];
+#pragma unroll
+            for (int o = 0; o < NE; ++o) d[o] = p[pos[o] < s1 ? pos[o] : 0];
+            any = false;
+#pragma unroll
+            for (int o = 0; o < NE; ++o) {
+                const bool live = pos[o] < s1;
+                const uint32_t x = d[o];
+                pos[o] += live ? 5 + (x & 3) + ((x >> 2) & 3) + ((x >> 4) & 3) + ((x >> 6) & 3) : 0;
+                n[o] += live ? 1u : 0u;
+                any |= live;
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < NE; ++o) {
+            s_ex[threadIdx.x][o] = (uint8_t)(pos[o] - s1);  // 0..16
+            s_n[threadIdx.x][o] = (uint16_t)n[o];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t o = 0;
+        uint64_t base = 0;
+        for (int t = 0; t < TPB; ++t) {
+            s_entry[t] = o;
+            s_base[t] = (uint32_t)base;
+            base += s_n[t][o];
+            o = s_ex[t][o];
+        }
+        payload_end[k] = (uint32_t)E;
+        if (o != 0 || base != T || (T && clen < 4 + bnd)) atomicAdd(err, 1u);
+    }
+    __syncthreads();
+    uint64_t pos = s0 + s_entry[threadIdx.x];
+    uint64_t i = s_base[threadIdx.x];
+    uint32_t* toff = tuple_off + tuple_base[k];
+    while (pos < s1 && i < T) {
+        toff[i++] = (uint32_t)pos;
         const uint32_t d = p[pos];
         pos += 5 + (d & 3) + ((d >> 2) & 3) + ((d >> 4) & 3) + ((d >> 6) & 3);
     }
-    payload_end[k] = pos;
 }
 
 __global__ void k_svi_decode_tuples(const uint8_t* in, const uint64_t* in_off, const uint64_t* tuple_base,
@@ -205,37 +282,57 @@ __global__ void k_svi_decode_tuples(const uint8_t* in, const uint64_t* in_off, c
 }
 
 // inclusive scan of one chunk's deltas (one block per chunk), first `count`
-// results to the output; bounds decoded alongside
+// results to the output; bounds decoded alongside.  The chunk is scanned in
+// tiles of 4*TPB deltas: each thread loads four consecutive deltas as one
+// 16-byte vector (coalesced), the block scans the per-thread sums, and the
+// running total carries from tile to tile.
 template <int TPB>
 __global__ __launch_bounds__(TPB) void k_svi_finish(const uint32_t* deltas, const uint64_t* tuple_base,
                                                     const uint32_t* counts, const uint32_t* select,
                                                     const uint64_t* value_starts, uint32_t* out,
                                                     const uint8_t* in, const uint64_t* in_off,
                                                     const uint32_t* payload_end, uint8_t* bounds_out) {
+    constexpr int NW = TPB / 64;
     const uint32_t k = blockIdx.x;
     const uint32_t c = select ? select[k] : k;
-    const uint64_t b = tuple_base[k] * 4, e = tuple_base[k + 1] * 4;
-    const uint64_t n = e - b;
-    const uint64_t per = (n + TPB - 1) / TPB;
-    const uint64_t i0 = b + threadIdx.x * per, i1 = (i0 + per < e) ? i0 + per : e;
-    uint32_t s = 0;
-    for (uint64_t i = i0; i < i1; ++i) s += deltas[i];
-    __shared__ uint32_t part[TPB];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < TPB; o <<= 1) {
-        const uint32_t x = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
-        __syncthreads();
-        part[threadIdx.x] += x;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;
+    const uint64_t t0 = tuple_base[k], t1 = tuple_base[k + 1];  // one tuple = four deltas, 16-byte aligned
     const uint32_t count = counts[c];
     uint32_t* o = out + value_starts[c];
-    for (uint64_t i = i0; i < i1; ++i) {
-        run += deltas[i];
-        const uint64_t j = i - b;
-        if (j < count) o[j] = run;
+    const uint4* d4 = reinterpret_cast<const uint4*>(deltas);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __shared__ uint32_t s_w[2][NW];
+    uint32_t carry = 0, par = 0;
+    for (uint64_t q0 = t0; q0 < t1; q0 += TPB, par ^= 1u) {
+        const uint64_t q = q0 + threadIdx.x;
+        uint4 v = q < t1 ? d4[q] : make_uint4(0u, 0u, 0u, 0u);
+        v.y += v.x;
+        v.z += v.y;
+        v.w += v.z;
+        const uint32_t incl = wave_incl_scan_u32(v.w, lane);
+        if (lane == 63) s_w[par][wave] = incl;
+        __syncthreads();
+        // s_w[par] is written again two tiles later, after the next tile's
+        // barrier, which every thread reaches only once done here
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t x = s_w[par][w];
+            before += (w < wave) ? x : 0u;
+            total += x;
+        }
+        const uint32_t base = carry + before + incl - v.w;
+        const uint64_t j = (q - t0) * 4;
+        if (j + 3 < count) {
+            o[j] = base + v.x;
+            o[j + 1] = base + v.y;
+            o[j + 2] = base + v.z;
+            o[j + 3] = base + v.w;
+        } else if (q < t1) {
+            const uint32_t r[4] = {v.x, v.y, v.z, v.w};
+            for (int e = 0; e < 4; ++e)
+                if (j + e < count) o[j + e] = base + r[e];
+        }
+        carry += total;
     }
     if (bounds_out) {
         const uint8_t* q = in + in_off[c] + payload_end[k];
@@ -286,9 +383,23 @@ __global__ void k_fc_bitmap_set(const uint32_t* v, const uint64_t* starts, const
     const uint64_t b = starts[c], e = starts[c + 1];
     const uint32_t f = first[c];
     unsigned long long* w = words + word_base[k];
-    for (uint64_t i = b + 1 + threadIdx.x; i < e; i += blockDim.x) {
-        const uint64_t val = (uint64_t)v[i] - f - 1;
-        atomicOr(&w[val >> 6], 1ull << (val & 63));
+    // each thread ORs a run of RUN consecutive values, one atomic per word the
+    // run touches (sorted values put neighbours in the same word)
+    constexpr uint64_t RUN = 32;
+    for (uint64_t r0 = b + 1 + (uint64_t)threadIdx.x * RUN; r0 < e; r0 += (uint64_t)blockDim.x * RUN) {
+        const uint64_t r1 = r0 + RUN < e ? r0 + RUN : e;
+        uint64_t cw = ~0ull;
+        unsigned long long m = 0;
+        for (uint64_t i = r0; i < r1; ++i) {
+            const uint64_t val = (uint64_t)v[i] - f - 1;
+            if ((val >> 6) != cw) {
+                if (m) atomicOr(&w[cw], m);
+                cw = val >> 6;
+                m = 0;
+            }
+            m |= 1ull << (val & 63);
+        }
+        if (m) atomicOr(&w[cw], m);
     }
 }
 
@@ -509,14 +620,22 @@ int svi_decode_chunks(const uint8_t* d_in, const uint64_t* d_in_off, const std::
         t += (uint64_t)((cnt + 7) / 8 * 8) / 4;
     }
     tb[n] = t;
-    DevBuf d_sel, d_tb, d_toff, d_end, d_deltas;
+    DevBuf d_sel, d_tb, d_toff, d_end, d_deltas, d_err;
     if (int r = to_device(d_sel, sel, s)) return r;
     if (int r = to_device(d_tb, tb, s)) return r;
     if (int r = d_toff.alloc((t + 1) * 4, s)) return r;
     if (int r = d_end.alloc((size_t)n * 4, s)) return r;
     if (int r = d_deltas.alloc((t + 1) * 16, s)) return r;
-    hipLaunchKernelGGL(k_svi_walk, dim3(blocks_for(n, 64)), dim3(64), 0, s, d_in, d_in_off, d_tb.as<uint64_t>(),
-                       d_sel.as<uint32_t>(), n, d_toff.as<uint32_t>(), d_end.as<uint32_t>());
+    const std::vector<uint32_t> zero(1, 0u);
+    if (int r = to_device(d_err, zero, s)) return r;
+    hipLaunchKernelGGL(k_svi_walk_par<1024>, dim3(n), dim3(1024), 0, s, d_in, d_in_off, d_tb.as<uint64_t>(),
+                       d_sel.as<uint32_t>(), d_bounds ? 1 : 0, d_toff.as<uint32_t>(), d_end.as<uint32_t>(),
+                       d_err.as<uint32_t>());
+    PS_HIP_CHECK(hipGetLastError());
+    std::vector<uint32_t> bad;
+    if (int r = to_host(bad, d_err.p, 1, s)) return r;
+    if (bad[0])
+        return fail(PS_ERR_INVALID_ARGUMENT, "%u StreamVInt chunks do not end where their tuples end", bad[0]);
     if (t)
         hipLaunchKernelGGL(k_svi_decode_tuples, dim3(blocks_for(t, 256)), dim3(256), 0, s, d_in, d_in_off,
                            d_tb.as<uint64_t>(), d_sel.as<uint32_t>(), n, t, d_toff.as<uint32_t>(),

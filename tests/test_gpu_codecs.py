@@ -105,6 +105,21 @@ def test_svint_reference_frontier_case(C):
     assert np.array_equal(back[0], vals) and np.array_equal(bb[0], bnd)
 
 
+@pytest.mark.parametrize("cut", [1, 5, 16])
+def test_svint_decode_rejects_truncated_chunk(C, ps, cut):
+    # a chunk whose bytes end before its last tuple (or inside it) is refused
+    # before any value is decoded; the chunk after it is intact
+    rng = np.random.default_rng(13)
+    chunks = [sorted_unique(rng, 5000, 1 << 32), sorted_unique(rng, 3000, 1 << 32)]
+    data, off = C.svint_encode(chunks)
+    bad = np.concatenate([data[:off[1] - cut], data[off[1]:]])
+    boff = np.array([0, off[1] - cut, off[2] - cut], np.uint64)
+    with pytest.raises(ps.PSError, match="tuples end"):
+        C.svint_decode(bad, boff)
+    back, _ = C.svint_decode(data, off)  # the valid stream still decodes
+    assert np.array_equal(back[0], chunks[0]) and np.array_equal(back[1], chunks[1])
+
+
 def test_svint_rejects_oversized_chunk(C, ps):
     with pytest.raises(ps.PSError):
         C.svint_encode([np.arange((1 << 18) + 1, dtype=np.uint32)])
