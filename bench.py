@@ -102,9 +102,56 @@ def cpu_baseline(psa, puzzle, target_seconds=12.0):
         first += span
     span = first
     children = ctypes.c_uint64(nchildren)
-    return {"value": parents / dt, "unit": "expanded states/s", "cores": 1, "kind": "port",
-            "sample": f"first {span} indexes of layer {best_layer} of the {n}-pancake BFS "
-                      f"({parents} parents, {children.value} children marked, {dt:.1f} s, 1 thread)"}
+    single = {"value": parents / dt, "unit": "expanded states/s", "cores": 1, "kind": "port",
+              "sample": f"first {span} indexes of layer {best_layer} of the {n}-pancake BFS "
+                        f"({parents} parents, {children.value} children marked, {dt:.1f} s, 1 thread)"}
+    return single, cpu_baseline_parallel(op, frontier, N, best_layer, n)
+
+
+def cpu_baseline_parallel(op, frontier, N, layer, n, target_seconds=10.0):
+    """The same expand+mark on every host core the box grants (at most 16),
+    ParallelExec-style (Common/ThreadUtil.h:19-36): threads take 2^22-index
+    slices of the layer from a shared counter, each marking into its own next
+    array, until target_seconds of wall time."""
+    import ctypes
+    import threading
+
+    import numpy as np
+    import oracle_lib as O
+    L = O.lib()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(16, cores))
+    step = 1 << 22
+    lock = threading.Lock()
+    state = {"next": 0, "parents": 0, "children": 0}
+    deadline = time.perf_counter() + target_seconds
+
+    def worker():
+        nxt = np.zeros_like(frontier)
+        ch = ctypes.c_uint64(0)
+        while time.perf_counter() < deadline:
+            with lock:
+                first = state["next"] % N
+                state["next"] += step
+            span = min(step, N - first)
+            got = L.or_bfs3_expand_range(ctypes.byref(op), frontier, nxt, first, span, ctypes.byref(ch))
+            with lock:
+                state["parents"] += got
+                state["children"] += ch.value
+
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=worker) for _ in range(cores)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    dt = time.perf_counter() - t0
+    return {"value": state["parents"] / dt, "unit": "expanded states/s", "cores": cores, "kind": "port",
+            "sample": f"2^22-index slices of layer {layer} of the {n}-pancake BFS on {cores} threads "
+                      f"({state['parents']} parents, {state['children']} children marked, {dt:.1f} s)"}
 
 
 def copy_calibration(psa, nbytes, reps=5):
@@ -504,10 +551,11 @@ def main():
         seg_step = segment_step_bench(psa, max(2, args.steps))
         log("segment step:", seg_step)
 
-    cpu = None
+    cpu = cpu_par = None
     if not args.no_cpu:
-        cpu = cpu_baseline(psa, p)
+        cpu, cpu_par = cpu_baseline(psa, p)
         log("cpu baseline:", cpu)
+        log("cpu baseline, all cores:", cpu_par)
 
     value = N / dt_b
     kernel = dominant_kernel(psa, opts)
@@ -546,6 +594,7 @@ def main():
         "serving": serving,
         "parity_with_oracle_fixture": parity,
         "cpu_baseline": cpu,
+        "cpu_baseline_parallel": cpu_par,
         "device": props["name"],
     }
     print(json.dumps(out))
