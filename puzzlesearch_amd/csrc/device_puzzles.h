@@ -381,11 +381,13 @@ struct Packed {
     // Slots whose child may lie outside the parent's segment of 2^seg_bits
     // indexes.  A horizontal move changes only the blank cell, index -+ 1
     // without carry, so it stays in any segment of 16 or more indexes.  With
-    // the 15-puzzle's own segments (seg_bits 32: the segment is the packed
-    // values of tile-list positions 12..14) a vertical move changes the
-    // segment only if it reorders a list position >= 12: the packed value of
-    // a position depends only on the set of tiles left of it.  Up from blank
-    // cell z reorders positions z-W .. z-1, down reorders z .. z+W-1.
+    // the 15-puzzle's own segments (seg_bits 32) the segment is the packed
+    // values of tile-list positions 12..14, which name the last three tiles of
+    // the list in order (the value of position 14 is its tile, that of 13 its
+    // tile less one if tile 14 is smaller, ...).  A vertical move rotates W
+    // consecutive list positions -- up from blank cell z positions z-W .. z-1,
+    // down z .. z+W-1 -- and puts another tile at the highest of them.  So it
+    // leaves the segment exactly when that position is 12 or more.
     __device__ __forceinline__ static uint32_t cross_slots(uint64_t index, int seg_bits) {
         const int z = (int)(index & 15u);
         uint32_t m = seg_bits >= 4 ? 0x3u : 0xFu;
@@ -394,6 +396,13 @@ struct Packed {
             if (z + W - 1 < 12) m &= ~0x2u;
         }
         return m;
+    }
+
+    // Slots whose child may lie inside the parent's segment: with the
+    // 15-puzzle's own segments, the complement of cross_slots (exact, above);
+    // otherwise every slot.
+    __device__ __forceinline__ static uint32_t in_slots(uint64_t index, int seg_bits) {
+        return (S == 16 && seg_bits == 32) ? (~cross_slots(index, seg_bits) & 0xFu) : 0xFu;
     }
 
     // slots that have a child: the move exists and its op bit is not set

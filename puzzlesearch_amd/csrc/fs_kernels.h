@@ -99,6 +99,13 @@ __device__ __forceinline__ uint32_t fs_live(uint32_t w, const unsigned long long
     return nz;
 }
 
+// puzzles that can name the slots whose child may leave the segment
+// (PZ::cross_slots(index, seg_bits)); the others expand every slot
+template <class PZ, class = void>
+struct HasCrossSlots : std::false_type {};
+template <class PZ>
+struct HasCrossSlots<PZ, std::void_t<decltype(PZ::cross_slots(0ull, 0))>> : std::true_type {};
+
 // One OR per flag word: consecutive children of a parent that share a word
 // (the horizontal moves of the sliding puzzles: index -+ 1) are merged before
 // the atomic.
@@ -113,8 +120,7 @@ struct FsMarkSink {
     __device__ __forceinline__ void flush() {
         if (!pend_m) return;
         gor(&flags[pend_w], pend_m);
-        uint8_t* d = &dirty[pend_w >> (FS_REGION_BITS - 3)];
-        if (!*d) *d = 1;
+        dirty[pend_w >> (FS_REGION_BITS - 3)] = 1;  // a plain store: no read to wait for
         pend_m = 0;
     }
     __device__ __forceinline__ void operator()(int, uint64_t child, uint32_t label) {
@@ -144,7 +150,10 @@ __global__ __launch_bounds__(TPB) void k_fs_mark_in(FsMarkArgs a) {
             continue;
         }
         if (a.odd) atomicOr(&a.odd[id >> 6], 1ull << (id & 63));
-        PZ::expand(a.base | id, (uint32_t)a.ops[i], a.prm, sink);
+        if constexpr (HasCrossSlots<PZ>::value)  // skip the moves that certainly leave the segment
+            PZ::expand(a.base | id, (uint32_t)a.ops[i], a.prm, sink, PZ::in_slots(a.base | id, a.seg_bits));
+        else
+            PZ::expand(a.base | id, (uint32_t)a.ops[i], a.prm, sink);
         sink.flush();
         sink.pend_w = ~0ull;
     }
@@ -351,12 +360,6 @@ struct CrossSink {
 
 constexpr uint32_t FS_CROSS_PER_BLOCK = 4096;  // parents per block of the cross pass
 
-// puzzles that can name the slots whose child may leave the segment
-// (PZ::cross_slots(index, seg_bits)); the others expand every slot
-template <class PZ, class = void>
-struct HasCrossSlots : std::false_type {};
-template <class PZ>
-struct HasCrossSlots<PZ, std::void_t<decltype(PZ::cross_slots(0ull, 0))>> : std::true_type {};
 
 // Block b expands parents [b*4096, (b+1)*4096) and writes their cross
 // children, compacted in (parent, slot) order, to the front of its own area

@@ -101,8 +101,7 @@ __global__ void k_fs_mark_xseg(FsXsegArgs a) {
         while (op + 1 < a.ops && a.op_starts[op + 1] <= i) ++op;
         const uint64_t c = a.idx[i] & a.seg_mask;
         gor(&a.flags[c >> 3], (1u << op) << ((c & 7) * 4));
-        uint8_t* d = &a.dirty[c >> FS_REGION_BITS];
-        if (!*d) *d = 1;
+        a.dirty[c >> FS_REGION_BITS] = 1;
     }
 }
 
