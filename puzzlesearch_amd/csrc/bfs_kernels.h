@@ -310,17 +310,25 @@ __device__ __forceinline__ void expand_tile(const ExpandArgs& a, uint64_t tile, 
 
     if (tid == 0) tally.scan += pw1 - pw0;
     const uint64_t win_lo = w0 * 64, win_hi = w1 * 64;
+    // states of the window below nbits: one unsigned compare places a child
+    // in the window; a child at or past nbits takes the other path and is reported
+    const uint64_t win_states = a.use_window ? (win_hi < a.nbits ? win_hi : a.nbits) - win_lo : 0;
     bool local_any = false;
+    uint32_t n_local = 0;  // children marked in the window (added to the tally at the end)
     auto mark = [&](uint64_t child) {
+        const uint64_t rel = child - win_lo;
+        if (rel < win_states) {
+            lor(&win[rel >> 5], 1u << (rel & 31));
+            local_any = true;
+            ++n_local;
+            return;
+        }
         if (child >= a.nbits) {
             atomicOr(&st->error, 1u);
             return;
         }
         ++tally.children;
-        if (a.use_window && child >= win_lo && child < win_hi) {
-            lor(&win[(child - win_lo) >> 5], 1u << (child & 31));
-            local_any = true;
-        } else {
+        {
             if (a.skip_visited && (((cur[child >> 6] | old[child >> 6]) >> (child & 63)) & 1ull)) return;
             if (a.xmark) a.xmark[child] = 1;
             else gor(&nxt[child >> 5], 1u << (child & 31));
@@ -369,6 +377,7 @@ __device__ __forceinline__ void expand_tile(const ExpandArgs& a, uint64_t tile, 
     }
     wave_sync();
     run_batch((uint32_t)qn);
+    tally.children += n_local;
     if (local_any) *s_local = 1;
     __syncthreads();
     if (a.use_window) {
