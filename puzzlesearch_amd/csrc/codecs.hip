@@ -200,6 +200,12 @@ __global__ __launch_bounds__(TPB) void k_svi_walk_par(const uint8_t* in, const u
     const uint64_t T = tuple_base[k + 1] - tuple_base[k];
     const uint64_t bnd = with_bounds ? T * 2 : 0;  // (T*4 padded values) / 2
     const uint64_t E = clen >= 4 + bnd ? clen - bnd : 4;
+    // T tuples take 5T..17T bytes: any other payload length is refused
+    // before it is walked (this also bounds the per-range tuple counts below)
+    if (E - 4 < 5 * T || E - 4 > 17 * T) {
+        if (threadIdx.x == 0) atomicAdd(err, 1u);
+        return;
+    }
     const uint64_t L = (E - 4 + TPB - 1) / TPB;
     const uint64_t s0 = 4 + (uint64_t)threadIdx.x * L;
     const uint64_t s1 = s0 + L < E ? s0 + L : (s0 < E ? E : s0);
@@ -249,7 +255,7 @@ __global__ __launch_bounds__(TPB) void k_svi_walk_par(const uint8_t* in, const u
             o = s_ex[t][o];
         }
         payload_end[k] = (uint32_t)E;
-        if (o != 0 || base != T || (T && clen < 4 + bnd)) atomicAdd(err, 1u);
+        if (o != 0 || base != T || clen < 4 + bnd) atomicAdd(err, 1u);
     }
     __syncthreads();
     uint64_t pos = s0 + s_entry[threadIdx.x];

@@ -105,13 +105,16 @@ def test_svint_reference_frontier_case(C):
     assert np.array_equal(back[0], vals) and np.array_equal(bb[0], bnd)
 
 
-@pytest.mark.parametrize("cut", [1, 5, 16])
+@pytest.mark.parametrize("cut", [1, 5, 16, "all_but_header"])
 def test_svint_decode_rejects_truncated_chunk(C, ps, cut):
-    # a chunk whose bytes end before its last tuple (or inside it) is refused
-    # before any value is decoded; the chunk after it is intact
+    # a chunk whose bytes end before its last tuple (or inside it, or right
+    # after its count) is refused before any value is decoded; the chunk
+    # after it is intact
     rng = np.random.default_rng(13)
     chunks = [sorted_unique(rng, 5000, 1 << 32), sorted_unique(rng, 3000, 1 << 32)]
     data, off = C.svint_encode(chunks)
+    if cut == "all_but_header":
+        cut = int(off[1]) - 4
     bad = np.concatenate([data[:off[1] - cut], data[off[1]:]])
     boff = np.array([0, off[1] - cut, off[2] - cut], np.uint64)
     with pytest.raises(ps.PSError, match="tuples end"):
