@@ -251,15 +251,17 @@ __global__ __launch_bounds__(TPB) void k_fs_write(FsScanArgs a) {
                 uint4 v[CH];
 #pragma unroll
                 for (uint32_t k = 0; k < CH; ++k) v[k] = f4[(k0 + k) * TPB + threadIdx.x];
-                uint32_t c[CH], incl[CH];
+                uint32_t c[CH], incl[CH], keep[CH][4];
 #pragma unroll
                 for (uint32_t k = 0; k < CH; ++k) {
                     const uint32_t ws[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
                     const uint64_t j0 = w0 + ((uint64_t)(k0 + k) * TPB + threadIdx.x) * 4;
                     c[k] = 0;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (ws[e]) c[k] += __popc(fs_live(ws[e], a.odd, j0 + e) & ~nib_full(ws[e], a.ops));
+                    for (int e = 0; e < 4; ++e) {
+                        keep[k][e] = ws[e] ? fs_live(ws[e], a.odd, j0 + e) & ~nib_full(ws[e], a.ops) : 0u;
+                        c[k] += __popc(keep[k][e]);
+                    }
                     incl[k] = wave_incl_scan(c[k], lane);
                     if (lane == 63) tot[par][k][wave] = incl[k];
                 }
@@ -279,8 +281,7 @@ __global__ __launch_bounds__(TPB) void k_fs_write(FsScanArgs a) {
                     const uint64_t j0 = w0 + ((uint64_t)(k0 + k) * TPB + threadIdx.x) * 4;
                     uint64_t pos = running + before + incl[k] - c[k];
                     for (int e = 0; e < 4; ++e) {
-                        if (!ws[e]) continue;
-                        uint32_t kb = fs_live(ws[e], a.odd, j0 + e) & ~nib_full(ws[e], a.ops);
+                        uint32_t kb = keep[k][e];
                         while (kb) {
                             const int b = __ffs(kb) - 1;  // bit 4k of nibble k
                             kb &= kb - 1;
