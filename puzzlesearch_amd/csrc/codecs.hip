@@ -81,32 +81,44 @@ __global__ void k_svi_tuple_sizes(const uint32_t* v, SviPlan pl, uint32_t* sizes
 }
 
 // exclusive scan of tuple sizes within each chunk (one block per chunk);
-// chunk_payload[c] = sum of the chunk's tuple sizes
+// chunk_payload[c] = sum of the chunk's tuple sizes.  The chunk is scanned
+// in tiles of 4*TPB sizes, four consecutive sizes per thread (a wave reads
+// 1 KB at a time), the running total carried from tile to tile.
 template <int TPB>
 __global__ __launch_bounds__(TPB) void k_chunk_scan_u32(uint32_t* vals, const uint64_t* base, uint64_t* chunk_total) {
+    constexpr int NW = TPB / 64;
     const uint32_t c = blockIdx.x;
     const uint64_t b = base[c], e = base[c + 1];
-    const uint64_t n = e - b;
-    const uint64_t per = (n + TPB - 1) / TPB;
-    const uint64_t i0 = b + threadIdx.x * per, i1 = (i0 + per < e) ? i0 + per : e;
-    uint64_t s = 0;
-    for (uint64_t i = i0; i < i1; ++i) s += vals[i];
-    __shared__ uint64_t part[TPB];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < TPB; o <<= 1) {
-        const uint64_t x = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __shared__ uint32_t s_w[2][NW];
+    uint32_t carry = 0, par = 0;
+    for (uint64_t q0 = b; q0 < e; q0 += 4 * TPB, par ^= 1u) {
+        const uint64_t i = q0 + 4 * (uint64_t)threadIdx.x;
+        uint32_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = i + k < e ? vals[i + k] : 0u;
+        const uint32_t s4 = x[0] + x[1] + x[2] + x[3];
+        const uint32_t incl = wave_incl_scan_u32(s4, lane);
+        if (lane == 63) s_w[par][wave] = incl;
         __syncthreads();
-        part[threadIdx.x] += x;
-        __syncthreads();
+        // s_w[par] is written again two tiles later, after the next tile's
+        // barrier, which every thread reaches only once done here
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t t = s_w[par][w];
+            before += (w < wave) ? t : 0u;
+            total += t;
+        }
+        uint32_t run = carry + before + incl - s4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i + k < e) vals[i + k] = run;
+            run += x[k];
+        }
+        carry += total;
     }
-    uint64_t run = part[threadIdx.x] - s;
-    for (uint64_t i = i0; i < i1; ++i) {
-        const uint32_t x = vals[i];
-        vals[i] = (uint32_t)run;
-        run += x;
-    }
-    if (threadIdx.x == TPB - 1) chunk_total[c] = part[TPB - 1];
+    if (threadIdx.x == 0) chunk_total[c] = carry;
 }
 
 __global__ void k_svi_write(const uint32_t* v, const uint8_t* bounds, SviPlan pl, const uint32_t* tuple_off,
