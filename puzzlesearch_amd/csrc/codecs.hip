@@ -60,7 +60,28 @@ struct SviPlan {
     const uint32_t* counts;
     uint32_t nchunks;
     uint64_t ntuples;
+    uint32_t max_tuples;  // of one chunk
+    int32_t grid2d;       // launches are (tuple blocks of a chunk) x (chunk): no search for the chunk
 };
+
+// The tuple a thread encodes and its chunk; false for a thread past the end.
+__device__ __forceinline__ bool svi_tuple(const SviPlan& pl, uint64_t& t, uint32_t& c) {
+    if (pl.grid2d) {
+        c = blockIdx.y;
+        t = pl.tuple_base[c] + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        return t < pl.tuple_base[c + 1];
+    }
+    t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pl.ntuples) return false;
+    c = chunk_of(pl.tuple_base, pl.nchunks, t);
+    return true;
+}
+
+// launch shape of the per-tuple kernels for a plan
+inline dim3 svi_grid(const SviPlan& pl) {
+    if (pl.grid2d) return dim3((pl.max_tuples + 255) / 256, pl.nchunks);
+    return dim3((uint32_t)((pl.ntuples + 255) / 256));
+}
 
 // delta of value j of chunk c (padding values are 0)
 __device__ __forceinline__ uint32_t delta_at(const uint32_t* v, uint64_t start, uint32_t count, uint32_t j) {
@@ -70,9 +91,9 @@ __device__ __forceinline__ uint32_t delta_at(const uint32_t* v, uint64_t start, 
 }
 
 __global__ void k_svi_tuple_sizes(const uint32_t* v, SviPlan pl, uint32_t* sizes) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= pl.ntuples) return;
-    const uint32_t c = chunk_of(pl.tuple_base, pl.nchunks, t);
+    uint64_t t;
+    uint32_t c;
+    if (!svi_tuple(pl, t, c)) return;
     const uint32_t lt = (uint32_t)(t - pl.tuple_base[c]);
     uint32_t sz = 1;
 #pragma unroll
@@ -123,9 +144,9 @@ __global__ __launch_bounds__(TPB) void k_chunk_scan_u32(uint32_t* vals, const ui
 
 __global__ void k_svi_write(const uint32_t* v, const uint8_t* bounds, SviPlan pl, const uint32_t* tuple_off,
                             const uint64_t* chunk_out, const uint64_t* chunk_payload, uint8_t* out) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= pl.ntuples) return;
-    const uint32_t c = chunk_of(pl.tuple_base, pl.nchunks, t);
+    uint64_t t;
+    uint32_t c;
+    if (!svi_tuple(pl, t, c)) return;
     const uint32_t lt = (uint32_t)(t - pl.tuple_base[c]);
     const uint64_t start = pl.starts[c];
     const uint32_t count = pl.counts[c];
@@ -584,9 +605,14 @@ int svi_sizes(const uint32_t* d_values, const uint8_t* d_bounds, const std::vect
     if (int r = to_device(d_counts, cnt, s)) return r;
     if (int r = d_toff.alloc((t + 1) * 4, s)) return r;
     if (int r = d_payload.alloc((n + 1) * 8, s)) return r;
-    pl = SviPlan{d_starts_sel.as<uint64_t>(), d_tbase.as<uint64_t>(), d_counts.as<uint32_t>(), n, t};
-    if (t) hipLaunchKernelGGL(k_svi_tuple_sizes, dim3(blocks_for(t, 256)), dim3(256), 0, s, d_values, pl,
-                              d_toff.as<uint32_t>());
+    uint64_t max_t = 0;
+    for (uint32_t k = 0; k < n; ++k) max_t = std::max<uint64_t>(max_t, tb[k + 1] - tb[k]);
+    // one block row per chunk when the rows of the widest chunk waste at most
+    // about as many threads as there are tuples
+    const bool g2 = n > 0 && n <= 65535 && ((max_t + 255) / 256) * 256 * n <= 2 * t + 256ull * n;
+    pl = SviPlan{d_starts_sel.as<uint64_t>(), d_tbase.as<uint64_t>(), d_counts.as<uint32_t>(), n, t,
+                 (uint32_t)max_t, g2 ? 1 : 0};
+    if (t) hipLaunchKernelGGL(k_svi_tuple_sizes, svi_grid(pl), dim3(256), 0, s, d_values, pl, d_toff.as<uint32_t>());
     if (n) hipLaunchKernelGGL(k_chunk_scan_u32<256>, dim3(n), dim3(256), 0, s, d_toff.as<uint32_t>(), pl.tuple_base,
                               d_payload.as<uint64_t>());
     PS_HIP_CHECK(hipGetLastError());
@@ -609,7 +635,7 @@ int svi_write(const uint32_t* d_values, const uint8_t* d_bounds, const SviPlan& 
     DevBuf d_off;
     if (int r = to_device(d_off, out_off_sel, s)) return r;
     if (pl.ntuples)
-        hipLaunchKernelGGL(k_svi_write, dim3(blocks_for(pl.ntuples, 256)), dim3(256), 0, s, d_values, d_bounds, pl,
+        hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, d_bounds, pl,
                            d_toff.as<uint32_t>(), d_off.as<uint64_t>(), d_payload.as<uint64_t>(), d_out);
     PS_HIP_CHECK(hipGetLastError());
     PS_HIP_CHECK(hipStreamSynchronize(s));
