@@ -301,12 +301,21 @@ __global__ __launch_bounds__(TPB) void k_svi_walk_par(const uint8_t* in, const u
     }
 }
 
+// grid2d: block row y decodes selected chunk y (no search for a tuple's chunk)
 __global__ void k_svi_decode_tuples(const uint8_t* in, const uint64_t* in_off, const uint64_t* tuple_base,
-                                    const uint32_t* select, uint32_t nsel, uint64_t ntuples,
+                                    const uint32_t* select, uint32_t nsel, uint64_t ntuples, int grid2d,
                                     const uint32_t* tuple_off, uint32_t* deltas) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntuples) return;
-    const uint32_t k = chunk_of(tuple_base, nsel, t);
+    uint64_t t;
+    uint32_t k;
+    if (grid2d) {
+        k = blockIdx.y;
+        t = tuple_base[k] + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= tuple_base[k + 1]) return;
+    } else {
+        t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= ntuples) return;
+        k = chunk_of(tuple_base, nsel, t);
+    }
     const uint32_t c = select ? select[k] : k;
     const uint8_t* p = in + in_off[c] + tuple_off[t];
     const uint32_t desc = p[0];
@@ -680,10 +689,14 @@ int svi_decode_chunks(const uint8_t* d_in, const uint64_t* d_in_off, const std::
     if (int r = to_host(bad, d_err.p, 1, s)) return r;
     if (bad[0])
         return fail(PS_ERR_INVALID_ARGUMENT, "%u StreamVInt chunks do not end where their tuples end", bad[0]);
-    if (t)
-        hipLaunchKernelGGL(k_svi_decode_tuples, dim3(blocks_for(t, 256)), dim3(256), 0, s, d_in, d_in_off,
-                           d_tb.as<uint64_t>(), d_sel.as<uint32_t>(), n, t, d_toff.as<uint32_t>(),
-                           d_deltas.as<uint32_t>());
+    if (t) {
+        uint64_t max_t = 0;
+        for (uint32_t k = 0; k < n; ++k) max_t = std::max<uint64_t>(max_t, tb[k + 1] - tb[k]);
+        const bool g2 = n <= 65535 && ((max_t + 255) / 256) * 256 * n <= 2 * t + 256ull * n;
+        const dim3 grid = g2 ? dim3((uint32_t)((max_t + 255) / 256), n) : dim3((uint32_t)blocks_for(t, 256));
+        hipLaunchKernelGGL(k_svi_decode_tuples, grid, dim3(256), 0, s, d_in, d_in_off, d_tb.as<uint64_t>(),
+                           d_sel.as<uint32_t>(), n, t, g2 ? 1 : 0, d_toff.as<uint32_t>(), d_deltas.as<uint32_t>());
+    }
     hipLaunchKernelGGL(k_svi_finish<256>, dim3(n), dim3(256), 0, s, d_deltas.as<uint32_t>(), d_tb.as<uint64_t>(),
                        d_counts_all, d_sel.as<uint32_t>(), d_value_starts, d_values, d_in, d_in_off,
                        d_end.as<uint32_t>(), d_bounds);
