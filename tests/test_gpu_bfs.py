@@ -27,6 +27,9 @@ MODES = {
     "batched_eager": dict(mode=psa.BFS_BATCHED, graph=0),
     "plain": dict(mode=psa.BFS_PLAIN),
     "batched_seg12": dict(mode=psa.BFS_BATCHED, segment_bits=12),
+    # tiles of 2^19 states over 2^12-state segments (128 segments a tile): the
+    # bookkeeping kernel lists such tiles without reading every segment
+    "batched_tile19_seg12": dict(mode=psa.BFS_BATCHED, segment_bits=12, tile_bits_log2=19),
     "plain_seg12": dict(mode=psa.BFS_PLAIN, segment_bits=12),
     "batched_nowindow": dict(mode=psa.BFS_BATCHED, tile_bits_log2=-1),
     "batched_tile12_256": dict(mode=psa.BFS_BATCHED, tile_bits_log2=12, block_threads=256),
