@@ -681,7 +681,7 @@ int svi_decode_chunks(const uint8_t* d_in, const uint64_t* d_in_off, const std::
     if (int r = d_deltas.alloc((t + 1) * 16, s)) return r;
     const std::vector<uint32_t> zero(1, 0u);
     if (int r = to_device(d_err, zero, s)) return r;
-    hipLaunchKernelGGL(k_svi_walk_par<1024>, dim3(n), dim3(1024), 0, s, d_in, d_in_off, d_tb.as<uint64_t>(),
+    hipLaunchKernelGGL(k_svi_walk_par<256>, dim3(n), dim3(256), 0, s, d_in, d_in_off, d_tb.as<uint64_t>(),
                        d_sel.as<uint32_t>(), d_bounds ? 1 : 0, d_toff.as<uint32_t>(), d_end.as<uint32_t>(),
                        d_err.as<uint32_t>());
     PS_HIP_CHECK(hipGetLastError());
