@@ -410,9 +410,14 @@ __global__ __launch_bounds__(TPB) void k_fc_inspect(const uint32_t* v, const uin
     }
     const uint32_t f = v[b], l = v[e - 1];
     int good = l > f;
-    for (uint64_t i = b + 1 + threadIdx.x; i < e && good; i += TPB) {
-        const uint32_t x = v[i];
-        if (!(x > f && x <= l)) good = 0;
+    // four independent loads per thread and step
+    for (uint64_t i = b + 1 + threadIdx.x; i < e && good; i += 4 * TPB) {
+        uint32_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = i + k * TPB < e ? v[i + k * TPB] : l;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (!(x[k] > f && x[k] <= l)) good = 0;
     }
     good = __syncthreads_and(good);
     if (threadIdx.x == 0) {
@@ -738,7 +743,7 @@ int fc_encode_chunks(const uint32_t* d_values, const std::vector<uint64_t>& star
     if (int r = d_last.alloc((size_t)nchunks * 4 + 4, s)) return r;
     if (int r = d_ok.alloc((size_t)nchunks * 4 + 4, s)) return r;
     if (nchunks)
-        hipLaunchKernelGGL(k_fc_inspect<256>, dim3(nchunks), dim3(256), 0, s, d_values, d_starts.as<uint64_t>(),
+        hipLaunchKernelGGL(k_fc_inspect<1024>, dim3(nchunks), dim3(1024), 0, s, d_values, d_starts.as<uint64_t>(),
                            d_first.as<uint32_t>(), d_last.as<uint32_t>(), d_ok.as<uint32_t>());
     PS_HIP_CHECK(hipGetLastError());
     std::vector<uint32_t> first, last, ok;
