@@ -133,17 +133,23 @@ int ps_pancake_xseg_postprocess(int32_t n, int32_t segment, uint32_t* d_idx, uin
  * BitArray::AndNot + AndNotAndCount (Common/BitArray.h:97-125). */
 int ps_bits_andnot_count(uint64_t* d_a, const uint64_t* d_excl1, const uint64_t* d_excl2, uint64_t nwords,
                          uint64_t* d_count, ps_stream s);
-/* *d_count (device) += popcount of the array (BitArray::BitsCount). */
+/* *d_count (device) += popcount of the array (BitArray::BitsCount, Common/BitArray.h:109-115). */
 int ps_bits_count(const uint64_t* d_a, uint64_t nwords, uint64_t* d_count, ps_stream s);
-/* *d_hash (device) += sum over nonzero words w_j of splitmix64(splitmix64(j + word_offset) ^ w_j). */
+/* *d_hash (device) += sum over nonzero words w_j of splitmix64(splitmix64(j + word_offset) ^ w_j):
+ * the frontier hash of every parity test; no reference counterpart. */
 int ps_bits_hash(const uint64_t* d_a, uint64_t nwords, uint64_t word_offset, uint64_t* d_hash, ps_stream s);
-/* BitArray::ScanBits (Common/BitArray.h:137-144) into an index list: writes
+/* BitArray::ScanBits (Common/BitArray.h:137-145) into an index list: writes
  * the set bit indexes (ascending) of [0, nbits) to d_out and their number to
- * *d_count (device u64).  Fails with PS_ERR_CAPACITY if more than cap. */
+ * *d_count (device u64).  Fails with PS_ERR_CAPACITY if more than cap (then
+ * the first cap indexes are written).  Allocates a per-call scratch array of
+ * one u64 per 2^18 bits and waits for the stream. */
 int ps_bits_scan(const uint64_t* d_a, uint64_t nbits, uint64_t* d_out, uint64_t cap, uint64_t* d_count,
                  ps_stream s);
-/* Sets bit idx[i] for every i (BitArray::Set in a loop). */
-int ps_bits_set_list(uint64_t* d_a, const uint64_t* d_idx, uint64_t count, ps_stream s);
+/* Sets bit idx[i] of an array of nbits bits for every i (BitArray::Set in a
+ * loop, Common/BitArray.h:81-83).  An index >= nbits is not written: the
+ * number of such indexes is added to *d_rejected (device u64; may be NULL). */
+int ps_bits_set_list(uint64_t* d_a, uint64_t nbits, const uint64_t* d_idx, uint64_t count, uint64_t* d_rejected,
+                     ps_stream s);
 
 /* ----------------------------------------------------------- codecs */
 /* Values are handled in chunks: chunk c is values[starts[c], starts[c+1]),
@@ -152,14 +158,14 @@ int ps_bits_set_list(uint64_t* d_a, const uint64_t* d_idx, uint64_t count, ps_st
  * the total, written by the encoders (device arrays of nchunks+1 u64).  A
  * chunk with no values encodes to no bytes, as in the reference. */
 
-/* StreamVInt::Encode (Common/StreamVInt.cpp:278-300) or, with d_bounds,
- * Encode(count, indexes, bounds, ...) (:322-347): u32 count, values delta-coded
+/* StreamVInt::Encode (Common/StreamVInt.cpp:209-231) or, with d_bounds,
+ * Encode(count, indexes, bounds, ...) (Common/StreamVInt.cpp:253-279): u32 count, values delta-coded
  * and padded with zeros to a multiple of 8 in 4-value tuples (descriptor byte +
  * 1..4 little-endian bytes each), then 4-bit bounds two per byte.  The input
  * values are not modified. */
 int ps_svint_encode(const uint32_t* d_values, const uint8_t* d_bounds, const uint64_t* d_starts, uint32_t nchunks,
                     uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, ps_stream s);
-/* StreamVInt::Decode (Common/StreamVInt.cpp:302-368) of nchunks concatenated
+/* StreamVInt::Decode (Common/StreamVInt.cpp:233-251, with bounds :280-299) of nchunks concatenated
  * chunks: chunk c is d_in[d_in_offsets[c] .. d_in_offsets[c+1]) (nchunks+1
  * device offsets, as the encoders write them; an empty chunk has no bytes).
  * Chunk c's values go to d_values + value_starts[c]; value_starts (nchunks+1,
@@ -168,7 +174,7 @@ int ps_svint_decode(const uint8_t* d_in, const uint64_t* d_in_offsets, uint32_t 
                     uint32_t* d_values, uint8_t* d_bounds, uint64_t values_cap, uint64_t* d_value_starts,
                     ps_stream s);
 /* OpBitsCompression<bits>::Encode / Decode, bits in {2, 4, 8}; count % 8 == 0
- * (Common/OpBitsCompression.cpp:65-83). */
+ * (Common/OpBitsCompression.cpp:55-75). */
 int ps_opbits_encode(int32_t bits, const uint8_t* d_ops, uint64_t count, uint8_t* d_out, ps_stream s);
 int ps_opbits_decode(int32_t bits, const uint8_t* d_in, uint64_t count, uint8_t* d_ops, ps_stream s);
 /* FrontierCompression::EncodeWithCheck (with_check = 1) or Encode per chunk:
@@ -181,7 +187,7 @@ int ps_fc_encode(const uint32_t* d_values, const uint64_t* d_starts, uint32_t nc
 int ps_fc_decode(const uint8_t* d_in, const uint64_t* d_in_offsets, uint32_t nchunks, uint32_t* d_values,
                  uint64_t values_cap, uint64_t* d_value_starts, ps_stream s);
 /* Groups cross-segment children by (op, target segment) like
- * SmartMultiplexor::Add (Common/Multiplexor.cpp:341-353), in canonical order:
+ * SmartMultiplexor::Add (Common/Multiplexor.cpp:200-212), in canonical order:
  * bucket b = op * nseg + segment holds the sorted in-segment indexes
  * d_sorted_idx[starts[b] .. starts[b+1]); d_bucket_starts has nops*nseg+1
  * entries. */

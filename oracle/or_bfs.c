@@ -107,7 +107,7 @@ done:
     return layers;
 }
 
-/* Common/InMemoryTwoBitBFS.cpp:35-80: UNVISITED=0, OLD=1, CUR=2, NEXT=3 with CUR
+/* Common/InMemoryTwoBitBFS.cpp:24-69: UNVISITED=0, OLD=1, CUR=2, NEXT=3 with CUR
  * and NEXT swapped after every layer. */
 static inline int tb_get(const uint64_t* a, uint64_t i) { return (int)((a[i >> 5] >> (2 * (i & 31))) & 3); }
 static inline void tb_set(uint64_t* a, uint64_t i, int v) {

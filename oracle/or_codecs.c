@@ -97,7 +97,7 @@ int or_svint_encode(int count, uint32_t* idx, uint8_t* buf) {
     return 4 + or_svint_encode_indexes(adjusted, idx, buf + 4);
 }
 
-/* Common/StreamVInt.cpp:302-320: *size receives the bytes consumed. */
+/* Common/StreamVInt.cpp:233-251: *size receives the bytes consumed. */
 int or_svint_decode(int* size, const uint8_t* buf, uint32_t* idx) {
     if (*size == 0) return 0;
     int count = (int)rd32(buf);
@@ -106,7 +106,7 @@ int or_svint_decode(int* size, const uint8_t* buf, uint32_t* idx) {
     return count;
 }
 
-/* Common/StreamVInt.cpp:322-368 */
+/* Common/StreamVInt.cpp:253-299 */
 int or_svint_encode_with_bounds(int count, uint32_t* idx, uint8_t* bounds, uint8_t* buf) {
     if (count == 0) return 0;
     int adjusted = (count + 7) / 8 * 8;
@@ -133,7 +133,7 @@ int or_svint_decode_with_bounds(int* size, const uint8_t* buf, uint32_t* idx, ui
 
 /* ------------------------------------------------------------ OpBits */
 
-/* Common/OpBitsCompression.cpp:15-83: eight op-flag bytes per tuple folded into
+/* Common/OpBitsCompression.cpp:10-76: eight op-flag bytes per tuple folded into
  * BITS bytes.  count must be a multiple of 8 (the reference stops otherwise). */
 int64_t or_opbits_encode(int bits, uint64_t count, const uint8_t* ops, uint8_t* buf) {
     if (count & 7) return -1;

@@ -75,7 +75,7 @@ int or_has_odd_cycles(const or_puzzle* p);
 uint64_t or_parse(const or_puzzle* p, const char* text);
 int or_to_string(const or_puzzle* p, uint64_t index, char* buf, int cap);
 
-/* Device I/O layout of PuzzleGpu::ExpandGpu (Puzzles/PuzzleGpu.cpp:93-122 and the
+/* Device I/O layout of PuzzleGpu::ExpandGpu (Puzzles/PuzzleGpu.cpp:53-82 and the
  * kernels it launches): in[i] = index << OperatorsCount() | usedOpBits,
  * out[i*S + k] = child << OperatorsCount() | opLabel, or ~0.  Returns S (slots per
  * parent) or -1 if the puzzle/mode pair is not valid. */
@@ -86,7 +86,7 @@ int or_slots_per_parent(const or_puzzle* p, int mode);
 int or_expand_one(const or_puzzle* p, int mode, uint64_t index, int op_bits,
                   uint64_t* children, int32_t* ops);
 /* FourPegHanoiOptimized::Expand (CPU class): restores symmetry only when a peg
- * became empty or non-empty (Puzzles/FourPegHanoiOptimized.cpp:222-247). */
+ * became empty or non-empty (Puzzles/FourPegHanoiOptimized.cpp:207-238). */
 int or_hanoi_expand_cpu_opt(int disks, int symmetry, uint64_t index, int op_bits,
                             uint64_t* children, int32_t* ops);
 /* PuzzleGpu::CombineIndexAndOpBits / SplitIndexAndOps (Puzzles/PuzzleGpu.cpp:35-51). */
@@ -108,7 +108,7 @@ uint64_t or_bits_count(const uint64_t* a, uint64_t nwords);
  * layer l's frontier bit array. */
 int or_bfs3(const or_puzzle* p, uint64_t start, uint64_t index_count, int max_layers,
             uint64_t* counts, uint64_t* hashes);
-/* InMemoryTwoBitBFS (Common/InMemoryTwoBitBFS.cpp:35-80).  hashes[l] = hash of the
+/* InMemoryTwoBitBFS (Common/InMemoryTwoBitBFS.cpp:24-69).  hashes[l] = hash of the
  * whole 2-bit array after layer l (value v of state i at bits 2i..2i+1, natural
  * order: bit 2i = v & 1). */
 int or_bfs2(const or_puzzle* p, uint64_t start, uint64_t index_count, int max_layers,
@@ -124,7 +124,7 @@ int or_bfs3_segmented(const or_puzzle* p, uint64_t start, int segment_bits, int 
 uint64_t or_bfs3_expand_range(const or_puzzle* p, const uint64_t* cur, uint64_t* next,
                               uint64_t first, uint64_t span, uint64_t* children_out);
 
-/* ---- StreamVInt (Common/StreamVInt.cpp:185-368) ---- */
+/* ---- StreamVInt (Common/StreamVInt.cpp:149-300) ---- */
 int or_svint_encode_tuple(const uint32_t* v4, uint8_t* buf);
 int or_svint_decode_tuple(const uint8_t* buf, uint32_t* v4);
 void or_svint_encode_bounds_tuple(const uint8_t* b8, uint8_t* buf);
@@ -138,7 +138,7 @@ int or_svint_decode(int* size, const uint8_t* buf, uint32_t* idx);
 int or_svint_encode_with_bounds(int count, uint32_t* idx, uint8_t* bounds, uint8_t* buf);
 int or_svint_decode_with_bounds(int* size, const uint8_t* buf, uint32_t* idx, uint8_t* bounds);
 
-/* ---- OpBitsCompression<2/4/8> (Common/OpBitsCompression.cpp:15-83) ---- */
+/* ---- OpBitsCompression<2/4/8> (Common/OpBitsCompression.cpp:10-76) ---- */
 int64_t or_opbits_encode(int bits, uint64_t count, const uint8_t* ops, uint8_t* buf);
 int64_t or_opbits_decode(int bits, uint64_t count, const uint8_t* buf, uint8_t* ops);
 

@@ -126,6 +126,32 @@ def build_host_tests():
     return HOST_TESTS_BIN
 
 
+REF_TESTS_SRC = os.path.join(ROOT, "tests", "cpp", "ref_binding_tests.cpp")
+REF_TESTS_BIN = os.path.join(ROOT, "tests", "cpp", "ref_binding_tests")
+ORACLE = os.path.join(ROOT, "oracle")
+
+
+def build_ref_binding_tests():
+    """The C++ test program of the reference-signature binding
+    (tests/cpp/ref_binding_tests): it checks the launchers against the CPU
+    oracle, so it links the oracle library (test infrastructure), built first."""
+    if not os.path.exists(REF_TESTS_SRC):
+        return None
+    rc, log = _run(["make", "-s", "-C", ORACLE, "liboracle.so"])
+    if rc != 0:
+        print(log, file=sys.stderr)
+        raise RuntimeError("build of oracle/liboracle.so failed")
+    tmp = REF_TESTS_BIN + ".tmp"
+    rc, log = _run([CXX, "-O1", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{HOST}", f"-I{ORACLE}", REF_TESTS_SRC,
+                    "-o", tmp, f"-L{OUT}", f"-L{ORACLE}", "-lpuzzlesearch_host", "-lpuzzlesearch_amd", "-loracle",
+                    "-Wl,-rpath,$ORIGIN/../../puzzlesearch_amd/lib", "-Wl,-rpath,$ORIGIN/../../oracle", "-lpthread"])
+    if rc != 0:
+        print(log, file=sys.stderr)
+        raise RuntimeError("build of tests/cpp/ref_binding_tests failed")
+    os.replace(tmp, REF_TESTS_BIN)
+    return REF_TESTS_BIN
+
+
 def build(jobs=None, only=None):
     jobs = jobs or min(8, os.cpu_count() or 4)
     out = []
@@ -135,9 +161,9 @@ def build(jobs=None, only=None):
         h = build_host(jobs)
         if h:
             out.append(h)
-            t = build_host_tests()
-            if t:
-                out.append(t)
+            for t in (build_host_tests(), build_ref_binding_tests()):
+                if t:
+                    out.append(t)
     return out
 
 

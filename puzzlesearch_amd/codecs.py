@@ -1,7 +1,7 @@
 """Device codecs through the C ABI: StreamVInt (Common/StreamVInt.h),
 OpBitsCompression (Common/OpBitsCompression.h), FrontierCompression
 (Common/FrontierCompression.h) and segment bucketing of cross-segment children
-(Common/Multiplexor.cpp:341-353).
+(Common/Multiplexor.cpp:200-212).
 
 Host arrays go in and come out; the work runs on the device.  Chunks follow
 the reference's framing: every chunk is encoded on its own (at most 2^18

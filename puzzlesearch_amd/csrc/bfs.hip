@@ -1,5 +1,5 @@
 // Device-resident BFS driver: InMemoryThreeBitBFS / InMemoryTwoBitBFS
-// (Common/InMemoryThreeBitBFS.cpp:4-48, Common/InMemoryTwoBitBFS.cpp:35-80)
+// (Common/InMemoryThreeBitBFS.cpp:4-48, Common/InMemoryTwoBitBFS.cpp:24-69)
 // with the whole layer loop on the GPU, optionally split into segments of
 // 2^segment_bits states as DiskBasedOptThreeBitBFS does
 // (Common/DiskBasedOptThreeBitBFS.cpp:54-144).
@@ -429,13 +429,16 @@ int destroy_bfs(ps_bfs* b) {
 // frontier, so it is exact only where every index of the space is a state of
 // the searched graph.  That excludes symmetry-reduced Hanoi (non-canonical
 // indexes), the packed sliding index (unused (segment, blank) codes) and the
-// pancake index of more than 12 pancakes (29-bit segments of 12! states).
-bool every_index_a_state(const ps_puzzle& p) {
+// pancake index of more than 12 pancakes (29-bit segments of 12! states).  An
+// index space of nbits indexes must also end at the last state: the 12-pancake
+// IndexesCount() is 2^29 > 12!, and an index >= n! would be ranked modulo n!
+// and inherit the neighbours of a real state.
+bool every_index_a_state(const ps_puzzle& p, uint64_t nbits) {
     switch (p.kind) {
-    case PS_PANCAKE: return p.size <= 12;
-    case PS_SLIDING: return true;
+    case PS_PANCAKE: return p.size <= 12 && nbits <= factorial(p.size);
+    case PS_SLIDING: return nbits <= factorial(p.size * p.height);
     case PS_HANOI4:
-    case PS_HANOI4_OPT: return (p.flags & PS_FLAG_SYMMETRY) == 0;
+    case PS_HANOI4_OPT: return (p.flags & PS_FLAG_SYMMETRY) == 0 && nbits <= (1ull << (2 * p.size));
     }
     return false;
 }
@@ -608,12 +611,12 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
     if (opt.bits == 2 && opt.mode == PS_BFS_PLAIN) opt.mode = PS_BFS_BATCHED;
     ps_puzzle_info info;
     if (int r = puzzle_query(p, &info)) return r;
-    if (opt.direction == PS_DIR_BOTTOM_UP && !every_index_a_state(*p))
-        return fail(PS_ERR_UNSUPPORTED,
-                    "bottom-up layers need a puzzle whose every index is a state (not symmetric Hanoi, "
-                    "packed sliding or more than 12 pancakes)");
     Geometry g;
     if (int r = geometry(p, &opt, info, &g)) return r;
+    if (opt.direction == PS_DIR_BOTTOM_UP && !every_index_a_state(*p, g.nbits))
+        return fail(PS_ERR_UNSUPPORTED,
+                    "bottom-up layers need a puzzle whose every index is a state (not symmetric Hanoi, "
+                    "packed sliding, more than 12 pancakes, or an index_count beyond the last state)");
     if (opt.cross_marks == PS_CROSS_LISTS) {
         if (opt.bits != 3 || p->kind != PS_PANCAKE || p->size < 10 || p->size > 12)
             return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS covers the three-bit search of 10..12 pancakes");
@@ -713,7 +716,7 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         else block_bits = puzzle_locality_block(p);
         b->use_window = opt.tile_bits_log2 >= 0 ? 1 : 0;
         b->pull_capable = (opt.direction != PS_DIR_TOP_DOWN && b->use_window && opt.mode == PS_BFS_BATCHED &&
-                           every_index_a_state(*p) && !b->lists) ? 1 : 0;
+                           every_index_a_state(*p, g.nbits) && !b->lists) ? 1 : 0;
         if (!b->use_window) block_bits = 1ull << 18;
         if (b->lists) block_bits = factorial(9);
         b->tile_words = std::max<uint64_t>(1, block_bits / 64);

@@ -54,11 +54,16 @@ __global__ __launch_bounds__(TPB) void k_hash(const uint64_t* a, uint64_t n, uin
     block_atomic_add<TPB>(s, h);
 }
 
-__global__ __launch_bounds__(TPB) void k_set_list(uint32_t* a, const uint64_t* idx, uint64_t n) {
+// Indexes >= nbits are not written; their number goes to *rejected (if given).
+__global__ __launch_bounds__(TPB) void k_set_list(uint32_t* a, uint64_t nbits, const uint64_t* idx, uint64_t n,
+                                                  unsigned long long* rejected) {
+    unsigned long long bad = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * TPB) {
         const uint64_t v = idx[i];
-        psk::gor(&a[v >> 5], 1u << (v & 31));
+        if (v < nbits) psk::gor(&a[v >> 5], 1u << (v & 31));
+        else ++bad;
     }
+    if (rejected) block_atomic_add<TPB>(bad, rejected);
 }
 
 // ScanBits into a list, in three passes over chunks of SCAN_CHUNK words:
@@ -197,12 +202,14 @@ extern "C" int ps_bits_hash(const uint64_t* d_a, uint64_t nwords, uint64_t word_
     PS_API_END
 }
 
-extern "C" int ps_bits_set_list(uint64_t* d_a, const uint64_t* d_idx, uint64_t count, ps_stream s) {
+extern "C" int ps_bits_set_list(uint64_t* d_a, uint64_t nbits, const uint64_t* d_idx, uint64_t count,
+                                uint64_t* d_rejected, ps_stream s) {
     PS_API_BEGIN
     if (int r = require_runtime()) return r;
     if (count == 0) return PS_OK;
     if (!d_a || !d_idx) return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
-    hipLaunchKernelGGL(k_set_list, dim3(grid_for(count, 4)), dim3(TPB), 0, stream_of(s), (uint32_t*)d_a, d_idx, count);
+    hipLaunchKernelGGL(k_set_list, dim3(grid_for(count, 4)), dim3(TPB), 0, stream_of(s), (uint32_t*)d_a, nbits, d_idx,
+                       count, (unsigned long long*)d_rejected);
     PS_HIP_CHECK(hipGetLastError());
     return PS_OK;
     PS_API_END

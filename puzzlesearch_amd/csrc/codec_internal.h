@@ -12,7 +12,7 @@ namespace ps {
 
 constexpr uint32_t MAX_CHUNK_VALUES = 256 * 1024;  // StreamVInt::MAX_INDEXES_COUNT
 
-// StreamVInt::Encode, with bounds when d_bounds is given (Common/StreamVInt.cpp:278-347)
+// StreamVInt::Encode, with bounds when d_bounds is given (Common/StreamVInt.cpp:209-279)
 int svi_encode_chunks(const uint32_t* d_values, const uint8_t* d_bounds, const std::vector<uint64_t>& starts,
                       uint8_t* d_out, uint64_t out_cap, std::vector<uint64_t>& off, hipStream_t s);
 // FrontierCompression::EncodeWithCheck (with_check) or Encode (Common/FrontierCompression.cpp:92-115);

@@ -1,9 +1,9 @@
 // Frontier codecs on the GPU, byte-identical to the reference CPU codecs:
-//   StreamVInt (Common/StreamVInt.cpp:185-368)
-//   OpBitsCompression<2/4/8> (Common/OpBitsCompression.cpp:15-83)
+//   StreamVInt (Common/StreamVInt.cpp:149-300)
+//   OpBitsCompression<2/4/8> (Common/OpBitsCompression.cpp:10-76)
 //   FrontierCompression bitmap-or-StreamVInt chunks (Common/FrontierCompression.cpp:28-150)
 //   bucketing of cross-segment children by (op, target segment), the grouping
-//   SmartMultiplexor::Add performs (Common/Multiplexor.cpp:266-274), in
+//   SmartMultiplexor::Add performs (Common/Multiplexor.cpp:200-212), in
 //   canonical (sorted) order.
 //
 // Values come in chunks (chunk c = values[starts[c], starts[c+1])), each encoded

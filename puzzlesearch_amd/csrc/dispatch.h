@@ -83,7 +83,7 @@ int dispatch(const ps_puzzle& p, int mode, Fn&& fn) {
     case PS_SLIDING_PACKED:
         if constexpr ((FAM & FAM_PACKED) != 0) {
             if (mode != PS_EXPAND_FULL) break;
-            // the instantiations of Puzzles/SlidingTilePuzzleOptimized.cpp:350-362
+            // the instantiations of Puzzles/SlidingTilePuzzleOptimized.cpp:297-309
             const int key = p.size * 16 + p.height;
             switch (key) {
             case 2 * 16 + 2: return fn(Tag<Packed<2, 2>>{});

@@ -1,5 +1,5 @@
 // Plain batch expansion: the device side of PuzzleGpu::Expand
-// (Puzzles/PuzzleGpu.cpp:93-122).  One thread per parent; the parent's slots
+// (Puzzles/PuzzleGpu.cpp:53-82).  One thread per parent; the parent's slots
 // are staged in LDS and written out with coalesced stores.
 #include <cstring>
 #include <stdexcept>

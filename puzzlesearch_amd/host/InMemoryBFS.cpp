@@ -1,4 +1,4 @@
-// In-memory BFS drivers (Common/InMemoryTwoBitBFS.cpp:35-80,
+// In-memory BFS drivers (Common/InMemoryTwoBitBFS.cpp:24-69,
 // Common/InMemoryThreeBitBFS.cpp:4-48) and their device-resident variants.
 #include "InMemoryBFS.h"
 

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "Buffer.h"
 #include "Util.h"
 
 // Common/Puzzle.h:11-14
@@ -42,7 +43,7 @@ public:
                         ExpandHint hint) = 0;
 
     // Common/Puzzle.h:39-44 (used by the segmented driver for cross-segment ids)
-    virtual void CrossSegmentPostProcess(int op, int segment, int segmentBits, std::vector<uint32_t>& expandedIndexes) {
+    virtual void CrossSegmentPostProcess(int op, int segment, int segmentBits, Buffer<uint32_t>& expandedIndexes) {
         (void)op;
         (void)segment;
         (void)segmentBits;

@@ -1,8 +1,10 @@
-// PuzzleGpu (Puzzles/PuzzleGpu.cpp:4-82) over the C ABI.
+// PuzzleGpu (Puzzles/PuzzleGpu.cpp:4-82) over the GPU.h shim.
 #include "PuzzleGpu.h"
 
 #include <cstdlib>
 #include <string>
+
+#include "GPU.h"
 
 std::atomic<uint64_t> ExpandBuffer::s_nodes{0};
 std::atomic<uint64_t> ExpandBuffer::s_nanos{0};
@@ -22,11 +24,14 @@ void PsCheck(int status, const char* what) {
 }
 
 // The library is initialised on first use by any GPU plugin (the reference's
-// CUDA runtime initialises lazily the same way).
+// CUDA runtime initialises lazily the same way), unless the program has
+// already called ps_init for a device of its choice.
 void EnsurePsRuntime() {
     static std::once_flag once;
     static int status = PS_OK;
     std::call_once(once, [] {
+        int32_t cus = 0;
+        if (ps_device_props(nullptr, 0, &cus, nullptr, nullptr) == PS_OK) return;  // already initialised
         const char* dev = std::getenv("PS_DEVICE");
         status = ps_init(dev ? std::atoi(dev) : 0);
     });
@@ -34,12 +39,18 @@ void EnsurePsRuntime() {
 }
 
 PuzzleGpu::Exec::Exec(int branchingFactor) {
-    EnsurePsRuntime();
-    PsCheck(ps_stream_create(&stream), "ps_stream_create");
-    PsCheck(ps_alloc((void**)&gpuSrc, MAX_INDEXES_BUFFER * 8), "ps_alloc");
-    PsCheck(ps_alloc((void**)&gpuDst, MAX_INDEXES_BUFFER * (uint64_t)branchingFactor * 8), "ps_alloc");
+    stream = CreateCudaStream();
+    try {
+        gpuSrc = CreateGPUBuffer((int)MAX_INDEXES_BUFFER);
+        gpuDst = CreateGPUBuffer((int)MAX_INDEXES_BUFFER * branchingFactor);
+    } catch (...) {
+        if (gpuSrc) ps_free(gpuSrc);
+        ps_stream_destroy(stream);
+        throw;
+    }
 }
 
+// Destructors do not throw: failures to release are ignored here.
 PuzzleGpu::Exec::~Exec() {
     ps_stream_destroy(stream);
     ps_free(gpuSrc);

@@ -1,20 +1,22 @@
 // GPU plugin base class, mirroring Puzzles/PuzzleGpu.h:9-40: a pool of
 // (stream, device input buffer, device output buffer) triples, the
 // index<<ops|opBits packing, and a virtual ExpandGpu that subclasses implement
-// with one ps_expand call.  Device work goes through the C ABI only.
+// with one launcher call (PancakeGpu.h, SlidingPuzzleGpu.h, HanoiTowersGPU.h).
+// Buffers, streams and copies go through the element-count shim of GPU.h.
 #pragma once
 
 #include <memory>
 #include <mutex>
 #include <vector>
 
+#include "GPU.h"
 #include "Puzzle.h"
 #include "puzzlesearch_amd.h"
 
-using CuStream = void*;  // Puzzles/GPU.h:8 (a ps_stream / hipStream_t)
-
 class PuzzleGpu : public Puzzle {
 protected:
+    // Puzzles/PuzzleGpu.cpp:4-15: a stream, MAX_INDEXES_BUFFER parents in and
+    // MAX_INDEXES_BUFFER * branchingFactor slots out.
     struct Exec {
         CuStream stream = nullptr;
         uint64_t* gpuSrc = nullptr;
@@ -32,7 +34,7 @@ public:
                 std::vector<uint64_t>& expandedIndexes, std::vector<int>& expandedOperators,
                 ExpandHint hint) override;
 
-    // Descriptor of this puzzle for the C ABI (device BFS, expand).
+    // Descriptor of this puzzle for the C ABI (device BFS, segment step).
     virtual ps_puzzle Descriptor() const = 0;
 
 protected:
@@ -42,7 +44,9 @@ protected:
     void ReleaseStream(Exec* stream);
     void CombineIndexAndOpBits(std::vector<uint64_t>& indexes, std::vector<int>& usedOperatorBits);
     void SplitIndexAndOps(std::vector<uint64_t>& expandedIndexes, std::vector<int>& expandedOperators);
-    // H2D of the packed parents, `launch`, D2H of slots*count results, unpack.
+    // Copy of the packed parents to an acquired Exec, `launch(src, dst, count,
+    // stream)`, copy back of slots * count results, unpack.  The Exec is
+    // released on every path, also when the launch throws.
     template <class Launch>
     void RunBatch(std::vector<uint64_t>& indexes, std::vector<uint64_t>& expandedIndexes,
                   std::vector<int>& expandedOperators, int slots, Launch&& launch);
@@ -56,15 +60,14 @@ private:
 template <class Launch>
 void PuzzleGpu::RunBatch(std::vector<uint64_t>& indexes, std::vector<uint64_t>& expandedIndexes,
                          std::vector<int>& expandedOperators, int slots, Launch&& launch) {
+    ensure(indexes.size() <= MAX_INDEXES_BUFFER && slots <= BranchingFactor());
     expandedIndexes.resize(indexes.size() * (size_t)slots);
     expandedOperators.resize(indexes.size() * (size_t)slots);
     Exec* ex = AquireStream();
     try {
-        PsCheck(ps_memcpy_h2d_async(ex->gpuSrc, indexes.data(), indexes.size() * 8, ex->stream), "copy to GPU");
+        CopyToGpu(indexes.data(), ex->gpuSrc, indexes.size(), ex->stream);
         launch(ex->gpuSrc, ex->gpuDst, (uint64_t)indexes.size(), ex->stream);
-        PsCheck(ps_memcpy_d2h_async(expandedIndexes.data(), ex->gpuDst, expandedIndexes.size() * 8, ex->stream),
-                "copy from GPU");
-        PsCheck(ps_stream_sync(ex->stream), "stream synchronize");
+        CopyFromGpu(ex->gpuDst, expandedIndexes.data(), expandedIndexes.size(), ex->stream);
     } catch (...) {
         ReleaseStream(ex);
         throw;

@@ -1,9 +1,14 @@
 // GPU puzzle plugins: host-side text formats and ranks, device expansion
-// through ps_expand.
+// through the launchers of PancakeGpu.h, SlidingPuzzleGpu.h and
+// HanoiTowersGPU.h.
 #include "Puzzles.h"
 
 #include <algorithm>
 #include <sstream>
+
+#include "HanoiTowersGPU.h"
+#include "PancakeGpu.h"
+#include "SlidingPuzzleGpu.h"
 
 namespace ps_host {
 
@@ -145,9 +150,10 @@ ps_puzzle PancakeOptimizedGPU::Descriptor() const {
     return ps_puzzle{PS_PANCAKE, m_Size, 0, m_InvIndex ? PS_FLAG_INV_INDEX : 0};
 }
 
+// The reference's body is ensure(false): its Expand calls the launchers
+// itself.  Here it expands with the full flip set, like PancakeExpand.
 void PancakeOptimizedGPU::ExpandGpu(uint64_t* gpuIndexes, uint64_t* gpuExpanded, uint64_t count, CuStream stream) {
-    ps_puzzle p = Descriptor();
-    PsCheck(ps_expand(&p, PS_EXPAND_FULL, gpuIndexes, gpuExpanded, count, stream), "ps_expand");
+    PancakeExpand(gpuIndexes, gpuExpanded, m_Size, count, stream);
 }
 
 // Puzzles/PancakeOptimizedGpu.cpp:36-79: with 29-bit segments and n > 12 the
@@ -158,31 +164,40 @@ void PancakeOptimizedGPU::Expand(std::vector<uint64_t>& indexes, std::vector<int
     SetupOutputBuffers(expandedIndexes, expandedOperators);
     if (indexes.empty()) return;
     CombineIndexAndOpBits(indexes, usedOperatorBits);
-    int mode = PS_EXPAND_FULL;
-    if (hint.SegmentBits == 29 && m_Size > 12) mode = hint.CrossSegment ? PS_EXPAND_CROSS_SEGMENT : PS_EXPAND_IN_SEGMENT;
-    ps_puzzle p = Descriptor();
-    const int slots = ps_expand_slots(&p, mode);
-    RunBatch(indexes, expandedIndexes, expandedOperators, slots,
-             [&](uint64_t* src, uint64_t* dst, uint64_t count, CuStream s) {
-                 PsCheck(ps_expand(&p, mode, src, dst, count, s), "ps_expand");
-             });
+    const int br = BranchingFactor();
+    const int n = m_Size;
+    const bool inv = m_InvIndex;
+    if (hint.SegmentBits == 29 && n > 12) {
+        if (hint.CrossSegment)
+            RunBatch(indexes, expandedIndexes, expandedOperators, br - 11,
+                     [&](uint64_t* src, uint64_t* dst, uint64_t count, CuStream s) {
+                         PancakeExpandCrossSegment(src, dst, n, inv, count, s);
+                     });
+        else
+            RunBatch(indexes, expandedIndexes, expandedOperators, 11,
+                     [&](uint64_t* src, uint64_t* dst, uint64_t count, CuStream s) {
+                         PancakeExpandInSegment(src, dst, n, count, s);
+                     });
+    } else {
+        RunBatch(indexes, expandedIndexes, expandedOperators, br,
+                 [&](uint64_t* src, uint64_t* dst, uint64_t count, CuStream s) { PancakeExpand(src, dst, n, count, s); });
+    }
 }
 
 // Puzzles/PancakeOptimizedGpu.cpp:21-34
 void PancakeOptimizedGPU::CrossSegmentPostProcess(int op, int segment, int segmentBits,
-                                                  std::vector<uint32_t>& expandedIndexes) {
+                                                  Buffer<uint32_t>& expandedIndexes) {
     (void)op;
     const bool invert = m_InvIndex && segmentBits == 29 && m_Size > 12;
-    if (!invert || expandedIndexes.empty()) return;
+    if (!invert || expandedIndexes.IsEmpty()) return;
+    // the u32 ids are staged in the input buffer of MAX_INDEXES_BUFFER u64
+    ensure(expandedIndexes.Size() <= 2 * MAX_INDEXES_BUFFER);
     Exec* ex = AquireStream();
     try {
-        const uint64_t bytes = expandedIndexes.size() * 4;
-        ensure(bytes <= MAX_INDEXES_BUFFER * 8);
-        PsCheck(ps_memcpy_h2d_async(ex->gpuSrc, expandedIndexes.data(), bytes, ex->stream), "copy to GPU");
-        PsCheck(ps_pancake_xseg_postprocess(m_Size, segment, (uint32_t*)ex->gpuSrc, expandedIndexes.size(), ex->stream),
-                "ps_pancake_xseg_postprocess");
-        PsCheck(ps_memcpy_d2h_async(expandedIndexes.data(), ex->gpuSrc, bytes, ex->stream), "copy from GPU");
-        PsCheck(ps_stream_sync(ex->stream), "stream synchronize");
+        uint32_t* gpu = (uint32_t*)ex->gpuSrc;
+        CopyToGpu(expandedIndexes.Buf(), gpu, expandedIndexes.Size(), ex->stream);
+        PancakeCrossSegmentPostProcessGPU(gpu, segment, m_Size, expandedIndexes.Size(), ex->stream);
+        CopyFromGpu(gpu, expandedIndexes.Buf(), expandedIndexes.Size(), ex->stream);
     } catch (...) {
         ReleaseStream(ex);
         throw;
@@ -219,9 +234,9 @@ uint64_t SlidingTilePuzzleGpu::Parse(std::string state) {
 
 ps_puzzle SlidingTilePuzzleGpu::Descriptor() const { return ps_puzzle{PS_SLIDING, m_Width, m_Height, 0}; }
 
+// Puzzles/SlidingTilePuzzleGpu.cpp:49-56
 void SlidingTilePuzzleGpu::ExpandGpu(uint64_t* gpuIndexes, uint64_t* gpuExpanded, uint64_t count, CuStream stream) {
-    ps_puzzle p = Descriptor();
-    PsCheck(ps_expand(&p, PS_EXPAND_FULL, gpuIndexes, gpuExpanded, count, stream), "ps_expand");
+    GpuSlidingTilePuzzleSimpleExpand(gpuIndexes, gpuExpanded, m_Width, m_Size, count, stream);
 }
 
 // ---------------------------------------------------------- sliding packed
@@ -331,11 +346,10 @@ uint64_t SlidingTilePuzzleOptimized<width, height>::Parse(std::string state) {
 template <int width, int height>
 void SlidingTilePuzzleOptimized<width, height>::ExpandGpu(uint64_t* gpuIndexes, uint64_t* gpuExpanded, uint64_t count,
                                                           CuStream stream) {
-    ps_puzzle p = Descriptor();
-    PsCheck(ps_expand(&p, PS_EXPAND_FULL, gpuIndexes, gpuExpanded, count, stream), "ps_expand");
+    GpuSlidingTilePuzzleOptimizedExpand<width, height>(gpuIndexes, gpuExpanded, count, stream);
 }
 
-// Puzzles/SlidingTilePuzzleOptimized.cpp:300-338: with 32-bit segments only
+// Puzzles/SlidingTilePuzzleOptimized.cpp:247-285: with 32-bit segments only
 // parents whose blank sits where a vertical move can change the segment are
 // expanded for the cross-segment pass.
 template <int width, int height>
@@ -419,9 +433,10 @@ ps_puzzle FourPegHanoiGPUBase::Descriptor() const {
     return ps_puzzle{m_Optimized ? PS_HANOI4_OPT : PS_HANOI4, m_Size, 0, m_UseSymmetry ? PS_FLAG_SYMMETRY : 0};
 }
 
+// Puzzles/FourPegHanoiGPU.cpp:12-25, Puzzles/FourPegHanoiOptimizedGPU.cpp:12-25
 void FourPegHanoiGPUBase::ExpandGpu(uint64_t* gpuIndexes, uint64_t* gpuExpanded, uint64_t count, CuStream stream) {
-    ps_puzzle p = Descriptor();
-    PsCheck(ps_expand(&p, PS_EXPAND_FULL, gpuIndexes, gpuExpanded, count, stream), "ps_expand");
+    if (m_Optimized) GpuHanoiTowersExpandOptimized(gpuIndexes, gpuExpanded, m_Size, m_UseSymmetry, count, stream);
+    else GpuHanoiTowersExpand(gpuIndexes, gpuExpanded, m_Size, m_UseSymmetry, count, stream);
 }
 
 // Puzzles/FourPegHanoiGPU.cpp:27-70: for the cross-segment pass with 32-bit

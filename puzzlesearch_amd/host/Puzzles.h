@@ -22,7 +22,7 @@ public:
     void Expand(std::vector<uint64_t>& indexes, std::vector<int>& usedOperatorBits,
                 std::vector<uint64_t>& expandedIndexes, std::vector<int>& expandedOperators,
                 ExpandHint hint) override;
-    void CrossSegmentPostProcess(int op, int segment, int segmentBits, std::vector<uint32_t>& expandedIndexes) override;
+    void CrossSegmentPostProcess(int op, int segment, int segmentBits, Buffer<uint32_t>& expandedIndexes) override;
     ps_puzzle Descriptor() const override;
     int Size() const { return m_Size; }
     bool InvIndex() const { return m_InvIndex; }

@@ -221,6 +221,24 @@ def test_pancake13_prefix(ps, golden, inv):
     assert (counts, hashes) == (oc, oh)
 
 
+def test_bottom_up_index_space_past_last_state(ps, golden):
+    """An index space longer than the state space (9-pancake with
+    index_count = 9! + 64, or the 12-pancake's 2^29 IndexesCount()) holds
+    indexes that are no state: bottom-up layers are refused there, and AUTO
+    stays top-down and equals the oracle."""
+    r = golden["pancake_9"]
+    n = math.factorial(9) + 64
+    with pytest.raises(psa.PSError, match="bottom-up"):
+        psa.Bfs(psa.pancake(9), index_count=n, direction=psa.DIR_BOTTOM_UP)
+    with pytest.raises(psa.PSError, match="bottom-up"):
+        psa.Bfs(psa.pancake(12), direction=psa.DIR_BOTTOM_UP)
+    with psa.Bfs(psa.pancake(9), index_count=n, direction=psa.DIR_AUTO) as b:
+        counts, hashes = b.run(0)
+    oc, oh = O.bfs3(O.pancake(9), 0, index_count=n)
+    assert counts == r["counts"] == oc
+    assert hashes == oh
+
+
 def test_pancake12_reference_sizing(ps):
     """Same search with PancakeOptimized's 2^29 index space: identical result."""
     if "pancake_12" not in FIX:

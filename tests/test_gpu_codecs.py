@@ -1,8 +1,8 @@
 """Device codecs against the CPU oracle codecs, byte for byte: StreamVInt with
-and without bounds (Common/StreamVInt.cpp:278-368), OpBits<2/4/8>
-(Common/OpBitsCompression.cpp:15-83), FrontierCompression Encode /
+and without bounds (Common/StreamVInt.cpp:209-300), OpBits<2/4/8>
+(Common/OpBitsCompression.cpp:10-76), FrontierCompression Encode /
 EncodeWithCheck / Decode (Common/FrontierCompression.cpp:92-125), and bucketing
-of cross-segment children by (op, segment) (Common/Multiplexor.cpp:341-353)
+of cross-segment children by (op, segment) (Common/Multiplexor.cpp:200-212)
 against a numpy sort.  Edge cases: empty chunks, counts not a multiple of 8,
 the 2^18-value maximum chunk, unsorted chunks, the reference's bitmap
 selection cases."""
