@@ -145,6 +145,17 @@ int ps_bits_hash(const uint64_t* d_a, uint64_t nwords, uint64_t word_offset, uin
  * one u64 per 2^18 bits and waits for the stream. */
 int ps_bits_scan(const uint64_t* d_a, uint64_t nbits, uint64_t* d_out, uint64_t cap, uint64_t* d_count,
                  ps_stream s);
+/* ps_bits_scan that allocates nothing and does not wait.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  Its chunk counts live in a
+ * static device array of 2^20 + 1 u64 (8 MiB, held by every device the library
+ * runs on until the process ends), so nbits <= 2^38 (larger: PS_ERR_INVALID_ARGUMENT).
+ * *d_count receives the number of set bits; when it exceeds cap only the first
+ * cap indexes are written, and the caller checks *d_count (no PS_ERR_CAPACITY).
+ * All calls on one device share that array: calls that overlap on one device
+ * race, so keep them on one stream or otherwise ordered. */
+int ps_bits_scan_noalloc(const uint64_t* d_a, uint64_t nbits, uint64_t* d_out, uint64_t cap, uint64_t* d_count,
+                         ps_stream s);
 /* Sets bit idx[i] of an array of nbits bits for every i (BitArray::Set in a
  * loop, Common/BitArray.h:81-83).  An index >= nbits is not written: the
  * number of such indexes is added to *d_rejected (device u64; may be NULL). */

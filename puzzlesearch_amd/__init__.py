@@ -168,6 +168,7 @@ _SIGS = {
     "ps_bits_count": (_i32, [_vp, _u64, _vp, _vp]),
     "ps_bits_hash": (_i32, [_vp, _u64, _u64, _vp, _vp]),
     "ps_bits_scan": (_i32, [_vp, _u64, _vp, _u64, _vp, _vp]),
+    "ps_bits_scan_noalloc": (_i32, [_vp, _u64, _vp, _u64, _vp, _vp]),
     "ps_bits_set_list": (_i32, [_vp, _u64, _vp, _u64, _vp, _vp]),
     "ps_svint_encode": (_i32, [_vp, _vp, _vp, ctypes.c_uint32, _vp, _u64, _vp, _vp]),
     "ps_svint_decode": (_i32, [_vp, _vp, ctypes.c_uint32, _i32, _vp, _vp, _u64, _vp, _vp]),

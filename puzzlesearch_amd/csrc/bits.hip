@@ -1,4 +1,6 @@
 // Device bit-array operations of the C ABI (Common/BitArray.h:69-169 on HBM).
+#include <mutex>
+
 #include "bfs_kernels.h"
 #include "ps_internal.h"
 
@@ -159,8 +161,48 @@ __global__ __launch_bounds__(TPB) void k_scan_write(const uint64_t* a, uint64_t 
 
 __global__ void k_copy_count(const unsigned long long* src, uint64_t* dst) { *dst = *src; }
 
+__global__ void k_set_u64(uint64_t* dst, uint64_t v) { *dst = v; }
+
 }  // namespace
 
+}  // namespace ps
+
+// Scratch memory of ps_bits_scan_noalloc: the per-chunk set-bit counts of the
+// scanned array, one u64 per SCAN_CHUNK words, then their total.  Every device
+// the library runs on holds its own copy (8 MiB) until the process ends.
+constexpr uint64_t NOALLOC_MAX_CHUNKS = 1ull << 20;  // arrays of up to 2^38 bits
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_MAX_CHUNKS + 1];
+
+namespace ps {
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+unsigned long long* g_scratch[MAX_DEVICES] = {};  // device address of `scratch`, per device
+}  // namespace
+
+int bits_scratch_init(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return fail(PS_ERR_UNSUPPORTED, "device %d >= %d", device, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device]) return PS_OK;
+    void* p = nullptr;
+    PS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = (unsigned long long*)p;
+    return PS_OK;
+}
+
+namespace {
+// ScanBits of nwords words into d_out with the chunk counts in tmp
+// (nchunks + 1 entries, all written here); the total goes to *d_count.
+int launch_scan(const uint64_t* d_a, uint64_t nwords, uint64_t nbits, unsigned long long* tmp, uint64_t nchunks,
+                uint64_t* d_out, uint64_t cap, uint64_t* d_count, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan_counts, dim3((uint32_t)nchunks), dim3(TPB), 0, st, d_a, nwords, nbits, tmp);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, tmp, nchunks);
+    hipLaunchKernelGGL(k_scan_write, dim3((uint32_t)nchunks), dim3(TPB), 0, st, d_a, nwords, nbits, tmp, d_out, cap);
+    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, tmp + nchunks, d_count);
+    PS_HIP_CHECK(hipGetLastError());
+    return PS_OK;
+}
+}  // namespace
 }  // namespace ps
 
 using namespace ps;
@@ -228,13 +270,14 @@ extern "C" int ps_bits_scan(const uint64_t* d_a, uint64_t nbits, uint64_t* d_out
     }
     if (!d_a || (cap && !d_out)) return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
     const uint64_t nchunks = (nwords + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (nchunks > 0xFFFFFFFFull) return fail(PS_ERR_INVALID_ARGUMENT, "array of %llu bits too large",
+                                            (unsigned long long)nbits);
     unsigned long long* tmp = nullptr;
     PS_HIP_CHECK(hipMallocAsync((void**)&tmp, (nchunks + 1) * 8, st));
-    hipLaunchKernelGGL(k_scan_counts, dim3((uint32_t)nchunks), dim3(TPB), 0, st, d_a, nwords, nbits, tmp);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, tmp, nchunks);
-    hipLaunchKernelGGL(k_scan_write, dim3((uint32_t)nchunks), dim3(TPB), 0, st, d_a, nwords, nbits, tmp, d_out, cap);
-    hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(1), 0, st, tmp + nchunks, d_count);
-    PS_HIP_CHECK(hipGetLastError());
+    if (int r = launch_scan(d_a, nwords, nbits, tmp, nchunks, d_out, cap, d_count, st)) {
+        (void)hipFreeAsync(tmp, st);
+        return r;
+    }
     uint64_t total = 0;
     PS_HIP_CHECK(hipMemcpyAsync(&total, tmp + nchunks, 8, hipMemcpyDeviceToHost, st));
     PS_HIP_CHECK(hipFreeAsync(tmp, st));
@@ -242,5 +285,38 @@ extern "C" int ps_bits_scan(const uint64_t* d_a, uint64_t nbits, uint64_t* d_out
     if (total > cap) return fail(PS_ERR_CAPACITY, "ps_bits_scan: %llu set bits exceed capacity %llu",
                                  (unsigned long long)total, (unsigned long long)cap);
     return PS_OK;
+    PS_API_END
+}
+
+// The same scan with its chunk counts in the static `scratch` array of this
+// device: no allocation and no wait, so a graph capture may hold it.  Clears
+// nothing first: every scratch entry a call reads it has written itself.
+extern "C" int ps_bits_scan_noalloc(const uint64_t* d_a, uint64_t nbits, uint64_t* d_out, uint64_t cap,
+                                    uint64_t* d_count, ps_stream s) {
+    PS_API_BEGIN
+    if (int r = require_runtime()) return r;
+    if (!d_count) return fail(PS_ERR_INVALID_ARGUMENT, "NULL count pointer");
+    hipStream_t st = stream_of(s);
+    const uint64_t nwords = (nbits + 63) / 64;
+    const uint64_t nchunks = (nwords + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (nchunks > NOALLOC_MAX_CHUNKS)
+        return fail(PS_ERR_INVALID_ARGUMENT,
+                    "ps_bits_scan_noalloc: %llu bits exceed the static scratch limit of 2^38 bits",
+                    (unsigned long long)nbits);
+    if (nwords == 0) {
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, d_count, 0ull);
+        PS_HIP_CHECK(hipGetLastError());
+        return PS_OK;
+    }
+    if (!d_a || (cap && !d_out)) return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
+    int dev = -1;
+    PS_HIP_CHECK(hipGetDevice(&dev));
+    unsigned long long* tmp = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        if (dev >= 0 && dev < MAX_DEVICES) tmp = g_scratch[dev];
+    }
+    if (!tmp) return fail(PS_ERR_NOT_INITIALIZED, "ps_bits_scan_noalloc: device %d was not set up by ps_init", dev);
+    return launch_scan(d_a, nwords, nbits, tmp, nchunks, d_out, cap, d_count, st);
     PS_API_END
 }

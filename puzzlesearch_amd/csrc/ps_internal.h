@@ -36,6 +36,11 @@ inline uint64_t factorial(int n) {
 
 int launch_fill(void* dst, uint8_t value, uint64_t bytes, hipStream_t s);
 
+// Looks up, once per device, the device address of the static scratch array
+// of the allocation-free bit scan (bits.hip); called by ps_init with the
+// device current.
+int bits_scratch_init(int device);
+
 }  // namespace ps
 
 #define PS_HIP_CHECK(expr)                                                                               \

@@ -119,6 +119,7 @@ int ps_init(int32_t device) {
     if (device < 0 || device >= n) return fail(PS_ERR_INVALID_ARGUMENT, "device %d out of range [0,%d)", device, n);
     PS_HIP_CHECK(hipSetDevice(device));
     PS_HIP_CHECK(hipFree(nullptr));  // force context creation
+    if (int r = bits_scratch_init(device)) return r;
     g_device = device;
     g_ready = true;
     clear_error();
