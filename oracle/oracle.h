@@ -188,6 +188,44 @@ int64_t or_fs_segment_step(const or_puzzle* p, int segment_bits, uint32_t segmen
                            const uint32_t* idx, const uint8_t* opbits, uint64_t n,
                            const uint32_t* xseg_idx, const uint64_t* xseg_op_starts, or_fs_io* io);
 
+/* ---- one segment step of the disk-based three-bit BFS ----
+ * DB_Opt3BitBFS_Solver::Expand(segment) (Common/DiskBasedOptThreeBitBFS.cpp:54-144)
+ * with in-memory lists.  Inputs: the segment's current and old frontiers (ids
+ * ascending) and the cross-segment marks of the previous step per op
+ * (xseg_idx[xseg_op_starts[op] .. xseg_op_starts[op+1])).  Outputs: the new
+ * frontier (ids ascending) and its FrontierCompression::Encode chunks of 2^18
+ * values (CompressedSegmentWriter); the cross-segment buckets in (op, seg)
+ * order, each as FrontierCompression::EncodeWithCheck chunks of 4096 values
+ * in emission order, concatenated in xs_bytes.  Returns the layer count of the
+ * segment or -1 (capacity or argument error). */
+typedef struct or_seg3_io {
+    uint32_t* fr_idx;
+    uint64_t fr_cap;
+    uint64_t fr_n;
+    uint8_t* fr_bytes;
+    uint64_t fr_bytes_cap;
+    uint64_t fr_nbytes;
+    uint32_t* bk_op;
+    uint32_t* bk_seg;
+    uint64_t* bk_values;
+    uint64_t* bk_nbytes;
+    uint64_t bk_cap;
+    uint64_t bk_n;
+    uint8_t* xs_bytes;
+    uint64_t xs_cap;
+    uint64_t xs_nbytes;
+    uint64_t children_in;
+    uint64_t children_cross;
+    uint64_t count;
+} or_seg3_io;
+
+int64_t or_bfs3_segment_step(const or_puzzle* p, int segment_bits, uint32_t segment, uint32_t nseg,
+                             const uint32_t* cur, uint64_t ncur, const uint32_t* old, uint64_t nold,
+                             const uint32_t* xseg_idx, const uint64_t* xseg_op_starts, or_seg3_io* io);
+/* The expansion mode of ExpandHint {segment_bits, cross} for this puzzle
+ * (Puzzles/PancakeOptimizedGpu.cpp:57-79). */
+int or_hint_mode(const or_puzzle* p, int segment_bits, int cross);
+
 #ifdef __cplusplus
 }
 #endif

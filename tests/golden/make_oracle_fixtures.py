@@ -30,6 +30,9 @@ FIXTURES = {
     "hanoi_12": (lambda: O.hanoi(12), "index:0", 0, True),
     "hanoi_12_sym": (lambda: O.hanoi(12, True), "index:0", 0, False),
     "hanoi_13": (lambda: O.hanoi(13), "index:0", 0, False),
+    # the Hanoi-16 batched search the benchmark times: 4^16 states, 162 layers
+    # (about 25 minutes on one core)
+    "hanoi_16": (lambda: O.hanoi(16), "index:0", 0, False),
     "sliding_packed_4x3": (lambda: O.sliding_packed(4, 3), "0 1 2 3 4 5 6 7 8 9 10 11", 0, False),
     "sliding_packed_5x2": (lambda: O.sliding_packed(5, 2), "0 1 2 3 4 5 6 7 8 9", 0, True),
 }

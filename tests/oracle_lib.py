@@ -78,9 +78,22 @@ class FsIo(ctypes.Structure):
                 ("children_in", _u64), ("children_cross", _u64), ("count", _u64)]
 
 
+class Seg3Io(ctypes.Structure):
+    """or_seg3_io (oracle.h)."""
+    _fields_ = [("fr_idx", ctypes.c_void_p), ("fr_cap", _u64), ("fr_n", _u64),
+                ("fr_bytes", ctypes.c_void_p), ("fr_bytes_cap", _u64), ("fr_nbytes", _u64),
+                ("bk_op", ctypes.c_void_p), ("bk_seg", ctypes.c_void_p), ("bk_values", ctypes.c_void_p),
+                ("bk_nbytes", ctypes.c_void_p), ("bk_cap", _u64), ("bk_n", _u64),
+                ("xs_bytes", ctypes.c_void_p), ("xs_cap", _u64), ("xs_nbytes", _u64),
+                ("children_in", _u64), ("children_cross", _u64), ("count", _u64)]
+
+
 _SIGS = {
     "or_fs_segment_step": (ctypes.c_int64, [_P, _i, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u8p, _u64,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(FsIo)]),
+    "or_bfs3_segment_step": (ctypes.c_int64, [_P, _i, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u64, _u32p, _u64,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Seg3Io)]),
+    "or_hint_mode": (_i, [_P, _i, _i]),
     "or_splitmix64": (_u64, [_u64]),
     "or_bits_hash": (_u64, [_u64p, _u64]),
     "or_children_hash": (_u64, [_u64p, _i32p, _u64]),
@@ -288,3 +301,45 @@ def fs_segment_step(p, segment_bits, segment, nseg, idx, opbits, xseg=None):
     return {"count": int(r), "fr_idx": fr_idx[:io.fr_n].copy(), "fr_ops": fr_ops[:io.fr_n].copy(),
             "fr_bytes": fr_bytes[:io.fr_nbytes].copy(), "buckets": buckets,
             "children_in": int(io.children_in), "children_cross": int(io.children_cross)}
+
+
+def bfs3_segment_step(p, segment_bits, segment, nseg, cur, old, xseg=None):
+    """One three-bit BFS segment step (or_bfs3_segment_step); returns a dict
+    with count, the new frontier ids, their FrontierCompression stream bytes,
+    and the buckets [(op, seg, nvalues, bytes)] in (op, seg) order."""
+    cur = np.ascontiguousarray(cur, dtype=np.uint32)
+    old = np.ascontiguousarray(old, dtype=np.uint32)
+    bf = int(lib().or_branching_factor(ctypes.byref(p)))
+    ops = int(lib().or_operators_count(ctypes.byref(p)))
+    xs_idx = xs_st = None
+    nx = 0
+    if xseg is not None:
+        xs_idx = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint32) for x in xseg] +
+                                                     [np.zeros(0, np.uint32)]), dtype=np.uint32)
+        xs_st = np.zeros(ops + 1, np.uint64)
+        xs_st[1:] = np.cumsum([len(x) for x in xseg])
+        nx = len(xs_idx)
+    cap = len(cur) * bf + nx + 8
+    fr_idx = np.zeros(cap, np.uint32)
+    fr_bytes = np.zeros(5 * cap + 64 * (cap // (1 << 18) + 1) + 64, np.uint8)
+    bk_cap = cap * bf + 8
+    bk_op = np.zeros(bk_cap, np.uint32)
+    bk_seg = np.zeros(bk_cap, np.uint32)
+    bk_values = np.zeros(bk_cap, np.uint64)
+    bk_nbytes = np.zeros(bk_cap, np.uint64)
+    xs_bytes = np.zeros(5 * cap * bf + 64 * (cap * bf // 4096 + 1) + 64, np.uint8)
+    io = Seg3Io(fr_idx.ctypes.data, cap, 0, fr_bytes.ctypes.data, len(fr_bytes), 0,
+                bk_op.ctypes.data, bk_seg.ctypes.data, bk_values.ctypes.data, bk_nbytes.ctypes.data, bk_cap, 0,
+                xs_bytes.ctypes.data, len(xs_bytes), 0, 0, 0, 0)
+    r = lib().or_bfs3_segment_step(ctypes.byref(p), segment_bits, segment, nseg, cur, len(cur), old, len(old),
+                                   xs_idx.ctypes.data if xs_idx is not None else None,
+                                   xs_st.ctypes.data if xs_st is not None else None, ctypes.byref(io))
+    assert r >= 0, "oracle three-bit segment step failed"
+    buckets = []
+    off = 0
+    for b in range(io.bk_n):
+        nb = int(bk_nbytes[b])
+        buckets.append((int(bk_op[b]), int(bk_seg[b]), int(bk_values[b]), xs_bytes[off:off + nb].copy()))
+        off += nb
+    return {"count": int(r), "fr_idx": fr_idx[:io.fr_n].copy(), "fr_bytes": fr_bytes[:io.fr_nbytes].copy(),
+            "buckets": buckets, "children_in": int(io.children_in), "children_cross": int(io.children_cross)}

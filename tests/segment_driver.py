@@ -103,3 +103,82 @@ def oracle_step(p, segment_bits, nseg):
     def step(s, fi, fo, xs):
         return O.fs_segment_step(p, segment_bits, s, nseg, fi, fo, xs)
     return step
+
+
+def decode_fc_stream(data):
+    """FrontierCompression chunks (bitmap or StreamVInt) back to their values,
+    as CompressedSegmentReader reads a segment (Common/CompressedFrontier.cpp:90-100)."""
+    out = []
+    pos = 0
+    pad = np.concatenate([np.asarray(data, np.uint8), np.zeros(64, np.uint8)])
+    buf = np.zeros((1 << 18) + 64, np.uint32)
+    while pos < len(data):
+        size = ctypes.c_int(len(data) - pos)
+        n = O.lib().or_fc_decode(ctypes.byref(size), pad[pos:], buf)
+        assert n > 0 and size.value > 0
+        out.append(buf[:n].copy())
+        pos += size.value
+    assert pos == len(data)
+    return np.concatenate(out) if out else np.zeros(0, np.uint32)
+
+
+def three_bit_search(p, start, segment_bits, step, max_layers=1000, check_streams=True):
+    """DiskBasedOptThreeBitBFS (Common/DiskBasedOptThreeBitBFS.cpp:164-252) with
+    the stores replaced by dicts: SetInitialNode writes the start state and
+    buckets its cross-segment children (:34-52); every layer runs the step on
+    every segment (:216-218) with its current and old frontier and the marks
+    bucketed for it by the previous layer, then rotates old <- cur <- new
+    (:182-187).  The new frontier and the buckets are read back from their
+    compressed streams, as the next layer's readers would.
+
+    `step(segment, cur, old, xseg_per_op)` returns a dict with count, fr_bytes,
+    buckets [(op, seg, nvalues, bytes)] and optionally fr_idx."""
+    ops = int(O.lib().or_operators_count(ctypes.byref(p)))
+    total = int(O.lib().or_indexes_count(ctypes.byref(p)))
+    nseg = max(1, (total + (1 << segment_bits) - 1) >> segment_bits)
+    mask = (1 << segment_bits) - 1
+    seg0 = start >> segment_bits
+    empty = np.zeros(0, np.uint32)
+    cur = {seg0: np.array([start & mask], np.uint32)}
+    old = {}
+    xseg = {}
+    ch = np.zeros(16, np.uint64)
+    chop = np.zeros(16, np.int32)
+    mode = O.lib().or_hint_mode(ctypes.byref(p), segment_bits, 1)
+    k = O.lib().or_expand_one(ctypes.byref(p), mode, start, 0, ch, chop)
+    for j in range(k):
+        s = int(ch[j]) >> segment_bits
+        if s != seg0:
+            xseg.setdefault(s, [[] for _ in range(ops)])[int(chop[j])].append(int(ch[j]) & mask)
+    counts = [1]
+    while len(counts) < max_layers:
+        new, new_x = {}, {}
+        layer = 0
+        for s in range(nseg):
+            xs = xseg.get(s)
+            xs_arrays = [np.asarray(x, np.uint32) for x in xs] if xs else None
+            r = step(s, cur.get(s, empty), old.get(s, empty), xs_arrays)
+            layer += r["count"]
+            if check_streams:
+                fr = decode_fc_stream(r["fr_bytes"])
+                if "fr_idx" in r:
+                    assert np.array_equal(fr, r["fr_idx"])
+            else:
+                fr = r["fr_idx"]
+            assert len(fr) == r["count"]
+            if len(fr):
+                new[s] = fr
+            for op, seg, nval, data in r["buckets"]:
+                vals = decode_bucket(data, nval)
+                new_x.setdefault(seg, [[] for _ in range(ops)])[op].extend(vals.tolist())
+        if layer == 0:
+            break
+        counts.append(layer)
+        old, cur, xseg = cur, new, new_x
+    return counts
+
+
+def oracle_step3(p, segment_bits, nseg):
+    def step(s, cur, old, xs):
+        return O.bfs3_segment_step(p, segment_bits, s, nseg, cur, old, xs)
+    return step
