@@ -13,8 +13,6 @@
 // chunk's tuple descriptors (one thread per chunk: a descriptor's position
 // depends on every earlier tuple's size), then decodes tuples in parallel and
 // undoes the delta coding with a scan per chunk.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -24,6 +22,7 @@
 #include "codec_internal.h"
 #include "devbuf.h"
 #include "ps_internal.h"
+#include "seg_sort.h"
 
 namespace ps {
 namespace {
@@ -565,25 +564,18 @@ __global__ void k_opbits_decode(int bits, const uint8_t* in, uint64_t groups, ui
 
 // ---------------------------------------------------------------- buckets
 
-__global__ void k_bucket_keys(const uint64_t* children, const uint8_t* ops, uint64_t n, uint64_t* keys) {
+// (op * nseg + segment) << 32 | in-segment index; a child outside the
+// segments or an op >= nops gets the key 0xFFFFFFFF, which the grouping drops
+// and counts
+__global__ void k_bucket_keys(const uint64_t* children, const uint8_t* ops, uint64_t n, int seg_bits, uint32_t nseg,
+                              uint32_t nops, unsigned long long* kv) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    // a child index beyond 2^56 would alias the op field: mark it invalid (op 255)
     const uint64_t c = children[i];
-    keys[i] = (c >> 56) ? (255ull << 56) : (((uint64_t)ops[i] << 56) | c);
-}
-
-__global__ void k_bucket_split(const uint64_t* keys, uint64_t n, int seg_bits, uint32_t nseg, uint32_t nops,
-                               uint32_t* idx_out, unsigned long long* bucket_counts) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t k = keys[i];
-    const uint64_t op = k >> 56;
-    const uint64_t child = k & ((1ull << 56) - 1);
-    const uint64_t seg = child >> seg_bits;
-    idx_out[i] = (uint32_t)(child & ((1ull << seg_bits) - 1));
-    // out-of-range keys are not counted; the host then reports the mismatch
-    if (op < nops && seg < nseg) atomicAdd(&bucket_counts[op * nseg + seg], 1ull);
+    const uint64_t seg = c >> seg_bits;
+    const uint32_t op = ops[i];
+    const uint64_t key = (op < nops && seg < nseg) ? (uint64_t)op * nseg + seg : 0xFFFFFFFFull;
+    kv[i] = (key << 32) | (c & ((1ull << seg_bits) - 1));
 }
 
 
@@ -969,29 +961,30 @@ int ps_bucket_by_segment(const uint64_t* d_children, const uint8_t* d_ops, uint6
         return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
     hipStream_t s = stream_of(st);
     const uint64_t nb = (uint64_t)nseg * nops;
-    DevBuf d_keys, d_sorted, d_counts, d_tmp;
-    if (int r = d_keys.alloc(count * 8, s)) return r;
-    if (int r = d_sorted.alloc(count * 8, s)) return r;
-    if (int r = d_counts.alloc((nb + 1) * 8, s)) return r;
-    if (int r = launch_fill(d_counts.p, 0, (nb + 1) * 8, s)) return r;
-    if (count) {
-        hipLaunchKernelGGL(k_bucket_keys, dim3(blocks_for(count, 256)), dim3(256), 0, s, d_children, d_ops, count,
-                           d_keys.as<uint64_t>());
-        size_t tmp_bytes = 0;
-        PS_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, d_keys.as<uint64_t>(),
-                                                       d_sorted.as<uint64_t>(), (int)count, 0, 64, s));
-        if (int r = d_tmp.alloc(tmp_bytes, s)) return r;
-        PS_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(d_tmp.p, tmp_bytes, d_keys.as<uint64_t>(),
-                                                       d_sorted.as<uint64_t>(), (int)count, 0, 64, s));
-        hipLaunchKernelGGL(k_bucket_split, dim3(blocks_for(count, 256)), dim3(256), 0, s, d_sorted.as<uint64_t>(),
-                           count, segment_bits, nseg, (uint32_t)nops, d_sorted_idx, d_counts.as<unsigned long long>());
-        PS_HIP_CHECK(hipGetLastError());
-    }
-    std::vector<uint64_t> counts;
-    if (int r = to_host(counts, d_counts.p, nb, s)) return r;
     std::vector<uint64_t> starts(nb + 1, 0);
-    for (uint64_t b = 0; b < nb; ++b) starts[b + 1] = starts[b] + counts[b];
-    if (starts[nb] != count) return fail(PS_ERR_INDEX_RANGE, "children outside [0, nseg) segments or ops >= nops");
+    if (count) {
+        // (op * nseg + segment) << 32 | in-segment index, sorted by index with
+        // 8-bit stable counting passes, then stably grouped by bucket: buckets
+        // in (op, segment) order, each ascending
+        DevBuf d_kv, d_tmp;
+        if (int r = d_kv.alloc(count * 8, s)) return r;
+        if (int r = d_tmp.alloc(count * 8, s)) return r;
+        hipLaunchKernelGGL(k_bucket_keys, dim3(blocks_for(count, 256)), dim3(256), 0, s, d_children, d_ops, count,
+                           segment_bits, nseg, (uint32_t)nops, d_kv.as<unsigned long long>());
+        PS_HIP_CHECK(hipGetLastError());
+        if (int r = stable_sort_by_value(d_kv.as<unsigned long long>(), d_tmp.as<unsigned long long>(), count,
+                                         segment_bits, s))
+            return r;
+        std::vector<uint32_t> keys;
+        std::vector<uint64_t> counts;
+        uint64_t dropped = 0;
+        if (int r = stable_group_by_key(d_kv.as<unsigned long long>(), count, (uint32_t)nb, d_sorted_idx, keys, counts,
+                                        &dropped, s))
+            return r;
+        if (dropped) return fail(PS_ERR_INDEX_RANGE, "children outside [0, nseg) segments or ops >= nops");
+        for (size_t k = 0; k < keys.size(); ++k) starts[keys[k] + 1] = counts[k];
+        for (uint64_t b = 0; b < nb; ++b) starts[b + 1] += starts[b];
+    }
     PS_HIP_CHECK(hipMemcpyAsync(d_bucket_starts, starts.data(), starts.size() * 8, hipMemcpyHostToDevice, s));
     PS_HIP_CHECK(hipStreamSynchronize(s));
     return PS_OK;

@@ -10,16 +10,14 @@
 //   stream   the new frontier as StreamVInt with bounds in 2^18-value chunks,
 //            as CompressedFrontierWriter writes it (Common/CompressedFrontier.cpp:54-75)
 //   cross    children of the written entries in other segments, one slot per
-//            move, compacted in emission order, stably sorted by
-//            (op, segment), and each (op, segment) bucket cut into 4096-value
+//            move, compacted in emission order, stably grouped by
+//            (op, segment) with a counting sort (seg_sort.h), and each bucket cut into 4096-value
 //            chunks compressed with FrontierCompression::EncodeWithCheck, as
 //            SmartMultiplexor flushes them (Common/Multiplexor.cpp:125-161)
 //
 // Emission order inside a bucket is (parent, slot) order, the order in which
 // ExpandBuffer hands children to SmartMultiplexor::Add for one segment, so the
 // bucket bytes equal the reference's for a segment step run on its own.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -33,6 +31,7 @@
 #include "devbuf.h"
 #include "fs_launch.h"
 #include "ps_internal.h"
+#include "seg_sort.h"
 
 namespace {
 
@@ -112,15 +111,6 @@ __global__ void k_fs_clear_odd(const uint32_t* idx, uint64_t n, uint64_t in_mask
     }
 }
 
-// sorted (key << 32 | value) -> key and value arrays
-__global__ void k_fs_split(const unsigned long long* kv, uint64_t n, uint32_t* keys, uint32_t* vals) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const unsigned long long x = kv[i];
-        keys[i] = (uint32_t)(x >> 32);
-        vals[i] = (uint32_t)x;
-    }
-}
-
 }  // namespace psk
 
 struct ps_fs_segment {
@@ -130,7 +120,7 @@ struct ps_fs_segment {
     bool own_stream = false;
     int seg_bits = 0;
     uint32_t nseg = 0;
-    int ops = 0, slots = 0, odd = 0, key_bits = 0;
+    int ops = 0, slots = 0, odd = 0;
     uint64_t nwords = 0, nregions = 0;
     uint32_t* flags = nullptr;
     uint8_t* dirty = nullptr;
@@ -139,8 +129,7 @@ struct ps_fs_segment {
     unsigned long long* out_off = nullptr;
     unsigned long long* counters = nullptr;
     unsigned long long* h_counters = nullptr;  // pinned
-    GrowBuf out_idx, out_ops, fr_bytes, slot_buf, compact, sorted, keys, vals, uniq, runs, nruns, xs_bytes, tmp,
-        xseg_starts, block_cnt, block_off;
+    GrowBuf out_idx, out_ops, fr_bytes, slot_buf, compact, vals, xs_bytes, xseg_starts, block_cnt, block_off;
     uint32_t cus = 256;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // results of the last step (host)
@@ -157,9 +146,8 @@ namespace {
 int destroy_fs(ps_fs_segment* w) {
     if (!w) return PS_OK;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
-    for (GrowBuf* g : {&w->out_idx, &w->out_ops, &w->fr_bytes, &w->slot_buf, &w->compact, &w->sorted, &w->keys,
-                       &w->vals, &w->uniq, &w->runs, &w->nruns, &w->xs_bytes, &w->tmp, &w->xseg_starts,
-                       &w->block_cnt, &w->block_off})
+    for (GrowBuf* g : {&w->out_idx, &w->out_ops, &w->fr_bytes, &w->slot_buf, &w->compact, &w->vals, &w->xs_bytes,
+                       &w->xseg_starts, &w->block_cnt, &w->block_off})
         g->release(w->stream);
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     if (w->flags) (void)hipFree(w->flags);
@@ -182,8 +170,6 @@ int read_counters(ps_fs_segment* w) {
     PS_HIP_CHECK(hipStreamSynchronize(w->stream));
     return PS_OK;
 }
-
-int tmp_for(ps_fs_segment* w, size_t bytes) { return w->tmp.ensure(bytes, w->stream); }
 
 int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const uint8_t* d_ops, uint64_t n,
              const uint32_t* d_xidx, const uint64_t* xstarts, ps_fs_result* res) {
@@ -243,10 +229,7 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
     const uint32_t grid_scan = (uint32_t)std::min<uint64_t>(w->nregions, (uint64_t)w->cus * 8);
     hipLaunchKernelGGL(psk::k_fs_count<FS_TPB>, dim3(grid_scan), dim3(FS_TPB), 0, s, sa);
     PS_HIP_CHECK(hipGetLastError());
-    size_t tb = 0;
-    PS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w->cnt_out, w->out_off, (int)w->nregions, s));
-    if (int r = tmp_for(w, tb)) return r;
-    PS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w->tmp.p, tb, w->cnt_out, w->out_off, (int)w->nregions, s));
+    if (int r = scan_excl_u64(w->cnt_out, w->nregions, w->out_off, s)) return r;
     if (int r = read_counters(w)) return r;
     if (w->h_counters[6]) {
         // the marks of the valid ids are still cleared by the write pass below
@@ -305,14 +288,11 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
         ca.counters = w->counters;
         const uint64_t nblk = (n_out + psk::FS_CROSS_PER_BLOCK - 1) / psk::FS_CROSS_PER_BLOCK;
         if (int r = w->block_cnt.ensure(nblk * 8, s)) return r;
-        if (int r = w->block_off.ensure(nblk * 8, s)) return r;
+        if (int r = w->block_off.ensure((nblk + 1) * 8, s)) return r;
         ca.block_count = w->block_cnt.as<unsigned long long>();
         if (int r = fs_cross(w->puzzle, ca, (uint32_t)nblk, s, nullptr)) return r;
-        PS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w->block_cnt.as<unsigned long long>(),
-                                                      w->block_off.as<unsigned long long>(), (int)nblk, s));
-        if (int r = tmp_for(w, tb)) return r;
-        PS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w->tmp.p, tb, w->block_cnt.as<unsigned long long>(),
-                                                      w->block_off.as<unsigned long long>(), (int)nblk, s));
+        if (int r = scan_excl_u64(w->block_cnt.as<unsigned long long>(), nblk, w->block_off.as<unsigned long long>(), s))
+            return r;
         if (int r = read_counters(w)) return r;
         if (w->h_counters[4])
             return fail(PS_ERR_INDEX_RANGE, "%llu children fall outside the %u segments",
@@ -322,7 +302,6 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
             return fail(PS_ERR_HIP, "%llu cross-segment slots filled for %llu cross-segment children",
                         (unsigned long long)w->h_counters[7], (unsigned long long)m);
         if (int r = w->compact.ensure(std::max<uint64_t>(m, 1) * 8, s)) return r;
-        if (int r = w->nruns.ensure(16, s)) return r;
         hipLaunchKernelGGL(psk::k_fs_compact<256>, dim3((uint32_t)nblk), dim3(256), 0, s,
                            (const unsigned long long*)w->slot_buf.as<unsigned long long>(),
                            (uint32_t)(psk::FS_CROSS_PER_BLOCK * w->slots),
@@ -338,34 +317,16 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
     std::vector<uint64_t> xoff;
     uint64_t xbytes = 0, nchunks_x = 0;
     if (m) {
-        if (int r = w->sorted.ensure(m * 8, s)) return r;
-        PS_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, w->compact.as<unsigned long long>(),
-                                                       w->sorted.as<unsigned long long>(), (int)m, 32,
-                                                       32 + w->key_bits, s));
-        if (int r = tmp_for(w, tb)) return r;
-        PS_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(w->tmp.p, tb, w->compact.as<unsigned long long>(),
-                                                       w->sorted.as<unsigned long long>(), (int)m, 32,
-                                                       32 + w->key_bits, s));
-        if (int r = w->keys.ensure(m * 4, s)) return r;
+        // stable grouping by (op, segment): emission order inside a bucket
         if (int r = w->vals.ensure(m * 4 + 64, s)) return r;
-        hipLaunchKernelGGL(psk::k_fs_split, dim3((uint32_t)std::min<uint64_t>(blocks_for(m, 256), w->cus * 8ull)),
-                           dim3(256), 0, s, w->sorted.as<unsigned long long>(), m, w->keys.as<uint32_t>(),
-                           w->vals.as<uint32_t>());
-        PS_HIP_CHECK(hipGetLastError());
-        const uint64_t maxb = std::min<uint64_t>(m, (uint64_t)w->ops * w->nseg);
-        if (int r = w->uniq.ensure(maxb * 4, s)) return r;
-        if (int r = w->runs.ensure(maxb * 4, s)) return r;
-        PS_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, w->keys.as<uint32_t>(), w->uniq.as<uint32_t>(),
-                                                           w->runs.as<uint32_t>(), w->nruns.as<int>(), (int)m, s));
-        if (int r = tmp_for(w, tb)) return r;
-        PS_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(w->tmp.p, tb, w->keys.as<uint32_t>(), w->uniq.as<uint32_t>(),
-                                                           w->runs.as<uint32_t>(), w->nruns.as<int>(), (int)m, s));
-        int nb = 0;
-        PS_HIP_CHECK(hipMemcpyAsync(&nb, w->nruns.p, 4, hipMemcpyDeviceToHost, s));
-        PS_HIP_CHECK(hipStreamSynchronize(s));
-        std::vector<uint32_t> uk, rc;
-        if (int r = to_host(uk, w->uniq.p, (size_t)nb, s)) return r;
-        if (int r = to_host(rc, w->runs.p, (size_t)nb, s)) return r;
+        std::vector<uint32_t> uk;
+        std::vector<uint64_t> rc;
+        uint64_t dropped = 0;
+        if (int r = stable_group_by_key(w->compact.as<unsigned long long>(), m, (uint32_t)(w->ops * w->nseg),
+                                        w->vals.as<uint32_t>(), uk, rc, &dropped, s))
+            return r;
+        if (dropped) return fail(PS_ERR_INTERNAL, "%llu cross-segment keys out of range", (unsigned long long)dropped);
+        const int nb = (int)uk.size();
         // 4096-value chunks inside every bucket
         std::vector<uint64_t> cstarts;
         std::vector<uint64_t> first_chunk(nb + 1);
@@ -440,7 +401,7 @@ int ps_fs_segment_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st,
     if (segment_bits < 6 || segment_bits > 32) return fail(PS_ERR_INVALID_ARGUMENT, "segment_bits must be 6..32");
     const uint64_t seg_size = 1ull << segment_bits;
     const uint64_t nseg64 = (info.indexes_count + seg_size - 1) / seg_size;
-    if (nseg64 == 0 || nseg64 * (uint64_t)info.operators_count > (1ull << 31))
+    if (nseg64 == 0 || nseg64 * (uint64_t)info.operators_count > (1ull << 26))  // (op, segment) bucket keys
         return fail(PS_ERR_INVALID_ARGUMENT, "%llu segments of 2^%d states are too many",
                     (unsigned long long)nseg64, segment_bits);
     ps_fs_segment* w = new (std::nothrow) ps_fs_segment();
@@ -455,9 +416,6 @@ int ps_fs_segment_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st,
     w->nseg = (uint32_t)nseg64;
     w->ops = info.operators_count;
     w->odd = info.has_odd_cycles;
-    int kb = 1;
-    while ((1ull << kb) <= (uint64_t)w->ops * w->nseg) ++kb;  // all-ones stays above every key
-    w->key_bits = kb;
     w->nwords = seg_size / 8;
     w->nregions = (seg_size + (1ull << psk::FS_REGION_BITS) - 1) >> psk::FS_REGION_BITS;
     w->cus = (uint32_t)hip_cu_count();

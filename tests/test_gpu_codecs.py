@@ -185,10 +185,17 @@ def test_fc_bitmap_selected_where_reference_selects_it(C):
     assert not data[int(off[1]) + 3] & 0x80
 
 
-def test_bucket_by_segment(C):
-    rng = np.random.default_rng(31)
-    seg_bits, nseg, nops = 20, 37, 4
-    n = 300000
+@pytest.mark.parametrize("seg_bits,nseg,nops,n", [
+    (20, 37, 4, 300000),      # few buckets, three 8-bit value passes
+    (32, 5, 2, 100000),       # full 32-bit values: four value passes
+    (7, 3000, 4, 400000),     # 12000 buckets: two grouping passes of at most 8192 keys
+    (12, 20000, 15, 2000),    # more buckets than entries: most are empty
+    (16, 1, 1, 1),            # one entry
+    (16, 4, 2, 0),            # no entries
+    (30, 2, 3, 3000000),      # several chunk sizes of the counting passes
+])
+def test_bucket_by_segment(C, seg_bits, nseg, nops, n):
+    rng = np.random.default_rng(31 + n)
     children = rng.integers(0, nseg << seg_bits, size=n, dtype=np.uint64)
     ops = rng.integers(0, nops, size=n, dtype=np.uint8)
     idx, starts = C.bucket_by_segment(children, ops, seg_bits, nseg, nops)
