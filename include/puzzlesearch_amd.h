@@ -252,6 +252,64 @@ int ps_fs_segment_frontier(const ps_fs_segment* w, const uint32_t** d_idx, const
 int ps_fs_segment_buckets(const ps_fs_segment* w, uint32_t* op, uint32_t* segment, uint64_t* values,
                           uint64_t* byte_offsets, uint64_t cap, const uint8_t** d_bytes);
 
+/* ------------------------------------ disk-segment step of the three-bit BFS */
+/* One segment step of DiskBasedOptThreeBitBFS, DB_Opt3BitBFS_Solver::Expand(segment)
+ * (Common/DiskBasedOptThreeBitBFS.cpp:54-144), for any puzzle: the incoming
+ * cross-segment marks (after Puzzle::CrossSegmentPostProcess) and the
+ * in-segment children of the current frontier set NextArray, the old frontier
+ * clears it, and the scan of NextArray (excluding the current frontier for
+ * puzzles with odd cycles) gives the new layer of the segment.  Its ids are
+ * written ascending and as the FrontierCompression::Encode chunks of 2^18
+ * values that CompressedSegmentWriter stores; their cross-segment children go
+ * to (op, segment) buckets in emission order, each cut into
+ * FrontierCompression::EncodeWithCheck chunks of 4096 values as SmartMultiplexor
+ * flushes them.  The expansion follows the plugins' ExpandHint: more than 12
+ * pancakes in 29-bit segments use the in-segment and cross-segment flip sets.
+ * The workspace holds the segment's bit arrays (2^segment_bits bits each,
+ * two for puzzles with odd cycles) and grows its output buffers as needed; one
+ * step at a time per workspace. */
+typedef struct ps_seg3 ps_seg3;
+
+typedef struct ps_seg3_result {
+    uint64_t count;           /* states of the new layer in the segment (Expand's return value) */
+    uint64_t frontier_bytes;  /* their FrontierCompression stream */
+    uint64_t frontier_chunks; /* chunks of 2^18 values */
+    uint64_t children_in;     /* in-segment children marked */
+    uint64_t children_cross;  /* children bucketed for other segments */
+    uint64_t buckets;         /* nonempty (op, segment) buckets */
+    uint64_t cross_chunks;    /* FrontierCompression chunks of 4096 values over all buckets */
+    uint64_t cross_bytes;
+    uint64_t scanned_words;   /* NextArray words in the regions the scan visited */
+    uint64_t algorithmic_bytes; /* inputs + outputs + 8 per child + 16 per scanned word */
+    double device_ms;         /* first to last kernel of the step */
+    double wall_ms;
+} ps_seg3_result;
+
+int ps_seg3_create(const ps_puzzle* p, int32_t segment_bits, ps_stream s, ps_seg3** out);
+int ps_seg3_destroy(ps_seg3* w);
+int ps_seg3_info(const ps_seg3* w, uint32_t* segments, int32_t* operators, uint64_t* footprint_bytes);
+/* Inputs (device): the segment's current frontier d_cur[ncur] and old
+ * frontier d_old[nold] (in-segment ids, ascending, as CompressedSegmentReader
+ * yields them), and optionally the cross-segment marks for this segment,
+ * d_xseg[xseg_op_starts[op] .. xseg_op_starts[op+1]) for each op
+ * (xseg_op_starts: host, operators+1 entries).  Outputs stay in the workspace
+ * until the next step.  A segment with neither marks nor a current frontier
+ * has no data: count 0 and no output (:97). */
+int ps_seg3_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur, const uint32_t* d_old,
+                 uint64_t nold, const uint32_t* d_xseg, const uint64_t* xseg_op_starts, ps_seg3_result* res);
+/* The new frontier: device ids (ascending) and their stream; chunk_offsets
+ * (host, frontier_chunks + 1 entries) receives the chunk byte offsets.  A
+ * device pointer is NULL while the workspace has never held such output. */
+int ps_seg3_frontier(const ps_seg3* w, const uint32_t** d_idx, const uint8_t** d_bytes, uint64_t* chunk_offsets,
+                     uint64_t cap);
+/* The cross-segment buckets in (op, segment) order: bucket b holds values[b]
+ * children, the in-segment ids d_values[value_offsets[b] ..) of its target
+ * segment in emission order, compressed as d_bytes[byte_offsets[b] ..
+ * byte_offsets[b+1]) (host arrays of cap entries; the offset arrays need
+ * cap + 1).  Device pointers are NULL while no step has produced a child. */
+int ps_seg3_buckets(const ps_seg3* w, uint32_t* op, uint32_t* segment, uint64_t* values, uint64_t* value_offsets,
+                    uint64_t* byte_offsets, uint64_t cap, const uint32_t** d_values, const uint8_t** d_bytes);
+
 /* --------------------------------------------------- multi-search engine */
 /* Continuous batching of complete three-bit BFS jobs (InMemoryThreeBitBFS,
  * Common/InMemoryThreeBitBFS.cpp:4-48, for many puzzles at once).  No

@@ -119,6 +119,16 @@ class FsResult(ctypes.Structure):
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
+
+class Seg3Result(ctypes.Structure):
+    """ps_seg3_result"""
+    _fields_ = [(n, _u64) for n in ("count", "frontier_bytes", "frontier_chunks", "children_in", "children_cross",
+                                    "buckets", "cross_chunks", "cross_bytes", "scanned_words",
+                                    "algorithmic_bytes")] + [("device_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
 class BatchOptions(ctypes.Structure):
     """ps_batch_options"""
     _fields_ = [("max_jobs", _i32), ("segment_bits", _i32), ("hbm_budget", _u64), ("mode", _i32),
@@ -184,6 +194,12 @@ _SIGS = {
     "ps_fs_segment_frontier": (_i32, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
                                       _u64]),
     "ps_fs_segment_buckets": (_i32, [_vp, _vp, _vp, _vp, _vp, _u64, ctypes.POINTER(_vp)]),
+    "ps_seg3_create": (_i32, [_P, _i32, _vp, ctypes.POINTER(_vp)]),
+    "ps_seg3_destroy": (_i32, [_vp]),
+    "ps_seg3_info": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_i32), ctypes.POINTER(_u64)]),
+    "ps_seg3_step": (_i32, [_vp, ctypes.c_uint32, _vp, _u64, _vp, _u64, _vp, _vp, ctypes.POINTER(Seg3Result)]),
+    "ps_seg3_frontier": (_i32, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _u64]),
+    "ps_seg3_buckets": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _u64, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "ps_batch_options_default": (None, [ctypes.POINTER(BatchOptions)]),
     "ps_batch_create": (_i32, [ctypes.POINTER(BatchOptions), _vp, ctypes.POINTER(_vp)]),
     "ps_batch_destroy": (_i32, [_vp]),

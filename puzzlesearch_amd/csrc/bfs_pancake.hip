@@ -3,10 +3,12 @@
 #include "bfs_launch.h"
 #include "bfs_pancake_lists.h"
 #include "fs_launch.h"
+#include "seg3_launch.h"
 
 namespace ps {
 PS_DEFINE_BFS_FAMILY(pancake, FAM_PANCAKE)
 PS_DEFINE_FS_FAMILY(pancake, FAM_PANCAKE)
+PS_DEFINE_S3_FAMILY(pancake, FAM_PANCAKE)
 
 namespace {
 

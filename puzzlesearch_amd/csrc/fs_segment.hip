@@ -38,44 +38,7 @@ namespace {
 constexpr uint64_t FS_FRONTIER_CHUNK = 1ull << 18;  // StreamVInt::MAX_INDEXES_COUNT
 constexpr uint64_t FS_BUCKET_CHUNK = 1ull << 12;    // SmartMultiplexor small sector
 
-struct GrowBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    int ensure(size_t bytes, hipStream_t s) {
-        if (bytes <= cap && p) return PS_OK;
-        if (p) (void)hipFreeAsync(p, s);
-        p = nullptr;
-        cap = 0;
-        const size_t want = std::max<size_t>(bytes + bytes / 4, 256);
-        hipError_t e = hipMallocAsync(&p, want, s);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            return ps::fail(PS_ERR_OUT_OF_MEMORY, "segment step buffer of %zu bytes: %s", want, hipGetErrorString(e));
-        }
-        cap = want;
-        return PS_OK;
-    }
-    void release(hipStream_t s) {
-        if (p) (void)hipFreeAsync(p, s);
-        p = nullptr;
-        cap = 0;
-    }
-    // a buffer handed to a kernel must have been sized by ensure() first
-    template <class T>
-    T* as() const {
-        if (!p) {
-            std::fprintf(stderr, "puzzlesearch_amd: segment step used a buffer before sizing it\n");
-            std::abort();
-        }
-        return reinterpret_cast<T*>(p);
-    }
-    // the buffer for reporting to the caller: NULL while it was never sized (nothing in it)
-    template <class T>
-    T* ptr_or_null() const {
-        return reinterpret_cast<T*>(p);
-    }
-};
+using ps::GrowBuf;
 
 // [0] in-segment children [1] count [2] written [3] cross children [4] children
 // outside the segments [5] words scanned [6] frontier ids outside the segment

@@ -1,0 +1,333 @@
+// Kernels of one disk-segment step of the three-bit BFS,
+// DB_Opt3BitBFS_Solver::Expand(segment) (Common/DiskBasedOptThreeBitBFS.cpp:54-144).
+//
+// HBM layout of one segment (2^segment_bits states, at least one word):
+//   next   u64 bit array: the reference's IndexedBitArray NextArray; bit i of
+//          word i/64 is state i of the segment
+//   cur    u64 bit array of the current frontier, only for puzzles with odd
+//          cycles (CurArray)
+//   dirty  u8 per region of 2^16 states: set by every mark in the region, so
+//          the scan passes skip untouched regions (the role of
+//          IndexedBitArray's occupancy index, Common/BitArray.h:205-270)
+// Both arrays are all zero between steps: the scan clears next, and cur is
+// cleared word by word from the current frontier list.
+#pragma once
+
+#include "bfs_kernels.h"
+
+namespace psk {
+
+constexpr int S3_REGION_BITS = 16;                               // states per dirty region
+constexpr uint32_t S3_REGION_WORDS = 1u << (S3_REGION_BITS - 6);  // u64 words per region (1024)
+constexpr uint32_t S3_CROSS_PER_BLOCK = 4096;                    // parents per block of the cross pass
+
+// counters of a step: [0] in-segment children, [1] ids outside the segment,
+// [2] the layer count, [3] words scanned, [4] cross-segment children, [5]
+// children outside [0, nseg), [6] cross children written
+constexpr int S3_NCOUNTERS = 8;
+
+struct S3MarkArgs {
+    psd::PuzzleParams prm;
+    const uint32_t* idx;  // current frontier, in-segment ids
+    uint64_t n;
+    uint64_t base;  // segment << seg_bits
+    uint32_t segment;
+    int seg_bits;
+    uint32_t* next;  // the next array as u32 words
+    uint8_t* dirty;
+    unsigned long long* cur;  // NULL without odd cycles
+    unsigned long long* counters;
+};
+
+struct S3ScanArgs {
+    unsigned long long* next;
+    const unsigned long long* cur;  // NULL without odd cycles
+    uint8_t* dirty;
+    uint64_t nwords;
+    unsigned long long* cnt_out;        // per region: states of the new layer
+    unsigned long long* counters;
+    const unsigned long long* out_off;  // per region exclusive offsets (write pass)
+    uint32_t* out_idx;
+};
+
+struct S3CrossArgs {
+    psd::PuzzleParams prm;
+    const uint32_t* idx;  // the new frontier
+    uint64_t n;
+    uint64_t base;
+    uint32_t segment;
+    uint32_t nseg;
+    int seg_bits;
+    unsigned long long* slots;  // per block: (label * nseg + seg) << 32 | in-segment id, compacted
+    unsigned long long* counters;
+    unsigned long long* block_count;  // per block: cross children written
+};
+
+// One OR per next word: consecutive children of a parent in the same 32-bit
+// word are merged before the atomic.
+struct S3MarkSink {
+    uint32_t* next;
+    uint8_t* dirty;
+    uint64_t segment, in_mask;
+    int seg_bits;
+    uint64_t pend_w;
+    uint32_t pend_m;
+    unsigned long long marks;
+    __device__ __forceinline__ void flush() {
+        if (!pend_m) return;
+        gor(&next[pend_w], pend_m);
+        dirty[pend_w >> (S3_REGION_BITS - 5)] = 1;  // a plain store: no read to wait for
+        pend_m = 0;
+    }
+    __device__ __forceinline__ void operator()(int, uint64_t child, uint32_t) {
+        if ((child >> seg_bits) != segment) return;
+        const uint64_t c = child & in_mask;
+        ++marks;
+        const uint64_t w = c >> 5;
+        const uint32_t m = 1u << (c & 31);
+        if (w == pend_w) {
+            pend_m |= m;
+            return;
+        }
+        flush();
+        pend_w = w;
+        pend_m = m;
+    }
+};
+
+// In-segment children of the current frontier set next (and the frontier
+// sets cur with odd cycles), Common/DiskBasedOptThreeBitBFS.cpp:78-99.
+template <class PZ, int TPB>
+__global__ __launch_bounds__(TPB) void k_s3_mark_in(S3MarkArgs a) {
+    const uint64_t in_mask = (1ull << a.seg_bits) - 1ull;
+    S3MarkSink sink{a.next, a.dirty, a.segment, in_mask, a.seg_bits, ~0ull, 0u, 0ull};
+    unsigned long long bad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * TPB) {
+        const uint32_t id = a.idx[i];
+        if ((uint64_t)id > in_mask) {  // not an id of this segment: reported after the step
+            ++bad;
+            continue;
+        }
+        if (a.cur) atomicOr(&a.cur[id >> 6], 1ull << (id & 63));
+        PZ::expand(a.base | id, 0u, a.prm, sink);
+        sink.flush();
+        sink.pend_w = ~0ull;
+    }
+    block_add3<TPB>(sink.marks, bad, 0ull, &a.counters[0], &a.counters[1], nullptr);
+}
+
+// incoming cross-segment marks: NextArray.Set(idx) (:64-76)
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_s3_mark_list(const uint32_t* __restrict__ idx, uint64_t n, uint64_t in_mask, uint32_t* next,
+                               uint8_t* dirty, unsigned long long* bad_count) {
+    unsigned long long bad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t id = idx[i];
+        if ((uint64_t)id > in_mask) {
+            ++bad;
+            continue;
+        }
+        gor(&next[id >> 5], 1u << (id & 31));
+        dirty[id >> S3_REGION_BITS] = 1;
+    }
+    bad = wave_sum(bad);
+    if (bad && threadIdx.x % 64 == 0) atomicAdd(bad_count, bad);
+}
+
+// the old frontier: NextArray.Clear(idx) (:101-111); ids outside the segment
+// are counted by the mark pass of the frontier that wrote them
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_s3_clear_list(const uint32_t* __restrict__ idx, uint64_t n, uint64_t in_mask, uint32_t* next) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t id = idx[i];
+        if ((uint64_t)id <= in_mask)
+            __hip_atomic_fetch_and(&next[id >> 5], ~(1u << (id & 31)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// CurArray.Clear(): the words holding the current frontier's bits
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_s3_clear_cur(const uint32_t* __restrict__ idx, uint64_t n, uint64_t in_mask,
+                               unsigned long long* cur) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t id = idx[i];
+        if ((uint64_t)id <= in_mask) cur[id >> 6] = 0ull;
+    }
+}
+
+// thread t of a region's block owns words 4t .. 4t+3 of the region
+__device__ __forceinline__ void s3_load4(const S3ScanArgs& a, uint64_t w0, uint64_t rw, uint64_t (&nx)[4],
+                                         uint64_t (&live)[4]) {
+    const uint64_t j0 = (uint64_t)threadIdx.x * 4;
+    if (rw == S3_REGION_WORDS) {
+        const ulonglong2* n2 = reinterpret_cast<const ulonglong2*>(a.next + w0 + j0);
+        const ulonglong2 x0 = n2[0], x1 = n2[1];
+        nx[0] = x0.x;
+        nx[1] = x0.y;
+        nx[2] = x1.x;
+        nx[3] = x1.y;
+        if (a.cur) {
+            const ulonglong2* c2 = reinterpret_cast<const ulonglong2*>(a.cur + w0 + j0);
+            const ulonglong2 y0 = c2[0], y1 = c2[1];
+            live[0] = nx[0] & ~y0.x;
+            live[1] = nx[1] & ~y0.y;
+            live[2] = nx[2] & ~y1.x;
+            live[3] = nx[3] & ~y1.y;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) live[e] = nx[e];
+        }
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint64_t j = j0 + e;
+        nx[e] = (j < rw) ? a.next[w0 + j] : 0ull;
+        live[e] = (a.cur && j < rw) ? nx[e] & ~a.cur[w0 + j] : nx[e];
+    }
+}
+
+// pass 1 of ScanBitsAndClearWithExcl: per region, the states of the new layer
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_s3_count(S3ScanArgs a) {
+    static_assert(TPB * 4 == S3_REGION_WORDS, "a region is one block load");
+    const uint64_t nregions = (a.nwords + S3_REGION_WORDS - 1) / S3_REGION_WORDS;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __shared__ uint32_t red[2][TPB / 64];
+    unsigned long long total = 0, scanned = 0;  // thread 0
+    uint32_t par = 0;
+    for (uint64_t r = blockIdx.x; r < nregions; r += gridDim.x, par ^= 1u) {
+        const uint64_t w0 = r * S3_REGION_WORDS;
+        const uint64_t rw = (a.nwords - w0 < S3_REGION_WORDS) ? a.nwords - w0 : S3_REGION_WORDS;
+        uint32_t c = 0;
+        const bool dirty = a.dirty[r] != 0;
+        if (dirty) {
+            uint64_t nx[4], live[4];
+            s3_load4(a, w0, rw, nx, live);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) c += (uint32_t)__popcll(live[e]);
+        }
+        c = wave_sum(c);
+        if (lane == 0) red[par][wave] = c;
+        __syncthreads();
+        // red[par] is written again two regions later, after the next barrier,
+        // which thread 0 reaches only once it has read it here
+        if (threadIdx.x == 0) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int w = 0; w < TPB / 64; ++w) s += red[par][w];
+            a.cnt_out[r] = s;
+            total += s;
+            if (dirty) scanned += rw;
+        }
+    }
+    block_add3<TPB>(threadIdx.x == 0 ? total : 0ull, threadIdx.x == 0 ? scanned : 0ull, 0ull, &a.counters[2],
+                    &a.counters[3], nullptr);
+}
+
+// pass 2: the new frontier's ids in index order, and the clear of next and of
+// the region's dirty byte
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_s3_write(S3ScanArgs a) {
+    constexpr int NW = TPB / 64;
+    const uint64_t nregions = (a.nwords + S3_REGION_WORDS - 1) / S3_REGION_WORDS;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    __shared__ uint32_t tot[2][NW];
+    uint32_t par = 0;
+    for (uint64_t r = blockIdx.x; r < nregions; r += gridDim.x) {
+        if (!a.dirty[r]) continue;  // the same value for the whole block: it is cleared after the barrier below
+        const uint64_t w0 = r * S3_REGION_WORDS;
+        const uint64_t rw = (a.nwords - w0 < S3_REGION_WORDS) ? a.nwords - w0 : S3_REGION_WORDS;
+        uint64_t nx[4], live[4];
+        s3_load4(a, w0, rw, nx, live);
+        uint32_t c = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c += (uint32_t)__popcll(live[e]);
+        const uint32_t incl = wave_incl_scan(c, lane);
+        if (lane == 63) tot[par][wave] = incl;
+        __syncthreads();
+        // tot[par] is written again two regions later, after the next
+        // region's barrier, which every thread reaches only once done here
+        uint32_t before = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) before += (w < wave) ? tot[par][w] : 0u;
+        uint64_t pos = a.out_off[r] + before + incl - c;
+        const uint64_t j0 = w0 + (uint64_t)threadIdx.x * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            uint64_t v = live[e];
+            while (v) {
+                const int b = __ffsll((unsigned long long)v) - 1;
+                v &= v - 1;
+                a.out_idx[pos++] = (uint32_t)((j0 + e) * 64 + (uint64_t)b);
+            }
+            if (nx[e]) a.next[j0 + e] = 0ull;
+        }
+        if (threadIdx.x == 0) a.dirty[r] = 0;  // every thread read it before the barrier above
+        par ^= 1u;
+    }
+}
+
+// children of the new frontier that leave the segment, in slot order
+struct S3CrossSink {
+    uint64_t segment, nseg, in_mask;
+    int seg_bits;
+    unsigned long long bad;
+    uint32_t valid;             // bit k: slot k holds a cross child
+    unsigned long long v[16];   // by slot; PZ::SLOTS <= 16
+    __device__ __forceinline__ void operator()(int slot, uint64_t child, uint32_t label) {
+        const uint64_t s = child >> seg_bits;
+        if (s == segment) return;
+        if (s >= nseg) {
+            ++bad;
+            return;
+        }
+        v[slot] = (((uint64_t)label * nseg + s) << 32) | (child & in_mask);
+        valid |= 1u << slot;
+    }
+};
+
+// Block b expands parents [b*4096, (b+1)*4096) with the cross-segment hint
+// and writes their cross children, compacted in (parent, slot) order, to the
+// front of its own area slots[b*4096*SLOTS ...]; block_count[b] is how many.
+template <class PZ, int TPB>
+__global__ __launch_bounds__(TPB) void k_s3_cross(S3CrossArgs a) {
+    static_assert(PZ::SLOTS <= 16, "cross sink holds 16 slots");
+    constexpr int NW = TPB / 64;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    S3CrossSink sink{a.segment, a.nseg, (1ull << a.seg_bits) - 1ull, a.seg_bits, 0ull, 0u, {}};
+    const uint64_t p0 = (uint64_t)blockIdx.x * S3_CROSS_PER_BLOCK;
+    const uint64_t p1 = (p0 + S3_CROSS_PER_BLOCK < a.n) ? p0 + S3_CROSS_PER_BLOCK : a.n;
+    unsigned long long* out = a.slots + p0 * PZ::SLOTS;
+    __shared__ uint32_t tot[2][NW];
+    uint32_t par = 0, running = 0;
+    unsigned long long nx = 0;
+    for (uint64_t k0 = p0; k0 < p1; k0 += TPB, par ^= 1u) {
+        const uint64_t i = k0 + threadIdx.x;
+        sink.valid = 0;
+        if (i < p1) PZ::expand(a.base | a.idx[i], 0u, a.prm, sink);
+        const uint32_t c = (uint32_t)__popc(sink.valid);
+        nx += c;
+        const uint32_t incl = wave_incl_scan(c, lane);
+        if (lane == 63) tot[par][wave] = incl;
+        __syncthreads();
+        // tot[par] is written again two iterations later, after the next
+        // iteration's barrier, which every thread reaches only once done here
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t t = tot[par][w];
+            before += (w < wave) ? t : 0u;
+            total += t;
+        }
+        uint32_t pos = running + before + incl - c;
+#pragma unroll
+        for (int kk = 0; kk < PZ::SLOTS; ++kk)
+            if (sink.valid & (1u << kk)) out[pos++] = sink.v[kk];
+        running += total;
+    }
+    block_add3<TPB>(nx, sink.bad, nx, &a.counters[4], &a.counters[5], &a.counters[6]);
+    if (threadIdx.x == 0) a.block_count[blockIdx.x] = running;
+}
+
+}  // namespace psk
