@@ -12,8 +12,15 @@
 #include <string>
 #include <vector>
 
+#include <cstdlib>
+#include <stdexcept>
+#include <unistd.h>
+
+#include "CompressedFrontier.h"
+#include "DiskBasedBFS.h"
 #include "InMemoryBFS.h"
 #include "Puzzles.h"
+#include "Store.h"
 
 namespace {
 
@@ -92,6 +99,52 @@ const char* PANCAKE9 = "1 8 56 391 2278 10666 38015 93585 132697 79379 5804";   
 const char* HANOI10_SYM =                                                              // PuzzleSearchTests.cpp:372
     "1 1 1 3 5 7 11 16 24 36 55 56 81 137 172 162 183 293 435 506 588 562 779 1056 1519 1780 1930 1983 2249 2871 "
     "3957 5367 6165 6638 6425 6654 7608 9511 12500 15255 17690 18320 15339 10630 7614 5496 2214 334 52 4";
+const char* ST5x2 =                                                                    // PuzzleSearchTests.cpp:22
+    "1 2 3 6 11 19 30 44 68 112 176 271 411 602 851 1232 1783 2530 3567 4996 6838 9279 12463 16597 21848 28227 "
+    "35682 44464 54597 65966 78433 91725 104896 116966 126335 131998 133107 128720 119332 106335 91545 75742 60119 "
+    "45840 33422 23223 15140 9094 5073 2605 1224 528 225 75 20 2";
+const char* PANCAKE13_6 = "1 12 132 1451 14556 130096 1030505";                          // PuzzleSearchTests.cpp:569
+const char* HANOI21_10 = "1 3 6 12 30 30 66 96 126 210 330";                             // PuzzleSearchTests.cpp:477
+const char* HANOI21_SYM_10 = "1 1 1 3 5 7 11 16 24 36 55";                               // PuzzleSearchTests.cpp:487
+
+// a fresh directory for the stores of one test
+std::string TempDir(const char* tag) {
+    std::string t = std::string("/tmp/ps_host_tests_") + tag + "_XXXXXX";
+    std::vector<char> buf(t.begin(), t.end());
+    buf.push_back('\0');
+    const char* d = mkdtemp(buf.data());
+    if (!d) throw std::runtime_error("mkdtemp failed");
+    return d;
+}
+
+void RemoveDir(const std::string& dir) { std::system(("rm -rf '" + dir + "'").c_str()); }
+
+// CommonTests/StoreTests.cpp:9-37: 300 chunks of 10 KiB round-robin over the
+// segments, read back three times after rewinds, nothing left after DeleteAll
+void CheckStore(Store& store) {
+    constexpr size_t SIZE = 10 * 1024;
+    constexpr int BUFS = 300;
+    const int segments = store.MaxSegments();
+    std::vector<uint8_t> buf(SIZE);
+    for (int i = 0; i < BUFS; i++) {
+        std::memset(buf.data(), i & 255, SIZE);
+        store.Write(i % segments, buf.data(), SIZE);
+    }
+    EXPECT_EQ(store.TotalLength(), (uint64_t)SIZE * BUFS);
+    for (int tries = 0; tries < 3; tries++) {
+        bool same = true;
+        for (int i = 0; i < BUFS; i++) {
+            const size_t read = store.Read(i % segments, buf.data(), SIZE);
+            EXPECT_EQ(read, SIZE);
+            for (size_t j = 0; j < SIZE; j++) same &= buf[j] == (uint8_t)(i & 255);
+        }
+        EXPECT_EQ(same, true);
+        store.RewindAll();
+    }
+    store.DeleteAll();
+    for (int i = 0; i < segments; i++) EXPECT_EQ(store.Read(i, buf.data(), SIZE), (size_t)0);
+    EXPECT_EQ(store.TotalLength(), 0ull);
+}
 
 // ------------------------------------------------------------- CPU only
 
@@ -149,7 +202,150 @@ TEST_CPU(HanoiParse) {
     EXPECT_EQ(FourPegHanoiGPU(16, false).IndexesCount(), 1ull << 32);
 }
 
+TEST_CPU(StoreLayouts) {
+    // CommonTests/StoreTests.cpp:39-80: every file layout of StoreOptions
+    const std::string dir = TempDir("store");
+    {
+        auto single = Store::CreateSingleFileStore(50, dir + "/file");
+        CheckStore(single);
+        auto multi = Store::CreateMultiFileStore(50, dir + "/dir", "store");
+        CheckStore(multi);
+        auto seq = Store::CreateSequentialStore(50, {dir + "/s1", dir + "/s2", dir + "/s3"});
+        CheckStore(seq);
+        for (int files : {0, 1, 5}) {
+            StoreOptions opts;
+            opts.directories = {dir + "/d1", dir + "/d2", dir + "/d3"};
+            opts.filesPerPath = files;
+            auto store = Store::CreateFileStore(50, "store" + std::to_string(files), opts);
+            CheckStore(store);
+        }
+        // chunks keep their boundaries; Delete drops one segment only
+        auto s = Store::CreateSingleFileStore(3, dir + "/chunks");
+        const uint8_t a[3] = {1, 2, 3}, b[5] = {4, 5, 6, 7, 8};
+        s.Write(1, a, 3);
+        s.Write(2, b, 5);
+        s.Write(1, b, 5);
+        uint8_t r[16];
+        EXPECT_EQ(s.Read(1, r, 16), (size_t)3);
+        EXPECT_EQ((int)r[2], 3);
+        EXPECT_EQ(s.Read(1, r, 16), (size_t)5);
+        EXPECT_EQ((int)r[4], 8);
+        EXPECT_EQ(s.Read(1, r, 16), (size_t)0);
+        s.Delete(1);
+        EXPECT_EQ(s.HasData(1), false);
+        EXPECT_EQ(s.Length(2), 5ull);
+        EXPECT_EQ(s.Read(2, r, 16), (size_t)5);
+        StoreSet set;
+        set.Stores.emplace_back(Store::CreateSingleFileStore(2, dir + "/x0"));
+        set.Stores.emplace_back(Store::CreateSingleFileStore(2, dir + "/x1"));
+        set.Stores[1].Write(0, a, 3);
+        StoreSet other;
+        other.Stores.emplace_back(Store::CreateSingleFileStore(2, dir + "/y0"));
+        other.Stores.emplace_back(Store::CreateSingleFileStore(2, dir + "/y1"));
+        set.Swap(other);
+        EXPECT_EQ(other.TotalLength(), 3ull);
+        EXPECT_EQ(set.TotalLength(), 0ull);
+    }
+    RemoveDir(dir);
+}
+
+TEST_CPU(FrontierStreamChunks) {
+    // chunk lengths of FrontierCompression streams (Common/FrontierCompression.cpp:117-125):
+    // a bitmap chunk of 2 words holding first=100 and 3 more bits, then a
+    // StreamVInt chunk of 5 values (8 after padding: two tuples)
+    std::vector<uint8_t> s;
+    auto put32 = [&](uint32_t v) {
+        for (int i = 0; i < 4; i++) s.push_back((uint8_t)(v >> (8 * i)));
+    };
+    put32(2u | 0x80000000u);
+    put32(100);
+    const uint64_t w0 = 0x5ull, w1 = 1ull << 63;
+    for (uint64_t w : {w0, w1})
+        for (int i = 0; i < 8; i++) s.push_back((uint8_t)(w >> (8 * i)));
+    put32(5);
+    s.push_back(0x1B);  // lengths 4, 3, 2, 1 bytes
+    for (int i = 0; i < 10; i++) s.push_back(0);
+    s.push_back(0x00);  // four 1-byte values
+    for (int i = 0; i < 4; i++) s.push_back(0);
+    std::vector<uint64_t> off;
+    EXPECT_EQ(fcstream::ChunkBoundaries(s.data(), s.size(), off), 4ull + 5ull);
+    EXPECT_EQ(off.size(), (size_t)3);
+    EXPECT_EQ(off[1], 24ull);
+    EXPECT_EQ(off[2], (uint64_t)s.size());
+    bool thrown = false;
+    try {
+        fcstream::ChunkBoundaries(s.data(), s.size() - 1, off);
+    } catch (const std::runtime_error&) {
+        thrown = true;
+    }
+    EXPECT_EQ(thrown, true);
+}
+
 // ------------------------------------------------------------------ GPU
+
+TEST_GPU(DiskBasedOptThreeBitBFS_Reference) {
+    // PuzzlesTests/PuzzleSearchTests.cpp:258-333, 461-490, 553-590: the disk
+    // driver over file Stores, with the segment steps on the device
+    const std::string dir = TempDir("disk");
+    auto run = [&](Puzzle& p, const std::string& init, int segBits, size_t maxSteps, int filesPerPath) {
+        PuzzleOptions o;
+        o.segmentBits = segBits;
+        o.maxSteps = maxSteps;
+        o.storeOptions.directories = {dir + "/a", dir + "/b"};
+        o.storeOptions.filesPerPath = filesPerPath;
+        return Join(DiskBasedOptThreeBitBFS(p, init, o));
+    };
+    SlidingTilePuzzleGpu s52(5, 2);
+    EXPECT_EQ(run(s52, "0 1 2 3 4 5 6 7 8 9", 22, 10000, 5), std::string(ST5x2));
+    EXPECT_EQ(run(s52, "0 1 2 3 4 5 6 7 8 9", 19, 10000, 0), std::string(ST5x2));
+    SlidingTilePuzzleOptimized<5, 2> o52;
+    EXPECT_EQ(run(o52, "0 1 2 3 4 5 6 7 8 9", 22, 10000, 1), std::string(ST5x2));
+    EXPECT_EQ(run(o52, "0 1 2 3 4 5 6 7 8 9", 19, 10000, 5), std::string(ST5x2));
+    PancakeOptimizedGPU p9(9, false);
+    EXPECT_EQ(run(p9, p9.ToString(0), 15, 10000, 5), std::string(PANCAKE9));
+    PancakeOptimizedGPU p5(5, false);
+    EXPECT_EQ(run(p5, p5.ToString(0), 32, 10000, 5), std::string(PANCAKE5));
+    FourPegHanoiGPU h10(10, true);
+    EXPECT_EQ(run(h10, h10.ToString(0), 20, 10000, 5), std::string(HANOI10_SYM));
+    FourPegHanoiOptimizedGPU ho10(10, true);
+    EXPECT_EQ(run(ho10, ho10.ToString(0), 14, 10000, 0), std::string(HANOI10_SYM));
+    for (bool inv : {false, true}) {
+        PancakeOptimizedGPU p13(13, inv);
+        EXPECT_EQ(run(p13, p13.ToString(0), 29, 6, 5), std::string(PANCAKE13_6));
+    }
+    FourPegHanoiGPU h21(21, false), h21s(21, true);
+    EXPECT_EQ(run(h21, h21.ToString(0), 32, 10, 5), std::string(HANOI21_10));
+    EXPECT_EQ(run(h21s, h21s.ToString(0), 32, 10, 5), std::string(HANOI21_SYM_10));
+    RemoveDir(dir);
+}
+
+TEST_GPU(CompressedSegmentReaderWriter) {
+    // CompressedSegmentWriter's chunks read back by CompressedSegmentReader
+    // chunk by chunk (Common/CompressedFrontier.cpp:90-158), across Store pieces
+    const std::string dir = TempDir("frontier");
+    {
+        auto store = Store::CreateSingleFileStore(4, dir + "/f");
+        CompressedSegmentWriter w(store);
+        w.SetSegment(2);
+        std::vector<uint32_t> ids;
+        for (uint32_t i = 0; i < 700000; i++) ids.push_back(i * 3 + (i % 7 == 0));  // ascending
+        for (uint32_t v : ids) w.Add(v);
+        w.Flush();
+        CompressedSegmentReader r(store);
+        r.SetSegment(2);
+        std::vector<uint32_t> back;
+        for (;;) {
+            auto& b = r.Read();
+            if (b.IsEmpty()) break;
+            EXPECT_EQ(b.Size() <= (size_t)(1u << 18), true);
+            back.insert(back.end(), b.Buf(), b.Buf() + b.Size());
+        }
+        EXPECT_EQ(back == ids, true);
+        r.SetSegment(1);
+        EXPECT_EQ(r.Read().IsEmpty(), true);
+    }
+    RemoveDir(dir);
+}
 
 TEST_GPU(OptPancake5GPU) {
     // PuzzlesTests/PancakeTest.cpp:176-197
