@@ -203,6 +203,23 @@ def segment_step_bench(psa, steps, ppm=200000, seed=1234):
             dev.append(res.device_ms)
             wall.append(res.wall_ms)
         r = res.as_dict()
+        parity = None
+        fx_path = os.path.join(ROOT, "tests", "golden", "segment_fixtures.json")
+        if os.path.exists(fx_path):
+            with open(fx_path) as f:
+                fx = json.load(f).get(f"fifteen_{ppm}")
+            if fx and fx["seed"] == seed:
+                # the timed step's outputs against digests of the oracle step on the same frontier
+                import hashlib
+
+                def sha(a):
+                    return hashlib.sha256(memoryview(np.ascontiguousarray(a)).cast("B")).hexdigest()
+                o = w.outputs(res)
+                got = (r["count"], sha(o["fr_idx"]), sha(o["fr_ops"]), sha(o["fr_bytes"]),
+                       [[a, s, n, len(b), sha(b)] for a, s, n, b in o["buckets"]])
+                want = (fx["count"], fx["fr_idx"], fx["fr_ops"], fx["fr_bytes"], fx["buckets"])
+                assert got == want, "segment step outputs differ from the oracle step"
+                parity = True
     ms = sum(wall) / len(wall)
     dms = sum(dev) / len(dev)
     return {"workload": f"15-puzzle segment {seg}, blank cell {blank}, frontier Bernoulli(p={ppm / 1e6}) "
@@ -213,7 +230,7 @@ def segment_step_bench(psa, steps, ppm=200000, seed=1234):
             "hbm_frac_device": r["algorithmic_bytes"] / (dms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "count": r["count"], "frontier_count": r["frontier_count"], "frontier_bytes": r["frontier_bytes"],
             "children_in": r["children_in"], "children_cross": r["children_cross"], "buckets": r["buckets"],
-            "cross_bytes": r["cross_bytes"]}
+            "cross_bytes": r["cross_bytes"], "parity_with_oracle_step": parity}
 
 
 TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256], "grid_blocks": [0, 256, 1024],
@@ -347,8 +364,10 @@ def codec_bench(psa, n_chunks=256, reps=5, seed=7):
     return out
 
 
-def batched_vs_plain(psa, name, p, start, steps, fixture_counts=None):
-    """Batched persistent (tuned) vs plain per-segment launches of one search."""
+def batched_vs_plain(psa, name, p, start, steps, fixture=None):
+    """Batched persistent (tuned) vs plain per-segment launches of one search;
+    both must equal each other and, where an oracle fixture exists, its layer
+    counts and frontier hashes."""
     res = tuned(psa, p, start)
     opts = dict(res["options"])
     with psa.Bfs(p, graph=1, **opts) as b:
@@ -362,8 +381,10 @@ def batched_vs_plain(psa, name, p, start, steps, fixture_counts=None):
     out = {"workload": name, "states": n_states, "layers": len(counts), "tuned_options": opts,
            "batched_ms": dt_b * 1e3, "plain_ms": dt_p * 1e3, "plain_expand_launches": st_p.expand_launches,
            "batched_states_per_s": n_states / dt_b, "batched_vs_plain": dt_p / dt_b}
-    if fixture_counts is not None:
-        out["counts_match_reference"] = counts == fixture_counts
+    if fixture is not None:
+        assert counts == fixture["counts"], f"{name}: layer counts differ from the oracle fixture"
+        assert [str(h) for h in hashes] == fixture["hashes"], f"{name}: frontier hashes differ from the oracle fixture"
+        out["parity_with_oracle_fixture"] = True
     log(f"{name}: batched {dt_b * 1e3:.2f} ms, plain {dt_p * 1e3:.2f} ms ({st_p.expand_launches} launches)")
     return out
 
@@ -488,6 +509,7 @@ def main():
         parity = counts == fixture["counts"] and [str(h) for h in hashes] == fixture["hashes"]
     log(f"batched: {dt_b * 1e3:.2f} ms/search, layers={len(counts)}, sum={sum(counts)}, parity={parity}")
     assert sum(counts) == N, "layer counts do not cover the state space"
+    assert parity is not False, "layer counts or frontier hashes differ from the oracle fixture"
 
     counts_e, hashes_e, st_e, exp_launch_ms, exp_bytes_launch = eager_roofline(2)
     assert counts_e == counts and hashes_e == hashes
@@ -532,9 +554,9 @@ def main():
     if not args.no_batched_set:
         batched_set = [
             batched_vs_plain(psa, "three-bit BFS of the 4x3 sliding-tile puzzle (12! indexes, 239500800 reachable)",
-                             psa.sliding(4, 3), 0, max(2, args.steps // 2)),
+                             psa.sliding(4, 3), 0, max(2, args.steps // 2), load_fixture("sliding_4x3")),
             batched_vs_plain(psa, "three-bit BFS of 4-peg Hanoi with 16 disks (4^16 states)",
-                             psa.hanoi(16), 0, 2),
+                             psa.hanoi(16), 0, 2, load_fixture("hanoi_16")),
         ]
 
     serving = None

@@ -163,8 +163,8 @@ def _fixtures():
 FIX = _fixtures()
 
 
-@pytest.mark.parametrize("name", [n for n in ("pancake_10", "pancake_11", "hanoi_12", "hanoi_12_sym",
-                                                "sliding_packed_5x2") if n in FIX])
+@pytest.mark.parametrize("name", [n for n in ("pancake_10", "pancake_11", "hanoi_12", "hanoi_12_sym", "hanoi_13",
+                                                "sliding_packed_5x2", "sliding_packed_4x3") if n in FIX])
 @pytest.mark.parametrize("mode", ["batched_graph", "plain_seg20"])
 def test_medium_against_oracle_fixture(ps, name, mode):
     rec = FIX[name]
@@ -267,8 +267,9 @@ def test_sliding_4x3_full(ps, golden, shape, cross):
 def test_hanoi16_full(ps):
     """4-peg Hanoi, 16 disks, 4^16 states: every state is reached once; the
     search with byte cross-window marks and the batched search with atomic
-    marks (2^20-state segments, the tuned shape) agree in every layer count
-    and frontier hash."""
+    marks (2^20-state segments, the tuned shape the benchmark times) agree in
+    every layer count and frontier hash, and equal the oracle's 162 layers
+    (tests/golden/make_oracle_fixtures.py hanoi_16)."""
     with psa.Bfs(psa.hanoi(16), segment_bits=24) as b:
         counts, hashes = b.run(0)
         st = b.stats()
@@ -277,6 +278,9 @@ def test_hanoi16_full(ps):
     assert counts[:11] == [1, 3, 6, 12, 30, 30, 66, 96, 126, 210, 330]  # shared prefix with the 21-disk vector
     with psa.Bfs(psa.hanoi(16), segment_bits=20, block_threads=256, cross_marks=psa.CROSS_ATOMIC, graph=1) as b:
         assert b.run(0) == (counts, hashes)
+    if "hanoi_16" in FIX:
+        assert counts == FIX["hanoi_16"]["counts"]
+        assert [str(h) for h in hashes] == FIX["hanoi_16"]["hashes"]
 
 
 @pytest.mark.parametrize("vid,disks", [("hanoi_17_120", 17), ("hanoi_18_120", 18), ("hanoi_19_120", 19)])

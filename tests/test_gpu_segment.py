@@ -80,6 +80,50 @@ def test_fifteen_puzzle_segment(ps, ppm):
     assert st["children_in"] == ref["children_in"] and st["children_cross"] == ref["children_cross"]
 
 
+def _segment_fixtures():
+    import json
+    import os
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "segment_fixtures.json")
+    if not os.path.exists(p):
+        return {}
+    with open(p) as f:
+        return json.load(f)
+
+
+SEG_FIX = _segment_fixtures()
+
+
+@pytest.mark.parametrize("ppm", [10000, 50000, 200000])
+def test_fifteen_puzzle_segment_full_size(ps, ppm):
+    """The step the benchmark times -- seed 1234, every id of the segment's
+    blank cell (stride 1) -- against digests of the oracle step's outputs on
+    the same frontier (tests/golden/make_segment_fixtures.py): count, new
+    frontier ids, op bits and stream, and every bucket's compressed bytes."""
+    import hashlib
+    from puzzlesearch_amd.segment import FsSegment
+    key = f"fifteen_{ppm}"
+    if key not in SEG_FIX:
+        pytest.skip("segment fixture not generated")
+    rec = SEG_FIX[key]
+
+    def sha(a):
+        return hashlib.sha256(memoryview(np.ascontiguousarray(a)).cast("B")).hexdigest()
+
+    op = O.sliding_packed(4, 4)
+    seg, idx, ops = fifteen(ppm, 1, seed=rec["seed"])
+    assert (seg, len(idx)) == (rec["segment"], rec["parents"])
+    with FsSegment(_ps_puzzle(ps, op), 32) as w:
+        dev = w.step(seg, idx, ops)
+    assert dev["count"] == rec["count"]
+    assert len(dev["fr_idx"]) == rec["frontier"]
+    assert sha(dev["fr_idx"]) == rec["fr_idx"]
+    assert sha(dev["fr_ops"]) == rec["fr_ops"]
+    assert len(dev["fr_bytes"]) == rec["fr_nbytes"] and sha(dev["fr_bytes"]) == rec["fr_bytes"]
+    assert dev["stats"]["children_in"] == rec["children_in"]
+    assert dev["stats"]["children_cross"] == rec["children_cross"]
+    assert [[o, s, n, len(b), sha(b)] for o, s, n, b in dev["buckets"]] == rec["buckets"]
+
+
 def test_fifteen_puzzle_every_blank_cell(ps):
     # parents with the blank in every cell, so each of the cross pass's slot
     # skips (horizontal moves, vertical moves left of list position 12) is
