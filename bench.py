@@ -389,7 +389,7 @@ def batched_vs_plain(psa, name, p, start, steps, fixture=None):
     return out
 
 
-def serving_bench(psa, rates=(5.0, 20.0, 50.0), n_jobs=30, seed=2024):
+def serving_bench(psa, rates=(5.0, 20.0, 50.0), n_jobs=200, seed=2024):
     """Poisson streams of complete-BFS requests over the 16-kind serving mix;
     every result is checked against the same search run alone."""
     from puzzlesearch_amd import serve as S

@@ -317,15 +317,20 @@ int ps_seg3_buckets(const ps_seg3* w, uint32_t* op, uint32_t* segment, uint64_t*
  * of the puzzles compiled into the multi-search kernels (sliding tile with 8,
  * 9, 10 or 12 cells, pancake 8..12, 4-peg Hanoi up to 16 disks) are queued by
  * ps_batch_submit.  ps_batch_step admits queued jobs first come first served
- * while their HBM footprint fits the budget, advances every running job by one
- * layer, and retires finished jobs, releasing their memory.  Not thread-safe:
- * one caller drives a batch. */
+ * while their HBM footprint fits the budget, advances the running jobs, and
+ * retires finished jobs.  A finished job's device search is kept for the next
+ * job of the same puzzle and index space and released only when a queued job
+ * needs its memory (releasing device memory waits for the whole device).  Not
+ * thread-safe: one caller drives a batch. */
 typedef struct ps_batch ps_batch;
 
 enum { PS_JOB_QUEUED = 0, PS_JOB_RUNNING = 1, PS_JOB_DONE = 2, PS_JOB_FAILED = 3 };
 
-/* PS_BATCH_STREAMS: every job runs on its own stream and one layer of every
- * running job is enqueued before any is waited for, so their kernels overlap.
+/* PS_BATCH_STREAMS: every job runs on its own stream with its own completion
+ * event.  A step enqueues the next layers of each job that has none in flight
+ * (4 at first, then up to 32, replayed from the job's captured layer graph)
+ * and returns once at least one job's layers have completed, so a job is never
+ * held back by a larger one and the jobs' kernels overlap.
  * PS_BATCH_FUSED: all running jobs share one persistent expand, one finalize
  * and one bookkeeping launch per step; currently disabled (fails with
  * PS_ERR_UNSUPPORTED). */
@@ -336,7 +341,14 @@ typedef struct ps_batch_options {
     int32_t segment_bits;  /* segments of jobs larger than one segment (default 22; 0: none) */
     uint64_t hbm_budget;   /* bytes running jobs may hold (0: half of the free HBM at creation) */
     int32_t mode;          /* PS_BATCH_STREAMS (default) or PS_BATCH_FUSED */
-    int32_t reserved;
+    int32_t job_grid_blocks; /* PS_BATCH_STREAMS: most blocks of a job's persistent kernels, so that a
+                              * large job leaves room on the device for the others (0: no cap) */
+    int32_t reserved_cus;    /* PS_BATCH_STREAMS: CUs 0..reserved_cus-1 are kept free of jobs of at
+                              * least 2^25 states (their streams are CU-masked), so small jobs never
+                              * wait for a large job's persistent kernel (0: none) */
+    int32_t shortest_first;  /* PS_BATCH_STREAMS: a job whose unvisited states exceed this many times
+                              * those of the smallest running job gets no new layers, for at most
+                              * 20 ms at a time (default 4; 0: every job advances) */
 } ps_batch_options;
 
 typedef struct ps_batch_job_info {
@@ -354,7 +366,7 @@ typedef struct ps_batch_stats {
     uint64_t jobs_done;
     uint64_t states_done;  /* sum of the layer counts of finished jobs */
     uint64_t expanded;
-    uint64_t hbm_used, hbm_budget;
+    uint64_t hbm_used, hbm_budget; /* used: running jobs and the finished jobs' searches kept for reuse */
     double step_ms;        /* host time spent in steps that ran jobs */
 } ps_batch_stats;
 

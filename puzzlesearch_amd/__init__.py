@@ -132,7 +132,7 @@ class Seg3Result(ctypes.Structure):
 class BatchOptions(ctypes.Structure):
     """ps_batch_options"""
     _fields_ = [("max_jobs", _i32), ("segment_bits", _i32), ("hbm_budget", _u64), ("mode", _i32),
-                ("reserved", _i32)]
+                ("job_grid_blocks", _i32), ("reserved_cus", _i32), ("shortest_first", _i32)]
 
 
 class BatchJobInfo(ctypes.Structure):
@@ -435,10 +435,17 @@ class Batch:
     """Multi-search engine (ps_batch_*): continuous batching of complete
     three-bit BFS jobs of different puzzles."""
 
-    def __init__(self, max_jobs=64, segment_bits=22, hbm_budget=0, stream=None, mode=0):
+    def __init__(self, max_jobs=64, segment_bits=22, hbm_budget=0, stream=None, mode=0, job_grid_blocks=None,
+                 reserved_cus=None, shortest_first=None):
         o = BatchOptions()
         lib().ps_batch_options_default(ctypes.byref(o))
         o.max_jobs, o.segment_bits, o.hbm_budget, o.mode = max_jobs, segment_bits, hbm_budget, mode
+        if job_grid_blocks is not None:
+            o.job_grid_blocks = job_grid_blocks
+        if reserved_cus is not None:
+            o.reserved_cus = reserved_cus
+        if shortest_first is not None:
+            o.shortest_first = shortest_first
         h = _vp()
         check(lib().ps_batch_create(ctypes.byref(o), stream, ctypes.byref(h)))
         self.h = h.value

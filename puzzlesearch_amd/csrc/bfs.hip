@@ -42,6 +42,33 @@ uint64_t host_word_hash(uint64_t j, uint64_t w) { return w ? host_splitmix64(hos
 
 constexpr int MAX_LAYERS_CAP = ps::BFS_MAX_LAYERS;
 
+// The start of a search: the start state's word of the first array, its
+// segment's count, layer 0's count and hash, the first tile list entry and the
+// device state, written in order on the search's own stream (kernel arguments,
+// so no host buffer has to outlive the launch).
+struct ResetStart {
+    uint64_t* word_ptr;
+    uint64_t word;
+    unsigned long long* seg_count;  // NULL: none
+    unsigned long long* layer_counts;
+    unsigned long long* layer_hashes;
+    unsigned long long h0;
+    uint32_t* tile_list;  // NULL: none
+    uint32_t tile;
+    psk::BfsDevState* st;
+    psk::BfsDevState st_value;
+};
+
+__global__ void k_bfs_reset_start(ResetStart r) {
+    if (threadIdx.x != 0) return;
+    *r.word_ptr = r.word;
+    if (r.seg_count) *r.seg_count = 1ull;
+    *r.layer_counts = 1ull;
+    *r.layer_hashes = r.h0;
+    if (r.tile_list) *r.tile_list = r.tile;
+    *r.st = r.st_value;
+}
+
 }  // namespace
 
 
@@ -513,13 +540,34 @@ uint64_t footprint_bytes(const ps_puzzle* p, const ps_bfs_options* o, const Geom
 
 }  // namespace
 
+int enqueue_layers(ps_bfs* b, int n) {
+    const bool use_graph = b->opt.mode == PS_BFS_BATCHED && b->opt.graph;
+    b->collect_kernel_times = false;
+    if (b->fused && !b->primed)
+        if (int r = prime_fused(b)) return r;
+    if (!use_graph) {
+        for (int i = 0; i < n; ++i)
+            if (int r = launch_batched_layer(b, false)) return r;
+        return PS_OK;
+    }
+    if (int r = build_graph(b)) return r;
+    for (int i = 0; i < n; i += b->graph_layers) {
+        PS_HIP_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
+        b->stats.expand_launches += b->graph_layers;
+        b->stats.finalize_launches += b->bits == 3 && !b->fused ? b->graph_layers : 0;
+        b->stats.apply_launches += b->lists && !b->fused ? b->graph_layers : 0;
+    }
+    return PS_OK;
+}
+
 }  // namespace ps
 
 using namespace ps;
 
 // Reads the device state into the stats (call after the stream is idle).
 int ps::collect_stats(ps_bfs* b) {
-    PS_HIP_CHECK(hipMemcpy(b->h_st, b->st, sizeof(BfsDevState), hipMemcpyDeviceToHost));
+    PS_HIP_CHECK(hipMemcpyAsync(b->h_st, b->st, sizeof(BfsDevState), hipMemcpyDeviceToHost, b->stream));
+    PS_HIP_CHECK(hipStreamSynchronize(b->stream));
     const BfsDevState& st = *b->h_st;
     if (st.error & 1u) return fail(PS_ERR_INDEX_RANGE, "a child index exceeded the index space (%llu states)",
                                    (unsigned long long)b->nbits);
@@ -753,17 +801,17 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
                 LaunchCfg cs{b->tpb_expand, 0, b->lds_step, b->stream};
                 if (int r = pkl_step(*p, psk::StepArgs{}, cs, &occ_s)) return cleanup(r);
                 if (occ_s < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "list step kernel does not fit (LDS %zu bytes)", b->lds_step));
-                b->grid_step = (uint32_t)std::max<int64_t>(1, opt.grid_blocks > 0 ? opt.grid_blocks
-                                                                                : std::min<int64_t>((int64_t)occ_s * cus, (int64_t)b->xtiles));
+                const int64_t want_s = opt.grid_blocks > 0 ? (int64_t)opt.grid_blocks : (int64_t)occ_s * cus;
+                b->grid_step = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(want_s, (int64_t)b->xtiles));
             }
         } else {
             LaunchCfg c{b->tpb_expand, 0, b->lds_expand, b->stream};
             if (int r = bfs3_expand(*p, psk::ExpandArgs{}, c, &occ)) return cleanup(r);
         }
         if (occ < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "expand kernel does not fit (LDS %zu bytes)", b->lds_expand));
-        const int64_t want = (int64_t)occ * cus;
+        const int64_t want = opt.grid_blocks > 0 ? (int64_t)opt.grid_blocks : (int64_t)occ * cus;
         const int64_t tiles_live = b->lists ? (int64_t)b->xtiles : (int64_t)b->ntiles;
-        b->grid_expand = (uint32_t)std::max<int64_t>(1, opt.grid_blocks > 0 ? opt.grid_blocks : std::min<int64_t>(want, tiles_live));
+        b->grid_expand = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(want, tiles_live));
         b->chunks_per_seg = (b->seg_words + psk::FIN_CHUNK_WORDS - 1) / psk::FIN_CHUNK_WORDS;
         b->nchunks = (uint64_t)b->nseg * b->chunks_per_seg;
         b->grid_fin = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * 8, b->nchunks));
@@ -807,7 +855,6 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     b->primed = false;
     if (int r = launch_fill(b->seg_count, 0, (uint64_t)3 * b->nseg * 8, b->stream)) return r;
     if (int r = launch_fill(b->touched, 0, (uint64_t)((b->nseg + 31) / 32) * 4, b->stream)) return r;
-    PS_HIP_CHECK(hipStreamSynchronize(b->stream));
     // start state, layer 0
     uint64_t word_idx, word;
     if (b->bits == 2) {
@@ -817,19 +864,17 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
         word_idx = start / 64;
         word = 1ull << (start % 64);
     }
-    PS_HIP_CHECK(hipMemcpy(b->arr[0] + word_idx, &word, 8, hipMemcpyHostToDevice));
-    if (b->bits == 3) {
-        const uint64_t seg = b->nseg > 1 ? (start >> b->seg_shift) : 0;
-        const unsigned long long one = 1;
-        PS_HIP_CHECK(hipMemcpy(b->seg_count + seg, &one, 8, hipMemcpyHostToDevice));
-    }
-    const unsigned long long c0 = 1, h0 = host_word_hash(word_idx, word);
-    PS_HIP_CHECK(hipMemcpy(b->layer_counts, &c0, 8, hipMemcpyHostToDevice));
-    PS_HIP_CHECK(hipMemcpy(b->layer_hashes, &h0, 8, hipMemcpyHostToDevice));
+    ResetStart rs{};
+    rs.word_ptr = b->arr[0] + word_idx;
+    rs.word = word;
+    if (b->bits == 3) rs.seg_count = b->seg_count + (b->nseg > 1 ? (start >> b->seg_shift) : 0);
+    rs.layer_counts = b->layer_counts;
+    rs.layer_hashes = b->layer_hashes;
+    rs.h0 = host_word_hash(word_idx, word);
     BfsDevState st{};
     if (b->tile_list) {  // layer 0 visits the start state's tile
-        const uint32_t t0 = (uint32_t)(start / (b->tile_words * 64));
-        PS_HIP_CHECK(hipMemcpy(b->tile_list, &t0, 4, hipMemcpyHostToDevice));
+        rs.tile_list = b->tile_list;
+        rs.tile = (uint32_t)(start / (b->tile_words * 64));
         st.active_tiles = 1;
     }
     st.max_layers = (uint32_t)b->max_layers;
@@ -838,7 +883,11 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     st.direction = b->pull_capable ? (uint32_t)b->opt.direction : 0u;
     st.pull = st.direction == PS_DIR_BOTTOM_UP;
     st.done = b->max_layers <= 1 ? 1 : 0;
-    PS_HIP_CHECK(hipMemcpy(b->st, &st, sizeof(st), hipMemcpyHostToDevice));
+    rs.st = b->st;
+    rs.st_value = st;
+    hipLaunchKernelGGL(k_bfs_reset_start, dim3(1), dim3(64), 0, b->stream, rs);
+    PS_HIP_CHECK(hipGetLastError());
+    PS_HIP_CHECK(hipStreamSynchronize(b->stream));
     *b->h_st = st;
     b->stats = ps_bfs_stats{};
     b->ev_used = 0;

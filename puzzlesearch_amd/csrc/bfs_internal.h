@@ -101,5 +101,9 @@ int prime_fused(ps_bfs* b);
 int collect_stats(ps_bfs* b);
 // enqueues one complete layer (expand, finalize, bookkeeping) on b->stream
 int launch_batched_layer(ps_bfs* b, bool timed);
+// enqueues at least n more layers of a batched search on b->stream, through
+// its captured layer graph when it has one (whole replays of graph_layers
+// layers); layers after the last one are no-ops on the device
+int enqueue_layers(ps_bfs* b, int n);
 
 }  // namespace ps

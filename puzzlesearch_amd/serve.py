@@ -47,9 +47,11 @@ def draw(n_jobs, rate, seed):
     return [(float(t), MIX[int(k)]) for t, k in zip(np.cumsum(gaps), kinds)]
 
 
-def serve(rate, n_jobs, seed, max_jobs=64, segment_bits=22, hbm_budget=0, requests=None):
+def serve(rate, n_jobs, seed, max_jobs=64, segment_bits=22, hbm_budget=0, requests=None, job_grid_blocks=None,
+          reserved_cus=None, shortest_first=None):
     reqs = requests if requests is not None else draw(n_jobs, rate, seed)
-    with Batch(max_jobs=max_jobs, segment_bits=segment_bits, hbm_budget=hbm_budget) as b:
+    with Batch(max_jobs=max_jobs, segment_bits=segment_bits, hbm_budget=hbm_budget,
+               job_grid_blocks=job_grid_blocks, reserved_cus=reserved_cus, shortest_first=shortest_first) as b:
         t0 = time.perf_counter()
         pending = list(enumerate(reqs))
         arrival, ids = {}, {}

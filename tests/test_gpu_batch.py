@@ -35,8 +35,11 @@ def alone(ps, spec):
     return _alone[spec]
 
 
-def test_mix_equals_alone_and_oracle(ps):
-    with ps.Batch(max_jobs=16) as b:
+@pytest.mark.parametrize("policy", [{}, {"shortest_first": 0, "reserved_cus": 32}, {"shortest_first": 1},
+                                    {"job_grid_blocks": 64}])
+def test_mix_equals_alone_and_oracle(ps, policy):
+    # the scheduling options change only when each job's layers run
+    with ps.Batch(max_jobs=16, **policy) as b:
         ids = {}
         for spec in SMALL_MIX + SMALL_MIX[:3]:  # repeats run side by side too
             p, op, n = _puzzles(ps, *spec)
@@ -45,11 +48,11 @@ def test_mix_equals_alone_and_oracle(ps):
         while True:
             running, finished, queued = b.step()
             steps += 1
-            if steps % 50 == 0:
+            if steps % 2000 == 0:
                 print(f"step {steps}: running {running}, queued {queued}", flush=True)
             if running == 0 and queued == 0:
                 break
-            assert steps < 10000
+            assert steps < 1000000
         for jid, spec in ids.items():
             r = b.job(jid)
             assert r["state"] == ps.JOB_DONE, r
@@ -58,6 +61,42 @@ def test_mix_equals_alone_and_oracle(ps):
             p, op, n = _puzzles(ps, *spec)
             oc, oh = O.bfs3(op, 0, index_count=n)
             assert (r["counts"], r["hashes"]) == (oc, oh), spec
+
+
+def _oracle_fixture(name):
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_fixtures.json")
+    with open(path) as f:
+        return json.load(f)[name]
+
+
+def test_large_jobs_on_masked_streams(ps):
+    # pancake 11 and Hanoi 13 have at least 2^25 states: with reserved CUs they
+    # run on CU-masked streams, held back while the small jobs run
+    specs = {("pancake", 11, 0): "pancake_11", ("hanoi", 13, 0): "hanoi_13"}
+    small = [("sliding", 3, 3), ("pancake", 9, 0), ("hanoi", 10, 0)]
+    with ps.Batch(max_jobs=8, reserved_cus=32, shortest_first=4) as b:
+        ids = {}
+        for spec in list(specs) + small + list(specs):
+            p, op, n = _puzzles(ps, *spec)
+            ids[b.submit(p, 0, n)] = spec
+        steps = 0
+        while True:
+            running, finished, queued = b.step()
+            steps += 1
+            if running == 0 and queued == 0:
+                break
+            assert steps < 1000000
+        for jid, spec in ids.items():
+            r = b.job(jid)
+            assert r["state"] == ps.JOB_DONE, r
+            if spec in specs:
+                fx = _oracle_fixture(specs[spec])
+                assert r["counts"] == fx["counts"], spec
+                assert [str(h) for h in r["hashes"]] == fx["hashes"], spec
+            else:
+                assert (r["counts"], r["hashes"]) == alone(ps, spec), spec
 
 
 def test_admission_respects_budget(ps):
@@ -90,6 +129,9 @@ def test_fused_mode_is_refused(ps):
 
 
 def test_rejections(ps):
+    for bad in ({"shortest_first": -1}, {"reserved_cus": -1}, {"reserved_cus": 1 << 20}, {"job_grid_blocks": -5}):
+        with pytest.raises(ps.PSError):
+            ps.Batch(**bad)
     with ps.Batch(max_jobs=4, hbm_budget=1 << 20) as b:
         with pytest.raises(ps.PSError):
             b.submit(ps.pancake(13), 0)          # not in the multi-search table
