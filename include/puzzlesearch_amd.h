@@ -310,6 +310,58 @@ int ps_seg3_frontier(const ps_seg3* w, const uint32_t** d_idx, const uint8_t** d
 int ps_seg3_buckets(const ps_seg3* w, uint32_t* op, uint32_t* segment, uint64_t* values, uint64_t* value_offsets,
                     uint64_t* byte_offsets, uint64_t cap, const uint32_t** d_values, const uint8_t** d_bytes);
 
+/* ------------------------------------------- HanoiTowers vertical solver */
+/* 4-peg Hanoi states reduced by the symmetry of pegs 1..3 (pegs 1..3 ordered
+ * by bottom disk, largest first, empty pegs last); an index holds disk d's peg
+ * in bits 2d..2d+1. */
+
+/* kernel_expand_inseg / kernel_expand_inseg_without_smallest launched by
+ * GpuExpandInSegment<size> / GpuExpandInSegmentWithoutSmallest<size>
+ * (HanoiTowers/HanoiExpandGPU.cu:98-229, :231-275): for each id of the 2^32-state
+ * segment, d_children[i*6 + k] (or i*3 + k without the smallest disk's moves)
+ * is the in-segment id of the child of move k, moves (0,1) (0,2) (0,3) (1,2)
+ * (1,3) (2,3) (without: the three pairs of the pegs other than the smallest
+ * disk's), or the parent's own id where the child leaves the segment.
+ * disks 1..24; the segment must lie in the index space.  Enqueued on s. */
+int ps_hanoi_expand_in_segment(int32_t disks, uint32_t segment, const uint32_t* d_idx, uint64_t count,
+                               uint32_t* d_children, int32_t without_smallest, ps_stream s);
+
+/* HanoiTowers::Expand(indexes, children) (HanoiTowers/HanoiTowers.cpp:152-172,
+ * :322-327): d_children[i*6 + k] is the reduced child of d_idx[i] by move k,
+ * moves (0,1) (0,2) (0,3) (1,2) (1,3) (2,3); a move with no disk to move gives
+ * the parent itself.  Indexes must lie in the disks' index space. */
+int ps_hanoi_expand(int32_t disks, const uint64_t* d_idx, uint64_t count, uint64_t* d_children, ps_stream s);
+
+/* HanoiSearch<size> (HanoiTowers/HanoiSearch.cpp:46-290): the three-bit BFS of
+ * the reduced states over segments of 2^segment_bits states (even, 8..32; 0 =
+ * 32 as in the reference), with only the segments a layer reaches held in HBM
+ * (three bit arrays of 2^segment_bits bits each) instead of disk stores. */
+typedef struct ps_hanoi_search ps_hanoi_search;
+typedef struct ps_hanoi_search_stats {
+    uint64_t layers;
+    uint64_t segments_allocated;  /* segments held in HBM */
+    uint64_t expanded;            /* reduced parents expanded */
+    uint64_t children;            /* children generated (the four-bit shortcut counts 3) */
+    uint64_t listed_children;     /* children outside their parent's 2^18-state tile */
+    uint64_t algorithmic_bytes;
+    uint64_t hbm_bytes;
+    double device_ms, wall_ms;
+} ps_hanoi_search_stats;
+/* hbm_budget 0: three quarters of the free HBM at creation */
+int ps_hanoi_search_create(int32_t disks, int32_t segment_bits, uint64_t hbm_budget, ps_stream s,
+                           ps_hanoi_search** out);
+int ps_hanoi_search_destroy(ps_hanoi_search* h);
+/* Runs from `start` (a reduced state) for at most max_layers layers.
+ * restored[l] is the reference's result (layer counts restored to the
+ * unreduced puzzle with PegsMult, HanoiTowers/HanoiTowers.h:48-55; layer 0 is
+ * 1), reduced[l] the reduced states, hashes[l] (may be NULL) the hash of layer
+ * l's reduced frontier as a bit array over the whole index space (sum over
+ * nonzero words w_j of splitmix64(splitmix64(j) ^ w_j)).  Fails with
+ * PS_ERR_CAPACITY when a reached segment does not fit the budget. */
+int ps_hanoi_search_run(ps_hanoi_search* h, uint64_t start, int32_t max_layers, uint64_t* restored,
+                        uint64_t* reduced, uint64_t* hashes, int32_t cap, int32_t* n_layers);
+int ps_hanoi_search_stats_get(const ps_hanoi_search* h, ps_hanoi_search_stats* st);
+
 /* --------------------------------------------------- multi-search engine */
 /* Continuous batching of complete three-bit BFS jobs (InMemoryThreeBitBFS,
  * Common/InMemoryThreeBitBFS.cpp:4-48, for many puzzles at once).  No

@@ -95,6 +95,36 @@ void or_split_index_ops(int ops, const uint64_t* packed, uint64_t count, uint64_
 /* PancakeOptimized::CrossSegmentPostProcess, GPU form (Puzzles/PancakeGpu.cu:179-193). */
 void or_pancake_xseg_postprocess(int n, int segment, uint32_t* idx, uint64_t count);
 
+/* ---- HanoiTowers vertical solver (HanoiTowers/) ----
+ * States are 4-peg Hanoi indexes (disk d's peg in bits 2d..2d+1) reduced by
+ * the symmetry of pegs 1..3: pegs 1..3 ordered by bottom disk, largest first. */
+/* HanoiTowers::PegsMult (HanoiTowers/HanoiTowers.h:48-55): the unreduced states a
+ * reduced one stands for. */
+int or_hv_pegs_mult(uint64_t index);
+/* HanoiTowers::InSegmentOnly (:37-46) over the low seg_bits bits (32 in the
+ * reference): no disk above the segment's disks can move. */
+int or_hv_in_segment_only(uint64_t index, int seg_bits);
+/* HanoiTowers::Expand(index) (HanoiTowers/HanoiTowers.cpp:152-172): the six
+ * moves (0,1) (0,2) (0,3) (1,2) (1,3) (2,3), symmetry restored where a peg is
+ * or becomes empty. */
+void or_hv_expand(int disks, uint64_t index, uint64_t* out6);
+/* HanoiTowers::ExpandCrossSegment (:174-198): the children outside the segment
+ * of 2^seg_bits states, in move order; returns their number. */
+int or_hv_expand_cross_segment(int disks, int seg_bits, uint64_t segment, uint32_t index, uint64_t* out);
+/* kernel_expand_inseg / kernel_expand_inseg_without_smallest
+ * (HanoiTowers/HanoiExpandGPU.cu:98-229), 2^32-state segments: 6 (or 3) slots
+ * per parent, the parent's own id where the child leaves the segment. */
+void or_hv_expand_in_segment_slots(int disks, uint32_t segment, const uint32_t* idx, uint64_t count,
+                                   int without_smallest, uint32_t* out);
+/* HanoiSearch (HanoiTowers/HanoiSearch.cpp:46-290) with segments of 2^seg_bits
+ * states (even, 6..32; 32 in the reference) and in-memory stores.  restored[l]
+ * is the reference's result (PegsMult-restored counts, layer 0 = 1), reduced[l]
+ * the reduced states, hashes[l] (may be NULL) the hash of layer l's reduced
+ * frontier as a bit array over the whole index space.  Returns the number of
+ * layers, at most max_layers, or -1. */
+int or_hv_search(int disks, uint64_t start, int seg_bits, int max_layers, uint64_t* restored, uint64_t* reduced,
+                 uint64_t* hashes);
+
 /* ---- bit arrays (Common/BitArray.h:69-169) ---- */
 uint64_t or_bits_andnot_count(uint64_t* a, const uint64_t* excl, uint64_t nwords);
 void or_bits_andnot(uint64_t* a, const uint64_t* excl, uint64_t nwords);

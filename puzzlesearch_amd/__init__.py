@@ -129,6 +129,16 @@ class Seg3Result(ctypes.Structure):
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
+class HanoiSearchStats(ctypes.Structure):
+    """ps_hanoi_search_stats"""
+    _fields_ = [(n, _u64) for n in ("layers", "segments_allocated", "expanded", "children", "listed_children",
+                                    "algorithmic_bytes", "hbm_bytes")] + \
+        [("device_ms", ctypes.c_double), ("wall_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class BatchOptions(ctypes.Structure):
     """ps_batch_options"""
     _fields_ = [("max_jobs", _i32), ("segment_bits", _i32), ("hbm_budget", _u64), ("mode", _i32),
@@ -174,6 +184,12 @@ _SIGS = {
     "ps_copy_async": (_i32, [_vp, _vp, _u64, _vp]),
     "ps_expand": (_i32, [_P, _i32, _vp, _vp, _u64, _vp]),
     "ps_pancake_xseg_postprocess": (_i32, [_i32, _i32, _vp, _u64, _vp]),
+    "ps_hanoi_expand_in_segment": (_i32, [_i32, ctypes.c_uint32, _vp, _u64, _vp, _i32, _vp]),
+    "ps_hanoi_expand": (_i32, [_i32, _vp, _u64, _vp, _vp]),
+    "ps_hanoi_search_create": (_i32, [_i32, _i32, _u64, _vp, ctypes.POINTER(_vp)]),
+    "ps_hanoi_search_destroy": (_i32, [_vp]),
+    "ps_hanoi_search_run": (_i32, [_vp, _u64, _i32, _vp, _vp, _vp, _i32, ctypes.POINTER(_i32)]),
+    "ps_hanoi_search_stats_get": (_i32, [_vp, _vp]),
     "ps_bits_andnot_count": (_i32, [_vp, _vp, _vp, _u64, _vp, _vp]),
     "ps_bits_count": (_i32, [_vp, _u64, _vp, _vp]),
     "ps_bits_hash": (_i32, [_vp, _u64, _u64, _vp, _vp]),
@@ -503,5 +519,77 @@ class Batch:
         try:
             if self.h and _lib is not None:
                 _lib.ps_batch_destroy(self.h)
+        except Exception:
+            pass
+
+
+def hanoi_expand_in_segment(disks, segment, idx, without_smallest=False):
+    """GpuExpandInSegment[WithoutSmallest]<disks> on the device: 6 (or 3) u32
+    slots per in-segment id of the 2^32-state segment."""
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    per = 3 if without_smallest else 6
+    din = DeviceBuffer.from_array(idx)
+    dout = DeviceBuffer(max(8, len(idx) * per * 4))
+    check(lib().ps_hanoi_expand_in_segment(disks, int(segment), din.ptr, len(idx), dout.ptr,
+                                           1 if without_smallest else 0, None))
+    out = dout.download(np.uint32, len(idx) * per)
+    din.free()
+    dout.free()
+    return out
+
+
+def hanoi_expand(disks, idx):
+    """HanoiTowers::Expand on the device: six u64 reduced children per index."""
+    idx = np.ascontiguousarray(idx, dtype=np.uint64)
+    din = DeviceBuffer.from_array(idx)
+    dout = DeviceBuffer(max(8, len(idx) * 48))
+    check(lib().ps_hanoi_expand(disks, din.ptr, len(idx), dout.ptr, None))
+    out = dout.download(np.uint64, len(idx) * 6)
+    din.free()
+    dout.free()
+    return out
+
+
+class HanoiSearch:
+    """HanoiSearch<disks> (ps_hanoi_search_*): the symmetry-reduced three-bit
+    BFS of 4-peg Hanoi over segments of 2^segment_bits states held in HBM."""
+
+    def __init__(self, disks, segment_bits=32, hbm_budget=0, stream=None):
+        h = _vp()
+        check(lib().ps_hanoi_search_create(disks, segment_bits, hbm_budget, stream, ctypes.byref(h)))
+        self.h = h.value
+        self.disks = disks
+
+    def run(self, start=0, max_layers=4096):
+        """(restored counts, reduced counts, reduced-frontier hashes) per layer."""
+        restored = np.zeros(max_layers, np.uint64)
+        reduced = np.zeros(max_layers, np.uint64)
+        hashes = np.zeros(max_layers, np.uint64)
+        n = _i32()
+        check(lib().ps_hanoi_search_run(self.h, int(start), max_layers, restored.ctypes.data, reduced.ctypes.data,
+                                        hashes.ctypes.data, max_layers, ctypes.byref(n)))
+        k = n.value
+        return ([int(x) for x in restored[:k]], [int(x) for x in reduced[:k]], [int(x) for x in hashes[:k]])
+
+    def stats(self):
+        st = HanoiSearchStats()
+        check(lib().ps_hanoi_search_stats_get(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            check(lib().ps_hanoi_search_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            if self.h and _lib is not None:
+                _lib.ps_hanoi_search_destroy(self.h)
         except Exception:
             pass

@@ -5,6 +5,7 @@
 //
 //   host_tests --cpu   tests that need no GPU (Parse/ToString/metadata)
 //   host_tests --gpu   everything (plugin Expand on the GPU, the drivers)
+#include <algorithm>
 #include <cstring>
 #include <functional>
 #include <iostream>
@@ -18,6 +19,8 @@
 
 #include "CompressedFrontier.h"
 #include "DiskBasedBFS.h"
+#include "HanoiSearch.h"
+#include "HanoiTowers.h"
 #include "InMemoryBFS.h"
 #include "Puzzles.h"
 #include "Store.h"
@@ -106,6 +109,16 @@ const char* ST5x2 =                                                             
 const char* PANCAKE13_6 = "1 12 132 1451 14556 130096 1030505";                          // PuzzleSearchTests.cpp:569
 const char* HANOI21_10 = "1 3 6 12 30 30 66 96 126 210 330";                             // PuzzleSearchTests.cpp:477
 const char* HANOI21_SYM_10 = "1 1 1 3 5 7 11 16 24 36 55";                               // PuzzleSearchTests.cpp:487
+const char* HANOI17_120 =                                                               // HanoiSearchTests.cpp:21
+    "1 3 6 12 30 30 66 96 126 210 330 318 462 816 1032 936 1044 1752 2610 3036 3528 3306 4578 6318 9108 "
+    "10674 11580 11844 13374 17124 23664 32184 36984 39822 38544 39936 45936 57990 77262 96366 116052 "
+    "127092 122142 112086 116814 135078 157326 192498 246468 304650 354018 388128 415206 427710 411804 "
+    "402504 451236 521094 600684 714750 883602 1064970 1228656 1349322 1464336 1539426 1565634 1554408 "
+    "1610640 1703436 1890282 2144742 2533656 2988834 3506376 4017900 4551204 4999416 5378250 5623770 "
+    "5778096 5794866 5857608 6013998 6447654 7084050 7998492 9102018 10433148 11919366 13440804 14761050 "
+    "16074066 17531574 18802386 19692462 19862262 19671618 19386408 19539486 20568270 22353276 24582186 "
+    "27002898 29769948 32947008 36487218 40325658 44952132 49926018 54298296 57553218 59394084 60022506 "
+    "59865378 60725346 63215580 65921856 68036658 69281274 70563468";
 
 // a fresh directory for the stores of one test
 std::string TempDir(const char* tag) {
@@ -497,6 +510,70 @@ TEST_GPU(DeviceResidentDrivers) {
     GpuSearchOptions o;
     o.maxSteps = 3;
     EXPECT_EQ(Join(GpuInMemoryThreeBitBFS(p9, p9.ToString(0), o)), std::string("1 8 56 391"));
+}
+
+TEST_CPU(HanoiTowersVerticalMeta) {
+    // HanoiTowers/HanoiTowers.h:13-23, HanoiTowers.cpp:139-150
+    using H17 = HanoiTowers<17>;
+    EXPECT_EQ(H17::IndexesCount(), 1ull << 34);
+    EXPECT_EQ(H17::MaxSegments(), 4);
+    EXPECT_EQ(HanoiTowers<24>::MaxSegments(), 65536);
+    EXPECT_EQ(H17::ToString(0), std::string("1 2 3 4 5 6 7 8 9 10 11 12 13 14 15 16 17 0 0 0"));
+    EXPECT_EQ(H17::Parse(H17::ToString(0)), 0ull);
+    const uint64_t x = (1ull << 32) | (2ull << 2) | 1ull;  // disk 17 on peg 1, disk 2 on peg 2, disk 1 on peg 1
+    EXPECT_EQ(H17::ToString(x), std::string("3 4 5 6 7 8 9 10 11 12 13 14 15 16 0 1 17 0 2 0"));
+    EXPECT_EQ(H17::Parse(H17::ToString(x)), x);
+    EXPECT_EQ(H17::SplitIndex(x).first, 1);
+    EXPECT_EQ(H17::SplitIndex(x).second, (uint32_t)9);
+    EXPECT_EQ(H17::PegsMult(x), 6);
+    EXPECT_EQ(H17::PegsMult(1ull << 32), 3);  // only peg 1 used besides peg 0
+    EXPECT_EQ(H17::InSegmentOnly(0), false);
+    EXPECT_EQ(H17::InSegmentOnly(0x1B), true);  // disks 1..3 on pegs 3, 2, 1
+}
+
+TEST_GPU(HanoiTowersVerticalExpand) {
+    // slot form (HanoiExpandGPU.cu:98-229) against the full expansion
+    using H = HanoiTowers<17>;
+    std::vector<uint64_t> frontier{0}, all{0};
+    for (int layer = 0; layer < 12; layer++) {
+        std::vector<uint64_t> next;
+        H::Expand(frontier, next);
+        frontier.clear();
+        for (uint64_t c : next)
+            if (std::find(all.begin(), all.end(), c) == all.end()) {
+                all.push_back(c);
+                frontier.push_back(c);
+            }
+    }
+    std::vector<uint32_t> ids;
+    for (uint64_t s : all) ids.push_back(uint32_t(s));
+    std::vector<uint32_t> six, three;
+    H::ExpandInSegment(0, ids.size(), ids.data(), six);
+    GpuExpandInSegmentWithoutSmallest<17>(0, ids.size(), ids.data(), three);
+    EXPECT_EQ(six.size(), ids.size() * 6);
+    EXPECT_EQ(three.size(), ids.size() * 3);
+    std::vector<uint64_t> full;
+    H::Expand(all, full);
+    bool same = true;
+    for (size_t i = 0; i < ids.size(); i++)
+        for (int k = 0; k < 6; k++) {
+            const uint64_t c = full[i * 6 + k];
+            same &= six[i * 6 + k] == ((c >> 32) == 0 ? uint32_t(c) : ids[i]);
+        }
+    EXPECT_EQ(same, true);
+    std::vector<uint64_t> cross;
+    H::ExpandCrossSegment(0, ids, cross);
+    EXPECT_EQ(cross.size(), (size_t)0);  // disk 17 cannot move in 12 moves
+}
+
+TEST_GPU(HanoiSearch17) {
+    // HanoiTowersTests/HanoiSearchTests.cpp:15-23
+    HanoiTowers<17> towers;
+    std::string initial = towers.ToString(0);
+    SearchOptions opts;
+    opts.maxSteps = 120;
+    auto result = HanoiSearch<towers.Size>(initial, opts);
+    EXPECT_EQ(Join(result), std::string(HANOI17_120));
 }
 
 TEST_GPU(ErrorsBecomeExceptions) {

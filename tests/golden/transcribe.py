@@ -68,6 +68,16 @@ SPECS = [
          puzzle=dict(kind="hanoi", disks=18, symmetry=False), start="index:0", partial=True),
     dict(id="hanoi_19_120", file="HanoiTowersTests/HanoiSearchTests.cpp", test="Search19",
          puzzle=dict(kind="hanoi", disks=19, symmetry=False), start="index:0", partial=True),
+    dict(id="hanoi_20_120", file="HanoiTowersTests/HanoiSearchTests.cpp", test="Search20",
+         puzzle=dict(kind="hanoi", disks=20, symmetry=False), start="index:0", partial=True),
+    dict(id="hanoi_21_120", file="HanoiTowersTests/HanoiSearchTests.cpp", test="Search21",
+         puzzle=dict(kind="hanoi", disks=21, symmetry=False), start="index:0", partial=True),
+    dict(id="hanoi_22_120", file="HanoiTowersTests/HanoiSearchTests.cpp", test="Search22",
+         puzzle=dict(kind="hanoi", disks=22, symmetry=False), start="index:0", partial=True),
+    dict(id="hanoi_23_120", file="HanoiTowersTests/HanoiSearchTests.cpp", test="Search23",
+         puzzle=dict(kind="hanoi", disks=23, symmetry=False), start="index:0", partial=True),
+    dict(id="hanoi_24_120", file="HanoiTowersTests/HanoiSearchTests.cpp", test="Search24",
+         puzzle=dict(kind="hanoi", disks=24, symmetry=False), start="index:0", partial=True),
     dict(id="sliding_5x3_prefix", file="PuzzlesTests/PuzzleSearchTests.cpp", test="OptFrontierSearch_5x3_1th",
          puzzle=dict(kind="sliding", width=5, height=3), start="0 1 2 3 4 5 6 7 8 9 10 11 12 13 14",
          partial=True),

@@ -145,6 +145,12 @@ _SIGS = {
     "or_fc_encode": (_i, [_i, _u32p, _u8p]),
     "or_fc_encode_with_check": (_i, [_i, _u32p, _u8p]),
     "or_fc_decode": (_i, [ctypes.POINTER(_i), _u8p, _u32p]),
+    "or_hv_pegs_mult": (_i, [_u64]),
+    "or_hv_in_segment_only": (_i, [_u64, _i]),
+    "or_hv_expand": (None, [_i, _u64, _u64p]),
+    "or_hv_expand_cross_segment": (_i, [_i, _i, _u64, ctypes.c_uint32, _u64p]),
+    "or_hv_expand_in_segment_slots": (None, [_i, ctypes.c_uint32, _u32p, _u64, _i, _u32p]),
+    "or_hv_search": (_i, [_i, _u64, _i, _i, _u64p, _u64p, ctypes.c_void_p]),
 }
 
 
@@ -343,3 +349,44 @@ def bfs3_segment_step(p, segment_bits, segment, nseg, cur, old, xseg=None):
         off += nb
     return {"count": int(r), "fr_idx": fr_idx[:io.fr_n].copy(), "fr_bytes": fr_bytes[:io.fr_nbytes].copy(),
             "buckets": buckets, "children_in": int(io.children_in), "children_cross": int(io.children_cross)}
+
+
+# ------------------------------------------------- HanoiTowers vertical solver
+
+def hv_pegs_mult(index):
+    return int(lib().or_hv_pegs_mult(int(index)))
+
+
+def hv_in_segment_only(index, seg_bits=32):
+    return bool(lib().or_hv_in_segment_only(int(index), seg_bits))
+
+
+def hv_expand(disks, index):
+    out = np.zeros(6, np.uint64)
+    lib().or_hv_expand(disks, int(index), out)
+    return [int(x) for x in out]
+
+
+def hv_expand_cross_segment(disks, seg_bits, segment, index):
+    out = np.zeros(6, np.uint64)
+    n = lib().or_hv_expand_cross_segment(disks, seg_bits, int(segment), int(index), out)
+    return [int(x) for x in out[:n]]
+
+
+def hv_expand_in_segment_slots(disks, segment, idx, without_smallest=False):
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    out = np.zeros(len(idx) * (3 if without_smallest else 6), np.uint32)
+    lib().or_hv_expand_in_segment_slots(disks, int(segment), idx, len(idx), int(without_smallest), out)
+    return out
+
+
+def hv_search(disks, start=0, seg_bits=32, max_layers=4096, with_hashes=True):
+    """HanoiSearch restated: (restored counts, reduced counts, reduced-frontier hashes)."""
+    restored = np.zeros(max_layers, np.uint64)
+    reduced = np.zeros(max_layers, np.uint64)
+    hashes = np.zeros(max_layers, np.uint64)
+    n = lib().or_hv_search(disks, int(start), seg_bits, max_layers, restored, reduced,
+                           hashes.ctypes.data if with_hashes else None)
+    assert n > 0, "oracle vertical Hanoi search failed"
+    return ([int(x) for x in restored[:n]], [int(x) for x in reduced[:n]],
+            [int(x) for x in hashes[:n]] if with_hashes else None)
