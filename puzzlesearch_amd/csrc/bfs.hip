@@ -52,6 +52,7 @@ struct ResetStart {
     unsigned long long* seg_count;  // NULL: none
     unsigned long long* layer_counts;
     unsigned long long* layer_hashes;
+    uint64_t* visited_word;  // NULL: no visited array
     unsigned long long h0;
     uint32_t* tile_list;  // NULL: none
     uint32_t tile;
@@ -62,6 +63,7 @@ struct ResetStart {
 __global__ void k_bfs_reset_start(ResetStart r) {
     if (threadIdx.x != 0) return;
     *r.word_ptr = r.word;
+    if (r.visited_word) *r.visited_word = r.word;
     if (r.seg_count) *r.seg_count = 1ull;
     *r.layer_counts = 1ull;
     *r.layer_hashes = r.h0;
@@ -161,6 +163,21 @@ psk::StepArgs step_args(ps_bfs* b, bool first) {
     a.odd = b->info.has_odd_cycles;
     a.hash = b->opt.hash;
     a.first = first ? 1 : 0;
+    a.role = 0;
+    a.visited = b->visited;
+    a.nbits = b->nbits;
+    return a;
+}
+
+psk::PullArgs pull_args(ps_bfs* b) {
+    psk::PullArgs a{};
+    for (int i = 0; i < 3; ++i) a.bits[i] = b->arr[i];
+    a.st = b->st;
+    a.visited = b->visited;
+    a.wflags[0] = b->wflags;
+    a.wflags[1] = b->wflags + b->xtiles;
+    a.tiles = b->xtiles;
+    a.nbits = b->nbits;
     return a;
 }
 
@@ -233,6 +250,15 @@ int launch_batched_layer(ps_bfs* b, bool timed) {
             if (int r = record(b, &f0)) return r;
         LaunchCfg cs{b->tpb_expand, b->grid_step, b->lds_step, b->stream};
         if (int r = pkl_step(b->puzzle, step_args(b, false), cs, nullptr)) return r;
+        if (b->visited) {
+            // a split layer: its expansion top-down (role 1) or bottom-up; each
+            // of the two exits at once unless the device chose it
+            psk::StepArgs a1 = step_args(b, false);
+            a1.role = 1;
+            if (int r = pkl_step(b->puzzle, a1, cs, nullptr)) return r;
+            LaunchCfg cp{b->tpb_expand, b->grid_pull, b->lds_pull, b->stream};
+            if (int r = pkl_pull(b->puzzle, pull_args(b), cp, nullptr)) return r;
+        }
         if (timed) {
             if (int r = record(b, &f1)) return r;
             if (b->collect_kernel_times) b->expand_spans.push_back({f0, f1});
@@ -432,6 +458,7 @@ int destroy_bfs(ps_bfs* b) {
     if (b->xcounts) (void)hipFree(b->xcounts);
     if (b->xflags) (void)hipFree(b->xflags);
     if (b->xlists1) (void)hipFree(b->xlists1);
+    if (b->visited) (void)hipFree(b->visited);
     if (b->xcounts1) (void)hipFree(b->xcounts1);
     if (b->xflags1) (void)hipFree(b->xflags1);
     if (b->wflags) (void)hipFree(b->wflags);
@@ -526,8 +553,10 @@ uint64_t footprint_bytes(const ps_puzzle* p, const ps_bfs_options* o, const Geom
         arrays += entries * 4 + (uint64_t)tiles * slots * 4 + tiles;
         // batched mode: a second parity of lists, counts and flags, tile flags,
         // counts and the list of tiles a launch visits
-        if (o->mode == PS_BFS_BATCHED)
+        if (o->mode == PS_BFS_BATCHED) {
             arrays += entries * 4 + (uint64_t)tiles * slots * 4 + tiles + 2ull * tiles + 12ull * tiles + 4ull * tiles;
+            if (o->direction == PS_DIR_AUTO) arrays += g.nwords * 8;  // the visited states
+        }
     } else if (o->bits == 3 && o->mode == PS_BFS_BATCHED) {
         // the list of tiles a top-down expand visits: 4 bytes per tile, tiles
         // of at least 64 words unless tile_bits_log2 asks for smaller ones
@@ -670,8 +699,10 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
             return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS covers the three-bit search of 10..12 pancakes");
         if (g.nbits < factorial(p->size))
             return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS needs index_count >= %d!", p->size);
-        if (opt.direction != PS_DIR_TOP_DOWN)
-            return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS runs top-down layers only");
+        if (opt.direction == PS_DIR_BOTTOM_UP || (opt.direction == PS_DIR_AUTO && opt.mode != PS_BFS_BATCHED))
+            return fail(PS_ERR_UNSUPPORTED,
+                        "PS_CROSS_LISTS runs top-down layers, or with PS_DIR_AUTO in batched mode bottom-up ones "
+                        "where the frontier outnumbers the unvisited states");
         if (opt.tile_bits_log2 != 0)
             return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS uses its own tiles of 9! states (tile_bits_log2 = 0)");
     }
@@ -731,6 +762,8 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
             if (int r = dalloc((void**)&b->wflags, 2ull * b->xtiles)) return cleanup(r);
             if (int r = dalloc((void**)&b->tcount, 12ull * b->xtiles)) return cleanup(r);
             if (int r = dalloc((void**)&b->active, 4ull * b->xtiles)) return cleanup(r);
+            if (opt.direction == PS_DIR_AUTO)
+                if (int r = dalloc((void**)&b->visited, b->nwords * 8)) return cleanup(r);
         }
     }
     if (int r = dalloc((void**)&b->seg_count, (uint64_t)3 * b->nseg * 8)) return cleanup(r);
@@ -803,6 +836,17 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
                 if (occ_s < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "list step kernel does not fit (LDS %zu bytes)", b->lds_step));
                 const int64_t want_s = opt.grid_blocks > 0 ? (int64_t)opt.grid_blocks : (int64_t)occ_s * cus;
                 b->grid_step = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(want_s, (int64_t)b->xtiles));
+                if (b->visited) {
+                    if (b->tpb_expand != 512 && b->tpb_expand != 1024)
+                        return cleanup(fail(PS_ERR_UNSUPPORTED, "bottom-up list layers run 512 or 1024 threads"));
+                    b->lds_pull = pkl_pull_lds_bytes(*p, b->tpb_expand);
+                    int occ_p = 0;
+                    LaunchCfg cp{b->tpb_expand, 0, b->lds_pull, b->stream};
+                    if (int r = pkl_pull(*p, psk::PullArgs{}, cp, &occ_p)) return cleanup(r);
+                    if (occ_p < 1)
+                        return cleanup(fail(PS_ERR_UNSUPPORTED, "bottom-up list kernel does not fit (LDS %zu bytes)", b->lds_pull));
+                    b->grid_pull = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)occ_p * cus, (int64_t)b->xtiles));
+                }
             }
         } else {
             LaunchCfg c{b->tpb_expand, 0, b->lds_expand, b->stream};
@@ -851,6 +895,8 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
         if (int r = launch_fill(b->xflags1, 0, b->xtiles, b->stream)) return r;
         if (int r = launch_fill(b->wflags, 0, 2ull * b->xtiles, b->stream)) return r;
         if (int r = launch_fill(b->tcount, 0, 12ull * b->xtiles, b->stream)) return r;
+        if (b->visited)
+            if (int r = launch_fill(b->visited, 0, b->nwords * 8, b->stream)) return r;
     }
     b->primed = false;
     if (int r = launch_fill(b->seg_count, 0, (uint64_t)3 * b->nseg * 8, b->stream)) return r;
@@ -869,6 +915,7 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     rs.word = word;
     if (b->bits == 3) rs.seg_count = b->seg_count + (b->nseg > 1 ? (start >> b->seg_shift) : 0);
     rs.layer_counts = b->layer_counts;
+    if (b->visited) rs.visited_word = b->visited + word_idx;
     rs.layer_hashes = b->layer_hashes;
     rs.h0 = host_word_hash(word_idx, word);
     BfsDevState st{};
@@ -880,7 +927,7 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     st.max_layers = (uint32_t)b->max_layers;
     st.layers_recorded = 1;
     st.visited = 1;
-    st.direction = b->pull_capable ? (uint32_t)b->opt.direction : 0u;
+    st.direction = (b->pull_capable || b->visited) ? (uint32_t)b->opt.direction : 0u;
     st.pull = st.direction == PS_DIR_BOTTOM_UP;
     st.done = b->max_layers <= 1 ? 1 : 0;
     rs.st = b->st;

@@ -64,6 +64,9 @@ struct ps_bfs {
     int fused = 0;
     bool primed = false;
     uint32_t* xlists1 = nullptr;  // parity 1 (parity 0: xlists, xcounts, xflags)
+    uint64_t* visited = nullptr;  // fused lists with PS_DIR_AUTO: every state finalized so far
+    uint32_t grid_pull = 0;
+    size_t lds_pull = 0;
     uint32_t* xcounts1 = nullptr;
     uint8_t* xflags1 = nullptr;
     uint8_t* wflags = nullptr;    // [2][tile]
@@ -95,6 +98,7 @@ psk::LayerEndArgs layer_end_args(ps_bfs* b);
 psk::ListArgs list_args(ps_bfs* b);
 psk::ApplyArgs apply_args(ps_bfs* b);
 psk::StepArgs step_args(ps_bfs* b, bool first);
+psk::PullArgs pull_args(ps_bfs* b);
 // expands the start layer of a fused search (once after each reset)
 int prime_fused(ps_bfs* b);
 // reads the device state into b->stats (stream idle)

@@ -48,7 +48,8 @@ struct BfsDevState {
     uint32_t pull;       // the current layer is expanded bottom-up (see pull_tile)
     uint32_t direction;  // 0: always top-down, 1: choose per layer, 2: always bottom-up
     uint32_t active_tiles;  // tiles the next fused list launch visits (StepArgs::active)
-    uint32_t reserved;
+    uint32_t split;         // fused list search: the next launch only finalizes; the layer after it is
+                            // then expanded top-down or bottom-up by the launches that follow
     unsigned long long visited;      // states in the layers recorded so far
     unsigned long long pull_words;   // words scanned by bottom-up layers (old and cur read)
     unsigned long long pull_probes;  // cur bits read outside the tile by bottom-up layers
@@ -109,6 +110,19 @@ struct StepArgs {
     int32_t odd;
     int32_t hash;
     int32_t first;        // 1: expand the start layer only
+    int32_t role;         // 0: finalize (and expand unless st->split); 1: expand only (after a split finalize)
+    uint64_t* visited;    // every state of the layers finalized so far (NULL: top-down only)
+    uint64_t nbits;
+};
+
+// bottom-up layers of the fused list search (k_pkl_pull)
+struct PullArgs {
+    uint64_t* bits[3];
+    BfsDevState* st;
+    const uint64_t* visited;
+    uint8_t* wflags[2];
+    uint32_t tiles;
+    uint64_t nbits;
 };
 
 struct FinalizeArgs {

@@ -30,6 +30,10 @@ PS_DECLARE_BFS_FAMILY(hanoi)
 int pkl_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::ListArgs& x, const LaunchCfg& c, int* occ);
 int pkl_apply(const ps_puzzle& p, const psk::ApplyArgs& a, const LaunchCfg& c, int* occ);
 int pkl_step(const ps_puzzle& p, const psk::StepArgs& a, const LaunchCfg& c, int* occ);
+// bottom-up expansion of a split layer (k_pkl_pull; it exits at once unless the
+// finalize launch before it chose that direction)
+int pkl_pull(const ps_puzzle& p, const psk::PullArgs& a, const LaunchCfg& c, int* occ);
+size_t pkl_pull_lds_bytes(const ps_puzzle& p, int tpb);
 // bookkeeping between two k_pkl_step launches (advance: also the layer end)
 int pkl_layer_end(const psk::LayerEndArgs& a, const psk::StepArgs& s, int advance, hipStream_t stream);
 // LDS bytes of the kernels; 0 when the puzzle has no list kernels

@@ -58,7 +58,19 @@ int pkl_apply(const ps_puzzle& p, const psk::ApplyArgs& a, const LaunchCfg& c, i
 
 int pkl_step(const ps_puzzle& p, const psk::StepArgs& a, const LaunchCfg& c, int* occ) {
     if (!pkl_size(p)) return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
-    switch (p.size * 10000 + c.tpb) {
+    const int key = p.size * 10000 + c.tpb;
+    if (a.role == 1) {
+        switch (key) {
+        case 100512: return launch_pkl(psk::k_pkl_step<10, 512, 1>, c, occ, a);
+        case 101024: return launch_pkl(psk::k_pkl_step<10, 1024, 1>, c, occ, a);
+        case 110512: return launch_pkl(psk::k_pkl_step<11, 512, 1>, c, occ, a);
+        case 111024: return launch_pkl(psk::k_pkl_step<11, 1024, 1>, c, occ, a);
+        case 120512: return launch_pkl(psk::k_pkl_step<12, 512, 1>, c, occ, a);
+        case 121024: return launch_pkl(psk::k_pkl_step<12, 1024, 1>, c, occ, a);
+        }
+        return fail(PS_ERR_INVALID_ARGUMENT, "split list layers run 512 or 1024 threads, not %d", c.tpb);
+    }
+    switch (key) {
     case 100256: return launch_pkl(psk::k_pkl_step<10, 256>, c, occ, a);
     case 100512: return launch_pkl(psk::k_pkl_step<10, 512>, c, occ, a);
     case 101024: return launch_pkl(psk::k_pkl_step<10, 1024>, c, occ, a);
@@ -70,6 +82,29 @@ int pkl_step(const ps_puzzle& p, const psk::StepArgs& a, const LaunchCfg& c, int
     case 121024: return launch_pkl(psk::k_pkl_step<12, 1024>, c, occ, a);
     }
     return fail(PS_ERR_INVALID_ARGUMENT, "block_threads %d not in {256, 512, 1024}", c.tpb);
+}
+
+int pkl_pull(const ps_puzzle& p, const psk::PullArgs& a, const LaunchCfg& c, int* occ) {
+    if (!pkl_size(p)) return fail(PS_ERR_UNSUPPORTED, "cross-tile lists cover 10..12 pancakes");
+    switch (p.size * 10000 + c.tpb) {
+    case 100512: return launch_pkl(psk::k_pkl_pull<10, 512>, c, occ, a);
+    case 101024: return launch_pkl(psk::k_pkl_pull<10, 1024>, c, occ, a);
+    case 110512: return launch_pkl(psk::k_pkl_pull<11, 512>, c, occ, a);
+    case 111024: return launch_pkl(psk::k_pkl_pull<11, 1024>, c, occ, a);
+    case 120512: return launch_pkl(psk::k_pkl_pull<12, 512>, c, occ, a);
+    case 121024: return launch_pkl(psk::k_pkl_pull<12, 1024>, c, occ, a);
+    }
+    return fail(PS_ERR_INVALID_ARGUMENT, "the bottom-up list kernel runs 512 or 1024 threads, not %d", c.tpb);
+}
+
+size_t pkl_pull_lds_bytes(const ps_puzzle& p, int tpb) {
+    if (!pkl_size(p)) return 0;
+    switch (p.size) {
+    case 10: return psk::pkl_pull_lds<10>(tpb);
+    case 11: return psk::pkl_pull_lds<11>(tpb);
+    case 12: return psk::pkl_pull_lds<12>(tpb);
+    }
+    return 0;
 }
 
 int pkl_layer_end(const psk::LayerEndArgs& a, const psk::StepArgs& s, int advance, hipStream_t stream) {

@@ -333,6 +333,15 @@ __device__ __forceinline__ void pkl_gather(uint32_t t, uint32_t* counts, const u
 
 // Takes the next of n tiles (~0 when none is left): a plain read first, so
 // blocks arriving after the last tile make no atomic.  Thread 0 only.
+// After a split finalize: the layer just finalized (its count in acc_count)
+// is expanded bottom-up when it outnumbers the states not yet visited, which
+// bound the bottom-up work (Beamer's direction-optimizing BFS).  Every block
+// of the launches after the finalize reads the same values.
+__device__ __forceinline__ bool pkl_pull_now(const BfsDevState* st, uint64_t nbits) {
+    const unsigned long long cnt = st->acc_count, vis = st->visited + cnt;
+    return st->direction == 1u && cnt > (nbits > vis ? nbits - vis : 0ull);
+}
+
 __device__ __forceinline__ unsigned long long pkl_take_tile(unsigned long long* counter, unsigned long long n) {
     const unsigned long long taken = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (taken >= n) return ~0ull;
@@ -520,7 +529,7 @@ __global__ __launch_bounds__(TPB) void k_pkl_apply(ApplyArgs a) {
 // blocks per CU, as the LDS allows (at most 128 VGPRs; unbounded, the compiler
 // takes more and leaves one 512-thread block per CU).  1024-thread blocks keep
 // the finalized frontier words in LDS and expand from there.
-template <int N, int TPB>
+template <int N, int TPB, int ROLE = 0>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_pkl_step(StepArgs a) {
     using G = pkl::Geo<N>;
     constexpr int NW = TPB / 64;
@@ -530,6 +539,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     extern __shared__ uint32_t smem[];
     BfsDevState* st = a.st;
     if (st->done) return;
+    const bool first = a.first != 0;
+    // split layers: the role-0 launch only finalizes; a role-1 launch then
+    // expands top-down unless the pull launch expands bottom-up
+    const bool split = !first && st->split != 0u;
+    constexpr bool expand_only = ROLE == 1;
+    if (expand_only && (!split || pkl_pull_now(st, a.nbits))) return;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     PklExpLds lds;
     lds.win = smem;
@@ -548,7 +563,6 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
         lds.cnt = shared_region + (qbytes > mbytes ? qbytes : mbytes) / 4;
     }
     uint8_t* meta = reinterpret_cast<uint8_t*>(shared_region);  // tab3 + 504 words: 8-byte aligned
-    const bool first = a.first != 0;
     const uint32_t L = st->layer;
     const uint32_t nvisit = first ? (a.first_tile < a.tiles ? 1u : 0u) : st->active_tiles;
     const uint64_t* cur = a.bits[L % 3];
@@ -556,7 +570,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     uint64_t* old = a.bits[(L + 2) % 3];
     // the expansion: frontier, its next array, list parity
     const uint64_t* front = first ? a.bits[0] : nw;
-    const bool front_lds = FRONT_LDS && !first;
+    const bool front_lds = FRONT_LDS && !first && !expand_only;
     uint32_t* enext = reinterpret_cast<uint32_t*>(first ? a.bits[1] : old);
     const int gp = (int)(L & 1u), xp = first ? 0 : (int)((L + 1u) & 1u);
     uint32_t* tc_new = a.tcount + (uint64_t)(first ? 0 : (L + 1) % 3) * a.tiles;
@@ -572,8 +586,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
     bool tables = false;
     unsigned long long parents = 0;
     uint32_t err = 0;
-    // the first launch has no bookkeeping launch after it to reset tile_counter
-    unsigned long long* counter = first ? &st->chunk_counter : &st->tile_counter;
+    // the first launch has no bookkeeping launch after it to reset tile_counter;
+    // an expand-only launch follows the finalize launch of the same layer
+    unsigned long long* counter = (first || expand_only) ? &st->chunk_counter : &st->tile_counter;
     for (;;) {
         if (tid == 0) {
             const unsigned long long k = pkl_take_tile(counter, nvisit);
@@ -590,7 +605,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
         const uint32_t T = (uint32_t)tile;
         const uint64_t w0 = tile * pkl::kTileWords, w1 = w0 + pkl::kTileWords;
         bool expand = true;
-        if (!first) {
+        if (expand_only) {
+            // the tile's frontier count, written by the finalize launch
+            expand = tc_new[T] != 0u;
+        } else if (!first) {
             // 1. finalize the tile's next layer
             unsigned long long c = 0, h = 0;
             if (s_xf | s_wf) {
@@ -603,6 +621,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
                 ulonglong2* nw2 = reinterpret_cast<ulonglong2*>(nw + w0);
                 ulonglong2* old2 = reinterpret_cast<ulonglong2*>(old + w0);
                 const ulonglong2* cur2 = reinterpret_cast<const ulonglong2*>(cur + w0);
+                ulonglong2* vis2 = a.visited ? reinterpret_cast<ulonglong2*>(a.visited + w0) : nullptr;
                 for (uint32_t q = tid; q < pkl::kTileWords / 2; q += TPB) {
                     const ulonglong2 vn = nw2[q], vo = old2[q];
                     const ulonglong2 vc = a.odd ? cur2[q] : make_ulonglong2(0ull, 0ull);
@@ -612,6 +631,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
                     v.y = (vn.y | g.y) & ~vo.y & ~vc.y;
                     nw2[q] = v;
                     old2[q] = make_ulonglong2(0ull, 0ull);
+                    if (vis2 && (v.x | v.y)) {
+                        const ulonglong2 vv = vis2[q];
+                        vis2[q] = make_ulonglong2(vv.x | v.x, vv.y | v.y);
+                    }
                     if (FRONT_LDS) reinterpret_cast<ulonglong2*>(flds)[q] = v;
                     c += (unsigned long long)(__popcll(v.x) + __popcll(v.y));
                     if (a.hash) h += word_hash(w0 + 2 * q, v.x) + word_hash(w0 + 2 * q + 1, v.y);
@@ -644,7 +667,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
                 s_cnt = sc;
             }
             __syncthreads();
-            expand = s_cnt != 0;
+            expand = s_cnt != 0 && !split;
         }
         if (expand) {
             // 2. expand the tile's frontier
@@ -703,6 +726,128 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
                     &st->expanded, &st->scan_words);
 }
 
+// Bottom-up expansion of a split layer (direction-optimizing BFS): after the
+// finalize launch, every state not yet visited looks for a neighbour in the
+// layer just finalized -- flips are involutions, so its neighbours are its
+// children.  Flips of <= 9 elements stay in the tile and are tested in an LDS
+// copy of the tile's frontier words; only a state none of them reaches tests
+// its flips of 10..N elements, one read of the frontier each (the list
+// arithmetic gives those children as global ranks).  A state with a
+// neighbour in the frontier is in the next layer (the graph has no shorter
+// path to it); its bit goes to an LDS window stored into the tile's next
+// words, which nothing else writes.  No lists are filled.
+template <int N>
+inline size_t pkl_pull_lds(int tpb) {
+    return (size_t)2 * pkl::kTileWords * 4 + (size_t)pkl::kTileWords * 8 + PKL_TAB_BYTES +
+           (size_t)(tpb / 64) * PKL_QCAP * 4;
+}
+
+template <int N, int TPB>
+__global__ __launch_bounds__(TPB) void k_pkl_pull(PullArgs a) {
+    using G = pkl::Geo<N>;
+    constexpr int NW = TPB / 64;
+    constexpr int CROSS = G::CROSS;
+    constexpr uint32_t WIN = 2 * pkl::kTileWords;
+    extern __shared__ uint32_t smem[];
+    BfsDevState* st = a.st;
+    if (st->done || st->split == 0u || !pkl_pull_now(st, a.nbits)) return;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    uint32_t* win = smem;
+    uint64_t* flds = reinterpret_cast<uint64_t*>(smem + WIN);  // WIN * 4 is a multiple of 16
+    pkl::Tab6* tab6 = reinterpret_cast<pkl::Tab6*>(smem + 2 * WIN);
+    uint32_t* tab6lo = reinterpret_cast<uint32_t*>(tab6 + 720);
+    uint32_t* tab3 = tab6lo + 720;
+    uint32_t* q = tab3 + 504 + wave * PKL_QCAP;
+    const uint32_t L = st->layer;
+    const uint64_t* front = a.bits[(L + 1) % 3];             // the layer just finalized
+    uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 2) % 3]);  // cleared by that launch
+    const int xp = (int)((L + 1u) & 1u);
+    __shared__ unsigned long long s_tile;
+    __shared__ uint32_t s_any;
+    for (uint32_t l = tid; l < 720; l += TPB) tab6[l] = pkl::tab6_entry(l, &tab6lo[l]);
+    for (uint32_t hh = tid; hh < 504; hh += TPB) tab3[hh] = pkl::tab3_entry(hh);
+    unsigned long long words = 0, probes = 0;
+    for (;;) {
+        if (tid == 0) s_tile = pkl_take_tile(&st->chunk_counter, a.tiles);
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        if (tile >= a.tiles) break;
+        const uint32_t T = (uint32_t)tile;
+        const uint64_t w0 = tile * pkl::kTileWords;
+        for (uint32_t i = tid; i < WIN; i += TPB) win[i] = 0;
+        for (uint32_t i = tid; i < pkl::kTileWords; i += TPB) flds[i] = front[w0 + i];
+        if (tid == 0) s_any = 0;
+        uint32_t dp[3];
+        pkl::tile_digits<N>(T, dp);
+        const pkl::TileK tk = pkl::tile_consts<N>(dp);
+        __syncthreads();
+        if (tid == 0) words += pkl::kTileWords;
+        bool any = false;
+        auto probe = [&](uint32_t off) {
+            bool found = false;
+            uint32_t xc[CROSS];
+            pkl::children_row<N>(
+                pkl::tab_row(tab6, tab6lo, tab3, off), tk, off,
+                [&](uint32_t c) { found |= ((flds[c >> 6] >> (c & 63)) & 1ull) != 0ull; },
+                [&](int k, uint32_t, uint32_t child) { xc[k - 10] = child; });
+            if (!found) {
+#pragma unroll
+                for (int f = 0; f < CROSS; ++f) {
+                    const uint64_t g = xc[f];  // the child's rank
+                    if (g >= a.nbits) continue;
+                    ++probes;
+                    found |= ((front[g >> 6] >> (g & 63)) & 1ull) != 0ull;
+                }
+            }
+            if (found) {
+                lor(&win[off >> 5], 1u << (off & 31));
+                any = true;
+            }
+        };
+        // the tile's unvisited states, compacted per wave as the expansion does
+        int qn = 0;
+        for (uint64_t g = (uint64_t)wave * 64; g < pkl::kTileWords; g += (uint64_t)NW * 64) {
+            const uint64_t wi = g + lane;
+            uint64_t bits = (wi < pkl::kTileWords) ? ~a.visited[w0 + wi] : 0ull;
+            const uint32_t c = (uint32_t)__popcll(bits);
+            const uint32_t incl = wave_incl_scan(c, lane);
+            const int total = (int)__shfl(incl, 63, 64);
+            int off = qn + (int)(incl - c);
+            int end_all = qn + total;
+            const uint32_t base = (uint32_t)(wi * 64);
+            for (;;) {
+                while (bits && off < PKL_QCAP) {
+                    q[off++] = base + (uint32_t)(__ffsll((unsigned long long)bits) - 1);
+                    bits &= bits - 1;
+                }
+                if (end_all < PKL_QCAP) {
+                    qn = end_all;
+                    break;
+                }
+                wave_sync();
+                for (uint32_t j = lane; j < (uint32_t)PKL_QCAP; j += 64) probe(q[j]);
+                wave_sync();
+                off -= PKL_QCAP;
+                end_all -= PKL_QCAP;
+                qn = 0;
+                if (end_all == 0) break;
+            }
+        }
+        wave_sync();
+        for (uint32_t j = lane; j < (uint32_t)qn; j += 64) probe(q[j]);
+        if (any) s_any = 1;
+        __syncthreads();
+        uint32_t* dst = nxt + w0 * 2;
+        for (uint32_t i = tid; i < WIN; i += TPB) {
+            const uint32_t v = win[i];
+            if (v) dst[i] = v;
+        }
+        if (tid == 0 && s_any) a.wflags[xp][T] = 1;
+        __syncthreads();
+    }
+    block_add3<TPB>(words, probes, 0ull, &st->pull_words, &st->pull_probes, nullptr);
+}
+
 // Bookkeeping between two launches of k_pkl_step (one block): with advance,
 // the layer end of every batched search (layer_end_body); then, unless the
 // search is done, the list of tiles the next launch visits -- those that a
@@ -722,6 +867,15 @@ __global__ __launch_bounds__(TPB) void k_pkl_layer_end(LayerEndArgs a, StepArgs 
     __syncthreads();
     if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const uint32_t L = __hip_atomic_load(&st->layer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        // split the next layer (finalize, then choose the direction of its
+        // expansion) once the frontier could outgrow the unvisited states: the
+        // next layer is at most a few times the current one
+        const unsigned long long cnt = advance ? a.layer_counts[L] : 0ull;
+        const unsigned long long vis = st->visited;
+        const unsigned long long unv = a.nbits > vis ? a.nbits - vis : 0ull;
+        st->split = (advance && st->direction == 1u && 4ull * cnt > unv) ? 1u : 0u;
+    }
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     const int gp = (int)(L & 1u);
     const uint8_t* xf = s.xflags[gp];

@@ -62,10 +62,6 @@ __device__ __forceinline__ void hv_from_index(uint64_t index, uint64_t mask, HvS
     for (int p = 0; p < 4; ++p) s.top[p] = hv_low(s.pegs[p]);
 }
 
-__device__ __forceinline__ uint64_t hv_to_index(const HvState& s) {
-    return s.pegs[1] | (s.pegs[2] << 1) | (s.pegs[3] * 3);
-}
-
 // the smaller top disk of pegs a, b changes peg; pegs 1..3 are put back in
 // bottom-disk order when one of the two pegs was or became empty
 __device__ __forceinline__ uint64_t hv_child(const HvState& s, int a, int b) {
