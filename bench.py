@@ -240,10 +240,24 @@ TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256]
 PANCAKE_CANDIDATES = dict(TUNE_CANDIDATES, cross_marks=[2, 0, 1], block_threads=[0, 512, 256])
 
 
-def dominant_kernel(psa, opts):
-    """The kernel the roofline covers: batched searches with cross-tile lists
-    run one fused gather/finalize/expand kernel per layer."""
-    return "k_pkl_step" if opts.get("cross_marks") == psa.CROSS_LISTS else "k_bfs3_expand"
+def kernel_rooflines(psa, opts, st):
+    """Per-kernel achieved bandwidth of an eager (event-timed) run: batched
+    searches with cross-tile lists run the fused gather/finalize/expand kernel
+    k_pkl_step per layer and, in layers the device expands bottom-up, k_pkl_pull
+    after a finalize-only k_pkl_step.  Each kernel's algorithmic bytes (the
+    per-unit figures of the design notes times the units it processed) over
+    its summed HIP-event time.  Returns {kernel: {...}}, dominant first."""
+    out = {}
+    main = "k_pkl_step" if opts.get("cross_marks") == psa.CROSS_LISTS else "k_bfs3_expand"
+    if st.expand_launches:
+        out[main] = {"launches": st.expand_launches, "ms_total": st.expand_ms, "bytes_total": st.expand_bytes}
+    if st.pull_launches:
+        out["k_pkl_pull"] = {"launches": st.pull_launches, "ms_total": st.pull_ms, "bytes_total": st.pull_bytes}
+    for k, v in out.items():
+        v["avg_launch_ms"] = v["ms_total"] / v["launches"]
+        v["algorithmic_bytes_per_launch"] = v["bytes_total"] / v["launches"]
+        v["achieved_GBps"] = v["bytes_total"] / (v["ms_total"] * 1e-3) / 1e9 if v["ms_total"] > 0 else 0.0
+    return dict(sorted(out.items(), key=lambda kv: -kv[1]["ms_total"]))
 
 
 def tuned(psa, p, start, index_count=0, candidates=None):
@@ -430,6 +444,9 @@ def main():
     ap.add_argument("--cross-marks", choices=["auto", "bytes", "atomic", "lists"], default="auto",
                     help="children outside the tile: auto = the tuner's choice (with --profile or --no-tune: "
                          "lists for 10..12 pancakes, else bytes)")
+    ap.add_argument("--direction", type=int, default=-1, choices=[-1, 0, 1],
+                    help="layer direction without the tuner: 0 top-down, 1 auto (bottom-up where the frontier "
+                         "outnumbers the unvisited states); -1: auto with cross-tile lists, else top-down")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-segment-step", action="store_true")
@@ -475,8 +492,12 @@ def main():
     cm = {"bytes": psa.CROSS_BYTES, "atomic": psa.CROSS_ATOMIC, "lists": psa.CROSS_LISTS}.get(args.cross_marks)
     if cm is None:
         cm = psa.CROSS_LISTS if 10 <= n <= 12 else psa.CROSS_BYTES
+    direction = args.direction
+    if direction < 0:
+        # the tuner's pick for the 12-pancake with lists: bottom-up where the frontier outnumbers the unvisited
+        direction = psa.DIR_AUTO if cm == psa.CROSS_LISTS else psa.DIR_TOP_DOWN
     opts = {"segment_bits": args.segment_bits, "cross_marks": cm, "grid_blocks": args.grid_blocks,
-            "block_threads": args.block_threads}
+            "block_threads": args.block_threads, "direction": direction}
     tune_res = None
     if not args.no_tune and not args.profile and args.cross_marks == "auto":
         tune_res = tuned(psa, p, 0, index_count=N, candidates=PANCAKE_CANDIDATES if 10 <= n <= 12 else None)
@@ -489,15 +510,12 @@ def main():
             for _ in range(runs):
                 ce, he = b.run(0)
             st = b.stats()
-        ms = st.expand_ms / max(1, st.expand_launches)
-        by = st.expand_bytes / max(1, st.expand_launches)
-        return ce, he, st, ms, by
+        return ce, he, st, kernel_rooflines(psa, opts, st)
 
     if args.profile:
-        ce, he, st_e, ms, by = eager_roofline(args.steps)
-        print(json.dumps({"profile": dominant_kernel(psa, opts), "launches_last_run": st_e.expand_launches,
-                          "avg_launch_ms": ms, "algorithmic_bytes_per_launch": by,
-                          "achieved_GBps": by / (ms * 1e-3) / 1e9, "counts": ce}))
+        ce, he, st_e, kr = eager_roofline(args.steps)
+        print(json.dumps({"profile": "12-pancake batched search, eager", "options": eager_opts,
+                          "kernels_last_run": kr, "counts": ce}))
         return
 
     # batched persistent path, layer loop in a hipGraph
@@ -511,23 +529,37 @@ def main():
     assert sum(counts) == N, "layer counts do not cover the state space"
     assert parity is not False, "layer counts or frontier hashes differ from the oracle fixture"
 
-    counts_e, hashes_e, st_e, exp_launch_ms, exp_bytes_launch = eager_roofline(2)
+    counts_e, hashes_e, st_e, kr = eager_roofline(2)
     assert counts_e == counts and hashes_e == hashes
-    achieved = exp_bytes_launch / (exp_launch_ms * 1e-3) / 1e9
-    log(f"{dominant_kernel(psa, opts)}: {st_e.expand_launches} launches, avg {exp_launch_ms:.3f} ms, "
-        f"{exp_bytes_launch / 1e9:.3f} GB/launch -> {achieved:.0f} GB/s; apply {st_e.apply_ms:.2f} ms, "
-        f"finalize {st_e.finalize_ms:.2f} ms total")
+    kernel = next(iter(kr))
+    dom = kr[kernel]
+    exp_launch_ms, exp_bytes_launch, achieved = dom["avg_launch_ms"], dom["algorithmic_bytes_per_launch"], \
+        dom["achieved_GBps"]
+    for k, v in kr.items():
+        log(f"{k}: {v['launches']} launches, {v['ms_total']:.3f} ms total, avg {v['avg_launch_ms']:.3f} ms, "
+            f"{v['algorithmic_bytes_per_launch'] / 1e9:.3f} GB/launch -> {v['achieved_GBps']:.0f} GB/s")
 
-    # the same batched path with the other ways of marking children outside the tile
+    # the same batched path top-down only (every layer expands its frontier,
+    # the reference algorithm's work), and with the other ways of marking
+    # children outside the tile (top-down)
     other_marks = {}
-    for name, mode in (("byte_cross_marks", psa.CROSS_BYTES), ("atomic_cross_marks", psa.CROSS_ATOMIC)):
-        if opts.get("cross_marks") == mode:
+    top_down_bytes = None
+    variants = [("top_down", opts.get("cross_marks"))] if opts.get("direction", 0) != 0 else []
+    variants += [("byte_cross_marks", psa.CROSS_BYTES), ("atomic_cross_marks", psa.CROSS_ATOMIC)]
+    for name, mode in variants:
+        if name != "top_down" and opts.get("cross_marks") == mode:
             continue
-        with psa.Bfs(p, index_count=N, graph=1, **dict(opts, cross_marks=mode)) as b:
+        with psa.Bfs(p, index_count=N, graph=1, **dict(opts, cross_marks=mode, direction=psa.DIR_TOP_DOWN)) as b:
             dt_x, counts_x, hashes_x = timed_runs(psa, b, 0, max(1, args.steps // 2), 1)
+            st_x = b.stats()
         assert counts_x == counts and hashes_x == hashes
-        other_marks[name] = {"value": N / dt_x, "ms_per_step": dt_x * 1e3}
+        other_marks[name] = {"value": N / dt_x, "ms_per_step": dt_x * 1e3,
+                             "algorithmic_bytes_per_search": st_x.algorithmic_bytes}
+        if name == "top_down" or (top_down_bytes is None and opts.get("direction", 0) == 0):
+            top_down_bytes = st_x.algorithmic_bytes
         log(f"batched, {name.replace('_', ' ')}: {dt_x * 1e3:.2f} ms/search")
+    if top_down_bytes is None:
+        top_down_bytes = st_b.algorithmic_bytes
 
     plain = None
     if not args.no_plain:
@@ -536,7 +568,8 @@ def main():
         # segment of the 2^20..2^26 range and the plain path's fastest
         plain = {}
         for key, sb in (("", args.segment_bits), ("seg26_", 26)):
-            popts = dict(eager_opts, segment_bits=sb)
+            # the reference's per-segment loop expands every layer top-down
+            popts = dict(eager_opts, segment_bits=sb, direction=psa.DIR_TOP_DOWN)
             with psa.Bfs(p, index_count=N, mode=psa.BFS_PLAIN, **popts) as b:
                 dt_p, counts_p, hashes_p = timed_runs(psa, b, 0, max(1, args.steps // 2), 1)
                 st_p = b.stats()
@@ -580,7 +613,6 @@ def main():
         log("cpu baseline, all cores:", cpu_par)
 
     value = N / dt_b
-    kernel = dominant_kernel(psa, opts)
     traffic = traffic_from_profile(f"{kernel}_pancake{n}_traffic.txt")
     out = {
         "metric": "expanded states/s (batched persistent three-bit BFS, 12-pancake)",
@@ -605,7 +637,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "avg_launch_ms": exp_launch_ms,
                      "algorithmic_bytes_per_launch": exp_bytes_launch},
+        "kernels": kr,
         "whole_search_hbm_frac": st_b.algorithmic_bytes / dt_b / 1e9 / HBM_PEAK_GBS,
+        # the bytes the top-down (reference) algorithm moves for this search,
+        # over the wall time of the search as run
+        "top_down_algorithmic_bytes_per_search": top_down_bytes,
+        "top_down_bytes_hbm_frac": top_down_bytes / dt_b / 1e9 / HBM_PEAK_GBS,
         **other_marks,
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,

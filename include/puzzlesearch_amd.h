@@ -501,6 +501,10 @@ typedef struct ps_bfs_stats {
     uint64_t apply_launches;      /* batched PS_CROSS_LISTS runs one fused kernel per layer, counted
                                      in expand_launches (apply and finalize launches stay 0) */
     uint64_t list_entries;        /* cross-tile list entries written (PS_CROSS_LISTS), all layers */
+    double pull_ms;               /* summed event time of the bottom-up kernel of batched PS_CROSS_LISTS
+                                     runs (k_pkl_pull; its launches are not in expand_launches) */
+    uint64_t pull_launches;
+    uint64_t pull_bytes;          /* algorithmic bytes of those launches (expand_bytes excludes them) */
 } ps_bfs_stats;
 
 void ps_bfs_options_default(ps_bfs_options* o);

@@ -82,7 +82,8 @@ class BfsStats(ctypes.Structure):
                 ("finalize_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("pull_words", ctypes.c_uint64), ("pull_probes", ctypes.c_uint64),
                 ("apply_ms", ctypes.c_double), ("apply_launches", ctypes.c_uint64),
-                ("list_entries", ctypes.c_uint64)]
+                ("list_entries", ctypes.c_uint64), ("pull_ms", ctypes.c_double),
+                ("pull_launches", ctypes.c_uint64), ("pull_bytes", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

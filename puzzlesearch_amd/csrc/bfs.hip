@@ -245,25 +245,37 @@ int launch_batched_layer(ps_bfs* b, bool timed) {
     if (b->fused) {
         if (!b->primed)
             if (int r = prime_fused(b)) return r;
-        size_t f0 = 0, f1 = 0;
+        size_t f0 = 0, fm = 0, f1 = 0;
         if (timed)
             if (int r = record(b, &f0)) return r;
         LaunchCfg cs{b->tpb_expand, b->grid_step, b->lds_step, b->stream};
         if (int r = pkl_step(b->puzzle, step_args(b, false), cs, nullptr)) return r;
+        b->stats.expand_launches++;
         if (b->visited) {
             // a split layer: its expansion top-down (role 1) or bottom-up; each
             // of the two exits at once unless the device chose it
             psk::StepArgs a1 = step_args(b, false);
             a1.role = 1;
             if (int r = pkl_step(b->puzzle, a1, cs, nullptr)) return r;
+            b->stats.expand_launches++;
+            if (timed)
+                if (int r = record(b, &fm)) return r;
             LaunchCfg cp{b->tpb_expand, b->grid_pull, b->lds_pull, b->stream};
             if (int r = pkl_pull(b->puzzle, pull_args(b), cp, nullptr)) return r;
+            b->stats.pull_launches++;
         }
         if (timed) {
             if (int r = record(b, &f1)) return r;
-            if (b->collect_kernel_times) b->expand_spans.push_back({f0, f1});
+            if (b->collect_kernel_times) {
+                // k_pkl_step launches and the k_pkl_pull launch timed apart
+                if (b->visited) {
+                    b->expand_spans.push_back({f0, fm});
+                    b->pull_spans.push_back({fm, f1});
+                } else {
+                    b->expand_spans.push_back({f0, f1});
+                }
+            }
         }
-        b->stats.expand_launches++;
         return launch_layer_end(b);
     }
     size_t e0 = 0, e1 = 0, e2 = 0, ea = 0;
@@ -637,12 +649,23 @@ int ps::collect_stats(ps_bfs* b) {
         PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
         ams += ms;
     }
+    double pms = 0;
+    for (auto& sp : b->pull_spans) {
+        float ms = 0;
+        PS_HIP_CHECK(hipEventElapsedTime(&ms, b->ev_pool[sp.first], b->ev_pool[sp.second]));
+        pms += ms;
+    }
     b->stats.expand_ms = ems;
     b->stats.finalize_ms = fms;
     b->stats.apply_ms = ams;
+    b->stats.pull_ms = pms;
     b->stats.list_entries = b->lists ? st.expanded * (uint64_t)(b->puzzle.size - 9) : 0;
-    // the fused kernel also finalizes: its bytes are the whole search's
-    if (b->fused) b->stats.expand_bytes = b->stats.algorithmic_bytes;
+    // the fused kernel also finalizes: its bytes are the whole search's but
+    // for the bottom-up layers of k_pkl_pull
+    if (b->fused) {
+        b->stats.pull_bytes = 16ull * st.pull_words + 8ull * st.pull_probes;
+        b->stats.expand_bytes = b->stats.algorithmic_bytes - b->stats.pull_bytes;
+    }
     return PS_OK;
 }
 
@@ -941,6 +964,7 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
     b->expand_spans.clear();
     b->fin_spans.clear();
     b->apply_spans.clear();
+    b->pull_spans.clear();
     b->layers_done = 1;
     b->start_index = start;
     return PS_OK;

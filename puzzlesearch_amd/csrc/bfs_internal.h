@@ -82,7 +82,7 @@ struct ps_bfs {
     hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    std::vector<std::pair<size_t, size_t>> expand_spans, fin_spans, apply_spans;
+    std::vector<std::pair<size_t, size_t>> expand_spans, fin_spans, apply_spans, pull_spans;
     bool collect_kernel_times = true;
     ps_bfs_stats stats{};
     int layers_done = 0;

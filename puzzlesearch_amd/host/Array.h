@@ -1,5 +1,5 @@
 // Fixed-size zero-initialised host array with the interface of the reference's
-// TArray<T> (Common/Array.h:7-70): Size, SetSize (reallocates and zeroes),
+// TArray<T> (Common/Array.h:9-64): Size, SetSize (reallocates and zeroes),
 // operator[], Buf, Clear; movable, not copyable.
 #pragma once
 

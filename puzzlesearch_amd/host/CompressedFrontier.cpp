@@ -272,7 +272,7 @@ void CompressedCrossSegmentReader::SetSegment(int segment) {
     m_LastOp = -1;
 }
 
-// Common/CompressedFrontier.cpp:176-188
+// Common/CompressedFrontier.cpp:155-166
 Buffer<uint32_t>& CompressedCrossSegmentReader::Read(int op) {
     m_IndexBuffer.Clear();
     if (m_Next + 1 >= m_Chunks.size()) {

@@ -1,5 +1,5 @@
 // Compressed segment streams over a Store, with the interface of the
-// reference's Common/CompressedFrontier.h:57-132 and the same bytes: a
+// reference's Common/CompressedFrontier.h:60-124 and the same bytes: a
 // segment's stream is a sequence of FrontierCompression chunks (a bitmap chunk
 // `u32 words | 0x80000000, u32 first, words x u64`, or a StreamVInt chunk `u32
 // count` + 4-value tuples), each of at most 2^18 values, written to the Store
@@ -100,7 +100,7 @@ private:
     fcstream::DeviceBytes m_DevIds, m_Scratch;
 };
 
-// Common/CompressedFrontier.h:106-132: the cross-segment stream of every op
+// Common/CompressedFrontier.h:105-124: the cross-segment stream of every op
 class CompressedCrossSegmentReader {
 public:
     explicit CompressedCrossSegmentReader(StoreSet& storeSet);

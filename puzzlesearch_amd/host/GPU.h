@@ -9,7 +9,7 @@
 // reference's ERR throws with cudaGetErrorString (Puzzles/GPU.cu:5-10).  The
 // library is initialised on the first call (device $PS_DEVICE, default 0) if
 // the caller has not called ps_init, as the CUDA runtime initialises lazily.
-// The reference's __device__ rank/unrank helpers (Puzzles/GPU.h:29-35) have no
+// The reference's __device__ rank/unrank helpers (Puzzles/GPU.h:26-32) have no
 // host counterpart here: the device kernels behind the launchers implement
 // their own ranking.
 #pragma once
