@@ -15,7 +15,7 @@
 //     launch has a time limit;
 //   * each job's layer counts and frontier hashes are compared with the
 //     oracle's InMemoryThreeBitBFS (Common/InMemoryThreeBitBFS.cpp:4-48).
-// Usage: multi_emu [--grid G] [--max-steps S] kind:a:b ...   (kind = pancake,
+// Usage: multi_emu [--grid G] [--max-steps S] [--segment-bits B] kind:a:b ...   (kind = pancake,
 // sliding, hanoi; b = height for sliding).  Exit status 0 = every job equal
 // to the oracle.
 #include "simt_emu.h"
@@ -90,6 +90,8 @@ struct Job {
     int steps = 0;
 };
 
+int g_segment_bits = 22;  // the batch's segment_bits: segments for searches of more states
+
 bool make_job(const std::string& spec, Job* j) {
     char kind[32] = {};
     int a = 0, b = 0;
@@ -113,7 +115,7 @@ bool make_job(const std::string& spec, Job* j) {
     if (j->kind < 0) return false;
     j->odd = or_has_odd_cycles(&j->op);
     // geometry of ps_bfs_create (csrc/bfs.hip) for the fused batch options
-    const int segment_bits = j->nbits > (1ull << 22) ? 22 : 0;
+    const int segment_bits = (g_segment_bits > 0 && j->nbits > (1ull << g_segment_bits)) ? g_segment_bits : 0;
     j->nwords = (j->nbits + 63) / 64;
     if (segment_bits == 0) {
         j->nseg = 1;
@@ -221,6 +223,7 @@ int main(int argc, char** argv) {
         const std::string s = argv[i];
         if (s == "--grid" && i + 1 < argc) grid = std::atoi(argv[++i]);
         else if (s == "--max-steps" && i + 1 < argc) max_steps = std::atoi(argv[++i]);
+        else if (s == "--segment-bits" && i + 1 < argc) g_segment_bits = std::atoi(argv[++i]);
         else specs.push_back(s);
     }
     simt::lds = []() { return reinterpret_cast<unsigned char*>(psk::smem); };
