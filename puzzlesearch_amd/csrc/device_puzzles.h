@@ -281,101 +281,105 @@ struct Sliding {
 
 // ---------------------------------------------------------- sliding (packed)
 
-// SlidingTilePuzzleOptimized<W,H>: tiles[0..size-2] are the non-blank tiles
-// (tile-1) in cell order, tiles[15] the blank cell; packed value of a tile =
-// number of smaller tiles to its left; tiles[0], tiles[1] are implied by parity
-// (Puzzles/SlidingPuzzleGpu.cu:108-207).
+// SlidingTilePuzzleOptimized<W,H>: the index of a state is its tile list (the
+// non-blank tiles in cell order) with each position's packed value -- the
+// number of smaller tiles to its left, in [0, position] -- as mixed-radix
+// digits of positions 2..11, the values of positions 12..14 as the segment's
+// nibbles and the blank cell in the low four bits; positions 0 and 1 are
+// implied by parity (the index format of Puzzles/SlidingPuzzleGpu.cu:142-159
+// and Puzzles/SlidingTilePuzzleOptimized.cpp:79-192).
 template <int W, int H>
 struct Packed {
     static constexpr int OPS = 4;
     static constexpr int SLOTS = 4;
     static constexpr int S = W * H;
 
-    __device__ __forceinline__ static void from_index(uint64_t index, uint32_t (&t)[16]) {
-        const uint32_t seg = (uint32_t)(index >> 32);
+    // The index as 4-bit nibbles: nibble q = packed value of list position q
+    // for q = 2..14 (positions 12..14 are the segment's nibbles), nibbles 0
+    // and 1 zero, nibble 15 the blank cell.
+    __device__ __forceinline__ static uint64_t nibbles(uint64_t index) {
         uint32_t idx = (uint32_t)index;
-        t[12] = seg & 15;
-        t[13] = (seg >> 4) & 15;
-        t[14] = seg >> 8;
-        t[15] = idx % 16;
-        idx /= 16;
-        t[0] = 0;
-        t[1] = 0;
+        uint64_t nib = ((uint64_t)(idx & 15u) << 60) | (((index >> 32) & 0xFFFull) << 48);
+        idx >>= 4;
 #pragma unroll
         for (int i = 2; i < 12; ++i) {
-            t[i] = idx % (uint32_t)(i + 1);
-            idx /= (uint32_t)(i + 1);
+            const uint32_t q = idx / (uint32_t)(i + 1);
+            nib |= (uint64_t)(idx - q * (uint32_t)(i + 1)) << (4 * i);
+            idx = q;
         }
+        return nib;
     }
 
-    __device__ __forceinline__ static uint64_t to_index(const uint32_t (&t)[16]) {
-        const uint64_t seg = t[12] | (t[13] << 4) | (t[14] << 8);
+    __device__ __forceinline__ static uint64_t index_of(uint64_t nib) {
         uint32_t idx = 0;
 #pragma unroll
-        for (int i = 11; i >= 2; --i) idx = idx * (uint32_t)(i + 1) + t[i];
-        return (seg << 32) | (uint64_t)(idx * 16u + t[15]);
+        for (int i = 11; i >= 2; --i) idx = idx * (uint32_t)(i + 1) + (uint32_t)((nib >> (4 * i)) & 15u);
+        return (((nib >> 48) & 0xFFFull) << 32) | (uint64_t)(idx * 16u + (uint32_t)(nib >> 60));
     }
 
-    __device__ __forceinline__ static void unpack(uint32_t (&a)[16]) {
-        uint32_t inv = 0;
-#pragma unroll
-        for (int i = 1; i < S - 1; ++i)
-#pragma unroll
-            for (int j = 0; j < i; ++j) {
-                const uint32_t x = (a[j] >= a[i]) ? 1u : 0u;
-                a[j] += x;
-                inv ^= x;
-            }
-        const bool inv_even = (inv & 1) == 0;
-        const bool row_even = ((a[15] / W) & 1) == 0;
+    // Children of the vertical moves without unpacking the permutation.  A
+    // vertical move rotates W consecutive tile-list positions, so only their
+    // packed values change, and by the relative order of those W tiles alone:
+    // the unpack step applied inside the window to the window's packed values
+    // gives the tiles' ranks within the list prefix ending at the window.
+    // Position 0's packed value is 0 and position 1's is the parity bit that
+    // unpack restores (the number of inversions is sum(i - v_i)).  Slot and
+    // label order as expand: up (slot 0, OP_DOWN), then down (slot 1, OP_UP).
+    template <class F>
+    __device__ __forceinline__ static void vertical(uint64_t index, bool up, bool dn, F&& f) {
+        uint64_t nib = nibbles(index);
+        const int z = (int)(nib >> 60);
+        constexpr uint32_t C = (uint32_t)((S - 2) * (S - 1) / 2 - 1);  // sum of i over 2..S-2
+        const uint32_t inv = (uint32_t)__popcll(nib & 0x0111111111111100ull) + 1u + C;
+        const bool inv_even = (inv & 1u) == 0;
+        const bool row_even = ((z / W) & 1) == 0;
         const bool restore = (W % 2 == 0) ? ((!inv_even) == row_even) : !inv_even;
-        if (restore) {
-            const uint32_t x = a[0];
-            a[0] = a[1];
-            a[1] = x;
+        nib |= (uint64_t)(restore ? 1u : 0u) << 4;
+        constexpr uint64_t WMASK = (W == 16) ? ~0ull : ((1ull << (4 * W)) - 1ull);
+        if (up) {  // the tile at position z-W moves to z-1, the W-1 after it move down one
+            const int p = z - W;
+            const uint64_t win = nib >> (4 * p);
+            uint32_t v[W], R[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) R[k] = v[k] = (uint32_t)((win >> (4 * k)) & 15u);
+#pragma unroll
+            for (int i = 1; i < W; ++i)
+#pragma unroll
+                for (int j = 0; j < i; ++j) R[j] += (R[j] >= R[i]) ? 1u : 0u;
+            uint64_t w = 0;
+            uint32_t below = 0;
+#pragma unroll
+            for (int k = 1; k < W; ++k) {
+                w |= (uint64_t)(v[k] - (R[0] < R[k] ? 1u : 0u)) << (4 * (k - 1));
+                below += (R[k] < R[0]) ? 1u : 0u;
+            }
+            w |= (uint64_t)(v[0] + below) << (4 * (W - 1));
+            uint64_t c = (nib & ~(WMASK << (4 * p))) | (w << (4 * p));
+            c = (c & ~(15ull << 60)) | ((uint64_t)(z - W) << 60);
+            f(0, index_of(c), OP_DOWN);
         }
-    }
-
-    __device__ __forceinline__ static void pack(uint32_t (&a)[16]) {
-        a[0] = 0;
-        a[1] = 0;
+        if (dn) {  // the tile at position z+W-1 moves to z, the W-1 before it move up one
+            const int p = z;
+            const uint64_t win = nib >> (4 * p);
+            uint32_t v[W], R[W];
 #pragma unroll
-        for (int i = S - 2; i > 2; --i)
+            for (int k = 0; k < W; ++k) R[k] = v[k] = (uint32_t)((win >> (4 * k)) & 15u);
 #pragma unroll
-            for (int j = 2; j < i; ++j) a[j] -= (a[j] >= a[i]) ? 1u : 0u;
-    }
-
-    // blank moves up: the tile above (list position z-W) is re-inserted at z-1
-    __device__ __forceinline__ static void rotate_up(uint32_t (&a)[16]) {
-        const int z = (int)a[15];
-        uint32_t b[16];
-        uint32_t moved = 0;
+            for (int i = 1; i < W; ++i)
 #pragma unroll
-        for (int i = 0; i < 15; ++i) moved = (i == z - W) ? a[i] : moved;
+                for (int j = 0; j < i; ++j) R[j] += (R[j] >= R[i]) ? 1u : 0u;
+            uint64_t w = 0;
+            uint32_t below = 0;
 #pragma unroll
-        for (int i = 0; i < 15; ++i) {
-            const uint32_t nxt = (i + 1 < 15) ? a[i + 1] : 0;
-            b[i] = (i >= z - W && i < z - 1) ? nxt : (i == z - 1) ? moved : a[i];
+            for (int k = 0; k < W - 1; ++k) {
+                w |= (uint64_t)(v[k] + (R[W - 1] < R[k] ? 1u : 0u)) << (4 * (k + 1));
+                below += (R[k] < R[W - 1]) ? 1u : 0u;
+            }
+            w |= (uint64_t)(v[W - 1] - below);
+            uint64_t c = (nib & ~(WMASK << (4 * p))) | (w << (4 * p));
+            c = (c & ~(15ull << 60)) | ((uint64_t)(z + W) << 60);
+            f(1, index_of(c), OP_UP);
         }
-#pragma unroll
-        for (int i = 0; i < 15; ++i) a[i] = b[i];
-        a[15] = (uint32_t)(z - W);
-    }
-
-    __device__ __forceinline__ static void rotate_dn(uint32_t (&a)[16]) {
-        const int z = (int)a[15];
-        uint32_t b[16];
-        uint32_t moved = 0;
-#pragma unroll
-        for (int i = 0; i < 15; ++i) moved = (i == z + W - 1) ? a[i] : moved;
-#pragma unroll
-        for (int i = 0; i < 15; ++i) {
-            const uint32_t prv = (i > 0) ? a[i - 1] : 0;
-            b[i] = (i > z && i <= z + W - 1) ? prv : (i == z) ? moved : a[i];
-        }
-#pragma unroll
-        for (int i = 0; i < 15; ++i) a[i] = b[i];
-        a[15] = (uint32_t)(z + W);
     }
 
     // Slots whose child may lie outside the parent's segment of 2^seg_bits
@@ -424,27 +428,7 @@ struct Packed {
         const int blank = (int)(index & 15u);
         const bool up = (want & 1u) && blank >= W && !(op_bits & (1u << OP_UP));
         const bool dn = (want & 2u) && blank < S - W && !(op_bits & (1u << OP_DOWN));
-        if (up || dn) {
-            uint32_t a[16];
-            from_index(index, a);
-            unpack(a);
-            if (up) {
-                uint32_t b[16];
-#pragma unroll
-                for (int i = 0; i < 16; ++i) b[i] = a[i];
-                rotate_up(b);
-                pack(b);
-                f(0, to_index(b), OP_DOWN);
-            }
-            if (dn) {
-                uint32_t b[16];
-#pragma unroll
-                for (int i = 0; i < 16; ++i) b[i] = a[i];
-                rotate_dn(b);
-                pack(b);
-                f(1, to_index(b), OP_UP);
-            }
-        }
+        if (up || dn) vertical(index, up, dn, f);
         if ((want & 4u) && blank % W > 0 && !(op_bits & (1u << OP_LEFT))) f(2, index - 1, OP_RIGHT);
         if ((want & 8u) && blank % W < W - 1 && !(op_bits & (1u << OP_RIGHT))) f(3, index + 1, OP_LEFT);
     }
