@@ -220,8 +220,40 @@ def segment_step_bench(psa, steps, ppm=200000, seed=1234):
                 want = (fx["count"], fx["fr_idx"], fx["fr_ops"], fx["fr_bytes"], fx["buckets"])
                 assert got == want, "segment step outputs differ from the oracle step"
                 parity = True
+        # the same frontier as the disk loop stores it: StreamVInt chunks of
+        # 2^18 ids with the op bits as 4-bit bounds; decoded on the device,
+        # then the same step (checked against the first run's outputs)
+        lib = psa.lib()
+        nf = len(idx)
+        per = 1 << 18
+        nch = (nf + per - 1) // per
+        starts = np.minimum(np.arange(nch + 1, dtype=np.uint64) * np.uint64(per), np.uint64(nf))
+        d_st = psa.DeviceBuffer.from_array(starts)
+        cap = 4 * nch + 17 * nf // 4 + nf // 2 + 4096
+        d_stream = psa.DeviceBuffer(cap)
+        d_soff = psa.DeviceBuffer(8 * (nch + 1))
+        psa.check(lib.ps_svint_encode(d_idx.ptr, d_ops.ptr, d_st.ptr, nch, d_stream.ptr, cap, d_soff.ptr, None))
+        stream_bytes = int(d_soff.download(np.uint64, nch + 1)[-1])
+        d_idx2 = psa.DeviceBuffer(4 * nf + 64)
+        d_ops2 = psa.DeviceBuffer(nf + 64)
+        d_vs = psa.DeviceBuffer(8 * (nch + 1))
+        ref_count = r["count"]
+        swall = []
+        for _ in range(steps):
+            psa.sync()
+            t0 = time.perf_counter()
+            psa.check(lib.ps_svint_decode(d_stream.ptr, d_soff.ptr, nch, 1, d_idx2.ptr, d_ops2.ptr, nf, d_vs.ptr,
+                                          None))
+            res2 = w.step_device(seg, d_idx2.ptr, d_ops2.ptr, nf)
+            swall.append((time.perf_counter() - t0) * 1e3)
+        assert res2.as_dict()["count"] == ref_count
+        if parity:
+            o2 = w.outputs(res2)
+            assert (sha(o2["fr_idx"]), sha(o2["fr_ops"]), sha(o2["fr_bytes"])) == (fx["fr_idx"], fx["fr_ops"],
+                                                                                   fx["fr_bytes"])
     ms = sum(wall) / len(wall)
     dms = sum(dev) / len(dev)
+    sms = sum(swall) / len(swall)
     return {"workload": f"15-puzzle segment {seg}, blank cell {blank}, frontier Bernoulli(p={ppm / 1e6}) "
                         f"over {W.FIFTEEN_M} ids, seed {seed}",
             "parents": len(idx), "value": len(idx) / (ms * 1e-3), "unit": "expanded states/s",
@@ -230,7 +262,10 @@ def segment_step_bench(psa, steps, ppm=200000, seed=1234):
             "hbm_frac_device": r["algorithmic_bytes"] / (dms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "count": r["count"], "frontier_count": r["frontier_count"], "frontier_bytes": r["frontier_bytes"],
             "children_in": r["children_in"], "children_cross": r["children_cross"], "buckets": r["buckets"],
-            "cross_bytes": r["cross_bytes"], "parity_with_oracle_step": parity}
+            "cross_bytes": r["cross_bytes"], "parity_with_oracle_step": parity,
+            "from_stream": {"input": f"StreamVInt chunks of 2^18 ids with 4-bit op bounds, {stream_bytes} bytes",
+                            "ms_per_step": sms, "value": len(idx) / (sms * 1e-3),
+                            "decode_and_step_parity": parity}}
 
 
 TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256], "grid_blocks": [0, 256, 1024],
@@ -296,7 +331,8 @@ def profile_set(psa, name, runs):
 
 def codec_bench(psa, n_chunks=256, reps=5, seed=7):
     """Device-resident codec throughput on frontier-like chunks of 2^18 sorted
-    ids (the reference's StreamVInt chunk size): StreamVInt with 4-bit bounds,
+    ids (the reference's StreamVInt chunk size): StreamVInt with 4-bit bounds
+    (gaps of mean 16, and of mean 160 for tuples of mixed lengths),
     FrontierCompression (gaps of mean 16: StreamVInt chunks; mean 3: bitmap
     chunks) and OpBits4.  Each figure is bytes in + bytes out over the call's
     wall time (the encoders read chunk sizes back to the host once per call);
@@ -347,6 +383,20 @@ def codec_bench(psa, n_chunks=256, reps=5, seed=7):
                     "encode_GBps": moved / t_enc / 1e9, "decode_GBps": moved / t_dec / 1e9,
                     "encode_hbm_frac": moved / t_enc / 1e9 / HBM_PEAK_GBS,
                     "decode_hbm_frac": moved / t_dec / 1e9 / HBM_PEAK_GBS}
+
+    # StreamVInt with bounds, gaps of mean 160: about a fifth of the deltas take two bytes, so tuples of
+    # 5..9 bytes mix and every chunk takes the decoder's descriptor walk (at mean 16 every tuple has
+    # five bytes and the decoder indexes them directly)
+    vals = frontier(160)
+    d_v.upload(vals)
+    enc = lambda: psa.check(lib.ps_svint_encode(d_v.ptr, d_b.ptr, d_st.ptr, n_chunks, d_out.ptr, cap, d_off.ptr, None))
+    t_enc = timed(enc)
+    nbytes = int(d_off.download(np.uint64, n_chunks + 1)[-1])
+    t_dec = timed(dec)
+    assert np.array_equal(d_dec.download(np.uint32, n), vals) and np.array_equal(d_decb.download(np.uint8, n), bounds)
+    moved = 5 * n + nbytes
+    out["svint_mixed"] = {"stream_bytes": nbytes, "encode_ms": t_enc * 1e3, "decode_ms": t_dec * 1e3,
+                          "encode_GBps": moved / t_enc / 1e9, "decode_GBps": moved / t_dec / 1e9}
 
     # FrontierCompression, StreamVInt-coded (mean gap 16) and bitmap-coded (mean gap 3) chunks
     for name, gap in (("fc_svint", 16), ("fc_bitmap", 3)):

@@ -735,11 +735,16 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
 // arithmetic gives those children as global ranks).  A state with a
 // neighbour in the frontier is in the next layer (the graph has no shorter
 // path to it); its bit goes to an LDS window stored into the tile's next
-// words, which nothing else writes.  No lists are filled.
+// words, which nothing else writes.  No lists are filled.  Each state is
+// first tested against its flips of 2..6 elements, whose tile offsets the
+// six-position table row gives directly; only the states none of them reaches
+// (a fraction of them in the dense layers a split expands) are queued per wave
+// for the full test, which computes the flips of 7..9 elements by the sweep.
+constexpr int PKL_QB = 128;  // per-wave queue of states the table check leaves to the full check
 template <int N>
 inline size_t pkl_pull_lds(int tpb) {
     return (size_t)2 * pkl::kTileWords * 4 + (size_t)pkl::kTileWords * 8 + PKL_TAB_BYTES +
-           (size_t)(tpb / 64) * PKL_QCAP * 4;
+           (size_t)(tpb / 64) * (PKL_QCAP + PKL_QB) * 4;
 }
 
 template <int N, int TPB>
@@ -758,6 +763,7 @@ __global__ __launch_bounds__(TPB) void k_pkl_pull(PullArgs a) {
     uint32_t* tab6lo = reinterpret_cast<uint32_t*>(tab6 + 720);
     uint32_t* tab3 = tab6lo + 720;
     uint32_t* q = tab3 + 504 + wave * PKL_QCAP;
+    uint32_t* qb = tab3 + 504 + NW * PKL_QCAP + wave * PKL_QB;
     const uint32_t L = st->layer;
     const uint64_t* front = a.bits[(L + 1) % 3];             // the layer just finalized
     uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 2) % 3]);  // cleared by that launch
@@ -804,6 +810,46 @@ __global__ __launch_bounds__(TPB) void k_pkl_pull(PullArgs a) {
                 any = true;
             }
         };
+        // the table check: flips of 2..6 elements; a state it leaves goes to
+        // the wave's second queue, which is drained 64 states at a time
+        auto quick = [&](uint32_t off) {
+            const uint32_t h = off / 720u;
+            const uint32_t l = off - h * 720u;
+            const pkl::Tab6 t = tab6[l];
+            const uint32_t b6 = off - l;
+            const uint32_t c[5] = {b6 + (t.c234 & 1023u), b6 + ((t.c234 >> 10) & 1023u), b6 + ((t.c234 >> 20) & 1023u),
+                                   b6 + (t.c56 & 1023u), b6 + ((t.c56 >> 10) & 1023u)};
+            bool found = false;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) found |= ((flds[c[k] >> 6] >> (c[k] & 63u)) & 1ull) != 0ull;
+            return found;
+        };
+        int qbn = 0;  // wave-uniform fill of qb
+        auto run_queue = [&](uint32_t n) {
+            for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                const bool has = j < n;
+                const uint32_t off = has ? q[j] : 0u;
+                const bool found = has && quick(off);
+                if (found) {
+                    lor(&win[off >> 5], 1u << (off & 31));
+                    any = true;
+                }
+                const bool rest = has && !found;
+                const unsigned long long m = __ballot(rest);
+                if (rest) qb[qbn + __popcll(m & ((1ull << lane) - 1ull))] = off;
+                qbn += __popcll(m);
+                if (qbn >= 64) {
+                    wave_sync();
+                    probe(qb[lane]);
+                    const uint32_t tail = (uint32_t)(qbn - 64) > (uint32_t)lane ? qb[64 + lane] : 0u;
+                    wave_sync();
+                    if ((uint32_t)(qbn - 64) > (uint32_t)lane) qb[lane] = tail;
+                    qbn -= 64;
+                    wave_sync();
+                }
+            }
+        };
         // the tile's unvisited states, compacted per wave as the expansion does
         int qn = 0;
         for (uint64_t g = (uint64_t)wave * 64; g < pkl::kTileWords; g += (uint64_t)NW * 64) {
@@ -825,7 +871,7 @@ __global__ __launch_bounds__(TPB) void k_pkl_pull(PullArgs a) {
                     break;
                 }
                 wave_sync();
-                for (uint32_t j = lane; j < (uint32_t)PKL_QCAP; j += 64) probe(q[j]);
+                run_queue((uint32_t)PKL_QCAP);
                 wave_sync();
                 off -= PKL_QCAP;
                 end_all -= PKL_QCAP;
@@ -834,7 +880,9 @@ __global__ __launch_bounds__(TPB) void k_pkl_pull(PullArgs a) {
             }
         }
         wave_sync();
-        for (uint32_t j = lane; j < (uint32_t)qn; j += 64) probe(q[j]);
+        run_queue((uint32_t)qn);
+        wave_sync();
+        if ((uint32_t)lane < (uint32_t)qbn) probe(qb[lane]);
         if (any) s_any = 1;
         __syncthreads();
         uint32_t* dst = nxt + w0 * 2;

@@ -9,10 +9,9 @@
 // Values come in chunks (chunk c = values[starts[c], starts[c+1])), each encoded
 // independently and concatenated, as a writer appends encoded chunks to a
 // segment file.  Encoding is parallel per 4-value tuple: tuple sizes, a scan
-// per chunk for byte offsets, then the writes.  Decoding first walks each
-// chunk's tuple descriptors (one thread per chunk: a descriptor's position
-// depends on every earlier tuple's size), then decodes tuples in parallel and
-// undoes the delta coding with a scan per chunk.
+// per chunk for byte offsets, then the writes.  Decoding is one pass per chunk
+// (k_svi_decode_chunk): LDS-staged tiles, a parallel walk of the tuple
+// descriptors composed by a block scan, the deltas undone by a second scan.
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -210,181 +209,354 @@ __global__ void k_read_counts(const uint8_t* in, const uint64_t* in_off, uint32_
     }
 }
 
-// Descriptor walk of one chunk per block, in parallel: a chunk's tuples
-// (descriptor byte + 1..4 value bytes each, 5..17 bytes) tile its payload
-// [4, E), E = chunk bytes - bound bytes.  Thread t owns payload bytes
-// [s0, s1); the first tuple starting at or after s0 starts at s0 + o,
-// o in [0, 16].  Each thread walks its range from all 17 entry offsets at once
-// (independent loads), recording where each walk leaves the range and how many
-// tuples it passed; thread 0 chains the ranges from entry 0; then every thread
-// walks its range once more from its true entry, writing the tuple offsets.
-// A chunk whose tuples do not end exactly at E counts in *err.
-template <int TPB>
-__global__ __launch_bounds__(TPB) void k_svi_walk_par(const uint8_t* in, const uint64_t* in_off,
-                                                      const uint64_t* tuple_base, const uint32_t* select,
-                                                      int with_bounds, uint32_t* tuple_off, uint32_t* payload_end,
-                                                      uint32_t* err) {
-    constexpr int NE = 17;
-    const uint32_t k = blockIdx.x;
-    const uint32_t c = select ? select[k] : k;
+// One pass per chunk: a block decodes one StreamVInt chunk, its payload
+// [4, E) (E = chunk bytes - bound bytes) in tiles of SVD_TILE bytes staged in
+// LDS (with SVD_LOOK bytes past the tile, where a tuple starting in the tile
+// may end).  A tuple is a descriptor byte and four values of 1..4 bytes
+// (5..17 bytes), so where the tuples of a stretch of bytes start depends on
+// every earlier descriptor:
+//  1 thread t owns the tile's bytes [ts + t*B, ts + (t+1)*B) and walks them from
+//    all 17 offsets at which the first tuple starting in them can begin,
+//    recording for each where the walk leaves its bytes (0..16 past their end)
+//    and how many tuples start in them: a map entry offset -> (exit, tuples);
+//  2 a block scan composes the maps (Hillis-Steele over the threads, 17
+//    entries each), so every thread learns, from the tile's entry offset, the
+//    offset of its first tuple and the tuple's index in the tile;
+//  3 every thread decodes its tuples' deltas into LDS, a block scan of the
+//    per-thread sums undoes the delta coding (the running value carries from
+//    tile to tile), and the tile's values are stored with coalesced writes;
+//  4 the last tile must end exactly at E, after exactly T = ceil(count/8)*2
+//    tuples; a chunk that does not (or whose payload length no T tuples can
+//    fill) is refused and counted in *err, its values undefined.
+constexpr int SVD_TPB = 512;
+constexpr uint32_t SVD_B = 32;                        // payload bytes per thread and tile
+constexpr uint32_t SVD_TILE = SVD_B * SVD_TPB;        // 16 KB
+constexpr uint32_t SVD_LOOK = 16;                     // bytes past the tile a tuple can reach
+constexpr uint32_t SVD_NE = 17;                       // entry offsets 0..16
+constexpr uint32_t SVD_MAXT = (SVD_TILE + 4) / 5;     // tuples starting in one tile, at most
+// staged bytes are laid out with a pad word after every 8 words, so the
+// 32-byte stretches of consecutive threads start in different LDS banks
+constexpr uint32_t SVD_STAGE_WORDS = (SVD_TILE + SVD_LOOK + 4) / 4;
+constexpr uint32_t SVD_PAD_WORDS = (SVD_STAGE_WORDS + 7) / 8 * 9 + 1;
+constexpr size_t SVD_LDS = (size_t)SVD_PAD_WORDS * 4 + (size_t)2 * SVD_TPB * SVD_NE * 4 + (size_t)SVD_MAXT * 16;
+
+__device__ __forceinline__ uint32_t svd_pw(uint32_t w) { return (w >> 3) * 9u + (w & 7u); }
+__device__ __forceinline__ uint32_t svd_pb(uint32_t b) { return (b >> 5) * 36u + (b & 31u); }
+
+__device__ __forceinline__ uint32_t svd_len(uint32_t d) {
+    // 5 + the four 2-bit length fields
+    return 5u + (uint32_t)__popc(d & 0x55u) + 2u * (uint32_t)__popc(d & 0xAAu);
+}
+
+// exclusive block sum of v over SVD_TPB threads; *total receives the sum
+__device__ __forceinline__ uint32_t svd_block_excl(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    constexpr int NW = SVD_TPB / 64;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint32_t incl = wave_incl_scan_u32(v, lane);
+    __syncthreads();
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const uint32_t x = s_w[w];
+        before += (w < wave) ? x : 0u;
+        tot += x;
+    }
+    *total = tot;
+    return before + incl - v;
+}
+
+__global__ __launch_bounds__(SVD_TPB) void k_svi_decode_chunk(const uint8_t* in, const uint64_t* in_off,
+                                                               const uint32_t* select, const uint64_t* value_starts,
+                                                               int with_bounds, uint32_t* out, uint8_t* bounds_out,
+                                                               uint32_t* err) {
+    extern __shared__ uint32_t svd_smem[];
+    uint8_t* buf = reinterpret_cast<uint8_t*>(svd_smem);  // staged bytes, from an aligned address
+    uint32_t* map0 = svd_smem + SVD_PAD_WORDS;
+    uint32_t* map1 = map0 + SVD_TPB * SVD_NE;
+    uint32_t* vals = map1 + SVD_TPB * SVD_NE;
+    __shared__ uint32_t s_w[SVD_TPB / 64];
+    __shared__ uint32_t s_bad;
+    const uint32_t c = select[blockIdx.x];
     const uint8_t* p = in + in_off[c];
     const uint64_t clen = in_off[c + 1] - in_off[c];
-    const uint64_t T = tuple_base[k + 1] - tuple_base[k];
-    const uint64_t bnd = with_bounds ? T * 2 : 0;  // (T*4 padded values) / 2
-    const uint64_t E = clen >= 4 + bnd ? clen - bnd : 4;
-    // T tuples take 5T..17T bytes: any other payload length is refused
-    // before it is walked (this also bounds the per-range tuple counts below)
-    if (E - 4 < 5 * T || E - 4 > 17 * T) {
-        if (threadIdx.x == 0) atomicAdd(err, 1u);
+    const int tid = threadIdx.x;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    if (clen < 4) {
+        if (tid == 0) atomicAdd(err, 1u);
         return;
     }
-    const uint64_t L = (E - 4 + TPB - 1) / TPB;
-    const uint64_t s0 = 4 + (uint64_t)threadIdx.x * L;
-    const uint64_t s1 = s0 + L < E ? s0 + L : (s0 < E ? E : s0);
-    __shared__ uint8_t s_ex[TPB][NE];
-    __shared__ uint16_t s_n[TPB][NE];  // tuples of a range: at most its bytes / 5
-    __shared__ uint32_t s_entry[TPB], s_base[TPB];
-    {
-        uint64_t pos[NE];
-        uint32_t n[NE];
-#pragma unroll
-        for (int o = 0; o < NE; ++o) {
-            pos[o] = s0 + (uint64_t)o;
-            n[o] = 0;
-        }
-        // all 17 descriptor loads of a step are issued before any is used; a
-        // walk that has left the range reads byte 0 of the chunk instead
-        // (a selected chunk is not empty) and does not move
-        bool any = s0 < s1;
-        while (any) {
-            uint32_t d[NE];
-#pragma unroll
-            for (int o = 0; o < NE; ++o) d[o] = p[pos[o] < s1 ? pos[o] : 0];
-            any = false;
-#pragma unroll
-            for (int o = 0; o < NE; ++o) {
-                const bool live = pos[o] < s1;
-                const uint32_t x = d[o];
-                pos[o] += live ? 5 + (x & 3) + ((x >> 2) & 3) + ((x >> 4) & 3) + ((x >> 6) & 3) : 0;
-                n[o] += live ? 1u : 0u;
-                any |= live;
-            }
-        }
-#pragma unroll
-        for (int o = 0; o < NE; ++o) {
-            s_ex[threadIdx.x][o] = (uint8_t)(pos[o] - s1);  // 0..16
-            s_n[threadIdx.x][o] = (uint16_t)n[o];
-        }
+    const uint32_t count = rd32(p);
+    const uint64_t T = (uint64_t)((count + 7u) / 8u) * 2u;  // tuples: count padded to a multiple of 8, 4 per tuple
+    const uint64_t bnd = with_bounds ? T * 2 : 0;
+    const uint64_t E = clen >= 4 + bnd ? clen - bnd : 4;
+    if (clen < 4 + bnd || E - 4 < 5 * T || E - 4 > 17 * T) {
+        if (tid == 0) atomicAdd(err, 1u);
+        return;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t o = 0;
-        uint64_t base = 0;
-        for (int t = 0; t < TPB; ++t) {
-            s_entry[t] = o;
-            s_base[t] = (uint32_t)base;
-            base += s_n[t][o];
-            o = s_ex[t][o];
-        }
-        payload_end[k] = (uint32_t)E;
-        if (o != 0 || base != T || clen < 4 + bnd) atomicAdd(err, 1u);
-    }
-    __syncthreads();
-    uint64_t pos = s0 + s_entry[threadIdx.x];
-    uint64_t i = s_base[threadIdx.x];
-    uint32_t* toff = tuple_off + tuple_base[k];
-    while (pos < s1 && i < T) {
-        toff[i++] = (uint32_t)pos;
-        const uint32_t d = p[pos];
-        pos += 5 + (d & 3) + ((d >> 2) & 3) + ((d >> 4) & 3) + ((d >> 6) & 3);
-    }
-}
-
-// grid2d: block row y decodes selected chunk y (no search for a tuple's chunk)
-__global__ void k_svi_decode_tuples(const uint8_t* in, const uint64_t* in_off, const uint64_t* tuple_base,
-                                    const uint32_t* select, uint32_t nsel, uint64_t ntuples, int grid2d,
-                                    const uint32_t* tuple_off, uint32_t* deltas) {
-    uint64_t t;
-    uint32_t k;
-    if (grid2d) {
-        k = blockIdx.y;
-        t = tuple_base[k] + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (t >= tuple_base[k + 1]) return;
-    } else {
-        t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if (t >= ntuples) return;
-        k = chunk_of(tuple_base, nsel, t);
-    }
-    const uint32_t c = select ? select[k] : k;
-    const uint8_t* p = in + in_off[c] + tuple_off[t];
-    const uint32_t desc = p[0];
-    int pos = 1;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const uint32_t len = 1 + ((desc >> (2 * j)) & 3);
-        uint32_t v = 0;
-        for (uint32_t b = 0; b < len; ++b) v |= (uint32_t)p[pos++] << (8 * b);
-        deltas[t * 4 + j] = v;
-    }
-}
-
-// inclusive scan of one chunk's deltas (one block per chunk), first `count`
-// results to the output; bounds decoded alongside.  The chunk is scanned in
-// tiles of 4*TPB deltas: each thread loads four consecutive deltas as one
-// 16-byte vector (coalesced), the block scans the per-thread sums, and the
-// running total carries from tile to tile.
-template <int TPB>
-__global__ __launch_bounds__(TPB) void k_svi_finish(const uint32_t* deltas, const uint64_t* tuple_base,
-                                                    const uint32_t* counts, const uint32_t* select,
-                                                    const uint64_t* value_starts, uint32_t* out,
-                                                    const uint8_t* in, const uint64_t* in_off,
-                                                    const uint32_t* payload_end, uint8_t* bounds_out) {
-    constexpr int NW = TPB / 64;
-    const uint32_t k = blockIdx.x;
-    const uint32_t c = select ? select[k] : k;
-    const uint64_t t0 = tuple_base[k], t1 = tuple_base[k + 1];  // one tuple = four deltas, 16-byte aligned
-    const uint32_t count = counts[c];
     uint32_t* o = out + value_starts[c];
-    const uint4* d4 = reinterpret_cast<const uint4*>(deltas);
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    __shared__ uint32_t s_w[2][NW];
-    uint32_t carry = 0, par = 0;
-    for (uint64_t q0 = t0; q0 < t1; q0 += TPB, par ^= 1u) {
-        const uint64_t q = q0 + threadIdx.x;
-        uint4 v = q < t1 ? d4[q] : make_uint4(0u, 0u, 0u, 0u);
-        v.y += v.x;
-        v.z += v.y;
-        v.w += v.z;
-        const uint32_t incl = wave_incl_scan_u32(v.w, lane);
-        if (lane == 63) s_w[par][wave] = incl;
-        __syncthreads();
-        // s_w[par] is written again two tiles later, after the next tile's
-        // barrier, which every thread reaches only once done here
-        uint32_t before = 0, total = 0;
+    uint32_t entry = 0;   // offset of the tile's first tuple past the tile start
+    uint64_t tbase = 0;   // tuples before the tile
+    uint32_t carry = 0;   // running value before the tile
+    // the staged words of a tile: [ts, min(te + LOOK, clen)) from the aligned
+    // address below ts, bytes outside the chunk as 0; the next tile's words
+    // are loaded into registers while the current tile is decoded
+    constexpr uint32_t NWORDS = SVD_STAGE_WORDS;
+    constexpr uint32_t WPT = (NWORDS + SVD_TPB - 1) / SVD_TPB;
+    auto load_tile = [&](uint64_t ts, uint32_t (&r)[WPT]) {
+        const uint64_t te = ts + SVD_TILE < E ? ts + SVD_TILE : E;
+        const uint64_t se = te + SVD_LOOK < clen ? te + SVD_LOOK : clen;
+        const uintptr_t lo = reinterpret_cast<uintptr_t>(p + ts), hi = reinterpret_cast<uintptr_t>(p + se);
+        const uintptr_t a0 = lo & ~(uintptr_t)3;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            const uint32_t x = s_w[par][w];
-            before += (w < wave) ? x : 0u;
-            total += x;
+        for (uint32_t k = 0; k < WPT; ++k) {
+            const uint32_t w = tid + k * SVD_TPB;
+            const uintptr_t a = a0 + (uintptr_t)w * 4;
+            uint32_t x = 0;
+            if (w < NWORDS && a + 4 <= hi && a >= lo) {
+                x = *reinterpret_cast<const uint32_t*>(a);
+            } else if (w < NWORDS) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if (a + b >= lo && a + b < hi) x |= (uint32_t)*reinterpret_cast<const uint8_t*>(a + b) << (8 * b);
+            }
+            r[k] = x;
         }
-        const uint32_t base = carry + before + incl - v.w;
-        const uint64_t j = (q - t0) * 4;
-        if (j + 3 < count) {
-            o[j] = base + v.x;
-            o[j + 1] = base + v.y;
-            o[j + 2] = base + v.z;
-            o[j + 3] = base + v.w;
-        } else if (q < t1) {
-            const uint32_t r[4] = {v.x, v.y, v.z, v.w};
-            for (int e = 0; e < 4; ++e)
-                if (j + e < count) o[j + e] = base + r[e];
+    };
+    uint32_t staged[WPT];
+    // Every delta after the first of one byte (dense frontiers; the first is
+    // the chunk's first value): when the payload is the first tuple and then
+    // exactly 5 bytes per tuple, and those descriptors are all 0, tuple t >= 1
+    // starts at 4 + L0 + 5(t-1), so no walk is needed.  Each step stages
+    // 512 x 6 tuples (15360 bytes) and every thread decodes six; a nonzero
+    // descriptor sends the chunk to the walk below, which rewrites its values.
+    const uint32_t L0 = svd_len(p[4]);
+    if (4 + (uint64_t)L0 <= E && E - 4 - L0 == 5 * (T - 1)) {
+        constexpr uint32_t K = 6;
+        constexpr uint64_t STEP = (uint64_t)SVD_TPB * K;
+        const uint64_t T1 = T - 1;
+        uint32_t carry5 = 0;
+        {  // the first tuple, read directly
+            const uint32_t d = p[4];
+            uint32_t k = 5, run = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t len = 1u + ((d >> (2 * j)) & 3u);
+                uint32_t x = 0;
+                for (uint32_t i = 0; i < len; ++i) x |= (uint32_t)p[k + i] << (8 * i);
+                k += len;
+                run += x;
+                if (tid == 0 && (uint32_t)j < count) o[j] = run;
+            }
+            carry5 = run;
         }
-        carry += total;
+        bool bad = false;
+        load_tile(4 + L0, staged);
+        for (uint64_t t0 = 0; t0 < T1; t0 += STEP) {
+            const uint64_t ts = 4 + L0 + 5 * t0;
+            const uint32_t sh = (uint32_t)(reinterpret_cast<uintptr_t>(p + ts) & 3u);
+#pragma unroll
+            for (uint32_t k = 0; k < WPT; ++k) {
+                const uint32_t w = tid + k * SVD_TPB;
+                if (w < NWORDS) svd_smem[svd_pw(w)] = staged[k];
+            }
+            __syncthreads();
+            if (t0 + STEP < T1) load_tile(4 + L0 + 5 * (t0 + STEP), staged);
+            // my six tuples: bytes [30 tid, 30 tid + 30) of the staged step
+            const uint32_t b0 = sh + 30u * tid;
+            uint32_t wv[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) wv[k] = svd_smem[svd_pw((b0 >> 2) + k)];
+            const uint32_t r8 = (b0 & 3u) * 8u;
+            uint32_t v[K * 4];
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < K; ++j) {
+                // bytes 5j .. 5j+4 of my stretch: descriptor, then four one-byte deltas
+                const uint32_t o0 = r8 + 40u * j;                    // bit offset of the descriptor
+                const uint64_t lo = (uint64_t)wv[o0 >> 5] | ((uint64_t)wv[(o0 >> 5) + 1] << 32);
+                const uint64_t x = lo >> (o0 & 31u);                // descriptor in bits 0..7
+                const bool live = t0 + (uint64_t)tid * K + j < T1;
+                bad |= live && (x & 255u) != 0u;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t d = live ? (uint32_t)(x >> (8 * (e + 1))) & 255u : 0u;
+                    sum += d;
+                    v[j * 4 + e] = sum;
+                }
+            }
+            uint32_t tot;
+            const uint32_t base = carry5 + svd_block_excl(sum, s_w, &tot);
+            const uint64_t j0 = (1 + t0 + (uint64_t)tid * K) * 4;
+#pragma unroll
+            for (uint32_t e = 0; e < K * 4; ++e)
+                if (j0 + e < count) o[j0 + e] = base + v[e];
+            carry5 += tot;
+            __syncthreads();  // the staged bytes are rewritten by the next step
+        }
+        if (bad) s_bad = 1;
+        __syncthreads();
+        if (!s_bad) {
+            if (bounds_out) {
+                const uint8_t* q = p + E;
+                uint8_t* bo = bounds_out + value_starts[c];
+                for (uint64_t g = tid; g * 8 < count; g += SVD_TPB) {
+                    const uint64_t stored = rd32(q + g * 4);
+                    const uint64_t all = (stored & 0x0F0F0F0Full) | ((stored & 0xF0F0F0F0ull) << 28);
+                    for (int e = 0; e < 8; ++e) {
+                        const uint64_t idx = g * 8 + e;
+                        if (idx < count) bo[idx] = (uint8_t)(all >> (8 * e));
+                    }
+                }
+            }
+            return;
+        }
+        __syncthreads();
+        if (tid == 0) s_bad = 0;
+        __syncthreads();
     }
+    load_tile(4, staged);
+    for (uint64_t ts = 4; ts < E; ts += SVD_TILE) {
+        const uint64_t te = ts + SVD_TILE < E ? ts + SVD_TILE : E;
+        const uint32_t sh = (uint32_t)(reinterpret_cast<uintptr_t>(p + ts) & 3u);
+#pragma unroll
+        for (uint32_t k = 0; k < WPT; ++k) {
+            const uint32_t w = tid + k * SVD_TPB;
+            if (w < NWORDS) svd_smem[svd_pw(w)] = staged[k];
+        }
+        __syncthreads();
+        if (te < E) load_tile(te, staged);
+        // 1 the walk map of this thread's bytes [rs, re), positions relative to ts
+        const uint32_t tl = (uint32_t)(te - ts);
+        const uint32_t rs = tid * SVD_B < tl ? tid * SVD_B : tl;
+        const uint32_t re = rs + SVD_B < tl ? rs + SVD_B : tl;
+        // A tile whose tuples from the entry on are all of 5 bytes (every
+        // descriptor on the chain entry, entry+5, .. is 0) needs no walk: the
+        // block checks the bytes of the entry's residue class mod 5 at or
+        // after the entry.
+        __shared__ uint32_t s_nz;
+        if (tid == 0) s_nz = 0u;
+        __syncthreads();
+        {
+            const uint32_t r5 = entry % 5u;
+            bool nz = false;
+            for (uint32_t i = rs + (r5 + 5u - rs % 5u) % 5u; i < re; i += 5u)
+                nz |= i >= entry && buf[svd_pb(sh + i)] != 0u;
+            if (__ballot(nz) != 0ull && (tid % 64) == 0) s_nz = 1u;
+        }
+        __syncthreads();
+        const bool uniform = entry < tl && s_nz == 0u;
+        uint32_t my_entry, my_t, tile_tuples, next_entry;
+        if (uniform) {
+            // chain positions entry + 5k: the first at or after rs, and how many come before it
+            const uint32_t k0 = rs > entry ? (rs - entry + 4u) / 5u : 0u;
+            my_entry = entry + 5u * k0 - rs;
+            my_t = k0;
+            tile_tuples = (tl - entry + 4u) / 5u;
+            next_entry = entry + 5u * tile_tuples - tl;
+        } else {
+            {
+                // from the last byte back: f(i) = where the walk from a tuple
+                // starting at byte i leaves [rs, re) and how many tuples start
+                // on the way, f(i) = f(i + len(i)) plus one tuple; kept in the
+                // value buffer, free until the tuples are decoded
+                uint16_t* fd = reinterpret_cast<uint16_t*>(vals);  // [SVD_B][SVD_TPB]
+                for (int i = (int)re - 1; i >= (int)rs; --i) {
+                    const uint32_t j = (uint32_t)i + svd_len(buf[svd_pb(sh + (uint32_t)i)]);
+                    const uint32_t f = j >= re ? ((1u << 8) | (j - re))
+                                               : (uint32_t)fd[(j - rs) * SVD_TPB + tid] + (1u << 8);
+                    fd[((uint32_t)i - rs) * SVD_TPB + tid] = (uint16_t)f;
+                }
+#pragma unroll
+                for (uint32_t e = 0; e < SVD_NE; ++e) {
+                    const uint32_t i = rs + e;
+                    map0[tid * SVD_NE + e] = i >= re ? i - re : (uint32_t)fd[e * SVD_TPB + tid];
+                }
+            }
+            __syncthreads();
+            // 2 inclusive composition scan: map[t] = the walk of threads 0..t
+            uint32_t* A = map0;
+            uint32_t* Bm = map1;
+            for (int d = 1; d < SVD_TPB; d <<= 1) {
+#pragma unroll
+                for (uint32_t e = 0; e < SVD_NE; ++e) {
+                    uint32_t r = A[tid * SVD_NE + e];
+                    if (tid >= d) {
+                        const uint32_t f = A[(tid - d) * SVD_NE + e];
+                        const uint32_t g = A[tid * SVD_NE + (f & 255u)];
+                        r = ((f >> 8) + (g >> 8)) << 8 | (g & 255u);
+                    }
+                    Bm[tid * SVD_NE + e] = r;
+                }
+                __syncthreads();
+                uint32_t* t = A;
+                A = Bm;
+                Bm = t;
+            }
+            const uint32_t before = tid ? A[(tid - 1) * SVD_NE + entry] : entry;  // (tuples << 8) | my entry
+            my_entry = tid ? (before & 255u) : entry;
+            my_t = tid ? (before >> 8) : 0u;
+            const uint32_t last = A[(SVD_TPB - 1) * SVD_NE + entry];
+            tile_tuples = last >> 8;
+            next_entry = last & 255u;
+        }
+        // 3 deltas of this thread's tuples into LDS, then the values
+        uint32_t sum = 0;
+        uint32_t nt = 0;
+        for (uint32_t pos = rs + my_entry; pos < re; ++nt) {
+            const uint32_t q = my_t + nt;
+            // bytes pos .. pos+16 of the tuple from five aligned LDS words
+            const uint32_t qb = sh + pos, r8 = (qb & 3u) * 8u;
+            const uint32_t wq = qb >> 2;
+            const uint64_t w01 = (uint64_t)svd_smem[svd_pw(wq)] | ((uint64_t)svd_smem[svd_pw(wq + 1)] << 32);
+            const uint64_t w23 = (uint64_t)svd_smem[svd_pw(wq + 2)] | ((uint64_t)svd_smem[svd_pw(wq + 3)] << 32);
+            const uint32_t w4 = svd_smem[svd_pw(wq + 4)];
+            const uint64_t b0 = r8 ? (w01 >> r8) | (w23 << (64u - r8)) : w01;                  // bytes 0..7
+            const uint64_t b1 = r8 ? (w23 >> r8) | ((uint64_t)w4 << (64u - r8)) : w23;         // bytes 8..15
+            const uint32_t b16 = (w4 >> r8) & 255u;                                            // byte 16
+            const uint32_t d = (uint32_t)(b0 & 255u);
+            const uint64_t xl = (b0 >> 8) | (b1 << 56);             // value bytes 0..7 (tuple bytes 1..8)
+            const uint64_t xh = (b1 >> 8) | ((uint64_t)b16 << 56);  // value bytes 8..15
+            uint32_t o8 = 0;  // value byte offset
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t len = 1u + ((d >> (2 * j)) & 3u);
+                const uint32_t sb = o8 * 8u;
+                const uint64_t x = o8 < 8u ? ((xl >> sb) | (sb ? xh << (64u - sb) : 0ull)) : (xh >> (sb - 64u));
+                const uint32_t v = (uint32_t)x & (len == 4u ? 0xFFFFFFFFu : ((1u << (8u * len)) - 1u));
+                o8 += len;
+                if (q < SVD_MAXT) vals[q * 4 + j] = v;
+                sum += v;
+            }
+            pos += 1u + o8;
+        }
+        uint32_t tile_sum;
+        uint32_t run = carry + svd_block_excl(sum, s_w, &tile_sum);
+        for (uint32_t i = 0; i < nt && my_t + i < SVD_MAXT; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                run += vals[(my_t + i) * 4 + j];
+                vals[(my_t + i) * 4 + j] = run;
+            }
+        __syncthreads();
+        const uint64_t v0 = tbase * 4;  // value index of the tile's first value
+        const uint32_t nv = (tile_tuples < SVD_MAXT ? tile_tuples : SVD_MAXT) * 4;
+        for (uint32_t i = tid; i < nv; i += SVD_TPB)
+            if (v0 + i < count) o[v0 + i] = vals[i];
+        if (tile_tuples > SVD_MAXT && tid == 0) s_bad = 1;
+        entry = next_entry;
+        tbase += tile_tuples;
+        carry += tile_sum;
+        __syncthreads();  // buf, maps and vals are rewritten by the next tile
+    }
+    // 4 the tuples end exactly at E, T of them
+    if (tid == 0 && (entry != 0 || tbase != T || s_bad)) atomicAdd(err, 1u);
     if (bounds_out) {
-        const uint8_t* q = in + in_off[c] + payload_end[k];
+        const uint8_t* q = p + E;
         uint8_t* bo = bounds_out + value_starts[c];
-        for (uint64_t g = threadIdx.x; g * 8 < count; g += TPB) {
+        for (uint64_t g = tid; g * 8 < count; g += SVD_TPB) {
             const uint64_t stored = rd32(q + g * 4);
             const uint64_t all = (stored & 0x0F0F0F0Full) | ((stored & 0xF0F0F0F0ull) << 28);
             for (int j = 0; j < 8; ++j) {
@@ -655,50 +827,34 @@ int read_starts(std::vector<uint64_t>& starts, const uint64_t* d_starts, uint32_
     return PS_OK;
 }
 
-// Decodes the SVI chunks `sel` of a concatenated stream into values.
+// Decodes the SVI chunks `sel` (each with a nonzero count) of a concatenated
+// stream into values, one block per chunk.
 int svi_decode_chunks(const uint8_t* d_in, const uint64_t* d_in_off, const std::vector<uint32_t>& sel,
                       const std::vector<uint32_t>& counts_all, const uint32_t* d_counts_all,
                       const uint64_t* d_value_starts, uint32_t* d_values, uint8_t* d_bounds, hipStream_t s) {
+    (void)d_counts_all;
     const uint32_t n = (uint32_t)sel.size();
     if (n == 0) return PS_OK;
-    std::vector<uint64_t> tb(n + 1);
-    uint64_t t = 0;
-    for (uint32_t k = 0; k < n; ++k) {
-        const uint32_t cnt = counts_all[sel[k]];
-        if (cnt > MAX_CHUNK_VALUES) return fail(PS_ERR_INVALID_ARGUMENT, "stream chunk claims %u values", cnt);
-        tb[k] = t;
-        t += (uint64_t)((cnt + 7) / 8 * 8) / 4;
+    for (uint32_t k = 0; k < n; ++k)
+        if (counts_all[sel[k]] > MAX_CHUNK_VALUES)
+            return fail(PS_ERR_INVALID_ARGUMENT, "stream chunk claims %u values", counts_all[sel[k]]);
+    static bool lds_set = false;
+    if (!lds_set) {
+        PS_HIP_CHECK(hipFuncSetAttribute((const void*)k_svi_decode_chunk, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)SVD_LDS));
+        lds_set = true;
     }
-    tb[n] = t;
-    DevBuf d_sel, d_tb, d_toff, d_end, d_deltas, d_err;
+    DevBuf d_sel, d_err;
     if (int r = to_device(d_sel, sel, s)) return r;
-    if (int r = to_device(d_tb, tb, s)) return r;
-    if (int r = d_toff.alloc((t + 1) * 4, s)) return r;
-    if (int r = d_end.alloc((size_t)n * 4, s)) return r;
-    if (int r = d_deltas.alloc((t + 1) * 16, s)) return r;
     const std::vector<uint32_t> zero(1, 0u);
     if (int r = to_device(d_err, zero, s)) return r;
-    hipLaunchKernelGGL(k_svi_walk_par<256>, dim3(n), dim3(256), 0, s, d_in, d_in_off, d_tb.as<uint64_t>(),
-                       d_sel.as<uint32_t>(), d_bounds ? 1 : 0, d_toff.as<uint32_t>(), d_end.as<uint32_t>(),
-                       d_err.as<uint32_t>());
+    hipLaunchKernelGGL(k_svi_decode_chunk, dim3(n), dim3(SVD_TPB), SVD_LDS, s, d_in, d_in_off, d_sel.as<uint32_t>(),
+                       d_value_starts, d_bounds ? 1 : 0, d_values, d_bounds, d_err.as<uint32_t>());
     PS_HIP_CHECK(hipGetLastError());
     std::vector<uint32_t> bad;
     if (int r = to_host(bad, d_err.p, 1, s)) return r;
     if (bad[0])
         return fail(PS_ERR_INVALID_ARGUMENT, "%u StreamVInt chunks do not end where their tuples end", bad[0]);
-    if (t) {
-        uint64_t max_t = 0;
-        for (uint32_t k = 0; k < n; ++k) max_t = std::max<uint64_t>(max_t, tb[k + 1] - tb[k]);
-        const bool g2 = n <= 65535 && ((max_t + 255) / 256) * 256 * n <= 2 * t + 256ull * n;
-        const dim3 grid = g2 ? dim3((uint32_t)((max_t + 255) / 256), n) : dim3((uint32_t)blocks_for(t, 256));
-        hipLaunchKernelGGL(k_svi_decode_tuples, grid, dim3(256), 0, s, d_in, d_in_off, d_tb.as<uint64_t>(),
-                           d_sel.as<uint32_t>(), n, t, g2 ? 1 : 0, d_toff.as<uint32_t>(), d_deltas.as<uint32_t>());
-    }
-    hipLaunchKernelGGL(k_svi_finish<256>, dim3(n), dim3(256), 0, s, d_deltas.as<uint32_t>(), d_tb.as<uint64_t>(),
-                       d_counts_all, d_sel.as<uint32_t>(), d_value_starts, d_values, d_in, d_in_off,
-                       d_end.as<uint32_t>(), d_bounds);
-    PS_HIP_CHECK(hipGetLastError());
-    PS_HIP_CHECK(hipStreamSynchronize(s));
     return PS_OK;
 }
 

@@ -105,11 +105,42 @@ def test_svint_reference_frontier_case(C):
     assert np.array_equal(back[0], vals) and np.array_equal(bb[0], bnd)
 
 
+@pytest.mark.parametrize("kind", ["uniform5", "uniform9", "uniform17", "mixed_multiple", "mixed_multiple_big"])
+def test_svint_uniform_and_mixed_tuple_lengths(C, kind):
+    # Chunks whose tuples all have one length are decoded by direct indexing;
+    # a payload whose length is a multiple of the tuple count but whose tuples
+    # differ in length (half of 5 bytes, half of 13: 9 on average) must take
+    # the descriptor walk.  Both against the oracle's bytes and values.
+    rng = np.random.default_rng(sum(map(ord, kind)))
+    n = 70000 if kind.endswith("big") else 4096
+    if kind == "uniform5":
+        gaps = rng.integers(1, 256, size=n)
+    elif kind == "uniform9":
+        gaps = rng.integers(256, 65536, size=n)
+    elif kind == "uniform17":
+        gaps = rng.integers(1 << 24, 1 << 25, size=n)
+    else:  # tuples alternate between four 1-byte and four 3-byte deltas
+        gaps = np.where((np.arange(n) // 4) % 2 == 0, rng.integers(1, 256, size=n),
+                        rng.integers(1 << 16, 1 << 17, size=n))
+    vals = (np.cumsum(gaps.astype(np.uint64)) % (1 << 32)).astype(np.uint32)  # wraps: deltas keep their size
+    bounds = rng.integers(0, 16, size=n, dtype=np.uint8)
+    data, off = C.svint_encode([vals], [bounds])
+    assert np.array_equal(data, oracle_svi(vals, bounds))
+    T = (n + 7) // 8 * 2
+    payload = len(data) - 4 - 2 * T
+    if kind.startswith("mixed"):
+        assert payload % T == 0 and payload // T == 9  # a multiple of T, tuples of two lengths
+    else:
+        assert payload == {"uniform5": 5, "uniform9": 9, "uniform17": 17}[kind] * T
+    back, bb = C.svint_decode(data, off, with_bounds=True)
+    assert np.array_equal(back[0], vals) and np.array_equal(bb[0], bounds)
+
+
 @pytest.mark.parametrize("cut", [1, 5, 16, "all_but_header"])
 def test_svint_decode_rejects_truncated_chunk(C, ps, cut):
     # a chunk whose bytes end before its last tuple (or inside it, or right
-    # after its count) is refused before any value is decoded; the chunk
-    # after it is intact
+    # after its count) is refused (the call fails); the chunk after it is
+    # intact
     rng = np.random.default_rng(13)
     chunks = [sorted_unique(rng, 5000, 1 << 32), sorted_unique(rng, 3000, 1 << 32)]
     data, off = C.svint_encode(chunks)
