@@ -816,7 +816,7 @@ int svi_write(const uint32_t* d_values, const uint8_t* d_bounds, const SviPlan& 
         hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, d_bounds, pl,
                            d_toff.as<uint32_t>(), d_off.as<uint64_t>(), d_payload.as<uint64_t>(), d_out);
     PS_HIP_CHECK(hipGetLastError());
-    PS_HIP_CHECK(hipStreamSynchronize(s));
+    if (int r = stream_wait(s)) return r;
     return PS_OK;
 }
 
@@ -952,7 +952,7 @@ int fc_encode_chunks(const uint32_t* d_values, const std::vector<uint64_t>& star
                            (uint32_t)bm.size(), d_wb.as<uint64_t>(), d_first.as<uint32_t>(),
                            d_words.as<unsigned long long>(), d_offsets, d_out);
         PS_HIP_CHECK(hipGetLastError());
-        PS_HIP_CHECK(hipStreamSynchronize(s));
+        if (int r = stream_wait(s)) return r;
     }
     return PS_OK;
 }
@@ -974,7 +974,7 @@ int ps_svint_encode(const uint32_t* d_values, const uint8_t* d_bounds, const uin
     if (int r = read_starts(starts, d_starts, nchunks, s)) return r;
     if (int r = svi_encode_chunks(d_values, d_bounds, starts, d_out, out_cap, off, s)) return r;
     PS_HIP_CHECK(hipMemcpyAsync(d_offsets, off.data(), off.size() * 8, hipMemcpyHostToDevice, s));
-    PS_HIP_CHECK(hipStreamSynchronize(s));
+    if (int r = stream_wait(s)) return r;
     return PS_OK;
     PS_API_END
 }
@@ -1049,7 +1049,7 @@ int ps_fc_encode(const uint32_t* d_values, const uint64_t* d_starts, uint32_t nc
     std::vector<uint64_t> starts, off;
     if (int r = read_starts(starts, d_starts, nchunks, s)) return r;
     if (int r = fc_encode_chunks(d_values, starts, with_check, d_out, out_cap, off, d_offsets, s)) return r;
-    PS_HIP_CHECK(hipStreamSynchronize(s));
+    if (int r = stream_wait(s)) return r;
     return PS_OK;
     PS_API_END
 }
@@ -1142,7 +1142,7 @@ int ps_bucket_by_segment(const uint64_t* d_children, const uint8_t* d_ops, uint6
         for (uint64_t b = 0; b < nb; ++b) starts[b + 1] += starts[b];
     }
     PS_HIP_CHECK(hipMemcpyAsync(d_bucket_starts, starts.data(), starts.size() * 8, hipMemcpyHostToDevice, s));
-    PS_HIP_CHECK(hipStreamSynchronize(s));
+    if (int r = stream_wait(s)) return r;
     return PS_OK;
     PS_API_END
 }

@@ -40,6 +40,41 @@ struct DevBuf {
     }
 };
 
+// Polls `e` until the work before it has finished (see stream_wait).
+inline int event_wait(hipEvent_t e) {
+    hipError_t q;
+    while ((q = hipEventQuery(e)) == hipErrorNotReady) {
+    }
+    if (q != hipSuccess) return fail(PS_ERR_HIP, "waiting for the stream failed: %s", hipGetErrorString(q));
+    return PS_OK;
+}
+
+// Waits until the work queued on `s` so far has finished, polling an event
+// from this thread instead of sleeping in hipStreamSynchronize: the steps that
+// read a count back between launches resume within microseconds of the copy
+// instead of after an interrupt wake-up (measured 30-390 us per wait).  One
+// event per thread and device, kept for the life of the thread.
+inline int stream_wait(hipStream_t s) {
+    constexpr int MAX_DEV = 64;
+    thread_local hipEvent_t ev[MAX_DEV] = {};
+    int dev = 0;
+    PS_HIP_CHECK(s ? hipStreamGetDevice(s, &dev) : hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEV) return fail(PS_ERR_HIP, "device %d out of range", dev);
+    if (!ev[dev]) {
+        int cur = 0;
+        PS_HIP_CHECK(hipGetDevice(&cur));
+        if (cur != dev) PS_HIP_CHECK(hipSetDevice(dev));
+        const hipError_t e = hipEventCreateWithFlags(&ev[dev], hipEventDisableTiming);
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess) {
+            ev[dev] = nullptr;
+            return fail(PS_ERR_HIP, "event creation failed: %s", hipGetErrorString(e));
+        }
+    }
+    PS_HIP_CHECK(hipEventRecord(ev[dev], s));
+    return event_wait(ev[dev]);
+}
+
 template <class T>
 inline int to_device(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
     if (int r = b.alloc(v.size() * sizeof(T), s)) return r;
@@ -51,7 +86,7 @@ template <class T>
 inline int to_host(std::vector<T>& v, const void* d, size_t n, hipStream_t s) {
     v.resize(n);
     if (n) PS_HIP_CHECK(hipMemcpyAsync(v.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, s));
-    PS_HIP_CHECK(hipStreamSynchronize(s));
+    if (int r = stream_wait(s)) return r;
     return PS_OK;
 }
 

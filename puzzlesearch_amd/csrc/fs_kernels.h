@@ -1,26 +1,44 @@
 // Kernels of one disk-segment step of the frontier search with 4-bit op flags,
 // DB_OptFS_Solver<4>::Expand(segment) (Common/DiskBasedOptFrontierSearch.cpp:53-132).
 //
-// HBM layout of one segment (2^segment_bits states):
-//   flags  u32[2^segment_bits / 8]: the IndexedArray<4> of the reference
-//          (Common/BitArray.h:283-314): state i owns bits 4*(i%8)..4*(i%8)+3 of
-//          word i/8, bit `op` set when a child arrived by the move labelled op
-//   dirty  u8 per region of 2^16 states: set by any mark in the region, so the
-//          scan passes skip untouched regions (the role of IndexedArray's
-//          occupancy index)
-//   odd    u64 bit array of the current frontier, only for puzzles with odd
-//          cycles (the reference's FrontierArray)
+// The op flags of a segment (2^segment_bits states, 4 bits each: the
+// IndexedArray<4> of the reference, Common/BitArray.h:283-314) are never held
+// in HBM.  The step bins its marks instead:
+//   count  each workgroup takes a contiguous share of the frontier and of the
+//          incoming cross-segment marks and counts their in-segment marks per
+//          region (regions of 2^FS_BIN_BITS states) in LDS, one row of a
+//          (workgroup, region) table;
+//   place  after a prefix sum down every region's column and over the regions,
+//          each workgroup writes the same marks to its own part of their
+//          region's range of a mark list (state in region << 2 | move), taking
+//          positions from LDS cursors;
+//   region one workgroup per region ORs the region's marks into an LDS copy
+//          of its flags (128 KB, with a bitmap of the words holding a mark)
+//          and counts the entries, then, after a prefix sum of the
+//          per-region output counts, rebuilds them and writes the region's
+//          (id, op bits) in id order.
+// For puzzles with odd cycles the current frontier's bits (`odd`, a u64 bit
+// array of the segment) exclude its entries, as the reference's
+// FrontierArray does.
 #pragma once
 
 #include "bfs_kernels.h"
 
 namespace psk {
 
-constexpr int FS_REGION_BITS = 16;                               // states per dirty region
-constexpr uint32_t FS_REGION_WORDS = 1u << (FS_REGION_BITS - 3);  // u32 flag words per region
-constexpr uint64_t FS_NONE = ~0ull;                              // empty cross-segment slot
+constexpr int FS_BIN_BITS = 18;   // states per region: 2^18, flags of 128 KB
+constexpr uint64_t FS_NONE = ~0ull;  // empty cross-segment slot
+constexpr int FS_REGION_TPB = 1024;  // thread t of a region workgroup owns flag words [32t, 32t + 32)
+static_assert((1 << FS_BIN_BITS) / 8 <= 32 * FS_REGION_TPB, "a region's flag words fit its workgroup");
+constexpr uint32_t FS_MAX_REGIONS = 1u << (32 - FS_BIN_BITS);  // LDS cursors of one workgroup: 64 KB
+constexpr int FS_BIN_TPB = 1024;
+constexpr uint32_t FS_MAX_BIN_BLOCKS = 512;  // rows of the (workgroup, region) table
+constexpr int FS_MK = 8;                      // marks a thread of the region pass loads at once
+constexpr int FS_COL_WAVES = 16;              // column pass over the table: waves per workgroup
+constexpr int FS_COL_ROWS = 32;               // and rows per wave
+static_assert(FS_COL_WAVES * FS_COL_ROWS == (int)FS_MAX_BIN_BLOCKS, "column pass covers every row");
 
-struct FsMarkArgs {
+struct FsBinArgs {
     psd::PuzzleParams prm;
     const uint32_t* idx;  // frontier, in-segment ids
     const uint8_t* ops;   // their op bits
@@ -28,31 +46,31 @@ struct FsMarkArgs {
     uint64_t base;  // segment << seg_bits
     uint32_t segment;
     int seg_bits;
-    uint32_t* flags;
-    uint8_t* dirty;
-    unsigned long long* odd;       // NULL without odd cycles
-    unsigned long long* counters;  // [0] in-segment children, [6] ids outside the segment
+    int rb;                       // log2(states per region) = min(FS_BIN_BITS, seg_bits)
+    uint32_t nregions;            // at most FS_MAX_REGIONS
+    uint64_t per, perx;           // parents and cross marks per workgroup
+    const uint32_t* xidx;         // incoming cross-segment marks, op by op
+    const uint64_t* xstarts;      // ops + 1 entries (device)
+    uint64_t nx;
+    int nops;
+    unsigned long long* odd;      // frontier bits (odd-cycle puzzles; count pass), else NULL
+    uint32_t* table;              // gridDim.x rows of nregions: count pass, the workgroup's marks per region;
+                                  // place pass, its first position relative to the region's start
+    const unsigned long long* rstart;  // place pass: nregions + 1 region starts in the mark list
+    uint32_t* marks;              // place pass: (state in region << 2) | move
+    unsigned long long* counters; // [0] in-segment children, [6] ids outside the segment (count pass)
 };
 
-struct FsXsegArgs {
-    const uint32_t* idx;  // cross-segment marks, per op
-    const uint64_t* op_starts;  // ops + 1 entries (device)
+struct FsRegionArgs {
+    uint32_t nregions;
+    const uint32_t* marks;
+    const unsigned long long* rstart;  // nregions + 1
+    const unsigned long long* odd;     // NULL without odd cycles
+    int rb;
     int ops;
-    uint64_t seg_mask;
-    uint32_t* flags;
-    uint8_t* dirty;
-};
-
-struct FsScanArgs {
-    const uint32_t* flags_c;
-    uint32_t* flags;
-    uint8_t* dirty;
-    const unsigned long long* odd;
-    uint64_t nwords;  // flag words of the segment
-    int ops;
-    unsigned long long* cnt_out;    // per region: entries written to the new frontier
-    unsigned long long* counters;   // [1] all nonzero entries (the layer count), [2] written, [5] words scanned
-    const unsigned long long* out_off;  // per region exclusive offsets (write pass)
+    unsigned long long* cnt_out;        // count pass: entries written per region
+    uint32_t* cnt_all;                  // count pass: entries per region
+    const unsigned long long* out_off;  // write pass: exclusive offsets
     uint32_t* out_idx;
     uint8_t* out_ops;
 };
@@ -106,237 +124,234 @@ struct HasCrossSlots : std::false_type {};
 template <class PZ>
 struct HasCrossSlots<PZ, std::void_t<decltype(PZ::cross_slots(0ull, 0))>> : std::true_type {};
 
-// One OR per flag word: consecutive children of a parent that share a word
-// (the horizontal moves of the sliding puzzles: index -+ 1) are merged before
-// the atomic.
-struct FsMarkSink {
-    uint32_t* flags;
-    uint8_t* dirty;
+// The in-segment children of one parent, by slot.
+template <int SLOTS>
+struct BinSink {
     uint64_t segment, in_mask;
     int seg_bits;
-    uint64_t pend_w;
-    uint32_t pend_m;
-    unsigned long long marks;
-    __device__ __forceinline__ void flush() {
-        if (!pend_m) return;
-        gor(&flags[pend_w], pend_m);
-        dirty[pend_w >> (FS_REGION_BITS - 3)] = 1;  // a plain store: no read to wait for
-        pend_m = 0;
-    }
-    __device__ __forceinline__ void operator()(int, uint64_t child, uint32_t label) {
+    uint32_t valid;  // bit k: slot k holds an in-segment child
+    uint32_t v[SLOTS];
+    uint32_t lab[SLOTS];
+    __device__ __forceinline__ void operator()(int slot, uint64_t child, uint32_t label) {
         if ((child >> seg_bits) != segment) return;
-        const uint64_t c = child & in_mask;
-        const uint32_t m = (1u << label) << ((c & 7) * 4);
-        ++marks;
-        if ((c >> 3) == pend_w) {
-            pend_m |= m;
-            return;
-        }
-        flush();
-        pend_w = c >> 3;
-        pend_m = m;
+        v[slot] = (uint32_t)(child & in_mask);
+        lab[slot] = label;
+        valid |= 1u << slot;
     }
 };
 
-template <class PZ, int TPB>
-__global__ __launch_bounds__(TPB) void k_fs_mark_in(FsMarkArgs a) {
+// Count (PLACE = false) or place (PLACE = true) the marks of the frontier's
+// in-segment children and of the incoming cross-segment marks.  Workgroup b
+// takes parents [b*per, (b+1)*per) and cross marks [b*perx, (b+1)*perx); the
+// place pass sees the same marks as the count pass, so each workgroup fills
+// exactly the positions its table row reserved.
+template <class PZ, int TPB, bool PLACE>
+__global__ __launch_bounds__(TPB) void k_fs_bin(FsBinArgs a) {
+    extern __shared__ uint32_t cur[];  // per region: marks counted, or the next position
+    uint32_t* row = a.table + (uint64_t)blockIdx.x * a.nregions;
+    for (uint32_t r = threadIdx.x; r < a.nregions; r += TPB)
+        cur[r] = PLACE ? (uint32_t)a.rstart[r] + row[r] : 0u;
+    __syncthreads();
     const uint64_t in_mask = (1ull << a.seg_bits) - 1ull;
-    FsMarkSink sink{a.flags, a.dirty, a.segment, in_mask, a.seg_bits, ~0ull, 0u, 0ull};
-    unsigned long long bad = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * TPB + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * TPB) {
+    const uint32_t lmask = (1u << a.rb) - 1u;
+    auto put = [&](uint32_t state, uint32_t label) {
+        uint32_t* c = &cur[state >> a.rb];
+        if constexpr (PLACE) a.marks[atomicAdd(c, 1u)] = ((state & lmask) << 2) | label;
+        else atomicAdd(c, 1u);
+    };
+    unsigned long long marks = 0, bad = 0;
+    const uint64_t p0 = (uint64_t)blockIdx.x * a.per < a.n ? (uint64_t)blockIdx.x * a.per : a.n;
+    const uint64_t p1 = p0 + a.per < a.n ? p0 + a.per : a.n;
+    for (uint64_t i = p0 + threadIdx.x; i < p1; i += TPB) {
+        BinSink<PZ::SLOTS> sink{a.segment, in_mask, a.seg_bits, 0u, {}, {}};
         const uint32_t id = a.idx[i];
         if ((uint64_t)id > in_mask) {  // not an id of this segment: reported after the step
-            ++bad;
+            if (!PLACE) ++bad;
             continue;
         }
-        if (a.odd) atomicOr(&a.odd[id >> 6], 1ull << (id & 63));
+        if (!PLACE && a.odd) atomicOr(&a.odd[id >> 6], 1ull << (id & 63));
         if constexpr (HasCrossSlots<PZ>::value)  // skip the moves that certainly leave the segment
             PZ::expand(a.base | id, (uint32_t)a.ops[i], a.prm, sink, PZ::in_slots(a.base | id, a.seg_bits));
         else
             PZ::expand(a.base | id, (uint32_t)a.ops[i], a.prm, sink);
-        sink.flush();
-        sink.pend_w = ~0ull;
+        marks += (unsigned long long)__popc(sink.valid);
+#pragma unroll
+        for (int k = 0; k < PZ::SLOTS; ++k)
+            if ((sink.valid >> k) & 1u) put(sink.v[k], sink.lab[k]);
     }
-    block_add3<TPB>(sink.marks, bad, 0ull, &a.counters[0], &a.counters[6], nullptr);
-}
-
-
-// pass 1 of ScanBitsAndClear: per region, entries counted and entries written.
-// Persistent blocks take regions round-robin; a full region is read as
-// FS_REGION_WORDS/4/TPB 16-byte loads per thread, all issued before any is
-// used.  The totals go to the counters once per block.
-template <int TPB>
-__global__ __launch_bounds__(TPB) void k_fs_count(FsScanArgs a) {
-    constexpr uint32_t PER = FS_REGION_WORDS / 4 / TPB;
-    static_assert(PER * 4 * TPB == FS_REGION_WORDS, "a region is a whole number of block loads");
-    const uint64_t nregions = (a.nwords + FS_REGION_WORDS - 1) / FS_REGION_WORDS;
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    __shared__ uint32_t red[2][TPB / 64];
-    unsigned long long all = 0;       // per thread, over every region of the block
-    unsigned long long out_blk = 0, scanned = 0;  // thread 0
-    uint32_t par = 0;
-    for (uint64_t r = blockIdx.x; r < nregions; r += gridDim.x, par ^= 1u) {
-        const uint64_t w0 = r * FS_REGION_WORDS;
-        const uint64_t rw = (a.nwords - w0 < FS_REGION_WORDS) ? a.nwords - w0 : FS_REGION_WORDS;
-        uint32_t out = 0;
-        const bool dirty = a.dirty[r] != 0;
-        if (dirty && rw == FS_REGION_WORDS) {
-            const uint4* f4 = reinterpret_cast<const uint4*>(a.flags_c + w0);
-            uint4 v[PER];
-#pragma unroll
-            for (uint32_t k = 0; k < PER; ++k) v[k] = f4[k * TPB + threadIdx.x];
-#pragma unroll
-            for (uint32_t k = 0; k < PER; ++k) {
-                const uint32_t ws[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    if (!ws[e]) continue;
-                    const uint32_t live = fs_live(ws[e], a.odd, w0 + ((uint64_t)k * TPB + threadIdx.x) * 4 + e);
-                    all += __popc(live);
-                    out += __popc(live & ~nib_full(ws[e], a.ops));
-                }
-            }
-        } else if (dirty) {  // the whole segment is smaller than a region
-            for (uint64_t j = threadIdx.x; j < rw; j += TPB) {
-                const uint32_t w = a.flags_c[w0 + j];
-                if (!w) continue;
-                const uint32_t live = fs_live(w, a.odd, w0 + j);
-                all += __popc(live);
-                out += __popc(live & ~nib_full(w, a.ops));
-            }
-        }
-        out = wave_sum(out);
-        if (lane == 0) red[par][wave] = out;
+    const uint64_t x0 = (uint64_t)blockIdx.x * a.perx < a.nx ? (uint64_t)blockIdx.x * a.perx : a.nx;
+    const uint64_t x1 = x0 + a.perx < a.nx ? x0 + a.perx : a.nx;
+    for (uint64_t j = x0 + threadIdx.x; j < x1; j += TPB) {
+        uint32_t op = 0;
+        while ((int)op + 1 < a.nops && a.xstarts[op + 1] <= j) ++op;
+        put((uint32_t)(a.xidx[j] & in_mask), op);
+    }
+    if (!PLACE) {
         __syncthreads();
-        // red[par] is written again two regions later, after the next barrier,
-        // which thread 0 reaches only once it has read it here
-        if (threadIdx.x == 0) {
-            uint32_t so = 0;
-#pragma unroll
-            for (int w = 0; w < TPB / 64; ++w) so += red[par][w];
-            a.cnt_out[r] = so;
-            out_blk += so;
-            if (dirty) scanned += rw;
-        }
+        for (uint32_t r = threadIdx.x; r < a.nregions; r += TPB) row[r] = cur[r];
+        block_add3<TPB>(marks, bad, 0ull, &a.counters[0], &a.counters[6], nullptr);
     }
-    block_add3<TPB>(all, threadIdx.x == 0 ? out_blk : 0ull, threadIdx.x == 0 ? scanned : 0ull, &a.counters[1],
-                    &a.counters[2], &a.counters[5]);
 }
 
-// pass 2: ordered write of (id, op bits) for entries with an unused move, and
-// the clear of the region (flags words and dirty byte).  Persistent blocks
-// take regions round-robin.  In a full region thread t of load k owns words
-// 4*(k*TPB + t) .. +3, so (load, thread, word) order is index order: the
-// block scans the per-load counts of CH loads at a time after one barrier
-// and writes every entry at its sorted position.
-template <int TPB>
-__global__ __launch_bounds__(TPB) void k_fs_write(FsScanArgs a) {
-    constexpr uint32_t PER = FS_REGION_WORDS / 4 / TPB;
-    constexpr uint32_t CH = PER < 4 ? PER : 4;
-    static_assert(PER % CH == 0, "a region is a whole number of chunks");
+// One workgroup per region: its flags are built in LDS from its marks, then
+// counted (WRITE = false: all entries, and entries with an unused move, per
+// region) or written in id order (WRITE = true).  Workgroups are persistent
+// (workgroup b takes regions b, b + grid, ...) and load the next region's mark
+// range, output offset and first FS_MK marks per thread while they work on the
+// current one.  Beside the flags the workgroup keeps one bit per flag word
+// (set when a mark lands in the word); thread t owns flag words
+// [32t, 32t + 32), visits only its words with a mark, writes its entries in id
+// order after a workgroup scan of the per-thread counts, and clears its words
+// again for the next region.
+template <int TPB, bool WRITE>
+__global__ __launch_bounds__(TPB) void k_fs_region(FsRegionArgs a) {
+    extern __shared__ uint32_t fl[];  // rw flag words, then nbm words of the nonzero-word bitmap
     constexpr int NW = TPB / 64;
-    const uint64_t nregions = (a.nwords + FS_REGION_WORDS - 1) / FS_REGION_WORDS;
+    static_assert(NW <= 64, "one lane per wave total");
+    const uint32_t rw = (1u << a.rb) / 8u;  // flag words of the region, at most 32 * TPB
+    const uint32_t nbm = (rw + 31u) / 32u;
+    uint32_t* bm = fl + rw;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    __shared__ uint32_t tot[2][CH][NW];
-    uint32_t par = 0;
-    for (uint64_t r = blockIdx.x; r < nregions; r += gridDim.x) {
-        if (!a.dirty[r]) continue;  // the same value for the whole block: it is cleared after the barrier below
-        const uint64_t w0 = r * FS_REGION_WORDS;
-        const uint64_t rw = (a.nwords - w0 < FS_REGION_WORDS) ? a.nwords - w0 : FS_REGION_WORDS;
-        uint64_t running = a.out_off[r];
-        if (rw == FS_REGION_WORDS) {
-            uint4* f4 = reinterpret_cast<uint4*>(a.flags + w0);
-            // the region in PER/CH chunks of CH loads per thread, one barrier each
-            for (uint32_t k0 = 0; k0 < PER; k0 += CH, par ^= 1u) {
-                uint4 v[CH];
+    const uint32_t t32 = threadIdx.x * 32u;
+    // per-wave totals; written only after the barrier that opens a region, which
+    // every thread reaches once done reading the previous region's totals
+    __shared__ uint32_t s_w[NW];
+    for (uint32_t x = threadIdx.x; x < rw + nbm; x += TPB) fl[x] = 0u;
+    uint32_t c[FS_MK];
+    auto load = [&](uint64_t j0, uint64_t end) {
 #pragma unroll
-                for (uint32_t k = 0; k < CH; ++k) v[k] = f4[(k0 + k) * TPB + threadIdx.x];
-                uint32_t c[CH], incl[CH], keep[CH][4];
-#pragma unroll
-                for (uint32_t k = 0; k < CH; ++k) {
-                    const uint32_t ws[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
-                    const uint64_t j0 = w0 + ((uint64_t)(k0 + k) * TPB + threadIdx.x) * 4;
-                    c[k] = 0;
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        keep[k][e] = ws[e] ? fs_live(ws[e], a.odd, j0 + e) & ~nib_full(ws[e], a.ops) : 0u;
-                        c[k] += __popc(keep[k][e]);
-                    }
-                    incl[k] = wave_incl_scan(c[k], lane);
-                    if (lane == 63) tot[par][k][wave] = incl[k];
-                }
-                __syncthreads();
-                // tot[par] is written again two chunks later, after the next
-                // chunk's barrier, which every thread reaches only once done here
-#pragma unroll
-                for (uint32_t k = 0; k < CH; ++k) {
-                    uint32_t before = 0, total = 0;
-#pragma unroll
-                    for (int w = 0; w < NW; ++w) {
-                        const uint32_t t = tot[par][k][w];
-                        before += (w < wave) ? t : 0u;
-                        total += t;
-                    }
-                    const uint32_t ws[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
-                    const uint64_t j0 = w0 + ((uint64_t)(k0 + k) * TPB + threadIdx.x) * 4;
-                    uint64_t pos = running + before + incl[k] - c[k];
-                    for (int e = 0; e < 4; ++e) {
-                        uint32_t kb = keep[k][e];
-                        while (kb) {
-                            const int b = __ffs(kb) - 1;  // bit 4k of nibble k
-                            kb &= kb - 1;
-                            a.out_idx[pos] = (uint32_t)((j0 + e) * 8 + (uint64_t)(b >> 2));
-                            a.out_ops[pos] = (uint8_t)((ws[e] >> b) & 15u);
-                            ++pos;
-                        }
-                    }
-                    if (ws[0] | ws[1] | ws[2] | ws[3]) f4[(k0 + k) * TPB + threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
-                    running += total;
-                }
+        for (int k = 0; k < FS_MK; ++k) {
+            const uint64_t j = j0 + (uint64_t)k * TPB + threadIdx.x;
+            c[k] = j < end ? a.marks[j] : ~0u;  // a mark is below 2^20
+        }
+    };
+    // a word's entries: nonzero, not in the current frontier, some move unused
+    auto keep_of = [&](uint32_t f, uint64_t word) -> uint32_t {
+        return fs_live(f, a.odd, word) & ~nib_full(f, a.ops);
+    };
+    uint32_t r = blockIdx.x;
+    uint64_t m0 = 0, m1 = 0, off = 0;
+    if (r < a.nregions) {
+        m0 = a.rstart[r];
+        m1 = a.rstart[r + 1];
+        if (WRITE) off = a.out_off[r];
+        load(m0, m1);
+    }
+    for (; r < a.nregions; r += gridDim.x) {
+        const uint32_t rn = r + gridDim.x;
+        uint64_t n0 = 0, n1 = 0, noff = 0;
+        if (rn < a.nregions) {
+            n0 = a.rstart[rn];
+            n1 = a.rstart[rn + 1];
+            if (WRITE) noff = a.out_off[rn];
+        }
+        if (m0 == m1) {  // no mark: nothing to count or write
+            if (!WRITE && threadIdx.x == 0) {
+                a.cnt_out[r] = 0;
+                a.cnt_all[r] = 0;
             }
-            if (threadIdx.x == 0) a.dirty[r] = 0;  // every thread read it before the first barrier above
+            load(n0, n1);
+            m0 = n0;
+            m1 = n1;
+            off = noff;
             continue;
         }
-        // the whole segment is smaller than a region (nregions == 1): thread t
-        // of iteration `it` owns words it*4*TPB + 4t .. +3, one block scan per iteration
-        for (uint64_t it = 0; it * 4 * TPB < rw; ++it) {
-            const uint64_t j0 = it * 4 * TPB + (uint64_t)threadIdx.x * 4;
-            uint32_t ws[4] = {0u, 0u, 0u, 0u}, keep[4] = {0u, 0u, 0u, 0u};
-            uint32_t c = 0;
+        __syncthreads();  // flags and bitmap are clear (first region, or cleared by their owners)
+        for (uint64_t j0 = m0;;) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const uint64_t j = j0 + e;
-                ws[e] = (j < rw) ? a.flags[w0 + j] : 0u;
-                if (ws[e]) keep[e] = fs_live(ws[e], a.odd, w0 + j) & ~nib_full(ws[e], a.ops);
-                c += __popc(keep[e]);
+            for (int k = 0; k < FS_MK; ++k) {
+                if (c[k] == ~0u) continue;
+                const uint32_t st = c[k] >> 2;
+                const uint32_t w = st >> 3;
+                lor(&fl[w], (1u << (c[k] & 3u)) << ((st & 7u) * 4u));
+                lor(&bm[w >> 5], 1u << (w & 31u));
             }
-            const uint32_t incl = wave_incl_scan(c, lane);
-            if (lane == 63) tot[0][0][wave] = incl;
-            __syncthreads();
-            uint32_t before = 0, total = 0;
-            for (int k = 0; k < NW; ++k) {
-                const uint32_t t = tot[0][0][k];
-                before += (k < wave) ? t : 0u;
-                total += t;
-            }
-            uint64_t pos = running + before + incl - c;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                uint32_t kb = keep[e];
-                while (kb) {
-                    const int b = __ffs(kb) - 1;
-                    kb &= kb - 1;
-                    a.out_idx[pos] = (uint32_t)((w0 + j0 + e) * 8 + (uint64_t)(b >> 2));
-                    a.out_ops[pos] = (uint8_t)((ws[e] >> b) & 15u);
-                    ++pos;
-                }
-                if (ws[e]) a.flags[w0 + j0 + e] = 0u;
-            }
-            running += total;
-            __syncthreads();
+            j0 += (uint64_t)FS_MK * TPB;
+            if (j0 >= m1) break;
+            load(j0, m1);
         }
-        if (threadIdx.x == 0) a.dirty[r] = 0;
+        __syncthreads();
+        load(n0, n1);  // in flight during the scan
+        const uint64_t gw0 = (uint64_t)r * rw + t32;  // this thread's first flag word in the segment
+        const uint32_t mine = threadIdx.x < nbm ? bm[threadIdx.x] : 0u;
+        uint32_t out = 0, all = 0;
+        for (uint32_t m = mine; m;) {
+            const int w = __ffs(m) - 1;
+            m &= m - 1u;
+            const uint32_t f = fl[t32 + (uint32_t)w];
+            const uint32_t live = fs_live(f, a.odd, gw0 + (uint32_t)w);
+            all += (uint32_t)__popc(live);
+            out += (uint32_t)__popc(live & ~nib_full(f, a.ops));
+        }
+        if (!WRITE) {
+            __shared__ uint32_t s_all[NW];
+            const uint32_t wo = wave_sum(out), wa = wave_sum(all);
+            if (lane == 0) {
+                s_w[wave] = wo;
+                s_all[wave] = wa;
+            }
+            __syncthreads();
+            if (threadIdx.x < 64) {
+                const uint32_t so = wave_sum(lane < NW ? s_w[lane] : 0u);
+                const uint32_t sa = wave_sum(lane < NW ? s_all[lane] : 0u);
+                if (threadIdx.x == 0) {
+                    a.cnt_out[r] = so;
+                    a.cnt_all[r] = sa;
+                }
+            }
+        } else {
+            // workgroup exclusive scan of the per-thread entry counts
+            const uint32_t incl = wave_incl_scan(out, lane);
+            if (lane == 63) s_w[wave] = incl;
+            __syncthreads();
+            const uint32_t wt = lane < NW ? s_w[lane] : 0u;
+            const uint32_t wi = wave_incl_scan(wt, lane);
+            const uint32_t before = __shfl(wi - wt, wave, 64);
+            uint64_t pos = off + before + incl - out;
+            // one entry per iteration; a word is fetched when the last one is used up
+            uint32_t m = mine, kb = 0, f = 0, wbase = 0;
+            for (;;) {
+                if (!kb) {
+                    if (!m) break;
+                    const int w = __ffs(m) - 1;
+                    m &= m - 1u;
+                    f = fl[t32 + (uint32_t)w];
+                    kb = keep_of(f, gw0 + (uint32_t)w);
+                    wbase = (uint32_t)((gw0 + (uint32_t)w) * 8u);
+                    if (!kb) continue;
+                }
+                const int b = __ffs(kb) - 1;  // bit 4i of nibble i
+                kb &= kb - 1u;
+                a.out_idx[pos] = wbase + (uint32_t)(b >> 2);
+                a.out_ops[pos] = (uint8_t)((f >> b) & 15u);
+                ++pos;
+            }
+        }
+        // this thread's words are read only by this thread: clear them for the next region
+        for (uint32_t m = mine; m;) {
+            const int w = __ffs(m) - 1;
+            m &= m - 1u;
+            fl[t32 + (uint32_t)w] = 0u;
+        }
+        if (threadIdx.x < nbm) bm[threadIdx.x] = 0u;
+        m0 = n0;
+        m1 = n1;
+        off = noff;
     }
+}
+
+// Sums of the per-region counts: counters[1] = entries, [5] = flag words of
+// the regions with marks.  One block.
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_fs_region_totals(const uint32_t* cnt_all, const unsigned long long* rstart,
+                                                          uint32_t nregions, uint32_t rw,
+                                                          unsigned long long* counters) {
+    unsigned long long all = 0, scanned = 0;
+    for (uint32_t r = threadIdx.x; r < nregions; r += TPB) {
+        all += cnt_all[r];
+        scanned += rstart[r + 1] > rstart[r] ? rw : 0u;
+    }
+    block_add3<TPB>(all, scanned, 0ull, &counters[1], &counters[5], nullptr);
 }
 
 

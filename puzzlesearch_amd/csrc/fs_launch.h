@@ -12,7 +12,7 @@ namespace ps {
 constexpr int FS_TPB = 256;
 
 #define PS_DECLARE_FS_FAMILY(NAME)                                                                       \
-    int fs_mark_in_##NAME(const ps_puzzle& p, const psk::FsMarkArgs& a, uint32_t grid, hipStream_t s); \
+    int fs_bin_##NAME(const ps_puzzle& p, const psk::FsBinArgs& a, int place, uint32_t grid, hipStream_t s); \
     int fs_cross_##NAME(const ps_puzzle& p, const psk::FsCrossArgs& a, uint32_t grid, hipStream_t s, int* slots);
 PS_DECLARE_FS_FAMILY(pancake)
 PS_DECLARE_FS_FAMILY(sliding)
@@ -20,11 +20,17 @@ PS_DECLARE_FS_FAMILY(packed)
 PS_DECLARE_FS_FAMILY(hanoi)
 
 #define PS_DEFINE_FS_FAMILY(NAME, FAMV)                                                                       \
-    int fs_mark_in_##NAME(const ps_puzzle& p, const psk::FsMarkArgs& a, uint32_t grid, hipStream_t s) {    \
+    int fs_bin_##NAME(const ps_puzzle& p, const psk::FsBinArgs& a, int place, uint32_t grid, hipStream_t s) { \
         return dispatch<FAMV>(p, PS_EXPAND_FULL, [&](auto tag) -> int {                                     \
             using PZ = typename decltype(tag)::type;                                                        \
             if constexpr (PZ::OPS <= 4) {                                                                   \
-                hipLaunchKernelGGL((psk::k_fs_mark_in<PZ, FS_TPB>), dim3(grid), dim3(FS_TPB), 0, s, a);     \
+                const size_t lds = (size_t)a.nregions * 4;                                                  \
+                if (place)                                                                                  \
+                    hipLaunchKernelGGL((psk::k_fs_bin<PZ, psk::FS_BIN_TPB, true>), dim3(grid),                \
+                                       dim3(psk::FS_BIN_TPB), lds, s, a);                                   \
+                else                                                                                        \
+                    hipLaunchKernelGGL((psk::k_fs_bin<PZ, psk::FS_BIN_TPB, false>), dim3(grid),               \
+                                       dim3(psk::FS_BIN_TPB), lds, s, a);                                   \
                 PS_HIP_CHECK(hipGetLastError());                                                            \
                 return PS_OK;                                                                               \
             } else {                                                                                        \
@@ -52,12 +58,13 @@ PS_DECLARE_FS_FAMILY(hanoi)
         });                                                                                                 \
     }
 
-inline int fs_mark_in(const ps_puzzle& p, const psk::FsMarkArgs& a, uint32_t grid, hipStream_t s) {
+// place = 0: count the marks per (workgroup, region); 1: write them to their regions' ranges
+inline int fs_bin(const ps_puzzle& p, const psk::FsBinArgs& a, int place, uint32_t grid, hipStream_t s) {
     switch (family_of(p)) {
-    case FAM_PANCAKE: return fs_mark_in_pancake(p, a, grid, s);
-    case FAM_SLIDING: return fs_mark_in_sliding(p, a, grid, s);
-    case FAM_PACKED: return fs_mark_in_packed(p, a, grid, s);
-    case FAM_HANOI: return fs_mark_in_hanoi(p, a, grid, s);
+    case FAM_PANCAKE: return fs_bin_pancake(p, a, place, grid, s);
+    case FAM_SLIDING: return fs_bin_sliding(p, a, place, grid, s);
+    case FAM_PACKED: return fs_bin_packed(p, a, place, grid, s);
+    case FAM_HANOI: return fs_bin_hanoi(p, a, place, grid, s);
     }
     return fail(PS_ERR_UNSUPPORTED, "unknown puzzle family");
 }

@@ -2,11 +2,14 @@
 // DB_OptFS_Solver<4>::Expand(segment) (Common/DiskBasedOptFrontierSearch.cpp:53-132)
 // with the segment's frontier and incoming cross-segment marks already in HBM.
 //
-//   mark     in-segment children of the frontier and the incoming marks are
-//            OR-ed into the 4-bit op-flag array                     (:63-99)
-//   scan     ScanBitsAndClear in two passes over the touched regions: counts
-//            per region, an exclusive scan, then the ordered write of
-//            (id, op bits) for entries with an unused move and the clear (:116-123)
+//   mark     in-segment children of the frontier and the incoming marks, as
+//            the marks of the 4-bit op-flag array                    (:63-99):
+//            counted per region of 2^18 states, placed in a mark list by
+//            region (fs_kernels.h)
+//   scan     ScanBitsAndClear (:116-123) per region in LDS: the region's
+//            flags built from its marks and counted, an exclusive scan of
+//            the output counts, then the flags rebuilt and the (id, op bits)
+//            of the entries with an unused move written in id order
 //   stream   the new frontier as StreamVInt with bounds in 2^18-value chunks,
 //            as CompressedFrontierWriter writes it (Common/CompressedFrontier.cpp:54-75)
 //   cross    children of the written entries in other segments, one slot per
@@ -54,17 +57,39 @@ uint64_t svi_worst(uint64_t m, bool bounds) {
 
 namespace psk {
 
-// incoming cross-segment marks: NextArray.Set(idx, op) for every (op, idx)
-__global__ void k_fs_mark_xseg(FsXsegArgs a) {
-    const uint64_t total = a.op_starts[a.ops];
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        int op = 0;
-        while (op + 1 < a.ops && a.op_starts[op + 1] <= i) ++op;
-        const uint64_t c = a.idx[i] & a.seg_mask;
-        gor(&a.flags[c >> 3], (1u << op) << ((c & 7) * 4));
-        a.dirty[c >> FS_REGION_BITS] = 1;
+// Column pass over the (workgroup, region) table: rcount[r] = marks of region
+// r, and every entry replaced by the marks of the rows above it (the
+// workgroup's first position in the region).  Workgroups of 16 waves take 64
+// regions; wave w sums rows [32w, 32w + 32).
+__global__ __launch_bounds__(FS_COL_WAVES * 64) void k_fs_bin_columns(uint32_t* table, uint32_t rows,
+                                                                      uint32_t nregions, uint32_t* rcount) {
+    __shared__ uint32_t part[FS_COL_WAVES][64];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint32_t r = blockIdx.x * 64u + (uint32_t)lane;
+    const bool ok = r < nregions;
+    uint32_t v[FS_COL_ROWS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < FS_COL_ROWS; ++k) {
+        const uint32_t b = (uint32_t)(wave * FS_COL_ROWS + k);
+        v[k] = (ok && b < rows) ? table[(uint64_t)b * nregions + r] : 0u;
+        sum += v[k];
     }
+    part[wave][lane] = sum;
+    __syncthreads();
+    uint32_t run = 0, total = 0;
+    for (int w = 0; w < FS_COL_WAVES; ++w) {
+        const uint32_t t = part[w][lane];
+        run += (w < wave) ? t : 0u;
+        total += t;
+    }
+#pragma unroll
+    for (int k = 0; k < FS_COL_ROWS; ++k) {
+        const uint32_t b = (uint32_t)(wave * FS_COL_ROWS + k);
+        if (ok && b < rows) table[(uint64_t)b * nregions + r] = run;
+        run += v[k];
+    }
+    if (ok && wave == 0) rcount[r] = total;
 }
 
 __global__ void k_fs_clear_odd(const uint32_t* idx, uint64_t n, uint64_t in_mask, unsigned long long* odd) {
@@ -85,14 +110,17 @@ struct ps_fs_segment {
     uint32_t nseg = 0;
     int ops = 0, slots = 0, odd = 0;
     uint64_t nwords = 0, nregions = 0;
-    uint32_t* flags = nullptr;
-    uint8_t* dirty = nullptr;
+    int rb = 0;                                    // log2(states per region)
+    uint32_t* rcount = nullptr;                    // marks per region
+    unsigned long long* rstart = nullptr;          // nregions + 1
+    uint32_t* cnt_all = nullptr;                   // entries per region
     unsigned long long* odd_bits = nullptr;
     unsigned long long* cnt_out = nullptr;
     unsigned long long* out_off = nullptr;
     unsigned long long* counters = nullptr;
     unsigned long long* h_counters = nullptr;  // pinned
-    GrowBuf out_idx, out_ops, fr_bytes, slot_buf, compact, vals, xs_bytes, xseg_starts, block_cnt, block_off;
+    GrowBuf out_idx, out_ops, fr_bytes, slot_buf, compact, vals, xs_bytes, xseg_starts, block_cnt, block_off, marks,
+        bin_table;
     uint32_t cus = 256;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // results of the last step (host)
@@ -110,11 +138,12 @@ int destroy_fs(ps_fs_segment* w) {
     if (!w) return PS_OK;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     for (GrowBuf* g : {&w->out_idx, &w->out_ops, &w->fr_bytes, &w->slot_buf, &w->compact, &w->vals, &w->xs_bytes,
-                       &w->xseg_starts, &w->block_cnt, &w->block_off})
+                       &w->xseg_starts, &w->block_cnt, &w->block_off, &w->marks, &w->bin_table})
         g->release(w->stream);
     if (w->stream) (void)hipStreamSynchronize(w->stream);
-    if (w->flags) (void)hipFree(w->flags);
-    if (w->dirty) (void)hipFree(w->dirty);
+    if (w->rcount) (void)hipFree(w->rcount);
+    if (w->rstart) (void)hipFree(w->rstart);
+    if (w->cnt_all) (void)hipFree(w->cnt_all);
     if (w->odd_bits) (void)hipFree(w->odd_bits);
     if (w->cnt_out) (void)hipFree(w->cnt_out);
     if (w->out_off) (void)hipFree(w->out_off);
@@ -130,7 +159,18 @@ int destroy_fs(ps_fs_segment* w) {
 
 int read_counters(ps_fs_segment* w) {
     PS_HIP_CHECK(hipMemcpyAsync(w->h_counters, w->counters, NCOUNTERS * 8, hipMemcpyDeviceToHost, w->stream));
-    PS_HIP_CHECK(hipStreamSynchronize(w->stream));
+    return stream_wait(w->stream);
+}
+
+// the region kernels need more than 64 KB of LDS for regions of 2^18 states
+int region_lds_setup(size_t bytes) {
+    static size_t set = 0;
+    if (bytes <= 65536 || bytes <= set) return PS_OK;
+    PS_HIP_CHECK(hipFuncSetAttribute((const void*)psk::k_fs_region<psk::FS_REGION_TPB, false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    PS_HIP_CHECK(hipFuncSetAttribute((const void*)psk::k_fs_region<psk::FS_REGION_TPB, true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    set = bytes;
     return PS_OK;
 }
 
@@ -146,67 +186,89 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
     const uint32_t grid_par = (uint32_t)std::min<uint64_t>(blocks_for(n, FS_TPB), (uint64_t)w->cus * 8);
     uint64_t bad_ids = 0;
 
-    // 1 mark: in-segment children of the frontier, then the incoming marks
-    if (n) {
-        psk::FsMarkArgs a{};
-        a.prm = params_of(w->puzzle);
-        a.idx = d_idx;
-        a.ops = d_ops;
-        a.n = n;
-        a.base = base;
-        a.segment = segment;
-        a.seg_bits = w->seg_bits;
-        a.flags = w->flags;
-        a.dirty = w->dirty;
-        a.odd = w->odd ? w->odd_bits : nullptr;
-        a.counters = w->counters;
-        if (int r = fs_mark_in(w->puzzle, a, grid_par, s)) return r;
-    }
-    uint64_t nx = xstarts ? xstarts[w->ops] : 0;
+    // 1 mark: in-segment children of the frontier and the incoming marks,
+    //   counted per region, then placed in the mark list by region
+    const uint64_t nx = xstarts ? xstarts[w->ops] : 0;
+    const uint64_t max_marks = n * (uint64_t)w->slots + nx;
+    if (max_marks > 0xFFFFFFFFull)
+        return fail(PS_ERR_CAPACITY, "%llu marks in one step exceed 2^32", (unsigned long long)max_marks);
+    psk::FsBinArgs a{};
+    a.prm = params_of(w->puzzle);
+    a.idx = d_idx;
+    a.ops = d_ops;
+    a.n = n;
+    a.base = base;
+    a.segment = segment;
+    a.seg_bits = w->seg_bits;
+    a.rb = w->rb;
+    a.nregions = (uint32_t)w->nregions;
+    a.nx = nx;
+    a.nops = w->ops;
     if (nx) {
         if (int r = w->xseg_starts.ensure((size_t)(w->ops + 1) * 8, s)) return r;
         PS_HIP_CHECK(hipMemcpyAsync(w->xseg_starts.p, xstarts, (size_t)(w->ops + 1) * 8, hipMemcpyHostToDevice, s));
-        psk::FsXsegArgs x{};
-        x.idx = d_xidx;
-        x.op_starts = w->xseg_starts.as<uint64_t>();
-        x.ops = w->ops;
-        x.seg_mask = seg_mask;
-        x.flags = w->flags;
-        x.dirty = w->dirty;
-        hipLaunchKernelGGL(psk::k_fs_mark_xseg, dim3((uint32_t)std::min<uint64_t>(blocks_for(nx, 256), w->cus * 8ull)),
-                           dim3(256), 0, s, x);
-        PS_HIP_CHECK(hipGetLastError());
+        a.xidx = d_xidx;
+        a.xstarts = w->xseg_starts.as<uint64_t>();
+    }
+    a.odd = w->odd ? w->odd_bits : nullptr;
+    a.counters = w->counters;
+    // workgroups of the bin passes: two per CU, each with at least one pass of its threads
+    const uint32_t grid_bin = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>({blocks_for(std::max(n, nx), psk::FS_BIN_TPB), (uint64_t)w->cus * 2,
+                               (uint64_t)psk::FS_MAX_BIN_BLOCKS}));
+    a.per = (n + grid_bin - 1) / grid_bin;
+    a.perx = (nx + grid_bin - 1) / grid_bin;
+    if (int r = w->bin_table.ensure((size_t)grid_bin * w->nregions * 4, s)) return r;
+    a.table = w->bin_table.as<uint32_t>();
+    if (int r = fs_bin(w->puzzle, a, 0, grid_bin, s)) return r;
+    hipLaunchKernelGGL(psk::k_fs_bin_columns, dim3((uint32_t)((w->nregions + 63) / 64)),
+                       dim3(psk::FS_COL_WAVES * 64), 0, s, a.table, grid_bin, (uint32_t)w->nregions, w->rcount);
+    PS_HIP_CHECK(hipGetLastError());
+    if (int r = scan_excl_u32(w->rcount, w->nregions, w->rstart, s)) return r;
+    if (int r = w->marks.ensure(std::max<uint64_t>(max_marks, 1) * 4, s)) return r;
+    if (n || nx) {
+        a.odd = nullptr;
+        a.rstart = w->rstart;
+        a.marks = w->marks.as<uint32_t>();
+        if (int r = fs_bin(w->puzzle, a, 1, grid_bin, s)) return r;
     }
 
-    // 2 scan: count per region, exclusive scan, ordered write + clear
-    psk::FsScanArgs sa{};
-    sa.flags_c = w->flags;
-    sa.flags = w->flags;
-    sa.dirty = w->dirty;
-    sa.odd = w->odd ? w->odd_bits : nullptr;
-    sa.nwords = w->nwords;
-    sa.ops = w->ops;
-    sa.cnt_out = w->cnt_out;
-    sa.counters = w->counters;
-    sa.out_off = w->out_off;
-    const uint32_t grid_scan = (uint32_t)std::min<uint64_t>(w->nregions, (uint64_t)w->cus * 8);
-    hipLaunchKernelGGL(psk::k_fs_count<FS_TPB>, dim3(grid_scan), dim3(FS_TPB), 0, s, sa);
+    // 2 scan: per region, the flags built in LDS from its marks and counted;
+    //   an exclusive scan of the output counts; the ordered write
+    psk::FsRegionArgs ra{};
+    ra.nregions = (uint32_t)w->nregions;
+    ra.marks = w->marks.as<uint32_t>();
+    ra.rstart = w->rstart;
+    ra.odd = w->odd ? w->odd_bits : nullptr;
+    ra.rb = w->rb;
+    ra.ops = w->ops;
+    ra.cnt_out = w->cnt_out;
+    ra.cnt_all = w->cnt_all;
+    ra.out_off = w->out_off;
+    const uint32_t rw = (1u << w->rb) / 8u;
+    const size_t lds_region = ((size_t)rw + (rw + 31) / 32) * 4;  // flags and their nonzero-word bitmap
+    if (int r = region_lds_setup(lds_region)) return r;
+    const uint32_t grid_region = (uint32_t)std::min<uint64_t>(w->nregions, w->cus);
+    hipLaunchKernelGGL((psk::k_fs_region<psk::FS_REGION_TPB, false>), dim3(grid_region),
+                       dim3(psk::FS_REGION_TPB), lds_region, s, ra);
+    PS_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(psk::k_fs_region_totals<1024>, dim3(1), dim3(1024), 0, s, (const uint32_t*)w->cnt_all,
+                       (const unsigned long long*)w->rstart, (uint32_t)w->nregions, rw, w->counters);
     PS_HIP_CHECK(hipGetLastError());
     if (int r = scan_excl_u64(w->cnt_out, w->nregions, w->out_off, s)) return r;
+    PS_HIP_CHECK(hipMemcpyAsync(w->counters + 2, w->out_off + w->nregions, 8, hipMemcpyDeviceToDevice, s));
     if (int r = read_counters(w)) return r;
-    if (w->h_counters[6]) {
-        // the marks of the valid ids are still cleared by the write pass below
-        bad_ids = w->h_counters[6];
-    }
+    if (w->h_counters[6]) bad_ids = w->h_counters[6];
     const uint64_t children_in = w->h_counters[0];
     const uint64_t count = w->h_counters[1];
     const uint64_t n_out = w->h_counters[2];
     const uint64_t scanned = w->h_counters[5];
     if (int r = w->out_idx.ensure(std::max<uint64_t>(n_out, 1) * 4, s)) return r;
     if (int r = w->out_ops.ensure(std::max<uint64_t>(n_out, 1), s)) return r;
-    sa.out_idx = w->out_idx.as<uint32_t>();
-    sa.out_ops = w->out_ops.as<uint8_t>();
-    hipLaunchKernelGGL(psk::k_fs_write<FS_TPB>, dim3(grid_scan), dim3(FS_TPB), 0, s, sa);
+    ra.out_idx = w->out_idx.as<uint32_t>();
+    ra.out_ops = w->out_ops.as<uint8_t>();
+    hipLaunchKernelGGL((psk::k_fs_region<psk::FS_REGION_TPB, true>), dim3(grid_region),
+                       dim3(psk::FS_REGION_TPB), lds_region, s, ra);
     PS_HIP_CHECK(hipGetLastError());
     if (w->odd && n) {
         hipLaunchKernelGGL(psk::k_fs_clear_odd, dim3(grid_par), dim3(256), 0, s, d_idx, n, seg_mask, w->odd_bits);
@@ -320,7 +382,7 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
         }
     }
     PS_HIP_CHECK(hipEventRecord(w->ev1, s));
-    PS_HIP_CHECK(hipEventSynchronize(w->ev1));
+    if (int r = event_wait(w->ev1)) return r;
     float ms = 0.f;
     PS_HIP_CHECK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
 
@@ -380,7 +442,10 @@ int ps_fs_segment_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st,
     w->ops = info.operators_count;
     w->odd = info.has_odd_cycles;
     w->nwords = seg_size / 8;
-    w->nregions = (seg_size + (1ull << psk::FS_REGION_BITS) - 1) >> psk::FS_REGION_BITS;
+    w->rb = std::min(psk::FS_BIN_BITS, segment_bits);
+    w->nregions = seg_size >> w->rb;
+    if (w->nregions > psk::FS_MAX_REGIONS) return cleanup(fail(PS_ERR_INTERNAL, "%llu regions exceed %u",
+                                                              (unsigned long long)w->nregions, psk::FS_MAX_REGIONS));
     w->cus = (uint32_t)hip_cu_count();
     if (int r = fs_cross(*p, psk::FsCrossArgs{}, 0, nullptr, &w->slots)) return cleanup(r);
     if (st) {
@@ -400,8 +465,9 @@ int ps_fs_segment_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st,
         }
         return PS_OK;
     };
-    if (int r = dalloc((void**)&w->flags, w->nwords * 4)) return cleanup(r);
-    if (int r = dalloc((void**)&w->dirty, w->nregions)) return cleanup(r);
+    if (int r = dalloc((void**)&w->rcount, w->nregions * 4)) return cleanup(r);
+    if (int r = dalloc((void**)&w->rstart, w->nregions * 8 + 8)) return cleanup(r);
+    if (int r = dalloc((void**)&w->cnt_all, w->nregions * 4)) return cleanup(r);
     if (w->odd)
         if (int r = dalloc((void**)&w->odd_bits, (seg_size + 63) / 64 * 8)) return cleanup(r);
     if (int r = dalloc((void**)&w->cnt_out, w->nregions * 8)) return cleanup(r);
@@ -411,8 +477,6 @@ int ps_fs_segment_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st,
         return cleanup(fail(PS_ERR_OUT_OF_MEMORY, "pinned host allocation failed"));
     if (hipEventCreate(&w->ev0) != hipSuccess || hipEventCreate(&w->ev1) != hipSuccess)
         return cleanup(fail(PS_ERR_HIP, "event creation failed"));
-    if (int r = launch_fill(w->flags, 0, w->nwords * 4, w->stream)) return cleanup(r);
-    if (int r = launch_fill(w->dirty, 0, w->nregions, w->stream)) return cleanup(r);
     if (w->odd)
         if (int r = launch_fill(w->odd_bits, 0, (seg_size + 63) / 64 * 8, w->stream)) return cleanup(r);
     if (hipStreamSynchronize(w->stream) != hipSuccess) return cleanup(fail(PS_ERR_HIP, "stream synchronize failed"));
@@ -434,7 +498,7 @@ int ps_fs_segment_info(const ps_fs_segment* w, uint32_t* segments, int32_t* oper
     if (segments) *segments = w->nseg;
     if (operators) *operators = w->ops;
     if (footprint_bytes)
-        *footprint_bytes = w->nwords * 4 + w->nregions * 17 + (w->odd ? ((1ull << w->seg_bits) + 63) / 64 * 8 : 0);
+        *footprint_bytes = w->nregions * 32 + (w->odd ? ((1ull << w->seg_bits) + 63) / 64 * 8 : 0);
     return PS_OK;
     PS_API_END
 }

@@ -92,8 +92,7 @@ int destroy_s3(ps_seg3* w) {
 
 int read_counters(ps_seg3* w) {
     PS_HIP_CHECK(hipMemcpyAsync(w->h_counters, w->counters, psk::S3_NCOUNTERS * 8, hipMemcpyDeviceToHost, w->stream));
-    PS_HIP_CHECK(hipStreamSynchronize(w->stream));
-    return PS_OK;
+    return stream_wait(w->stream);
 }
 
 uint32_t grid_for(const ps_seg3* w, uint64_t n, uint32_t tpb) {
@@ -293,7 +292,7 @@ int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur,
         }
     }
     PS_HIP_CHECK(hipEventRecord(w->ev1, s));
-    PS_HIP_CHECK(hipEventSynchronize(w->ev1));
+    if (int r = event_wait(w->ev1)) return r;
     float ms = 0.f;
     PS_HIP_CHECK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
 

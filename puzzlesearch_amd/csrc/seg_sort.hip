@@ -342,7 +342,7 @@ int stable_group_by_key(const ull* d_kv, uint64_t m, uint32_t key_space, uint32_
     PS_HIP_CHECK(hipGetLastError());
     ull h[2] = {0, 0};
     PS_HIP_CHECK(hipMemcpyAsync(h, small.p, 16, hipMemcpyDeviceToHost, s));
-    PS_HIP_CHECK(hipStreamSynchronize(s));
+    if (int r = stream_wait(s)) return r;
     const uint32_t nd = (uint32_t)h[1];
     if (dropped) *dropped = h[0];
     if (nd == 0) return PS_OK;
