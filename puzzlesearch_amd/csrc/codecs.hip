@@ -140,11 +140,14 @@ __global__ __launch_bounds__(TPB) void k_chunk_scan_u32(uint32_t* vals, const ui
     if (threadIdx.x == 0) chunk_total[c] = carry;
 }
 
+// skip (optional): chunks written by another encoder (bitmap chunks)
 __global__ void k_svi_write(const uint32_t* v, const uint8_t* bounds, SviPlan pl, const uint32_t* tuple_off,
-                            const uint64_t* chunk_out, const uint64_t* chunk_payload, uint8_t* out) {
+                            const uint64_t* chunk_out, const uint64_t* chunk_payload, uint8_t* out,
+                            const uint32_t* skip) {
     uint64_t t;
     uint32_t c;
     if (!svi_tuple(pl, t, c)) return;
+    if (skip && skip[c]) return;
     const uint32_t lt = (uint32_t)(t - pl.tuple_base[c]);
     const uint64_t start = pl.starts[c];
     const uint32_t count = pl.counts[c];
@@ -646,6 +649,76 @@ __global__ void k_fc_bitmap_write(const uint32_t* select, uint32_t nsel, const u
     for (uint64_t i = threadIdx.x; i < nw * 8; i += blockDim.x) o[8 + i] = src[i];
 }
 
+// ---------------------------------------------------------------- stream-ordered chunk plans
+
+// chunk c's StreamVInt bytes: count, payload and (with bounds) the 4-bit
+// bounds of the count rounded up to 8; nothing for an empty chunk
+__global__ void k_svi_chunk_bytes(SviPlan pl, const uint64_t* payload, int with_bounds, uint64_t* bytes) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= pl.nchunks) return;
+    const uint64_t cnt = pl.counts[c];
+    bytes[c] = cnt ? 4 + payload[c] + (with_bounds ? (cnt + 7) / 8 * 8 / 2 : 0) : 0;
+}
+
+// FrontierCompression's choice per chunk (Common/FrontierCompression.cpp:92-115):
+// a bitmap when count > 128 and the bitmap has fewer bits than 10 per value
+// (minus 8 without the check); an unchecked bitmap chunk that is not sorted
+// counts in *err.
+__global__ void k_fc_plan(SviPlan pl, const uint64_t* payload, const uint32_t* first, const uint32_t* last,
+                          const uint32_t* ok, int with_check, uint32_t* is_bitmap, uint32_t* words, uint64_t* bytes,
+                          uint32_t* err) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= pl.nchunks) return;
+    const uint64_t count = pl.counts[c];
+    bool use_bitmap = false;
+    uint64_t nw = 0;
+    if (count > 128) {
+        const uint64_t min_bits = count * 10;
+        const uint64_t f = first[c], l = last[c];
+        const uint64_t bits = (l < f) ? ~0ull : ((l - f + 63) / 64 * 8 + 8) * 8;
+        if (with_check) use_bitmap = bits < min_bits && ok[c];
+        else use_bitmap = bits < min_bits - 8 && l >= f;
+        if (!with_check && use_bitmap && !ok[c]) atomicAdd(err, 1u);
+        if (use_bitmap) nw = (l - f + 63) / 64;
+    }
+    is_bitmap[c] = use_bitmap ? 1u : 0u;
+    words[c] = (uint32_t)nw;
+    bytes[c] = use_bitmap ? 8 + nw * 8 : (count ? 4 + payload[c] : 0);
+}
+
+// One block per chunk, bitmap chunks only: the bitmap of the values after the
+// first, built in LDS and written after the chunk's 8-byte header.
+constexpr uint32_t FC_LDS_WORDS = 2048;
+constexpr uint32_t FC_DEV_MAX_COUNT = 13107;  // a bitmap chunk of at most this many values has < 2048 words
+
+__global__ __launch_bounds__(256) void k_fc_bitmap_chunk(const uint32_t* v, SviPlan pl, const uint32_t* is_bitmap,
+                                                         const uint32_t* words, const uint32_t* first,
+                                                         const uint64_t* out_off, uint8_t* out) {
+    const uint32_t c = blockIdx.x;
+    if (!is_bitmap[c]) return;
+    __shared__ unsigned long long bits[FC_LDS_WORDS];
+    const uint32_t nw = words[c];
+    const uint32_t f = first[c];
+    const uint64_t b = pl.starts[c], e = b + pl.counts[c];
+    for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) bits[i] = 0ull;
+    __syncthreads();
+    for (uint64_t i = b + 1 + threadIdx.x; i < e; i += blockDim.x) {
+        const uint32_t val = v[i] - f - 1;  // in [0, 64 nw): every value lies in (first, last]
+        atomicOr(&bits[val >> 6], 1ull << (val & 63));
+    }
+    __syncthreads();
+    uint8_t* o = out + out_off[c];
+    if (threadIdx.x == 0) {
+        const uint32_t h = nw | CODEC_MAP;
+        for (int j = 0; j < 4; ++j) {
+            o[j] = (uint8_t)(h >> (8 * j));
+            o[4 + j] = (uint8_t)(f >> (8 * j));
+        }
+    }
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(bits);
+    for (uint32_t i = threadIdx.x; i < nw * 8; i += blockDim.x) o[8 + i] = src[i];
+}
+
 __global__ void k_fc_bitmap_count(const uint8_t* in, const uint64_t* in_off, const uint32_t* select, uint32_t nsel,
                                   const uint32_t* words, uint32_t* counts) {
     const uint32_t k = blockIdx.x;
@@ -814,7 +887,8 @@ int svi_write(const uint32_t* d_values, const uint8_t* d_bounds, const SviPlan& 
     if (int r = to_device(d_off, out_off_sel, s)) return r;
     if (pl.ntuples)
         hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, d_bounds, pl,
-                           d_toff.as<uint32_t>(), d_off.as<uint64_t>(), d_payload.as<uint64_t>(), d_out);
+                           d_toff.as<uint32_t>(), d_off.as<uint64_t>(), d_payload.as<uint64_t>(), d_out,
+                           (const uint32_t*)nullptr);
     PS_HIP_CHECK(hipGetLastError());
     if (int r = stream_wait(s)) return r;
     return PS_OK;
@@ -954,6 +1028,115 @@ int fc_encode_chunks(const uint32_t* d_values, const std::vector<uint64_t>& star
         PS_HIP_CHECK(hipGetLastError());
         if (int r = stream_wait(s)) return r;
     }
+    return PS_OK;
+}
+
+namespace {
+
+// Plan tables of every chunk of `starts` in one device buffer (chunk starts,
+// tuple bases, counts), copied from the host in one transfer.
+int svi_plan_all(const std::vector<uint64_t>& starts, hipStream_t s, DevBuf& tab, SviPlan& pl) {
+    const uint32_t n = (uint32_t)(starts.size() - 1);
+    std::vector<uint64_t> h((size_t)2 * (n + 1) + (n + 1) / 2 + 1, 0);
+    uint64_t* st = h.data();
+    uint64_t* tb = st + n + 1;
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(tb + n + 1);
+    uint64_t t = 0, max_t = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint64_t c0 = starts[k], c1 = starts[k + 1];
+        if (c1 < c0) return fail(PS_ERR_INVALID_ARGUMENT, "chunk starts must not decrease");
+        if (c1 - c0 > MAX_CHUNK_VALUES)
+            return fail(PS_ERR_INVALID_ARGUMENT, "chunk of %llu values exceeds %u", (unsigned long long)(c1 - c0),
+                        MAX_CHUNK_VALUES);
+        st[k] = c0;
+        cnt[k] = (uint32_t)(c1 - c0);
+        tb[k] = t;
+        const uint64_t nt = (uint64_t)((cnt[k] + 7) / 8 * 8) / 4;
+        t += nt;
+        max_t = std::max(max_t, nt);
+    }
+    st[n] = starts[n];
+    tb[n] = t;
+    if (int r = to_device(tab, h, s)) return r;
+    const bool g2 = n > 0 && n <= 65535 && ((max_t + 255) / 256) * 256 * n <= 2 * t + 256ull * n;
+    uint64_t* base = tab.as<uint64_t>();
+    pl = SviPlan{base, base + n + 1, reinterpret_cast<const uint32_t*>(base + 2 * (n + 1)), n, t, (uint32_t)max_t,
+                 g2 ? 1 : 0};
+    return PS_OK;
+}
+
+// tuple sizes and the per-chunk payload sums of a plan
+int svi_payload(const uint32_t* d_values, const SviPlan& pl, hipStream_t s, DevBuf& d_toff, DevBuf& d_pay) {
+    if (int r = d_toff.alloc((pl.ntuples + 1) * 4, s)) return r;
+    if (int r = d_pay.alloc(((uint64_t)pl.nchunks + 1) * 8, s)) return r;
+    if (pl.ntuples)
+        hipLaunchKernelGGL(k_svi_tuple_sizes, svi_grid(pl), dim3(256), 0, s, d_values, pl, d_toff.as<uint32_t>());
+    if (pl.nchunks)
+        hipLaunchKernelGGL(k_chunk_scan_u32<256>, dim3(pl.nchunks), dim3(256), 0, s, d_toff.as<uint32_t>(),
+                           pl.tuple_base, d_pay.as<uint64_t>());
+    PS_HIP_CHECK(hipGetLastError());
+    return PS_OK;
+}
+
+}  // namespace
+
+int svi_encode_chunks_dev(const uint32_t* d_values, const uint8_t* d_bounds, const std::vector<uint64_t>& starts,
+                          uint8_t* d_out, uint64_t* d_offsets, hipStream_t s) {
+    DevBuf tab, d_toff, d_pay, d_bytes;
+    SviPlan pl{};
+    if (int r = svi_plan_all(starts, s, tab, pl)) return r;
+    if (int r = svi_payload(d_values, pl, s, d_toff, d_pay)) return r;
+    const uint32_t n = pl.nchunks;
+    if (int r = d_bytes.alloc(((uint64_t)n + 1) * 8, s)) return r;
+    if (n)
+        hipLaunchKernelGGL(k_svi_chunk_bytes, dim3(blocks_for(n, 256)), dim3(256), 0, s, pl,
+                           (const uint64_t*)d_pay.as<uint64_t>(), d_bounds ? 1 : 0, d_bytes.as<uint64_t>());
+    PS_HIP_CHECK(hipGetLastError());
+    if (int r = scan_excl_u64(d_bytes.as<unsigned long long>(), n, (unsigned long long*)d_offsets, s)) return r;
+    if (pl.ntuples)
+        hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, d_bounds, pl,
+                           (const uint32_t*)d_toff.as<uint32_t>(), (const uint64_t*)d_offsets,
+                           (const uint64_t*)d_pay.as<uint64_t>(), d_out, (const uint32_t*)nullptr);
+    PS_HIP_CHECK(hipGetLastError());
+    return PS_OK;
+}
+
+int fc_encode_chunks_dev(const uint32_t* d_values, const std::vector<uint64_t>& starts, int with_check,
+                         uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, uint32_t* d_err, hipStream_t s) {
+    const uint32_t n = (uint32_t)(starts.size() - 1);
+    for (uint32_t c = 0; c < n; ++c)
+        if (starts[c + 1] - starts[c] > FC_DEV_MAX_COUNT) {  // bitmaps too wide for LDS: the host-planned encoder
+            std::vector<uint64_t> off;
+            return fc_encode_chunks(d_values, starts, with_check, d_out, out_cap, off, d_offsets, s);
+        }
+    DevBuf tab, d_toff, d_pay, d_fl, d_bytes;
+    SviPlan pl{};
+    if (int r = svi_plan_all(starts, s, tab, pl)) return r;
+    // first, last, ok, is_bitmap, words: five arrays of n
+    if (int r = d_fl.alloc((size_t)5 * n * 4 + 4, s)) return r;
+    uint32_t* first = d_fl.as<uint32_t>();
+    uint32_t* last = first + n;
+    uint32_t* ok = last + n;
+    uint32_t* is_bm = ok + n;
+    uint32_t* words = is_bm + n;
+    if (n)
+        hipLaunchKernelGGL(k_fc_inspect<1024>, dim3(n), dim3(1024), 0, s, d_values, pl.starts, first, last, ok);
+    if (int r = svi_payload(d_values, pl, s, d_toff, d_pay)) return r;
+    if (int r = d_bytes.alloc(((uint64_t)n + 1) * 8, s)) return r;
+    if (n)
+        hipLaunchKernelGGL(k_fc_plan, dim3(blocks_for(n, 256)), dim3(256), 0, s, pl,
+                           (const uint64_t*)d_pay.as<uint64_t>(), (const uint32_t*)first, (const uint32_t*)last,
+                           (const uint32_t*)ok, with_check, is_bm, words, d_bytes.as<uint64_t>(), d_err);
+    PS_HIP_CHECK(hipGetLastError());
+    if (int r = scan_excl_u64(d_bytes.as<unsigned long long>(), n, (unsigned long long*)d_offsets, s)) return r;
+    if (pl.ntuples)
+        hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, (const uint8_t*)nullptr, pl,
+                           (const uint32_t*)d_toff.as<uint32_t>(), (const uint64_t*)d_offsets,
+                           (const uint64_t*)d_pay.as<uint64_t>(), d_out, (const uint32_t*)is_bm);
+    if (n)
+        hipLaunchKernelGGL(k_fc_bitmap_chunk, dim3(n), dim3(256), 0, s, d_values, pl, (const uint32_t*)is_bm,
+                           (const uint32_t*)words, (const uint32_t*)first, (const uint64_t*)d_offsets, d_out);
+    PS_HIP_CHECK(hipGetLastError());
     return PS_OK;
 }
 

@@ -1,6 +1,7 @@
 // Scratch device buffers and host<->device vector copies for the host-side
-// orchestration code (codecs, segment step).  Allocation is stream-ordered
-// (hipMallocAsync / hipFreeAsync on the owning stream).
+// orchestration code (codecs, segment step).  Allocation is stream-ordered,
+// from the library's scratch pool, and freed with hipFreeAsync on the owning
+// stream.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,13 +27,7 @@ struct DevBuf {
     }
     int alloc(size_t bytes, hipStream_t st) {
         s = st;
-        hipError_t e = hipMallocAsync(&p, bytes ? bytes : 16, st);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            return fail(PS_ERR_OUT_OF_MEMORY, "device scratch of %zu bytes: %s", bytes, hipGetErrorString(e));
-        }
-        return PS_OK;
+        return scratch_alloc(&p, bytes ? bytes : 16, st);
     }
     template <class T>
     T* as() const {
@@ -94,6 +89,37 @@ inline uint32_t blocks_for(uint64_t n, uint32_t tpb) {
     return (uint32_t)std::max<uint64_t>(1, (n + tpb - 1) / tpb);
 }
 
+// A grow-only pinned host buffer of a workspace, the target of the
+// read-backs that complete with the step; regrown only between steps, when
+// no copy into it is pending.
+struct GrowHost {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap && p) return PS_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 4, 256);
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return fail(PS_ERR_OUT_OF_MEMORY, "pinned host buffer of %zu bytes", want);
+        }
+        cap = want;
+        return PS_OK;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
 // A grow-only device buffer of a workspace (segment steps): reallocated on
 // its stream, with a quarter to spare, only when a step needs more.
 struct GrowBuf {
@@ -105,12 +131,7 @@ struct GrowBuf {
         p = nullptr;
         cap = 0;
         const size_t want = std::max<size_t>(bytes + bytes / 4, 256);
-        hipError_t e = hipMallocAsync(&p, want, s);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            return fail(PS_ERR_OUT_OF_MEMORY, "segment step buffer of %zu bytes: %s", want, hipGetErrorString(e));
-        }
+        if (int r = scratch_alloc(&p, want, s)) return r;
         cap = want;
         return PS_OK;
     }

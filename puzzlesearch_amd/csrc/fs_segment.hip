@@ -28,6 +28,7 @@
 #include <cstring>
 #include <new>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "codec_internal.h"
@@ -46,7 +47,7 @@ using ps::GrowBuf;
 // [0] in-segment children [1] count [2] written [3] cross children [4] children
 // outside the segments [5] words scanned [6] frontier ids outside the segment
 // [7] cross-segment slots filled
-constexpr int NCOUNTERS = 8;
+constexpr int NCOUNTERS = 9;  // [8]: unsorted cross-segment bitmap chunks
 
 uint64_t svi_worst(uint64_t m, bool bounds) {
     const uint64_t adj = (m + 7) / 8 * 8;
@@ -120,9 +121,14 @@ struct ps_fs_segment {
     unsigned long long* counters = nullptr;
     unsigned long long* h_counters = nullptr;  // pinned
     GrowBuf out_idx, out_ops, fr_bytes, slot_buf, compact, vals, xs_bytes, xseg_starts, block_cnt, block_off, marks,
-        bin_table;
+        bin_table, fr_off, x_off;
+    ps::GrowHost h_fr_off, h_x_off;  // chunk offsets of the frontier and cross-segment streams (pinned)
     uint32_t cus = 256;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the cross-segment children are found, grouped and encoded on `side`
+    // while the new frontier is encoded on `stream`
+    hipStream_t side = nullptr;
+    hipEvent_t ev_written = nullptr, ev_side = nullptr;
     // results of the last step (host)
     std::vector<uint64_t> fr_chunk_off;
     std::vector<uint32_t> bk_op, bk_seg;
@@ -138,7 +144,8 @@ int destroy_fs(ps_fs_segment* w) {
     if (!w) return PS_OK;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     for (GrowBuf* g : {&w->out_idx, &w->out_ops, &w->fr_bytes, &w->slot_buf, &w->compact, &w->vals, &w->xs_bytes,
-                       &w->xseg_starts, &w->block_cnt, &w->block_off, &w->marks, &w->bin_table})
+                       &w->xseg_starts, &w->block_cnt, &w->block_off, &w->marks, &w->bin_table, &w->fr_off,
+                       &w->x_off})
         g->release(w->stream);
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     if (w->rcount) (void)hipFree(w->rcount);
@@ -149,17 +156,23 @@ int destroy_fs(ps_fs_segment* w) {
     if (w->out_off) (void)hipFree(w->out_off);
     if (w->counters) (void)hipFree(w->counters);
     if (w->h_counters) (void)hipHostFree(w->h_counters);
+    w->h_fr_off.release();
+    w->h_x_off.release();
+    if (w->side) (void)hipStreamSynchronize(w->side);
     if (w->ev0) (void)hipEventDestroy(w->ev0);
     if (w->ev1) (void)hipEventDestroy(w->ev1);
+    if (w->ev_written) (void)hipEventDestroy(w->ev_written);
+    if (w->ev_side) (void)hipEventDestroy(w->ev_side);
+    if (w->side) (void)hipStreamDestroy(w->side);
     if (w->own_stream && w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
     (void)hipGetLastError();
     return PS_OK;
 }
 
-int read_counters(ps_fs_segment* w) {
-    PS_HIP_CHECK(hipMemcpyAsync(w->h_counters, w->counters, NCOUNTERS * 8, hipMemcpyDeviceToHost, w->stream));
-    return stream_wait(w->stream);
+int read_counters(ps_fs_segment* w, hipStream_t s) {
+    PS_HIP_CHECK(hipMemcpyAsync(w->h_counters, w->counters, NCOUNTERS * 8, hipMemcpyDeviceToHost, s));
+    return stream_wait(s);
 }
 
 // the region kernels need more than 64 KB of LDS for regions of 2^18 states
@@ -257,7 +270,7 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
     PS_HIP_CHECK(hipGetLastError());
     if (int r = scan_excl_u64(w->cnt_out, w->nregions, w->out_off, s)) return r;
     PS_HIP_CHECK(hipMemcpyAsync(w->counters + 2, w->out_off + w->nregions, 8, hipMemcpyDeviceToDevice, s));
-    if (int r = read_counters(w)) return r;
+    if (int r = read_counters(w, s)) return r;
     if (w->h_counters[6]) bad_ids = w->h_counters[6];
     const uint64_t children_in = w->h_counters[0];
     const uint64_t count = w->h_counters[1];
@@ -281,25 +294,18 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
                     w->seg_bits);
     }
 
-    // 3 the new frontier stream
-    std::vector<uint64_t> starts;
-    for (uint64_t c = 0; c < n_out; c += FS_FRONTIER_CHUNK) starts.push_back(c);
-    starts.push_back(n_out);  // no chunk at all for an empty frontier
-    uint64_t fr_cap = 0;
-    for (size_t c = 0; c + 1 < starts.size(); ++c) fr_cap += svi_worst(starts[c + 1] - starts[c], true);
-    if (int r = w->fr_bytes.ensure(std::max<uint64_t>(fr_cap, 16), s)) return r;
-    if (int r = svi_encode_chunks(w->out_idx.as<uint32_t>(), w->out_ops.as<uint8_t>(), starts, w->fr_bytes.as<uint8_t>(),
-                                  w->fr_bytes.cap, w->fr_chunk_off, s))
-        return r;
-
-    // 4 cross-segment children of the written entries, bucketed by (op, segment)
+    // 3 cross-segment children of the written entries, found on the side
+    //   stream while the main stream encodes the new frontier (4)
     const uint64_t nslots = n_out * (uint64_t)w->slots;
     if (nslots > 0x7FFFFFFFull)
         return fail(PS_ERR_CAPACITY, "%llu cross-segment slots in one step exceed 2^31",
                     (unsigned long long)nslots);
-    uint64_t m = 0;
+    hipStream_t s2 = w->side;
+    PS_HIP_CHECK(hipEventRecord(w->ev_written, s));
+    PS_HIP_CHECK(hipStreamWaitEvent(s2, w->ev_written, 0));
+    uint64_t nblk = 0;
     if (n_out) {
-        if (int r = w->slot_buf.ensure(nslots * 8, s)) return r;
+        if (int r = w->slot_buf.ensure(nslots * 8, s2)) return r;
         psk::FsCrossArgs ca{};
         ca.prm = params_of(w->puzzle);
         ca.idx = w->out_idx.as<uint32_t>();
@@ -311,14 +317,35 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
         ca.seg_bits = w->seg_bits;
         ca.slots = w->slot_buf.as<unsigned long long>();
         ca.counters = w->counters;
-        const uint64_t nblk = (n_out + psk::FS_CROSS_PER_BLOCK - 1) / psk::FS_CROSS_PER_BLOCK;
-        if (int r = w->block_cnt.ensure(nblk * 8, s)) return r;
-        if (int r = w->block_off.ensure((nblk + 1) * 8, s)) return r;
+        nblk = (n_out + psk::FS_CROSS_PER_BLOCK - 1) / psk::FS_CROSS_PER_BLOCK;
+        if (int r = w->block_cnt.ensure(nblk * 8, s2)) return r;
+        if (int r = w->block_off.ensure((nblk + 1) * 8, s2)) return r;
         ca.block_count = w->block_cnt.as<unsigned long long>();
-        if (int r = fs_cross(w->puzzle, ca, (uint32_t)nblk, s, nullptr)) return r;
-        if (int r = scan_excl_u64(w->block_cnt.as<unsigned long long>(), nblk, w->block_off.as<unsigned long long>(), s))
+        if (int r = fs_cross(w->puzzle, ca, (uint32_t)nblk, s2, nullptr)) return r;
+        if (int r = scan_excl_u64(w->block_cnt.as<unsigned long long>(), nblk, w->block_off.as<unsigned long long>(), s2))
             return r;
-        if (int r = read_counters(w)) return r;
+    }
+
+    // 4 the new frontier stream (main stream); its chunk offsets come back with the step
+    std::vector<uint64_t> starts;
+    for (uint64_t c = 0; c < n_out; c += FS_FRONTIER_CHUNK) starts.push_back(c);
+    starts.push_back(n_out);  // no chunk at all for an empty frontier
+    uint64_t fr_cap = 0;
+    for (size_t c = 0; c + 1 < starts.size(); ++c) fr_cap += svi_worst(starts[c + 1] - starts[c], true);
+    const size_t nfo = starts.size();  // chunk offsets
+    if (int r = w->fr_bytes.ensure(std::max<uint64_t>(fr_cap, 16), s)) return r;
+    if (int r = w->fr_off.ensure(nfo * 8, s)) return r;
+    if (int r = w->h_fr_off.ensure(nfo * 8)) return r;
+    if (int r = svi_encode_chunks_dev(w->out_idx.as<uint32_t>(), w->out_ops.as<uint8_t>(), starts,
+                                      w->fr_bytes.as<uint8_t>(), w->fr_off.as<uint64_t>(), s))
+        return r;
+    PS_HIP_CHECK(hipMemcpyAsync(w->h_fr_off.p, w->fr_off.p, nfo * 8, hipMemcpyDeviceToHost, s));
+
+    // 5 the cross-segment children compacted, bucketed by (op, segment) and
+    //   encoded (side stream); the bucket byte offsets come back with the step
+    uint64_t m = 0;
+    if (n_out) {
+        if (int r = read_counters(w, s2)) return r;
         if (w->h_counters[4])
             return fail(PS_ERR_INDEX_RANGE, "%llu children fall outside the %u segments",
                         (unsigned long long)w->h_counters[4], w->nseg);
@@ -326,8 +353,8 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
         if (w->h_counters[7] != m)
             return fail(PS_ERR_HIP, "%llu cross-segment slots filled for %llu cross-segment children",
                         (unsigned long long)w->h_counters[7], (unsigned long long)m);
-        if (int r = w->compact.ensure(std::max<uint64_t>(m, 1) * 8, s)) return r;
-        hipLaunchKernelGGL(psk::k_fs_compact<256>, dim3((uint32_t)nblk), dim3(256), 0, s,
+        if (int r = w->compact.ensure(std::max<uint64_t>(m, 1) * 8, s2)) return r;
+        hipLaunchKernelGGL(psk::k_fs_compact<256>, dim3((uint32_t)nblk), dim3(256), 0, s2,
                            (const unsigned long long*)w->slot_buf.as<unsigned long long>(),
                            (uint32_t)(psk::FS_CROSS_PER_BLOCK * w->slots),
                            (const unsigned long long*)w->block_cnt.as<unsigned long long>(),
@@ -339,28 +366,28 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
     w->bk_seg.clear();
     w->bk_values.clear();
     w->bk_byte_off.assign(1, 0);
-    std::vector<uint64_t> xoff;
-    uint64_t xbytes = 0, nchunks_x = 0;
+    std::vector<uint64_t> first_chunk;
+    uint64_t xbytes = 0, nchunks_x = 0, x_cap = 0;
     if (m) {
         // stable grouping by (op, segment): emission order inside a bucket
-        if (int r = w->vals.ensure(m * 4 + 64, s)) return r;
+        if (int r = w->vals.ensure(m * 4 + 64, s2)) return r;
         std::vector<uint32_t> uk;
         std::vector<uint64_t> rc;
         uint64_t dropped = 0;
         if (int r = stable_group_by_key(w->compact.as<unsigned long long>(), m, (uint32_t)(w->ops * w->nseg),
-                                        w->vals.as<uint32_t>(), uk, rc, &dropped, s))
+                                        w->vals.as<uint32_t>(), uk, rc, &dropped, s2))
             return r;
         if (dropped) return fail(PS_ERR_INTERNAL, "%llu cross-segment keys out of range", (unsigned long long)dropped);
         const int nb = (int)uk.size();
         // 4096-value chunks inside every bucket
         std::vector<uint64_t> cstarts;
-        std::vector<uint64_t> first_chunk(nb + 1);
-        uint64_t pos = 0, cap = 0;
+        first_chunk.assign(nb + 1, 0);
+        uint64_t pos = 0;
         for (int b = 0; b < nb; ++b) {
             first_chunk[b] = cstarts.size();
             for (uint64_t c = 0; c < rc[b]; c += FS_BUCKET_CHUNK) {
                 cstarts.push_back(pos + c);
-                cap += svi_worst(std::min<uint64_t>(FS_BUCKET_CHUNK, rc[b] - c), false);
+                x_cap += svi_worst(std::min<uint64_t>(FS_BUCKET_CHUNK, rc[b] - c), false);
             }
             pos += rc[b];
         }
@@ -368,23 +395,45 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
         cstarts.push_back(pos);
         if (pos != m) return fail(PS_ERR_HIP, "bucket sizes %llu != %llu children", (unsigned long long)pos,
                                   (unsigned long long)m);
-        if (int r = w->xs_bytes.ensure(cap + 16, s)) return r;
-        if (int r = fc_encode_chunks(w->vals.as<uint32_t>(), cstarts, 1, w->xs_bytes.as<uint8_t>(), w->xs_bytes.cap,
-                                     xoff, nullptr, s))
-            return r;
         nchunks_x = cstarts.size() - 1;
-        xbytes = xoff.back();
+        if (int r = w->xs_bytes.ensure(x_cap + 16, s2)) return r;
+        if (int r = w->x_off.ensure((nchunks_x + 1) * 8, s2)) return r;
+        if (int r = w->h_x_off.ensure((nchunks_x + 1) * 8)) return r;
+        if (int r = fc_encode_chunks_dev(w->vals.as<uint32_t>(), cstarts, 1, w->xs_bytes.as<uint8_t>(),
+                                         w->xs_bytes.cap, w->x_off.as<uint64_t>(),
+                                         reinterpret_cast<uint32_t*>(w->counters + 8), s2))
+            return r;
+        PS_HIP_CHECK(hipMemcpyAsync(w->h_x_off.p, w->x_off.p, (nchunks_x + 1) * 8, hipMemcpyDeviceToHost, s2));
         for (int b = 0; b < nb; ++b) {
             w->bk_op.push_back(uk[b] / w->nseg);
             w->bk_seg.push_back(uk[b] % w->nseg);
             w->bk_values.push_back(rc[b]);
-            w->bk_byte_off.push_back(xoff[first_chunk[b + 1]]);
         }
     }
+    // the main stream waits for the side stream's work
+    PS_HIP_CHECK(hipEventRecord(w->ev_side, s2));
+    PS_HIP_CHECK(hipStreamWaitEvent(s, w->ev_side, 0));
+    PS_HIP_CHECK(hipMemcpyAsync(w->h_counters + 8, w->counters + 8, 8, hipMemcpyDeviceToHost, s));
     PS_HIP_CHECK(hipEventRecord(w->ev1, s));
     if (int r = event_wait(w->ev1)) return r;
     float ms = 0.f;
     PS_HIP_CHECK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+    const uint64_t* hfo = w->h_fr_off.as<uint64_t>();
+    w->fr_chunk_off.assign(hfo, hfo + nfo);
+    if (w->fr_chunk_off.back() > w->fr_bytes.cap)
+        return fail(PS_ERR_INTERNAL, "frontier stream of %llu bytes overran its %zu-byte buffer",
+                    (unsigned long long)w->fr_chunk_off.back(), w->fr_bytes.cap);
+    if (nchunks_x) {
+        const uint64_t* hxo = w->h_x_off.as<uint64_t>();
+        xbytes = hxo[nchunks_x];
+        if (xbytes > w->xs_bytes.cap)
+            return fail(PS_ERR_INTERNAL, "cross-segment stream of %llu bytes overran its %zu-byte buffer",
+                        (unsigned long long)xbytes, w->xs_bytes.cap);
+        if (w->h_counters[8])
+            return fail(PS_ERR_INTERNAL, "%llu cross-segment bitmap chunks are not sorted",
+                        (unsigned long long)w->h_counters[8]);
+        for (size_t b = 0; b + 1 < first_chunk.size(); ++b) w->bk_byte_off.push_back(hxo[first_chunk[b + 1]]);
+    }
 
     ps_fs_result r{};
     r.count = count;
@@ -475,8 +524,12 @@ int ps_fs_segment_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st,
     if (int r = dalloc((void**)&w->counters, NCOUNTERS * 8)) return cleanup(r);
     if (hipHostMalloc((void**)&w->h_counters, NCOUNTERS * 8, hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(PS_ERR_OUT_OF_MEMORY, "pinned host allocation failed"));
-    if (hipEventCreate(&w->ev0) != hipSuccess || hipEventCreate(&w->ev1) != hipSuccess)
+    if (hipEventCreate(&w->ev0) != hipSuccess || hipEventCreate(&w->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&w->ev_written, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&w->ev_side, hipEventDisableTiming) != hipSuccess)
         return cleanup(fail(PS_ERR_HIP, "event creation failed"));
+    if (hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(PS_ERR_HIP, "stream creation failed"));
     if (w->odd)
         if (int r = launch_fill(w->odd_bits, 0, (seg_size + 63) / 64 * 8, w->stream)) return cleanup(r);
     if (hipStreamSynchronize(w->stream) != hipSuccess) return cleanup(fail(PS_ERR_HIP, "stream synchronize failed"));

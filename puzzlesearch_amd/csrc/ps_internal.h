@@ -18,6 +18,8 @@ int require_runtime();
 // compute units and LDS bytes per block of the current device
 int hip_cu_count();
 size_t hip_lds_limit();
+// stream-ordered allocation from the library's scratch pool (freed with hipFreeAsync)
+int scratch_alloc(void** p, size_t bytes, hipStream_t s);
 
 // Host-side puzzle facts (Common/Puzzle.h:22-26 for each reference class).
 int puzzle_query(const ps_puzzle* p, ps_puzzle_info* out);

@@ -65,6 +65,48 @@ int hip_cu_count() {
     return v;
 }
 
+// The library's stream-ordered scratch pool on the current device.  Freed
+// blocks stay reserved in it up to 1 GiB, so the short-lived buffers of the
+// codec and sorting helpers are taken from the pool instead of being mapped
+// again on every call.
+hipMemPool_t scratch_pool() {
+    constexpr int MAX_DEV = 64;
+    static hipMemPool_t pools[MAX_DEV] = {};
+    static std::mutex m;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEV) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(m);
+    if (!pools[dev]) {
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &props) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        uint64_t keep = 1ull << 30;
+        if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) (void)hipGetLastError();
+        pools[dev] = pool;
+    }
+    return pools[dev];
+}
+
+int scratch_alloc(void** p, size_t bytes, hipStream_t s) {
+    hipMemPool_t pool = scratch_pool();
+    const hipError_t e = pool ? hipMallocFromPoolAsync(p, bytes, pool, s) : hipMallocAsync(p, bytes, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(PS_ERR_OUT_OF_MEMORY, "device scratch of %zu bytes: %s", bytes, hipGetErrorString(e));
+    }
+    return PS_OK;
+}
+
 size_t hip_lds_limit() {
     int dev = 0, v = 0;
     (void)hipGetDevice(&dev);
