@@ -18,16 +18,18 @@ namespace ps {
 
 struct DevBuf {
     void* p = nullptr;
+    size_t n = 0;
     hipStream_t s = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFreeAsync(p, s);
-    }
+    ~DevBuf() { scratch_free(p, n, s); }
     int alloc(size_t bytes, hipStream_t st) {
+        scratch_free(p, n, s);
+        p = nullptr;
         s = st;
-        return scratch_alloc(&p, bytes ? bytes : 16, st);
+        n = bytes ? bytes : 16;
+        return scratch_alloc(&p, n, st);
     }
     template <class T>
     T* as() const {
@@ -73,8 +75,7 @@ inline int stream_wait(hipStream_t s) {
 template <class T>
 inline int to_device(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
     if (int r = b.alloc(v.size() * sizeof(T), s)) return r;
-    if (!v.empty()) PS_HIP_CHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    return PS_OK;
+    return copy_to_device(b.p, v.data(), v.size() * sizeof(T), s);
 }
 
 template <class T>

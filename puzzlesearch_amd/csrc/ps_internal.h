@@ -18,8 +18,15 @@ int require_runtime();
 // compute units and LDS bytes per block of the current device
 int hip_cu_count();
 size_t hip_lds_limit();
-// stream-ordered allocation from the library's scratch pool (freed with hipFreeAsync)
+// stream-ordered allocation from the library's scratch pool; scratch_free
+// keeps small blocks for reuse on the same stream and hands the others to
+// hipFreeAsync
 int scratch_alloc(void** p, size_t bytes, hipStream_t s);
+void scratch_free(void* p, size_t bytes, hipStream_t s);
+// host-to-device copy in stream order; sources up to STAGE_MAX_BYTES go
+// through pinned staging, so the call does not wait for a staging copy
+constexpr size_t STAGE_MAX_BYTES = 4u << 20;
+int copy_to_device(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // Host-side puzzle facts (Common/Puzzle.h:22-26 for each reference class).
 int puzzle_query(const ps_puzzle* p, ps_puzzle_info* out);

@@ -9,6 +9,7 @@
 #include <mutex>
 #include <new>
 #include <stdexcept>
+#include <vector>
 
 #include "ps_internal.h"
 
@@ -96,7 +97,53 @@ hipMemPool_t scratch_pool() {
     return pools[dev];
 }
 
+namespace {
+// Small scratch blocks released by this thread, kept for reuse on the stream
+// they were released on (stream order makes the reuse safe there); every
+// entry point waits for its streams before returning, so between calls the
+// blocks are idle.  hipFreeAsync costs about 10 us of host time per call,
+// which the short-lived helper buffers of a segment step would pay a few
+// dozen times per step.
+struct CachedBlock {
+    hipStream_t s;
+    int dev;
+    void* p;
+    size_t bytes;
+};
+constexpr size_t CACHE_MAX_BLOCK = 1u << 20;
+constexpr size_t CACHE_MAX_BLOCKS = 64;
+thread_local std::vector<CachedBlock> t_cache;
+}  // namespace
+
+void scratch_free(void* p, size_t bytes, hipStream_t s) {
+    if (!p) return;
+    int dev = 0;
+    if (bytes <= CACHE_MAX_BLOCK && t_cache.size() < CACHE_MAX_BLOCKS && hipGetDevice(&dev) == hipSuccess) {
+        t_cache.push_back(CachedBlock{s, dev, p, bytes});
+        return;
+    }
+    (void)hipFreeAsync(p, s);
+}
+
 int scratch_alloc(void** p, size_t bytes, hipStream_t s) {
+    if (bytes <= CACHE_MAX_BLOCK && !t_cache.empty()) {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            size_t best = t_cache.size();
+            for (size_t i = 0; i < t_cache.size(); ++i) {
+                const CachedBlock& b = t_cache[i];
+                if (b.s == s && b.dev == dev && b.bytes >= bytes && b.bytes <= std::max<size_t>(2 * bytes, 4096) &&
+                    (best == t_cache.size() || b.bytes < t_cache[best].bytes))
+                    best = i;
+            }
+            if (best < t_cache.size()) {
+                *p = t_cache[best].p;
+                t_cache[best] = t_cache.back();
+                t_cache.pop_back();
+                return PS_OK;
+            }
+        }
+    }
     hipMemPool_t pool = scratch_pool();
     const hipError_t e = pool ? hipMallocFromPoolAsync(p, bytes, pool, s) : hipMallocAsync(p, bytes, s);
     if (e != hipSuccess) {
@@ -104,6 +151,75 @@ int scratch_alloc(void** p, size_t bytes, hipStream_t s) {
         *p = nullptr;
         return fail(PS_ERR_OUT_OF_MEMORY, "device scratch of %zu bytes: %s", bytes, hipGetErrorString(e));
     }
+    return PS_OK;
+}
+
+namespace {
+// Pinned staging for small host-to-device copies: a pageable source costs a
+// synchronous staging copy inside hipMemcpyAsync (about 60 us for 128 KB);
+// from pinned memory the copy is queued and the call returns at once.  A
+// ring of four buffers per thread, each reused only after the event recorded
+// behind its last copy has passed.
+struct Stage {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    int dev = -1;
+    bool pending = false;
+};
+thread_local Stage t_stage[4];
+thread_local int t_stage_next = 0;
+}  // namespace
+
+int copy_to_device(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return PS_OK;
+    if (bytes > STAGE_MAX_BYTES) {
+        PS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return PS_OK;
+    }
+    int dev = 0;
+    PS_HIP_CHECK(s ? hipStreamGetDevice(s, &dev) : hipGetDevice(&dev));
+    Stage& st = t_stage[t_stage_next];
+    t_stage_next = (t_stage_next + 1) % 4;
+    if (st.pending) {
+        hipError_t q;
+        while ((q = hipEventQuery(st.ev)) == hipErrorNotReady) {
+        }
+        if (q != hipSuccess) return fail(PS_ERR_HIP, "waiting for a staged copy failed: %s", hipGetErrorString(q));
+        st.pending = false;
+    }
+    if (st.ev && st.dev != dev) {
+        (void)hipEventDestroy(st.ev);
+        st.ev = nullptr;
+    }
+    if (!st.ev) {
+        int cur = 0;
+        PS_HIP_CHECK(hipGetDevice(&cur));
+        if (cur != dev) PS_HIP_CHECK(hipSetDevice(dev));
+        const hipError_t e = hipEventCreateWithFlags(&st.ev, hipEventDisableTiming);
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess) {
+            st.ev = nullptr;
+            return fail(PS_ERR_HIP, "event creation failed: %s", hipGetErrorString(e));
+        }
+        st.dev = dev;
+    }
+    if (st.cap < bytes) {
+        if (st.p) (void)hipHostFree(st.p);
+        st.p = nullptr;
+        st.cap = 0;
+        const size_t want = std::max<size_t>(bytes, 64 * 1024);
+        if (hipHostMalloc(&st.p, want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            st.p = nullptr;
+            return fail(PS_ERR_OUT_OF_MEMORY, "pinned staging of %zu bytes", want);
+        }
+        st.cap = want;
+    }
+    std::memcpy(st.p, src, bytes);
+    PS_HIP_CHECK(hipMemcpyAsync(dst, st.p, bytes, hipMemcpyHostToDevice, s));
+    PS_HIP_CHECK(hipEventRecord(st.ev, s));
+    st.pending = true;
     return PS_OK;
 }
 
