@@ -141,13 +141,19 @@ __global__ __launch_bounds__(TPB) void k_chunk_scan_u32(uint32_t* vals, const ui
 }
 
 // skip (optional): chunks written by another encoder (bitmap chunks)
+// out_cap != ~0: chunk_out holds nchunks + 1 offsets, and a chunk ending past
+// out_cap is not written (counted in *err by its first tuple's thread)
 __global__ void k_svi_write(const uint32_t* v, const uint8_t* bounds, SviPlan pl, const uint32_t* tuple_off,
                             const uint64_t* chunk_out, const uint64_t* chunk_payload, uint8_t* out,
-                            const uint32_t* skip) {
+                            const uint32_t* skip, uint64_t out_cap, uint32_t* err) {
     uint64_t t;
     uint32_t c;
     if (!svi_tuple(pl, t, c)) return;
     if (skip && skip[c]) return;
+    if (out_cap != ~0ull && chunk_out[c + 1] > out_cap) {
+        if (err && t == pl.tuple_base[c]) atomicAdd(err, 1u);
+        return;
+    }
     const uint32_t lt = (uint32_t)(t - pl.tuple_base[c]);
     const uint64_t start = pl.starts[c];
     const uint32_t count = pl.counts[c];
@@ -273,7 +279,7 @@ __device__ __forceinline__ uint32_t svd_block_excl(uint32_t v, uint32_t* s_w, ui
 __global__ __launch_bounds__(SVD_TPB) void k_svi_decode_chunk(const uint8_t* in, const uint64_t* in_off,
                                                                const uint32_t* select, const uint64_t* value_starts,
                                                                int with_bounds, uint32_t* out, uint8_t* bounds_out,
-                                                               uint32_t* err) {
+                                                               uint32_t* err, uint64_t values_cap) {
     extern __shared__ uint32_t svd_smem[];
     uint8_t* buf = reinterpret_cast<uint8_t*>(svd_smem);  // staged bytes, from an aligned address
     uint32_t* map0 = svd_smem + SVD_PAD_WORDS;
@@ -281,7 +287,17 @@ __global__ __launch_bounds__(SVD_TPB) void k_svi_decode_chunk(const uint8_t* in,
     uint32_t* vals = map1 + SVD_TPB * SVD_NE;
     __shared__ uint32_t s_w[SVD_TPB / 64];
     __shared__ uint32_t s_bad;
-    const uint32_t c = select[blockIdx.x];
+    // without `select` every chunk gets a block: empty chunks are skipped, and
+    // a chunk past values_cap (err[1]) or over MAX_CHUNK_VALUES (err[2]) is refused
+    const uint32_t c = select ? select[blockIdx.x] : blockIdx.x;
+    if (!select) {
+        const uint64_t v0 = value_starts[c], v1 = value_starts[c + 1];
+        if (v1 == v0) return;
+        if (v1 - v0 > MAX_CHUNK_VALUES || v1 > values_cap) {
+            if (threadIdx.x == 0) atomicAdd(err + (v1 - v0 > MAX_CHUNK_VALUES ? 2 : 1), 1u);
+            return;
+        }
+    }
     const uint8_t* p = in + in_off[c];
     const uint64_t clen = in_off[c + 1] - in_off[c];
     const int tid = threadIdx.x;
@@ -888,7 +904,7 @@ int svi_write(const uint32_t* d_values, const uint8_t* d_bounds, const SviPlan& 
     if (pl.ntuples)
         hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, d_bounds, pl,
                            d_toff.as<uint32_t>(), d_off.as<uint64_t>(), d_payload.as<uint64_t>(), d_out,
-                           (const uint32_t*)nullptr);
+                           (const uint32_t*)nullptr, ~0ull, (uint32_t*)nullptr);
     PS_HIP_CHECK(hipGetLastError());
     if (int r = stream_wait(s)) return r;
     return PS_OK;
@@ -898,6 +914,17 @@ int read_starts(std::vector<uint64_t>& starts, const uint64_t* d_starts, uint32_
     if (int r = to_host(starts, d_starts, (size_t)nchunks + 1, s)) return r;
     for (uint32_t c = 0; c < nchunks; ++c)
         if (starts[c + 1] < starts[c]) return fail(PS_ERR_INVALID_ARGUMENT, "chunk starts must not decrease");
+    return PS_OK;
+}
+
+// the decode kernel's LDS exceeds the default 64 KB limit
+int svd_setup() {
+    static bool lds_set = false;
+    if (!lds_set) {
+        PS_HIP_CHECK(hipFuncSetAttribute((const void*)k_svi_decode_chunk, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)SVD_LDS));
+        lds_set = true;
+    }
     return PS_OK;
 }
 
@@ -912,18 +939,13 @@ int svi_decode_chunks(const uint8_t* d_in, const uint64_t* d_in_off, const std::
     for (uint32_t k = 0; k < n; ++k)
         if (counts_all[sel[k]] > MAX_CHUNK_VALUES)
             return fail(PS_ERR_INVALID_ARGUMENT, "stream chunk claims %u values", counts_all[sel[k]]);
-    static bool lds_set = false;
-    if (!lds_set) {
-        PS_HIP_CHECK(hipFuncSetAttribute((const void*)k_svi_decode_chunk, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)SVD_LDS));
-        lds_set = true;
-    }
+    if (int r = svd_setup()) return r;
     DevBuf d_sel, d_err;
     if (int r = to_device(d_sel, sel, s)) return r;
     const std::vector<uint32_t> zero(1, 0u);
     if (int r = to_device(d_err, zero, s)) return r;
     hipLaunchKernelGGL(k_svi_decode_chunk, dim3(n), dim3(SVD_TPB), SVD_LDS, s, d_in, d_in_off, d_sel.as<uint32_t>(),
-                       d_value_starts, d_bounds ? 1 : 0, d_values, d_bounds, d_err.as<uint32_t>());
+                       d_value_starts, d_bounds ? 1 : 0, d_values, d_bounds, d_err.as<uint32_t>(), ~0ull);
     PS_HIP_CHECK(hipGetLastError());
     std::vector<uint32_t> bad;
     if (int r = to_host(bad, d_err.p, 1, s)) return r;
@@ -1081,7 +1103,7 @@ int svi_payload(const uint32_t* d_values, const SviPlan& pl, hipStream_t s, DevB
 }  // namespace
 
 int svi_encode_chunks_dev(const uint32_t* d_values, const uint8_t* d_bounds, const std::vector<uint64_t>& starts,
-                          uint8_t* d_out, uint64_t* d_offsets, hipStream_t s) {
+                          uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, uint32_t* d_err, hipStream_t s) {
     DevBuf tab, d_toff, d_pay, d_bytes;
     SviPlan pl{};
     if (int r = svi_plan_all(starts, s, tab, pl)) return r;
@@ -1096,7 +1118,7 @@ int svi_encode_chunks_dev(const uint32_t* d_values, const uint8_t* d_bounds, con
     if (pl.ntuples)
         hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, d_bounds, pl,
                            (const uint32_t*)d_toff.as<uint32_t>(), (const uint64_t*)d_offsets,
-                           (const uint64_t*)d_pay.as<uint64_t>(), d_out, (const uint32_t*)nullptr);
+                           (const uint64_t*)d_pay.as<uint64_t>(), d_out, (const uint32_t*)nullptr, out_cap, d_err);
     PS_HIP_CHECK(hipGetLastError());
     return PS_OK;
 }
@@ -1132,7 +1154,8 @@ int fc_encode_chunks_dev(const uint32_t* d_values, const std::vector<uint64_t>& 
     if (pl.ntuples)
         hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, (const uint8_t*)nullptr, pl,
                            (const uint32_t*)d_toff.as<uint32_t>(), (const uint64_t*)d_offsets,
-                           (const uint64_t*)d_pay.as<uint64_t>(), d_out, (const uint32_t*)is_bm);
+                           (const uint64_t*)d_pay.as<uint64_t>(), d_out, (const uint32_t*)is_bm, ~0ull,
+                           (uint32_t*)nullptr);
     if (n)
         hipLaunchKernelGGL(k_fc_bitmap_chunk, dim3(n), dim3(256), 0, s, d_values, pl, (const uint32_t*)is_bm,
                            (const uint32_t*)words, (const uint32_t*)first, (const uint64_t*)d_offsets, d_out);
@@ -1153,11 +1176,22 @@ int ps_svint_encode(const uint32_t* d_values, const uint8_t* d_bounds, const uin
     if (!d_starts || !d_offsets || (nchunks && (!d_values || !d_out)))
         return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
     hipStream_t s = stream_of(st);
-    std::vector<uint64_t> starts, off;
+    std::vector<uint64_t> starts;
     if (int r = read_starts(starts, d_starts, nchunks, s)) return r;
-    if (int r = svi_encode_chunks(d_values, d_bounds, starts, d_out, out_cap, off, s)) return r;
-    PS_HIP_CHECK(hipMemcpyAsync(d_offsets, off.data(), off.size() * 8, hipMemcpyHostToDevice, s));
+    // sizes, offsets and the write on the device; a chunk that would end past
+    // out_cap is not written, and the call reports the capacity error
+    DevBuf d_err;
+    if (int r = d_err.alloc(8, s)) return r;
+    if (int r = launch_fill(d_err.p, 0, 8, s)) return r;
+    if (int r = svi_encode_chunks_dev(d_values, d_bounds, starts, d_out, out_cap, d_offsets, d_err.as<uint32_t>(), s))
+        return r;
+    uint64_t tail[2] = {0, 0};
+    PS_HIP_CHECK(hipMemcpyAsync(&tail[0], d_offsets + nchunks, 8, hipMemcpyDeviceToHost, s));
+    PS_HIP_CHECK(hipMemcpyAsync(&tail[1], d_err.p, 8, hipMemcpyDeviceToHost, s));
     if (int r = stream_wait(s)) return r;
+    if (tail[1] || tail[0] > out_cap)
+        return fail(PS_ERR_CAPACITY, "encoded size %llu exceeds output capacity %llu", (unsigned long long)tail[0],
+                    (unsigned long long)out_cap);
     return PS_OK;
     PS_API_END
 }
@@ -1170,25 +1204,32 @@ int ps_svint_decode(const uint8_t* d_in, const uint64_t* d_in_offsets, uint32_t 
     if (!d_in_offsets || !d_value_starts || (nchunks && (!d_in || !d_values)) || (with_bounds && !d_bounds))
         return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
     hipStream_t s = stream_of(st);
-    DevBuf d_counts;
+    // counts, their prefix sums as the value starts, and every chunk decoded
+    // with its limits checked on the device; one wait for the error counts
+    DevBuf d_counts, d_err;
     if (int r = d_counts.alloc((size_t)nchunks * 4 + 4, s)) return r;
+    if (int r = d_err.alloc(16, s)) return r;
+    if (int r = launch_fill(d_err.p, 0, 16, s)) return r;
     if (nchunks)
         hipLaunchKernelGGL(k_read_counts, dim3(blocks_for(nchunks, 256)), dim3(256), 0, s, d_in, d_in_offsets, nchunks,
                            d_counts.as<uint32_t>(), (uint32_t*)nullptr);
     PS_HIP_CHECK(hipGetLastError());
-    std::vector<uint32_t> counts;
-    if (int r = to_host(counts, d_counts.p, nchunks, s)) return r;
-    std::vector<uint64_t> vs(nchunks + 1, 0);
-    for (uint32_t c = 0; c < nchunks; ++c) vs[c + 1] = vs[c] + counts[c];
-    if (vs[nchunks] > values_cap)
-        return fail(PS_ERR_CAPACITY, "%llu decoded values exceed capacity %llu", (unsigned long long)vs[nchunks],
-                    (unsigned long long)values_cap);
-    PS_HIP_CHECK(hipMemcpyAsync(d_value_starts, vs.data(), vs.size() * 8, hipMemcpyHostToDevice, s));
-    std::vector<uint32_t> sel;
-    for (uint32_t c = 0; c < nchunks; ++c)
-        if (counts[c]) sel.push_back(c);
-    return svi_decode_chunks(d_in, d_in_offsets, sel, counts, d_counts.as<uint32_t>(), d_value_starts, d_values,
-                             with_bounds ? d_bounds : nullptr, s);
+    if (int r = scan_excl_u32(d_counts.as<uint32_t>(), nchunks, (unsigned long long*)d_value_starts, s)) return r;
+    if (nchunks) {
+        if (int r = svd_setup()) return r;
+        hipLaunchKernelGGL(k_svi_decode_chunk, dim3(nchunks), dim3(SVD_TPB), SVD_LDS, s, d_in, d_in_offsets,
+                           (const uint32_t*)nullptr, (const uint64_t*)d_value_starts, with_bounds ? 1 : 0, d_values,
+                           with_bounds ? d_bounds : nullptr, d_err.as<uint32_t>(), values_cap);
+        PS_HIP_CHECK(hipGetLastError());
+    }
+    std::vector<uint32_t> bad;
+    if (int r = to_host(bad, d_err.p, 3, s)) return r;
+    if (bad[2]) return fail(PS_ERR_INVALID_ARGUMENT, "%u stream chunks claim more than %u values", bad[2], MAX_CHUNK_VALUES);
+    if (bad[1])
+        return fail(PS_ERR_CAPACITY, "decoded values exceed capacity %llu", (unsigned long long)values_cap);
+    if (bad[0])
+        return fail(PS_ERR_INVALID_ARGUMENT, "%u StreamVInt chunks do not end where their tuples end", bad[0]);
+    return PS_OK;
     PS_API_END
 }
 

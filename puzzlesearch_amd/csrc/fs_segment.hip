@@ -337,7 +337,8 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
     if (int r = w->fr_off.ensure(nfo * 8, s)) return r;
     if (int r = w->h_fr_off.ensure(nfo * 8)) return r;
     if (int r = svi_encode_chunks_dev(w->out_idx.as<uint32_t>(), w->out_ops.as<uint8_t>(), starts,
-                                      w->fr_bytes.as<uint8_t>(), w->fr_off.as<uint64_t>(), s))
+                                      w->fr_bytes.as<uint8_t>(), w->fr_bytes.cap, w->fr_off.as<uint64_t>(),
+                                      nullptr, s))
         return r;
     PS_HIP_CHECK(hipMemcpyAsync(w->h_fr_off.p, w->fr_off.p, nfo * 8, hipMemcpyDeviceToHost, s));
 
