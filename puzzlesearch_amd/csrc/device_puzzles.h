@@ -219,24 +219,38 @@ struct Sliding {
         }
     }
 
-    // rank of the permutation a with positions i and j swapped
-    __device__ __forceinline__ static R rank_swapped(const uint32_t (&a)[N], int i, int j) {
-        uint32_t seen = 0;
-        R r = 0;
-        uint32_t ai = 0, aj = 0;
+    // A vertical move swaps the blank with the tile `width` positions away.
+    // Only the Lehmer digits of the two positions p < q and of the width-1
+    // positions between them change: with t the moving tile and `less` the
+    // tiles between them smaller than t,
+    //   up   (blank q -> p): d'_p = 0, d'_q = d_p - 1 - less, d'_i = d_i - [a_i < t]
+    //   down (blank p -> q): d'_p = d_q + 1 + less, d'_q = 0, d'_i = d_i + [a_i < t]
+    // so the child's rank is the parent's plus the weighted digit changes
+    // (w_i = (N-1-i)!, walked down from w_q by w_i = w_{i+1} * (N-1-i)).
+    // an: the tiles of the permutation as nibbles, dn: the digits as nibbles.
+    __device__ __forceinline__ static R vertical(R rank, uint64_t an, uint64_t dn, int blank, int width, bool up) {
+        const int p = up ? blank - width : blank;
+        const int q = up ? blank : blank + width;
+        const uint32_t t = (uint32_t)((an >> (4 * (up ? p : q))) & 15u);
+        R wq = 0;
 #pragma unroll
-        for (int p = 0; p < N; ++p) {
-            ai = (p == i) ? a[p] : ai;
-            aj = (p == j) ? a[p] : aj;
+        for (int i = 0; i < N; ++i) wq = (i == q) ? (R)kFact[N - 1 - i] : wq;
+        R w = wq, sum = 0;
+        uint32_t less = 0;
+        for (int i = q - 1; i > p; --i) {
+            w *= (R)(N - 1 - i);
+            if ((uint32_t)((an >> (4 * i)) & 15u) < t) {
+                ++less;
+                sum += w;
+            }
         }
-#pragma unroll
-        for (int p = 0; p < N; ++p) {
-            const uint32_t v = (p == i) ? aj : (p == j) ? ai : a[p];
-            const uint32_t dig = v - __popc(seen & ((1u << v) - 1u));
-            seen |= 1u << v;
-            r = r * (R)(N - p) + (R)dig;
+        const R wp = w * (R)(N - 1 - p);
+        if (up) {
+            const R dp = (R)((dn >> (4 * p)) & 15u);
+            return rank - dp * wp - sum + (dp - 1 - (R)less) * wq;
         }
-        return r;
+        const R dq = (R)((dn >> (4 * q)) & 15u);
+        return rank + ((R)less + 1 + dq) * wp + sum - dq * wq;
     }
 
     template <class F>
@@ -256,10 +270,17 @@ struct Sliding {
         const bool lf = col > 0 && !(op_bits & (1u << OP_LEFT));
         const bool rt = col < width - 1 && !(op_bits & (1u << OP_RIGHT));
         if (up || dn) {
-            uint32_t a[N];
-            digits_to_perm<N>(d, a);
-            if (up) f(0, (uint64_t)rank_swapped(a, blank - width, blank), OP_DOWN);
-            if (dn) f(1, (uint64_t)rank_swapped(a, blank, blank + width), OP_UP);
+            static_assert(N <= 16, "tiles and digits as nibbles of 64 bits");
+            uint64_t tiles = 0xFEDCBA9876543210ull, an = 0, dn_ = 0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {  // digits -> tiles, as nibbles
+                const uint32_t sh = d[i] * 4;
+                an |= ((tiles >> sh) & 15ull) << (4 * i);
+                dn_ |= (uint64_t)d[i] << (4 * i);
+                tiles = ((tiles >> sh >> 4) << sh) | (tiles & ((1ull << sh) - 1ull));
+            }
+            if (up) f(0, (uint64_t)vertical(rank, an, dn_, blank, width, true), OP_DOWN);
+            if (dn) f(1, (uint64_t)vertical(rank, an, dn_, blank, width, false), OP_UP);
         }
         // horizontal moves touch two adjacent digits; w_p = (N-1-p)!
         uint32_t dl = 0, dr = 0;

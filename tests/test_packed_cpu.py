@@ -6,7 +6,11 @@ bit of list position 1; every slot of random states of all eleven shapes, with
 random used-operator bits, must equal the oracle's restatement of the
 reference's unpack / rotate / pack kernel (Puzzles/SlidingPuzzleGpu.cu:108-270).
 A dropped parity bit fails every shape (checked when the arithmetic was
-written)."""
+written).  The same program checks the simple Lehmer-rank encoding
+(psd::Sliding<N>), whose vertical moves change only the digits of the swapped
+window, on nine shapes against the oracle's slots of the reference's rank /
+swap / re-rank kernel; an off-by-one in the moved tile's new digit fails all
+nine."""
 import os
 import subprocess
 
