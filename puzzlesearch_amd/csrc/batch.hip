@@ -338,7 +338,9 @@ int retire(ps_batch* b, Job* j) {
 // than BATCH_SLOW_MS, which keeps the work queued ahead of a newly admitted job
 // on a shared hardware queue short.  A small job thus finishes at its own pace,
 // not at the pace of the largest one.
-constexpr double MAX_WAIT_MS = 2.0, BATCH_FAST_MS = 1.0, BATCH_SLOW_MS = 3.0;
+// (batches of 0.3-1 ms rather than 1-3 ms: p50 at 50 jobs/s 4.21 / 3.99 ms
+// instead of 4.62 / 4.24 ms on two request streams, the same mean)
+constexpr double MAX_WAIT_MS = 2.0, BATCH_FAST_MS = 0.3, BATCH_SLOW_MS = 1.0;
 
 int enqueue_job(Job* j) {
     if (int r = enqueue_layers(j->bfs, j->batch_layers)) return r;
