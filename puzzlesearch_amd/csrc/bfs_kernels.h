@@ -636,8 +636,9 @@ __device__ __forceinline__ void finalize_words(const uint64_t* __restrict__ cur,
                 }
                 a = fin_word(a, vo[u].x, vc[u].x, odd);
                 b = fin_word(b, vo[u].y, vc[u].y, odd);
-                *reinterpret_cast<ulonglong2*>(nw + j) = make_ulonglong2(a, b);
-                *reinterpret_cast<ulonglong2*>(old + j) = make_ulonglong2(0ull, 0ull);
+                // frontiers are sparse: store only the words that change
+                if (a != vn[u].x || b != vn[u].y) *reinterpret_cast<ulonglong2*>(nw + j) = make_ulonglong2(a, b);
+                if (vo[u].x | vo[u].y) *reinterpret_cast<ulonglong2*>(old + j) = make_ulonglong2(0ull, 0ull);
                 cnt += (unsigned long long)(__popcll(a) + __popcll(b));
                 if (hash) h += word_hash(j, a) + word_hash(j + 1, b);
             }
@@ -646,11 +647,12 @@ __device__ __forceinline__ void finalize_words(const uint64_t* __restrict__ cur,
     // an odd word count leaves one word
     if (((w1 - w0) & 1) && threadIdx.x == 0) {
         const uint64_t j = w1 - 1;
-        uint64_t v = nw[j];
+        const uint64_t n0 = nw[j], o = old[j];
+        uint64_t v = n0;
         if (XM) v |= fold_marks(xmark + j * 64);
-        v = fin_word(v, old[j], odd ? cur[j] : 0ull, odd);
-        nw[j] = v;
-        old[j] = 0;
+        v = fin_word(v, o, odd ? cur[j] : 0ull, odd);
+        if (v != n0) nw[j] = v;
+        if (o) old[j] = 0;
         cnt += (unsigned long long)__popcll(v);
         if (hash) h += word_hash(j, v);
     }
