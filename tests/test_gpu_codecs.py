@@ -159,6 +159,48 @@ def test_svint_rejects_oversized_chunk(C, ps):
         C.svint_encode([np.arange((1 << 18) + 1, dtype=np.uint32)])
 
 
+def test_svint_encode_capacity_is_a_bound(C, ps):
+    # an output capacity below the encoded size is refused, and no byte at or
+    # past the capacity is written: the chunks that would end past it are
+    # skipped on the device
+    rng = np.random.default_rng(21)
+    chunks = [sorted_unique(rng, 3000, 1 << 32), sorted_unique(rng, 5000, 1 << 32), sorted_unique(rng, 2000, 1 << 20)]
+    data, off = C.svint_encode(chunks)
+    cap = int(off[1]) + 10  # the first chunk fits, the second does not
+    vals = np.concatenate(chunks)
+    st = np.array([0, 3000, 8000, 10000], np.uint64)
+    d_v = ps.DeviceBuffer.from_array(vals)
+    d_st = ps.DeviceBuffer.from_array(st)
+    canary = np.full(len(data) + 64, 0xA5, np.uint8)
+    d_out = ps.DeviceBuffer.from_array(canary)
+    d_off = ps.DeviceBuffer(8 * 4)
+    with pytest.raises(ps.PSError, match="capacity") as e:
+        ps.check(ps.lib().ps_svint_encode(d_v.ptr, None, d_st.ptr, 3, d_out.ptr, cap, d_off.ptr, None))
+    assert e.value.code == ps.PS_ERR_CAPACITY
+    out = d_out.download(np.uint8, len(canary))
+    assert np.array_equal(out[:int(off[1])], data[:int(off[1])])
+    assert np.all(out[cap:] == 0xA5)
+
+
+def test_svint_decode_capacity_is_a_bound(C, ps):
+    # values_cap below the decoded count is refused, no value at or past
+    # values_cap is written, and the chunks that fit still decode
+    rng = np.random.default_rng(22)
+    chunks = [sorted_unique(rng, 3000, 1 << 32), sorted_unique(rng, 5000, 1 << 32)]
+    data, off = C.svint_encode(chunks)
+    d_in = ps.DeviceBuffer.from_array(np.concatenate([data, np.zeros(64, np.uint8)]))
+    d_off = ps.DeviceBuffer.from_array(off)
+    cap = 3000 + 4999
+    d_v = ps.DeviceBuffer.from_array(np.full(8064, 0xDEADBEEF, np.uint32))
+    d_vs = ps.DeviceBuffer(8 * 3)
+    with pytest.raises(ps.PSError, match="capacity") as e:
+        ps.check(ps.lib().ps_svint_decode(d_in.ptr, d_off.ptr, 2, 0, d_v.ptr, None, cap, d_vs.ptr, None))
+    assert e.value.code == ps.PS_ERR_CAPACITY
+    v = d_v.download(np.uint32, 8064)
+    assert np.array_equal(v[:3000], chunks[0])
+    assert np.all(v[3000:] == 0xDEADBEEF)
+
+
 @pytest.mark.parametrize("bits", [2, 4, 8])
 def test_opbits_matches_oracle(C, bits):
     rng = np.random.default_rng(bits)
