@@ -12,11 +12,15 @@
 //            of the entries with an unused move written in id order
 //   stream   the new frontier as StreamVInt with bounds in 2^18-value chunks,
 //            as CompressedFrontierWriter writes it (Common/CompressedFrontier.cpp:54-75)
-//   cross    children of the written entries in other segments, one slot per
-//            move, compacted in emission order, stably grouped by
-//            (op, segment) with a counting sort (seg_sort.h), and each bucket cut into 4096-value
-//            chunks compressed with FrontierCompression::EncodeWithCheck, as
-//            SmartMultiplexor flushes them (Common/Multiplexor.cpp:125-161)
+//   cross    (a second stream, beside `stream`) children of the written
+//            entries in other segments, compacted in emission order, stably
+//            grouped by (op, segment) with a counting sort (seg_sort.h), and
+//            each bucket cut into 4096-value chunks compressed with
+//            FrontierCompression::EncodeWithCheck, as SmartMultiplexor flushes
+//            them (Common/Multiplexor.cpp:125-161)
+//
+// The host waits for the region counts, the cross-child count and the bucket
+// list; the chunk offsets of both streams are read back with the step's end.
 //
 // Emission order inside a bucket is (parent, slot) order, the order in which
 // ExpandBuffer hands children to SmartMultiplexor::Add for one segment, so the
@@ -46,8 +50,8 @@ using ps::GrowBuf;
 
 // [0] in-segment children [1] count [2] written [3] cross children [4] children
 // outside the segments [5] words scanned [6] frontier ids outside the segment
-// [7] cross-segment slots filled
-constexpr int NCOUNTERS = 9;  // [8]: unsorted cross-segment bitmap chunks
+// [7] cross-segment slots filled [8] unsorted cross-segment bitmap chunks
+constexpr int NCOUNTERS = 9;
 
 uint64_t svi_worst(uint64_t m, bool bounds) {
     const uint64_t adj = (m + 7) / 8 * 8;

@@ -98,12 +98,12 @@ hipMemPool_t scratch_pool() {
 }
 
 namespace {
-// Small scratch blocks released by this thread, kept for reuse on the stream
-// they were released on (stream order makes the reuse safe there); every
-// entry point waits for its streams before returning, so between calls the
-// blocks are idle.  hipFreeAsync costs about 10 us of host time per call,
-// which the short-lived helper buffers of a segment step would pay a few
-// dozen times per step.
+// Small scratch blocks released by this thread, kept for reuse by this thread
+// on the stream they were released on: work queued later on that stream runs
+// after the work that used the block, as with hipFreeAsync followed by
+// hipMallocAsync on one stream.  hipFreeAsync costs about 10 us of host time
+// per call, which the short-lived helper buffers of a segment step would pay
+// a few dozen times per step.
 struct CachedBlock {
     hipStream_t s;
     int dev;
