@@ -268,6 +268,57 @@ def segment_step_bench(psa, steps, ppm=200000, seed=1234):
                             "decode_and_step_parity": parity}}
 
 
+def seg3_step_bench(psa, steps, ppm=200000, seed=1234, old_seed=4321):
+    """The three-bit BFS segment step (DB_Opt3BitBFS_Solver::Expand) on the same
+    15-puzzle segment: current frontier Bernoulli(p) of `seed`, old frontier
+    Bernoulli(p) of `old_seed` without the current ids, no incoming marks.
+    Marks into the segment's NextArray, the old frontier cleared, the scan,
+    the FrontierCompression stream and the cross-segment buckets."""
+    from puzzlesearch_amd import workloads as W
+    from puzzlesearch_amd.seg3 import Seg3
+    seg, blank = W.fifteen_segment(seed)
+    cur, _ = W.bernoulli_frontier(seed, ppm, blank)
+    old0, _ = W.bernoulli_frontier(old_seed, ppm, blank)
+    old = np.setdiff1d(old0, cur, assume_unique=True).astype(np.uint32)
+    d_cur = psa.DeviceBuffer.from_array(cur)
+    d_old = psa.DeviceBuffer.from_array(old)
+    with Seg3(psa.sliding_packed(4, 4), 32) as w:
+        w.step_device(seg, d_cur.ptr, len(cur), d_old.ptr, len(old))  # warm-up (grows the buffers)
+        dev, wall, res = [], [], None
+        for _ in range(steps):
+            res = w.step_device(seg, d_cur.ptr, len(cur), d_old.ptr, len(old))
+            dev.append(res.device_ms)
+            wall.append(res.wall_ms)
+        r = res.as_dict()
+        parity = None
+        fx_path = os.path.join(ROOT, "tests", "golden", "segment_fixtures.json")
+        if os.path.exists(fx_path):
+            with open(fx_path) as f:
+                fx = json.load(f).get(f"seg3_fifteen_{ppm}")
+            if fx and fx["seed"] == seed and fx["old_seed"] == old_seed:
+                # the timed step's outputs against digests of the oracle step on the same frontiers
+                import hashlib
+
+                def sha(a):
+                    return hashlib.sha256(memoryview(np.ascontiguousarray(a)).cast("B")).hexdigest()
+                o = w.outputs(res)
+                got = (r["count"], sha(o["fr_idx"]), sha(o["fr_bytes"]),
+                       [[a, sg, n, len(b), sha(b)] for a, sg, n, b in o["buckets"]])
+                want = (fx["count"], fx["fr_idx"], fx["fr_bytes"], fx["buckets"])
+                assert got == want, "three-bit segment step outputs differ from the oracle step"
+                parity = True
+    ms = sum(wall) / len(wall)
+    dms = sum(dev) / len(dev)
+    return {"workload": f"15-puzzle segment {seg}, blank cell {blank}, current frontier Bernoulli(p={ppm / 1e6}) "
+                        f"of seed {seed}, old frontier of seed {old_seed}, over {W.FIFTEEN_M} ids",
+            "parents": len(cur), "old": len(old), "value": len(cur) / (ms * 1e-3), "unit": "expanded states/s",
+            "ms_per_step": ms, "device_ms_per_step": dms, "algorithmic_bytes": r["algorithmic_bytes"],
+            "hbm_frac_device": r["algorithmic_bytes"] / (dms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "count": r["count"], "frontier_bytes": r["frontier_bytes"], "children_in": r["children_in"],
+            "children_cross": r["children_cross"], "buckets": r["buckets"], "cross_bytes": r["cross_bytes"],
+            "parity_with_oracle_step": parity}
+
+
 TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256], "grid_blocks": [0, 256, 1024],
                    "graph_layers": [4, 8, 16], "cross_marks": [0, 1], "tile_bits_log2": [0, 16],
                    "skip_visited": [0], "direction": [0, 1]}
@@ -511,6 +562,8 @@ def main():
                     help="only eager batched searches of one config-3 workload, --steps times (for rocprofv3 runs)")
     ap.add_argument("--profile-segment", action="store_true",
                     help="only the 15-puzzle segment step, --steps times (for rocprofv3 runs)")
+    ap.add_argument("--profile-seg3", action="store_true",
+                    help="only the 15-puzzle three-bit segment step, --steps times (for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a kernel copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -531,6 +584,9 @@ def main():
         return
     if args.profile_segment:
         print(json.dumps({"profile": "segment_step", **segment_step_bench(psa, args.steps)}))
+        return
+    if args.profile_seg3:
+        print(json.dumps({"profile": "seg3_step", **seg3_step_bench(psa, args.steps)}))
         return
     props = psa.device_props()
     log("device:", props)
@@ -651,10 +707,12 @@ def main():
         codecs = codec_bench(psa)
         log("codecs:", codecs)
 
-    seg_step = None
+    seg_step = seg3_step = None
     if not args.no_segment_step:
         seg_step = segment_step_bench(psa, max(2, args.steps))
         log("segment step:", seg_step)
+        seg3_step = seg3_step_bench(psa, max(2, args.steps))
+        log("three-bit segment step:", seg3_step)
 
     cpu = cpu_par = None
     if not args.no_cpu:
@@ -699,6 +757,7 @@ def main():
         "batched_vs_plain_seg26": (value / plain["seg26_value"]) if plain else None,
         "batched_set": batched_set,
         "segment_step": seg_step,
+        "seg3_step": seg3_step,
         "codecs": codecs,
         "serving": serving,
         "parity_with_oracle_fixture": parity,

@@ -6,8 +6,11 @@ layer count, the new frontier's size and the SHA-256 of its ids, op bits and
 StreamVInt stream, and for every (op, segment) bucket its size and the SHA-256
 of its compressed chunks.  The GPU step is compared with these at the size the
 benchmark times.  About 3 CPU-minutes and 10 GB of host memory at p = 20 %.
+With --three-bit it does the same for the three-bit segment step
+(or_bfs3_segment_step) on that segment, with an old frontier of its own seed.
 
     python tests/golden/make_segment_fixtures.py [ppm ...]
+    python tests/golden/make_segment_fixtures.py --three-bit [ppm ...]
 """
 import hashlib
 import json
@@ -18,6 +21,8 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import numpy as np  # noqa: E402
 
 import oracle_lib as O  # noqa: E402
 from puzzlesearch_amd import workloads as W  # noqa: E402
@@ -34,7 +39,46 @@ def digest(r):
             "buckets": [[op, seg, n, len(b), sha(b)] for op, seg, n, b in r["buckets"]]}
 
 
+def old_frontier(seed, ppm, blank, cur):
+    """The old frontier of the three-bit step's workload: a Bernoulli sample of
+    its own seed without the ids of the current frontier (a state is in one
+    layer only)."""
+    old, _ = W.bernoulli_frontier(seed, ppm, blank)
+    return np.setdiff1d(old, cur, assume_unique=True).astype(np.uint32)
+
+
+def three_bit(ppm):
+    """The three-bit segment step (or_bfs3_segment_step,
+    DB_Opt3BitBFS_Solver::Expand) on the same segment: current frontier of
+    seed 1234, old frontier of seed 4321 (minus the current one), no incoming
+    marks."""
+    dst = os.path.join(HERE, "segment_fixtures.json")
+    with open(dst) as f:
+        data = json.load(f)
+    seed, old_seed = 1234, 4321
+    seg, blank = W.fifteen_segment(seed)
+    t = time.time()
+    cur, _ = W.bernoulli_frontier(seed, ppm, blank)
+    old = old_frontier(old_seed, ppm, blank, cur)
+    r = O.bfs3_segment_step(O.sliding_packed(4, 4), 32, seg, 3805, cur, old)
+    rec = {"seed": seed, "old_seed": old_seed, "segment": seg, "blank": blank, "ppm": ppm, "cur": len(cur),
+           "old": len(old), "count": r["count"], "frontier": len(r["fr_idx"]), "fr_idx": sha(r["fr_idx"]),
+           "fr_bytes": sha(r["fr_bytes"]), "fr_nbytes": len(r["fr_bytes"]), "children_in": r["children_in"],
+           "children_cross": r["children_cross"],
+           "buckets": [[op, sg, n, len(b), sha(b)] for op, sg, n, b in r["buckets"]],
+           "oracle_seconds": round(time.time() - t, 1)}
+    data[f"seg3_fifteen_{ppm}"] = rec
+    print(f"three-bit ppm {ppm}: {len(cur)} current, {len(old)} old, count {r['count']}, "
+          f"{len(r['buckets'])} buckets, {rec['oracle_seconds']} s", flush=True)
+    with open(dst, "w") as f:
+        json.dump(data, f, indent=1)
+
+
 def main():
+    if sys.argv[1:2] == ["--three-bit"]:
+        for ppm in [int(x) for x in sys.argv[2:]] or [200000]:
+            three_bit(ppm)
+        return
     ppms = [int(x) for x in sys.argv[1:]] or [10000, 50000, 200000]
     dst = os.path.join(HERE, "segment_fixtures.json")
     data = {}

@@ -328,7 +328,7 @@ def bfs3_segment_step(p, segment_bits, segment, nseg, cur, old, xseg=None):
     cap = len(cur) * bf + nx + 8
     fr_idx = np.zeros(cap, np.uint32)
     fr_bytes = np.zeros(5 * cap + 64 * (cap // (1 << 18) + 1) + 64, np.uint8)
-    bk_cap = cap * bf + 8
+    bk_cap = min(cap * bf, ops * nseg) + 8  # at most one bucket per (op, segment)
     bk_op = np.zeros(bk_cap, np.uint32)
     bk_seg = np.zeros(bk_cap, np.uint32)
     bk_values = np.zeros(bk_cap, np.uint64)

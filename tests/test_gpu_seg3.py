@@ -112,6 +112,41 @@ def test_hanoi21_prefix_32bit_segments(S, golden):
     assert counts == r["counts"]
 
 
+def test_fifteen_puzzle_three_bit_step_full_size(S, ps):
+    """The step the benchmark times: the 15-puzzle segment of seed 1234 with
+    a Bernoulli(0.2) current frontier and an old frontier of seed 4321 without
+    the current ids, against digests of the oracle step's outputs on the same
+    inputs (tests/golden/make_segment_fixtures.py --three-bit): count, new
+    frontier ids and stream, and every bucket's compressed bytes."""
+    import hashlib
+    import json
+    import os
+    from puzzlesearch_amd import workloads as W
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "segment_fixtures.json")
+    with open(path) as f:
+        rec = json.load(f).get("seg3_fifteen_200000")
+    if rec is None:
+        pytest.skip("three-bit segment fixture not generated")
+
+    def sha(a):
+        return hashlib.sha256(memoryview(np.ascontiguousarray(a)).cast("B")).hexdigest()
+
+    seg3, _ = S
+    seg, blank = W.fifteen_segment(rec["seed"])
+    cur, _ = W.bernoulli_frontier(rec["seed"], rec["ppm"], blank)
+    old0, _ = W.bernoulli_frontier(rec["old_seed"], rec["ppm"], blank)
+    old = np.setdiff1d(old0, cur, assume_unique=True).astype(np.uint32)
+    assert (seg, len(cur), len(old)) == (rec["segment"], rec["cur"], rec["old"])
+    with seg3.Seg3(psa.sliding_packed(4, 4), 32) as w:
+        dev = w.step(seg, cur, old)
+    assert dev["count"] == rec["count"]
+    assert len(dev["fr_idx"]) == rec["frontier"] and sha(dev["fr_idx"]) == rec["fr_idx"]
+    assert len(dev["fr_bytes"]) == rec["fr_nbytes"] and sha(dev["fr_bytes"]) == rec["fr_bytes"]
+    assert dev["stats"]["children_in"] == rec["children_in"]
+    assert dev["stats"]["children_cross"] == rec["children_cross"]
+    assert [[o, s, n, len(b), sha(b)] for o, s, n, b in dev["buckets"]] == rec["buckets"]
+
+
 def test_no_data_out_of_range_and_reuse(S, ps):
     seg3, _ = S
     p = O.pancake(9)
