@@ -25,13 +25,13 @@ int fc_encode_chunks(const uint32_t* d_values, const std::vector<uint64_t>& star
 // nothing waits for the stream.  The caller keeps d_out large enough for the
 // worst case and reads the offsets back once the stream is done.
 // fc_encode_chunks_dev counts unsorted unchecked bitmap chunks in *d_err (the
-// host-planned encoder refuses them), and falls back to that encoder, which
-// waits, when a chunk is too long for a bitmap built in LDS.
+// host-planned encoder refuses them); bitmaps wider than LDS go through a
+// global scratch.
 // svi_encode_chunks_dev writes no chunk that would end past out_cap and
 // counts such chunks in *d_err (optional).
 int svi_encode_chunks_dev(const uint32_t* d_values, const uint8_t* d_bounds, const std::vector<uint64_t>& starts,
                           uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, uint32_t* d_err, hipStream_t s);
 int fc_encode_chunks_dev(const uint32_t* d_values, const std::vector<uint64_t>& starts, int with_check,
-                         uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, uint32_t* d_err, hipStream_t s);
+                         uint8_t* d_out, uint64_t* d_offsets, uint32_t* d_err, hipStream_t s);
 
 }  // namespace ps

@@ -702,16 +702,16 @@ __global__ void k_fc_plan(SviPlan pl, const uint64_t* payload, const uint32_t* f
     bytes[c] = use_bitmap ? 8 + nw * 8 : (count ? 4 + payload[c] : 0);
 }
 
-// One block per chunk, bitmap chunks only: the bitmap of the values after the
-// first, built in LDS and written after the chunk's 8-byte header.
-constexpr uint32_t FC_LDS_WORDS = 2048;
-constexpr uint32_t FC_DEV_MAX_COUNT = 13107;  // a bitmap chunk of at most this many values has < 2048 words
+// One block per chunk, bitmap chunks of at most FC_LDS_WORDS words only: the
+// bitmap of the values after the first, built in LDS and written after the
+// chunk's 8-byte header.
+constexpr uint32_t FC_LDS_WORDS = 2048;  // wider bitmaps go through global scratch (k_fc_bitmap_wide_*)
 
 __global__ __launch_bounds__(256) void k_fc_bitmap_chunk(const uint32_t* v, SviPlan pl, const uint32_t* is_bitmap,
                                                          const uint32_t* words, const uint32_t* first,
                                                          const uint64_t* out_off, uint8_t* out) {
     const uint32_t c = blockIdx.x;
-    if (!is_bitmap[c]) return;
+    if (!is_bitmap[c] || words[c] > FC_LDS_WORDS) return;
     __shared__ unsigned long long bits[FC_LDS_WORDS];
     const uint32_t nw = words[c];
     const uint32_t f = first[c];
@@ -733,6 +733,63 @@ __global__ __launch_bounds__(256) void k_fc_bitmap_chunk(const uint32_t* v, SviP
     }
     const uint8_t* src = reinterpret_cast<const uint8_t*>(bits);
     for (uint32_t i = threadIdx.x; i < nw * 8; i += blockDim.x) o[8 + i] = src[i];
+}
+
+// Bitmap chunks too wide for LDS (more than FC_LDS_WORDS words): the bits are
+// ORed into a zeroed global scratch at the chunk's word base, then copied
+// after the chunk's header.  One block per chunk; other chunks return.
+__global__ __launch_bounds__(256) void k_fc_bitmap_wide_set(const uint32_t* v, SviPlan pl, const uint32_t* is_bitmap,
+                                                            const uint32_t* words, const uint32_t* first,
+                                                            const unsigned long long* word_base,
+                                                            unsigned long long* scratch) {
+    const uint32_t c = blockIdx.x;
+    if (!is_bitmap[c] || words[c] <= FC_LDS_WORDS) return;
+    const uint32_t f = first[c];
+    const uint64_t b = pl.starts[c], e = b + pl.counts[c];
+    unsigned long long* w = scratch + word_base[c];
+    // runs of 32 sorted values, one atomic per word a run touches
+    constexpr uint64_t RUN = 32;
+    for (uint64_t r0 = b + 1 + (uint64_t)threadIdx.x * RUN; r0 < e; r0 += (uint64_t)blockDim.x * RUN) {
+        const uint64_t r1 = r0 + RUN < e ? r0 + RUN : e;
+        uint64_t cw = ~0ull;
+        unsigned long long m = 0;
+        for (uint64_t i = r0; i < r1; ++i) {
+            const uint64_t val = (uint64_t)v[i] - f - 1;
+            if ((val >> 6) != cw) {
+                if (m) atomicOr(&w[cw], m);
+                cw = val >> 6;
+                m = 0;
+            }
+            m |= 1ull << (val & 63);
+        }
+        if (m) atomicOr(&w[cw], m);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fc_bitmap_wide_write(const uint32_t* is_bitmap, const uint32_t* words,
+                                                              const uint32_t* first,
+                                                              const unsigned long long* word_base,
+                                                              const unsigned long long* scratch,
+                                                              const uint64_t* out_off, uint8_t* out) {
+    const uint32_t c = blockIdx.x;
+    if (!is_bitmap[c] || words[c] <= FC_LDS_WORDS) return;
+    const uint32_t nw = words[c];
+    uint8_t* o = out + out_off[c];
+    if (threadIdx.x == 0) {
+        const uint32_t h = nw | CODEC_MAP, f = first[c];
+        for (int j = 0; j < 4; ++j) {
+            o[j] = (uint8_t)(h >> (8 * j));
+            o[4 + j] = (uint8_t)(f >> (8 * j));
+        }
+    }
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(scratch + word_base[c]);
+    for (uint64_t i = threadIdx.x; i < (uint64_t)nw * 8; i += blockDim.x) o[8 + i] = src[i];
+}
+
+// words of the chunks k_fc_bitmap_wide_* handle, for the scan of their bases
+__global__ void k_fc_wide_words(uint32_t n, const uint32_t* is_bitmap, const uint32_t* words, uint32_t* wide) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) wide[c] = (is_bitmap[c] && words[c] > FC_LDS_WORDS) ? words[c] : 0u;
 }
 
 __global__ void k_fc_bitmap_count(const uint8_t* in, const uint64_t* in_off, const uint32_t* select, uint32_t nsel,
@@ -1124,23 +1181,19 @@ int svi_encode_chunks_dev(const uint32_t* d_values, const uint8_t* d_bounds, con
 }
 
 int fc_encode_chunks_dev(const uint32_t* d_values, const std::vector<uint64_t>& starts, int with_check,
-                         uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, uint32_t* d_err, hipStream_t s) {
+                         uint8_t* d_out, uint64_t* d_offsets, uint32_t* d_err, hipStream_t s) {
     const uint32_t n = (uint32_t)(starts.size() - 1);
-    for (uint32_t c = 0; c < n; ++c)
-        if (starts[c + 1] - starts[c] > FC_DEV_MAX_COUNT) {  // bitmaps too wide for LDS: the host-planned encoder
-            std::vector<uint64_t> off;
-            return fc_encode_chunks(d_values, starts, with_check, d_out, out_cap, off, d_offsets, s);
-        }
-    DevBuf tab, d_toff, d_pay, d_fl, d_bytes;
+    DevBuf tab, d_toff, d_pay, d_fl, d_bytes, d_wbase, d_scratch;
     SviPlan pl{};
     if (int r = svi_plan_all(starts, s, tab, pl)) return r;
-    // first, last, ok, is_bitmap, words: five arrays of n
-    if (int r = d_fl.alloc((size_t)5 * n * 4 + 4, s)) return r;
+    // first, last, ok, is_bitmap, words, wide words: six arrays of n
+    if (int r = d_fl.alloc((size_t)6 * n * 4 + 4, s)) return r;
     uint32_t* first = d_fl.as<uint32_t>();
     uint32_t* last = first + n;
     uint32_t* ok = last + n;
     uint32_t* is_bm = ok + n;
     uint32_t* words = is_bm + n;
+    uint32_t* wide = words + n;
     if (n)
         hipLaunchKernelGGL(k_fc_inspect<1024>, dim3(n), dim3(1024), 0, s, d_values, pl.starts, first, last, ok);
     if (int r = svi_payload(d_values, pl, s, d_toff, d_pay)) return r;
@@ -1160,6 +1213,32 @@ int fc_encode_chunks_dev(const uint32_t* d_values, const std::vector<uint64_t>& 
         hipLaunchKernelGGL(k_fc_bitmap_chunk, dim3(n), dim3(256), 0, s, d_values, pl, (const uint32_t*)is_bm,
                            (const uint32_t*)words, (const uint32_t*)first, (const uint64_t*)d_offsets, d_out);
     PS_HIP_CHECK(hipGetLastError());
+    // bitmap chunks wider than LDS: a bitmap has fewer than 10 bits per value,
+    // so the chunks of more than 64 * FC_LDS_WORDS / 10 values bound the scratch
+    uint64_t wide_cap = 0;
+    for (uint32_t c = 0; c < n; ++c) {
+        const uint64_t cnt = starts[c + 1] - starts[c];
+        if (cnt * 10 > 64ull * FC_LDS_WORDS) wide_cap += cnt * 10 / 64 + 1;
+    }
+    if (wide_cap) {
+        if (int r = d_wbase.alloc(((uint64_t)n + 1) * 8, s)) return r;
+        if (int r = d_scratch.alloc(wide_cap * 8, s)) return r;
+        if (int r = launch_fill(d_scratch.p, 0, wide_cap * 8, s)) return r;
+        hipLaunchKernelGGL(k_fc_wide_words, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const uint32_t*)is_bm,
+                           (const uint32_t*)words, wide);
+        PS_HIP_CHECK(hipGetLastError());
+        if (int r = scan_excl_u32(wide, n, d_wbase.as<unsigned long long>(), s)) return r;
+        hipLaunchKernelGGL(k_fc_bitmap_wide_set, dim3(n), dim3(256), 0, s, d_values, pl, (const uint32_t*)is_bm,
+                           (const uint32_t*)words, (const uint32_t*)first,
+                           (const unsigned long long*)d_wbase.as<unsigned long long>(),
+                           d_scratch.as<unsigned long long>());
+        hipLaunchKernelGGL(k_fc_bitmap_wide_write, dim3(n), dim3(256), 0, s, (const uint32_t*)is_bm,
+                           (const uint32_t*)words, (const uint32_t*)first,
+                           (const unsigned long long*)d_wbase.as<unsigned long long>(),
+                           (const unsigned long long*)d_scratch.as<unsigned long long>(), (const uint64_t*)d_offsets,
+                           d_out);
+        PS_HIP_CHECK(hipGetLastError());
+    }
     return PS_OK;
 }
 

@@ -13,10 +13,14 @@
 //   stream   the new layer as FrontierCompression::Encode chunks of 2^18
 //            values, as CompressedSegmentWriter writes it
 //            (Common/CompressedFrontier.cpp:128-158)
-//   cross    cross-segment children of the new layer in (parent, slot)
-//            order, stably grouped by (op, segment), each bucket cut into
-//            4096-value chunks compressed with FrontierCompression::EncodeWithCheck,
-//            as SmartMultiplexor flushes them (Common/Multiplexor.cpp:125-161)
+//   cross    (a second stream, beside `stream`) cross-segment children of
+//            the new layer in (parent, slot) order, stably grouped by
+//            (op, segment), each bucket cut into 4096-value chunks compressed
+//            with FrontierCompression::EncodeWithCheck, as SmartMultiplexor
+//            flushes them (Common/Multiplexor.cpp:125-161)
+//
+// Both streams are encoded with the per-chunk choice, sizes and offsets on the
+// device; the chunk offsets come back with the step's end.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -58,9 +62,14 @@ struct ps_seg3 {
     unsigned long long* out_off = nullptr;
     unsigned long long* counters = nullptr;
     unsigned long long* h_counters = nullptr;  // pinned
-    ps::GrowBuf out_idx, fr_bytes, xseg_copy, slot_buf, compact, vals, xs_bytes, block_cnt, block_off;
+    ps::GrowBuf out_idx, fr_bytes, xseg_copy, slot_buf, compact, vals, xs_bytes, block_cnt, block_off, fr_off, x_off;
+    ps::GrowHost h_fr_off, h_x_off;  // chunk offsets of the frontier and cross-segment streams (pinned)
     uint32_t cus = 256;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the cross-segment children are found, grouped and encoded on `side`
+    // while the new frontier is encoded on `stream`
+    hipStream_t side = nullptr;
+    hipEvent_t ev_written = nullptr, ev_side = nullptr;
     // results of the last step (host)
     std::vector<uint64_t> fr_chunk_off;
     std::vector<uint32_t> bk_op, bk_seg;
@@ -74,25 +83,31 @@ namespace {
 int destroy_s3(ps_seg3* w) {
     if (!w) return PS_OK;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
+    if (w->side) (void)hipStreamSynchronize(w->side);
     for (GrowBuf* g : {&w->out_idx, &w->fr_bytes, &w->xseg_copy, &w->slot_buf, &w->compact, &w->vals, &w->xs_bytes,
-                       &w->block_cnt, &w->block_off})
+                       &w->block_cnt, &w->block_off, &w->fr_off, &w->x_off})
         g->release(w->stream);
     if (w->stream) (void)hipStreamSynchronize(w->stream);
+    w->h_fr_off.release();
+    w->h_x_off.release();
     for (void* p : {(void*)w->next, (void*)w->cur, (void*)w->dirty, (void*)w->cnt_out, (void*)w->out_off,
                     (void*)w->counters})
         if (p) (void)hipFree(p);
     if (w->h_counters) (void)hipHostFree(w->h_counters);
     if (w->ev0) (void)hipEventDestroy(w->ev0);
     if (w->ev1) (void)hipEventDestroy(w->ev1);
+    if (w->ev_written) (void)hipEventDestroy(w->ev_written);
+    if (w->ev_side) (void)hipEventDestroy(w->ev_side);
+    if (w->side) (void)hipStreamDestroy(w->side);
     if (w->own_stream && w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
     (void)hipGetLastError();
     return PS_OK;
 }
 
-int read_counters(ps_seg3* w) {
-    PS_HIP_CHECK(hipMemcpyAsync(w->h_counters, w->counters, psk::S3_NCOUNTERS * 8, hipMemcpyDeviceToHost, w->stream));
-    return stream_wait(w->stream);
+int read_counters(ps_seg3* w, hipStream_t s) {
+    PS_HIP_CHECK(hipMemcpyAsync(w->h_counters, w->counters, psk::S3_NCOUNTERS * 8, hipMemcpyDeviceToHost, s));
+    return stream_wait(s);
 }
 
 uint32_t grid_for(const ps_seg3* w, uint64_t n, uint32_t tpb) {
@@ -172,7 +187,7 @@ int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur,
     hipLaunchKernelGGL(psk::k_s3_count<S3_TPB>, dim3(grid_scan), dim3(S3_TPB), 0, s, sa);
     PS_HIP_CHECK(hipGetLastError());
     if (int rc = scan_excl_u64(w->cnt_out, w->nregions, w->out_off, s)) return rc;
-    if (int rc = read_counters(w)) return rc;
+    if (int rc = read_counters(w, s)) return rc;
     const uint64_t children_in = w->h_counters[0];
     const uint64_t bad_ids = w->h_counters[1];
     const uint64_t count = w->h_counters[2];
@@ -192,29 +207,19 @@ int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur,
                     w->seg_bits);
     }
 
-    // 5 the new frontier stream
-    std::vector<uint64_t> starts;
-    uint64_t fr_cap = 0;
-    for (uint64_t c = 0; c < count; c += S3_FRONTIER_CHUNK) {
-        starts.push_back(c);
-        fr_cap += fc_worst(std::min<uint64_t>(S3_FRONTIER_CHUNK, count - c));
-    }
-    starts.push_back(count);  // no chunk at all for an empty layer
-    if (int rc = w->fr_bytes.ensure(std::max<uint64_t>(fr_cap, 16), s)) return rc;
-    if (int rc = fc_encode_chunks(w->out_idx.as<uint32_t>(), starts, 0, w->fr_bytes.as<uint8_t>(), w->fr_bytes.cap,
-                                  w->fr_chunk_off, nullptr, s))
-        return rc;
-
-    // 6 cross-segment children of the new layer, grouped by (op, segment)
+    // 5 cross-segment children of the new layer (side stream, beside 6)
     const uint64_t nslots = count * (uint64_t)w->slots_x;
     if (nslots > 0x7FFFFFFFull)
         return fail(PS_ERR_CAPACITY, "%llu cross-segment slots in one step exceed 2^31", (unsigned long long)nslots);
-    uint64_t m = 0;
+    hipStream_t s2 = w->side;
+    PS_HIP_CHECK(hipEventRecord(w->ev_written, s));
+    PS_HIP_CHECK(hipStreamWaitEvent(s2, w->ev_written, 0));
+    uint64_t nblk = 0;
     if (count && w->slots_x) {
-        if (int rc = w->slot_buf.ensure(nslots * 8, s)) return rc;
-        const uint64_t nblk = (count + psk::S3_CROSS_PER_BLOCK - 1) / psk::S3_CROSS_PER_BLOCK;
-        if (int rc = w->block_cnt.ensure(nblk * 8, s)) return rc;
-        if (int rc = w->block_off.ensure((nblk + 1) * 8, s)) return rc;
+        if (int rc = w->slot_buf.ensure(nslots * 8, s2)) return rc;
+        nblk = (count + psk::S3_CROSS_PER_BLOCK - 1) / psk::S3_CROSS_PER_BLOCK;
+        if (int rc = w->block_cnt.ensure(nblk * 8, s2)) return rc;
+        if (int rc = w->block_off.ensure((nblk + 1) * 8, s2)) return rc;
         psk::S3CrossArgs ca{};
         ca.prm = params_of(w->puzzle);
         ca.idx = w->out_idx.as<uint32_t>();
@@ -226,40 +231,65 @@ int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur,
         ca.slots = w->slot_buf.as<unsigned long long>();
         ca.counters = w->counters;
         ca.block_count = w->block_cnt.as<unsigned long long>();
-        if (int rc = s3_cross(w->puzzle, w->mode_x, ca, (uint32_t)nblk, s, nullptr)) return rc;
+        if (int rc = s3_cross(w->puzzle, w->mode_x, ca, (uint32_t)nblk, s2, nullptr)) return rc;
         if (int rc = scan_excl_u64(w->block_cnt.as<unsigned long long>(), nblk, w->block_off.as<unsigned long long>(),
-                                   s))
+                                   s2))
             return rc;
-        if (int rc = read_counters(w)) return rc;
-        if (w->h_counters[5])
-            return fail(PS_ERR_INDEX_RANGE, "%llu children fall outside the %u segments",
-                        (unsigned long long)w->h_counters[5], w->nseg);
-        m = w->h_counters[4];
-        if (w->h_counters[6] != m)
-            return fail(PS_ERR_INTERNAL, "%llu cross-segment children written for %llu counted",
-                        (unsigned long long)w->h_counters[6], (unsigned long long)m);
-        if (m) {
-            if (int rc = w->compact.ensure(m * 8, s)) return rc;
-            hipLaunchKernelGGL(psk::k_fs_compact<256>, dim3((uint32_t)nblk), dim3(256), 0, s,
-                               (const unsigned long long*)w->slot_buf.as<unsigned long long>(),
-                               (uint32_t)(psk::S3_CROSS_PER_BLOCK * w->slots_x),
-                               (const unsigned long long*)w->block_cnt.as<unsigned long long>(),
-                               (const unsigned long long*)w->block_off.as<unsigned long long>(),
-                               w->compact.as<unsigned long long>());
-            PS_HIP_CHECK(hipGetLastError());
-        }
     }
-    uint64_t xbytes = 0, nchunks_x = 0;
-    if (m) {
-        if (int rc = w->vals.ensure(m * 4 + 64, s)) return rc;
+
+    // 6 the new frontier stream (main stream); its chunk offsets come back with the step
+    std::vector<uint64_t> starts;
+    uint64_t fr_cap = 0;
+    for (uint64_t c = 0; c < count; c += S3_FRONTIER_CHUNK) {
+        starts.push_back(c);
+        fr_cap += fc_worst(std::min<uint64_t>(S3_FRONTIER_CHUNK, count - c));
+    }
+    starts.push_back(count);  // no chunk at all for an empty layer
+    const size_t nfo = starts.size();
+    if (int rc = w->fr_bytes.ensure(std::max<uint64_t>(fr_cap, 16), s)) return rc;
+    if (int rc = w->fr_off.ensure(nfo * 8, s)) return rc;
+    if (int rc = w->h_fr_off.ensure(nfo * 8)) return rc;
+    if (int rc = fc_encode_chunks_dev(w->out_idx.as<uint32_t>(), starts, 0, w->fr_bytes.as<uint8_t>(),
+                                      w->fr_off.as<uint64_t>(), reinterpret_cast<uint32_t*>(w->counters + 7), s))
+        return rc;
+    PS_HIP_CHECK(hipMemcpyAsync(w->h_fr_off.p, w->fr_off.p, nfo * 8, hipMemcpyDeviceToHost, s));
+
+    // 7 the cross-segment children compacted, grouped by (op, segment) and
+    //   encoded (side stream); the bucket byte offsets come back with the step
+    uint64_t m = 0, xbytes = 0, nchunks_x = 0;
+    std::vector<uint64_t> first_chunk;
+    const int cross_rc = [&]() -> int {
+        if (count && w->slots_x) {
+            if (int rc = read_counters(w, s2)) return rc;
+            if (w->h_counters[5])
+                return fail(PS_ERR_INDEX_RANGE, "%llu children fall outside the %u segments",
+                            (unsigned long long)w->h_counters[5], w->nseg);
+            m = w->h_counters[4];
+            if (w->h_counters[6] != m)
+                return fail(PS_ERR_INTERNAL, "%llu cross-segment children written for %llu counted",
+                            (unsigned long long)w->h_counters[6], (unsigned long long)m);
+            if (m) {
+                if (int rc = w->compact.ensure(m * 8, s2)) return rc;
+                hipLaunchKernelGGL(psk::k_fs_compact<256>, dim3((uint32_t)nblk), dim3(256), 0, s2,
+                                   (const unsigned long long*)w->slot_buf.as<unsigned long long>(),
+                                   (uint32_t)(psk::S3_CROSS_PER_BLOCK * w->slots_x),
+                                   (const unsigned long long*)w->block_cnt.as<unsigned long long>(),
+                                   (const unsigned long long*)w->block_off.as<unsigned long long>(),
+                                   w->compact.as<unsigned long long>());
+                PS_HIP_CHECK(hipGetLastError());
+            }
+        }
+        if (!m) return PS_OK;
+        if (int rc = w->vals.ensure(m * 4 + 64, s2)) return rc;
         std::vector<uint32_t> uk;
         std::vector<uint64_t> rc_;
         uint64_t dropped = 0;
         if (int rc = stable_group_by_key(w->compact.as<unsigned long long>(), m, (uint32_t)(w->ops * w->nseg),
-                                         w->vals.as<uint32_t>(), uk, rc_, &dropped, s))
+                                         w->vals.as<uint32_t>(), uk, rc_, &dropped, s2))
             return rc;
         if (dropped) return fail(PS_ERR_INTERNAL, "%llu cross-segment keys out of range", (unsigned long long)dropped);
-        std::vector<uint64_t> cstarts, first_chunk(uk.size() + 1);
+        std::vector<uint64_t> cstarts;
+        first_chunk.assign(uk.size() + 1, 0);
         uint64_t pos = 0, cap = 0;
         for (size_t b = 0; b < uk.size(); ++b) {
             first_chunk[b] = cstarts.size();
@@ -274,13 +304,14 @@ int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur,
         if (pos != m)
             return fail(PS_ERR_INTERNAL, "bucket sizes %llu != %llu children", (unsigned long long)pos,
                         (unsigned long long)m);
-        if (int rc = w->xs_bytes.ensure(cap + 16, s)) return rc;
-        std::vector<uint64_t> xoff;
-        if (int rc = fc_encode_chunks(w->vals.as<uint32_t>(), cstarts, 1, w->xs_bytes.as<uint8_t>(), w->xs_bytes.cap,
-                                      xoff, nullptr, s))
-            return rc;
         nchunks_x = cstarts.size() - 1;
-        xbytes = xoff.back();
+        if (int rc = w->xs_bytes.ensure(cap + 16, s2)) return rc;
+        if (int rc = w->x_off.ensure((nchunks_x + 1) * 8, s2)) return rc;
+        if (int rc = w->h_x_off.ensure((nchunks_x + 1) * 8)) return rc;
+        if (int rc = fc_encode_chunks_dev(w->vals.as<uint32_t>(), cstarts, 1, w->xs_bytes.as<uint8_t>(),
+                                          w->x_off.as<uint64_t>(), reinterpret_cast<uint32_t*>(w->counters + 7), s2))
+            return rc;
+        PS_HIP_CHECK(hipMemcpyAsync(w->h_x_off.p, w->x_off.p, (nchunks_x + 1) * 8, hipMemcpyDeviceToHost, s2));
         uint64_t vpos = 0;
         for (size_t b = 0; b < uk.size(); ++b) {
             w->bk_op.push_back(uk[b] / w->nseg);
@@ -288,13 +319,39 @@ int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur,
             w->bk_values.push_back(rc_[b]);
             vpos += rc_[b];
             w->bk_value_off.push_back(vpos);
-            w->bk_byte_off.push_back(xoff[first_chunk[b + 1]]);
         }
+        return PS_OK;
+    }();
+    if (cross_rc) {  // leave no work of this step queued behind the error
+        (void)hipStreamSynchronize(s2);
+        (void)hipStreamSynchronize(s);
+        (void)hipGetLastError();
+        return cross_rc;
     }
+    // the main stream waits for the side stream's work
+    PS_HIP_CHECK(hipEventRecord(w->ev_side, s2));
+    PS_HIP_CHECK(hipStreamWaitEvent(s, w->ev_side, 0));
+    PS_HIP_CHECK(hipMemcpyAsync(w->h_counters + 7, w->counters + 7, 8, hipMemcpyDeviceToHost, s));
     PS_HIP_CHECK(hipEventRecord(w->ev1, s));
     if (int r = event_wait(w->ev1)) return r;
     float ms = 0.f;
     PS_HIP_CHECK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+    const uint64_t* hfo = w->h_fr_off.as<uint64_t>();
+    w->fr_chunk_off.assign(hfo, hfo + nfo);
+    if (w->fr_chunk_off.back() > w->fr_bytes.cap)
+        return fail(PS_ERR_INTERNAL, "frontier stream of %llu bytes overran its %zu-byte buffer",
+                    (unsigned long long)w->fr_chunk_off.back(), w->fr_bytes.cap);
+    if (w->h_counters[7])
+        return fail(PS_ERR_INVALID_ARGUMENT, "%llu frontier chunks are not sorted",
+                    (unsigned long long)w->h_counters[7]);
+    if (nchunks_x) {
+        const uint64_t* hxo = w->h_x_off.as<uint64_t>();
+        xbytes = hxo[nchunks_x];
+        if (xbytes > w->xs_bytes.cap)
+            return fail(PS_ERR_INTERNAL, "cross-segment stream of %llu bytes overran its %zu-byte buffer",
+                        (unsigned long long)xbytes, w->xs_bytes.cap);
+        for (size_t b = 0; b + 1 < first_chunk.size(); ++b) w->bk_byte_off.push_back(hxo[first_chunk[b + 1]]);
+    }
 
     r.count = count;
     r.frontier_bytes = w->fr_chunk_off.back();
@@ -385,8 +442,12 @@ int ps_seg3_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st, ps_se
     if (int r = dalloc((void**)&w->counters, psk::S3_NCOUNTERS * 8)) return cleanup(r);
     if (hipHostMalloc((void**)&w->h_counters, psk::S3_NCOUNTERS * 8, hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(PS_ERR_OUT_OF_MEMORY, "pinned host allocation failed"));
-    if (hipEventCreate(&w->ev0) != hipSuccess || hipEventCreate(&w->ev1) != hipSuccess)
+    if (hipEventCreate(&w->ev0) != hipSuccess || hipEventCreate(&w->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&w->ev_written, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&w->ev_side, hipEventDisableTiming) != hipSuccess)
         return cleanup(fail(PS_ERR_HIP, "event creation failed"));
+    if (hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(PS_ERR_HIP, "stream creation failed"));
     if (int r = launch_fill(w->next, 0, w->nwords * 8, w->stream)) return cleanup(r);
     if (w->odd)
         if (int r = launch_fill(w->cur, 0, w->nwords * 8, w->stream)) return cleanup(r);

@@ -23,7 +23,8 @@ constexpr uint32_t S3_CROSS_PER_BLOCK = 4096;                    // parents per 
 
 // counters of a step: [0] in-segment children, [1] ids outside the segment,
 // [2] the layer count, [3] words scanned, [4] cross-segment children, [5]
-// children outside [0, nseg), [6] cross children written
+// children outside [0, nseg), [6] cross children written, [7] frontier chunks
+// that FrontierCompression::Encode would take as a bitmap but are not sorted
 constexpr int S3_NCOUNTERS = 8;
 
 struct S3MarkArgs {
