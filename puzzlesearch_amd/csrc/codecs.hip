@@ -617,54 +617,6 @@ __global__ __launch_bounds__(TPB) void k_fc_inspect(const uint32_t* v, const uin
     }
 }
 
-__global__ void k_fc_bitmap_set(const uint32_t* v, const uint64_t* starts, const uint32_t* select, uint32_t nsel,
-                                const uint64_t* word_base, const uint32_t* first, unsigned long long* words) {
-    // one block per selected chunk
-    const uint32_t k = blockIdx.x;
-    if (k >= nsel) return;
-    const uint32_t c = select[k];
-    const uint64_t b = starts[c], e = starts[c + 1];
-    const uint32_t f = first[c];
-    unsigned long long* w = words + word_base[k];
-    // each thread ORs a run of RUN consecutive values, one atomic per word the
-    // run touches (sorted values put neighbours in the same word)
-    constexpr uint64_t RUN = 32;
-    for (uint64_t r0 = b + 1 + (uint64_t)threadIdx.x * RUN; r0 < e; r0 += (uint64_t)blockDim.x * RUN) {
-        const uint64_t r1 = r0 + RUN < e ? r0 + RUN : e;
-        uint64_t cw = ~0ull;
-        unsigned long long m = 0;
-        for (uint64_t i = r0; i < r1; ++i) {
-            const uint64_t val = (uint64_t)v[i] - f - 1;
-            if ((val >> 6) != cw) {
-                if (m) atomicOr(&w[cw], m);
-                cw = val >> 6;
-                m = 0;
-            }
-            m |= 1ull << (val & 63);
-        }
-        if (m) atomicOr(&w[cw], m);
-    }
-}
-
-__global__ void k_fc_bitmap_write(const uint32_t* select, uint32_t nsel, const uint64_t* word_base,
-                                  const uint32_t* first, const unsigned long long* words, const uint64_t* out_off,
-                                  uint8_t* out) {
-    const uint32_t k = blockIdx.x;
-    if (k >= nsel) return;
-    const uint32_t c = select[k];
-    const uint64_t nw = word_base[k + 1] - word_base[k];
-    uint8_t* o = out + out_off[c];
-    if (threadIdx.x == 0) {
-        const uint32_t h = (uint32_t)nw | CODEC_MAP, f = first[c];
-        for (int j = 0; j < 4; ++j) {
-            o[j] = (uint8_t)(h >> (8 * j));
-            o[4 + j] = (uint8_t)(f >> (8 * j));
-        }
-    }
-    const uint8_t* src = reinterpret_cast<const uint8_t*>(words + word_base[k]);
-    for (uint64_t i = threadIdx.x; i < nw * 8; i += blockDim.x) o[8 + i] = src[i];
-}
-
 // ---------------------------------------------------------------- stream-ordered chunk plans
 
 // chunk c's StreamVInt bytes: count, payload and (with bounds) the 4-bit
@@ -709,9 +661,14 @@ constexpr uint32_t FC_LDS_WORDS = 2048;  // wider bitmaps go through global scra
 
 __global__ __launch_bounds__(256) void k_fc_bitmap_chunk(const uint32_t* v, SviPlan pl, const uint32_t* is_bitmap,
                                                          const uint32_t* words, const uint32_t* first,
-                                                         const uint64_t* out_off, uint8_t* out) {
+                                                         const uint64_t* out_off, uint8_t* out, uint64_t out_cap,
+                                                         uint32_t* err) {
     const uint32_t c = blockIdx.x;
     if (!is_bitmap[c] || words[c] > FC_LDS_WORDS) return;
+    if (out_off[c + 1] > out_cap) {  // not written; counted
+        if (threadIdx.x == 0 && err) atomicAdd(err, 1u);
+        return;
+    }
     __shared__ unsigned long long bits[FC_LDS_WORDS];
     const uint32_t nw = words[c];
     const uint32_t f = first[c];
@@ -770,9 +727,14 @@ __global__ __launch_bounds__(256) void k_fc_bitmap_wide_write(const uint32_t* is
                                                               const uint32_t* first,
                                                               const unsigned long long* word_base,
                                                               const unsigned long long* scratch,
-                                                              const uint64_t* out_off, uint8_t* out) {
+                                                              const uint64_t* out_off, uint8_t* out,
+                                                              uint64_t out_cap, uint32_t* err) {
     const uint32_t c = blockIdx.x;
     if (!is_bitmap[c] || words[c] <= FC_LDS_WORDS) return;
+    if (out_off[c + 1] > out_cap) {  // not written; counted
+        if (threadIdx.x == 0 && err) atomicAdd(err, 1u);
+        return;
+    }
     const uint32_t nw = words[c];
     uint8_t* o = out + out_off[c];
     if (threadIdx.x == 0) {
@@ -906,67 +868,6 @@ struct SviEncodeResult {
     std::vector<uint64_t> bytes;  // per selected chunk
 };
 
-int svi_sizes(const uint32_t* d_values, const uint8_t* d_bounds, const std::vector<uint64_t>& starts,
-              const std::vector<uint32_t>& sel, hipStream_t s, DevBuf& d_starts_sel, DevBuf& d_tbase, DevBuf& d_counts,
-              DevBuf& d_toff, DevBuf& d_payload, std::vector<uint64_t>& chunk_bytes, SviPlan& pl) {
-    const uint32_t n = (uint32_t)sel.size();
-    std::vector<uint64_t> st(n + 1), tb(n + 1);
-    std::vector<uint32_t> cnt(n);
-    uint64_t t = 0;
-    for (uint32_t k = 0; k < n; ++k) {
-        const uint64_t c0 = starts[sel[k]], c1 = starts[sel[k] + 1];
-        if (c1 - c0 > MAX_CHUNK_VALUES) return fail(PS_ERR_INVALID_ARGUMENT, "chunk of %llu values exceeds %u",
-                                                    (unsigned long long)(c1 - c0), MAX_CHUNK_VALUES);
-        st[k] = c0;
-        cnt[k] = (uint32_t)(c1 - c0);
-        tb[k] = t;
-        t += (uint64_t)((cnt[k] + 7) / 8 * 8) / 4;
-    }
-    st[n] = n ? starts[sel[n - 1] + 1] : 0;
-    tb[n] = t;
-    if (int r = to_device(d_starts_sel, st, s)) return r;
-    if (int r = to_device(d_tbase, tb, s)) return r;
-    if (int r = to_device(d_counts, cnt, s)) return r;
-    if (int r = d_toff.alloc((t + 1) * 4, s)) return r;
-    if (int r = d_payload.alloc((n + 1) * 8, s)) return r;
-    uint64_t max_t = 0;
-    for (uint32_t k = 0; k < n; ++k) max_t = std::max<uint64_t>(max_t, tb[k + 1] - tb[k]);
-    // one block row per chunk when the rows of the widest chunk waste at most
-    // about as many threads as there are tuples
-    const bool g2 = n > 0 && n <= 65535 && ((max_t + 255) / 256) * 256 * n <= 2 * t + 256ull * n;
-    pl = SviPlan{d_starts_sel.as<uint64_t>(), d_tbase.as<uint64_t>(), d_counts.as<uint32_t>(), n, t,
-                 (uint32_t)max_t, g2 ? 1 : 0};
-    if (t) hipLaunchKernelGGL(k_svi_tuple_sizes, svi_grid(pl), dim3(256), 0, s, d_values, pl, d_toff.as<uint32_t>());
-    if (n) hipLaunchKernelGGL(k_chunk_scan_u32<256>, dim3(n), dim3(256), 0, s, d_toff.as<uint32_t>(), pl.tuple_base,
-                              d_payload.as<uint64_t>());
-    PS_HIP_CHECK(hipGetLastError());
-    std::vector<uint64_t> payload;
-    if (int r = to_host(payload, d_payload.p, n, s)) return r;
-    chunk_bytes.resize(n);
-    for (uint32_t k = 0; k < n; ++k) {
-        if (cnt[k] == 0) {
-            chunk_bytes[k] = 0;  // StreamVInt::Encode writes nothing for an empty chunk
-            continue;
-        }
-        const uint64_t adjusted = (cnt[k] + 7) / 8 * 8;
-        chunk_bytes[k] = 4 + payload[k] + (d_bounds ? adjusted / 2 : 0);
-    }
-    return PS_OK;
-}
-
-int svi_write(const uint32_t* d_values, const uint8_t* d_bounds, const SviPlan& pl, DevBuf& d_toff, DevBuf& d_payload,
-              const std::vector<uint64_t>& out_off_sel, uint8_t* d_out, hipStream_t s) {
-    DevBuf d_off;
-    if (int r = to_device(d_off, out_off_sel, s)) return r;
-    if (pl.ntuples)
-        hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, d_bounds, pl,
-                           d_toff.as<uint32_t>(), d_off.as<uint64_t>(), d_payload.as<uint64_t>(), d_out,
-                           (const uint32_t*)nullptr, ~0ull, (uint32_t*)nullptr);
-    PS_HIP_CHECK(hipGetLastError());
-    if (int r = stream_wait(s)) return r;
-    return PS_OK;
-}
-
 int read_starts(std::vector<uint64_t>& starts, const uint64_t* d_starts, uint32_t nchunks, hipStream_t s) {
     if (int r = to_host(starts, d_starts, (size_t)nchunks + 1, s)) return r;
     for (uint32_t c = 0; c < nchunks; ++c)
@@ -1012,103 +913,6 @@ int svi_decode_chunks(const uint8_t* d_in, const uint64_t* d_in_off, const std::
 }
 
 }  // namespace
-
-int svi_encode_chunks(const uint32_t* d_values, const uint8_t* d_bounds, const std::vector<uint64_t>& starts,
-                      uint8_t* d_out, uint64_t out_cap, std::vector<uint64_t>& off, hipStream_t s) {
-    const uint32_t nchunks = (uint32_t)(starts.size() - 1);
-    std::vector<uint32_t> sel(nchunks);
-    for (uint32_t c = 0; c < nchunks; ++c) sel[c] = c;
-    DevBuf d_st, d_tb, d_cnt, d_toff, d_pay;
-    std::vector<uint64_t> bytes;
-    SviPlan pl{};
-    if (int r = svi_sizes(d_values, d_bounds, starts, sel, s, d_st, d_tb, d_cnt, d_toff, d_pay, bytes, pl)) return r;
-    off.assign(nchunks + 1, 0);
-    for (uint32_t c = 0; c < nchunks; ++c) off[c + 1] = off[c] + bytes[c];
-    if (off[nchunks] > out_cap)
-        return fail(PS_ERR_CAPACITY, "encoded size %llu exceeds output capacity %llu",
-                    (unsigned long long)off[nchunks], (unsigned long long)out_cap);
-    std::vector<uint64_t> off_sel(off.begin(), off.end() - 1);
-    return svi_write(d_values, d_bounds, pl, d_toff, d_pay, off_sel, d_out, s);
-}
-
-int fc_encode_chunks(const uint32_t* d_values, const std::vector<uint64_t>& starts, int with_check, uint8_t* d_out,
-                     uint64_t out_cap, std::vector<uint64_t>& off, uint64_t* d_offsets, hipStream_t s) {
-    const uint32_t nchunks = (uint32_t)(starts.size() - 1);
-    DevBuf d_starts, d_first, d_last, d_ok, d_off_own;
-    if (int r = to_device(d_starts, starts, s)) return r;
-    if (!d_offsets) {
-        if (int r = d_off_own.alloc(((size_t)nchunks + 1) * 8, s)) return r;
-        d_offsets = d_off_own.as<uint64_t>();
-    }
-    if (int r = d_first.alloc((size_t)nchunks * 4 + 4, s)) return r;
-    if (int r = d_last.alloc((size_t)nchunks * 4 + 4, s)) return r;
-    if (int r = d_ok.alloc((size_t)nchunks * 4 + 4, s)) return r;
-    if (nchunks)
-        hipLaunchKernelGGL(k_fc_inspect<1024>, dim3(nchunks), dim3(1024), 0, s, d_values, d_starts.as<uint64_t>(),
-                           d_first.as<uint32_t>(), d_last.as<uint32_t>(), d_ok.as<uint32_t>());
-    PS_HIP_CHECK(hipGetLastError());
-    std::vector<uint32_t> first, last, ok;
-    if (int r = to_host(first, d_first.p, nchunks, s)) return r;
-    if (int r = to_host(last, d_last.p, nchunks, s)) return r;
-    if (int r = to_host(ok, d_ok.p, nchunks, s)) return r;
-    // Common/FrontierCompression.cpp:92-115: bitmap when count > 128 and the
-    // bitmap has fewer bits than 10 per value (minus 8 without the check)
-    std::vector<uint32_t> bm, sv;
-    std::vector<uint64_t> bm_words;
-    for (uint32_t c = 0; c < nchunks; ++c) {
-        const uint64_t count = starts[c + 1] - starts[c];
-        bool use_bitmap = false;
-        if (count > 128) {
-            const uint64_t min_bits = count * 10;
-            const uint64_t f = first[c], l = last[c];
-            const uint64_t bits = (l < f) ? ~0ull : ((l - f + 63) / 64 * 8 + 8) * 8;
-            if (with_check) use_bitmap = bits < min_bits && ok[c];
-            else use_bitmap = bits < min_bits - 8 && l >= f;
-            if (!with_check && use_bitmap && !ok[c])
-                return fail(PS_ERR_INVALID_ARGUMENT, "chunk %u is not sorted; use the checked encoder", c);
-        }
-        if (use_bitmap) {
-            bm.push_back(c);
-            bm_words.push_back(((uint64_t)last[c] - first[c] + 63) / 64);
-        } else {
-            sv.push_back(c);
-        }
-    }
-    DevBuf d_st, d_tb, d_cnt, d_toff, d_pay;
-    std::vector<uint64_t> sv_bytes;
-    SviPlan pl{};
-    if (int r = svi_sizes(d_values, nullptr, starts, sv, s, d_st, d_tb, d_cnt, d_toff, d_pay, sv_bytes, pl)) return r;
-    std::vector<uint64_t> bytes(nchunks, 0);
-    for (size_t k = 0; k < sv.size(); ++k) bytes[sv[k]] = sv_bytes[k];
-    for (size_t k = 0; k < bm.size(); ++k) bytes[bm[k]] = 8 + bm_words[k] * 8;
-    off.assign(nchunks + 1, 0);
-    for (uint32_t c = 0; c < nchunks; ++c) off[c + 1] = off[c] + bytes[c];
-    if (off[nchunks] > out_cap)
-        return fail(PS_ERR_CAPACITY, "encoded size %llu exceeds output capacity %llu",
-                    (unsigned long long)off[nchunks], (unsigned long long)out_cap);
-    PS_HIP_CHECK(hipMemcpyAsync(d_offsets, off.data(), off.size() * 8, hipMemcpyHostToDevice, s));
-    std::vector<uint64_t> sv_off(sv.size());
-    for (size_t k = 0; k < sv.size(); ++k) sv_off[k] = off[sv[k]];
-    if (int r = svi_write(d_values, nullptr, pl, d_toff, d_pay, sv_off, d_out, s)) return r;
-    if (!bm.empty()) {
-        std::vector<uint64_t> wb(bm.size() + 1, 0);
-        for (size_t k = 0; k < bm.size(); ++k) wb[k + 1] = wb[k] + bm_words[k];
-        DevBuf d_sel, d_wb, d_words;
-        if (int r = to_device(d_sel, bm, s)) return r;
-        if (int r = to_device(d_wb, wb, s)) return r;
-        if (int r = d_words.alloc(wb.back() * 8 + 8, s)) return r;
-        if (int r = launch_fill(d_words.p, 0, wb.back() * 8 + 8, s)) return r;
-        hipLaunchKernelGGL(k_fc_bitmap_set, dim3((uint32_t)bm.size()), dim3(256), 0, s, d_values,
-                           d_starts.as<uint64_t>(), d_sel.as<uint32_t>(), (uint32_t)bm.size(), d_wb.as<uint64_t>(),
-                           d_first.as<uint32_t>(), d_words.as<unsigned long long>());
-        hipLaunchKernelGGL(k_fc_bitmap_write, dim3((uint32_t)bm.size()), dim3(256), 0, s, d_sel.as<uint32_t>(),
-                           (uint32_t)bm.size(), d_wb.as<uint64_t>(), d_first.as<uint32_t>(),
-                           d_words.as<unsigned long long>(), d_offsets, d_out);
-        PS_HIP_CHECK(hipGetLastError());
-        if (int r = stream_wait(s)) return r;
-    }
-    return PS_OK;
-}
 
 namespace {
 
@@ -1181,7 +985,8 @@ int svi_encode_chunks_dev(const uint32_t* d_values, const uint8_t* d_bounds, con
 }
 
 int fc_encode_chunks_dev(const uint32_t* d_values, const std::vector<uint64_t>& starts, int with_check,
-                         uint8_t* d_out, uint64_t* d_offsets, uint32_t* d_err, hipStream_t s) {
+                         uint8_t* d_out, uint64_t out_cap, uint64_t* d_offsets, uint32_t* d_err, hipStream_t s) {
+    uint32_t* d_cap_err = d_err ? d_err + 1 : nullptr;
     const uint32_t n = (uint32_t)(starts.size() - 1);
     DevBuf tab, d_toff, d_pay, d_fl, d_bytes, d_wbase, d_scratch;
     SviPlan pl{};
@@ -1207,11 +1012,12 @@ int fc_encode_chunks_dev(const uint32_t* d_values, const std::vector<uint64_t>& 
     if (pl.ntuples)
         hipLaunchKernelGGL(k_svi_write, svi_grid(pl), dim3(256), 0, s, d_values, (const uint8_t*)nullptr, pl,
                            (const uint32_t*)d_toff.as<uint32_t>(), (const uint64_t*)d_offsets,
-                           (const uint64_t*)d_pay.as<uint64_t>(), d_out, (const uint32_t*)is_bm, ~0ull,
-                           (uint32_t*)nullptr);
+                           (const uint64_t*)d_pay.as<uint64_t>(), d_out, (const uint32_t*)is_bm, out_cap,
+                           d_cap_err);
     if (n)
         hipLaunchKernelGGL(k_fc_bitmap_chunk, dim3(n), dim3(256), 0, s, d_values, pl, (const uint32_t*)is_bm,
-                           (const uint32_t*)words, (const uint32_t*)first, (const uint64_t*)d_offsets, d_out);
+                           (const uint32_t*)words, (const uint32_t*)first, (const uint64_t*)d_offsets, d_out, out_cap,
+                           d_cap_err);
     PS_HIP_CHECK(hipGetLastError());
     // bitmap chunks wider than LDS: a bitmap has fewer than 10 bits per value,
     // so the chunks of more than 64 * FC_LDS_WORDS / 10 values bound the scratch
@@ -1236,7 +1042,7 @@ int fc_encode_chunks_dev(const uint32_t* d_values, const std::vector<uint64_t>& 
                            (const uint32_t*)words, (const uint32_t*)first,
                            (const unsigned long long*)d_wbase.as<unsigned long long>(),
                            (const unsigned long long*)d_scratch.as<unsigned long long>(), (const uint64_t*)d_offsets,
-                           d_out);
+                           d_out, out_cap, d_cap_err);
         PS_HIP_CHECK(hipGetLastError());
     }
     return PS_OK;
@@ -1349,10 +1155,26 @@ int ps_fc_encode(const uint32_t* d_values, const uint64_t* d_starts, uint32_t nc
     if (!d_starts || !d_offsets || (nchunks && (!d_values || !d_out)))
         return fail(PS_ERR_INVALID_ARGUMENT, "NULL device pointer");
     hipStream_t s = stream_of(st);
-    std::vector<uint64_t> starts, off;
+    std::vector<uint64_t> starts;
     if (int r = read_starts(starts, d_starts, nchunks, s)) return r;
-    if (int r = fc_encode_chunks(d_values, starts, with_check, d_out, out_cap, off, d_offsets, s)) return r;
+    // the choice, sizes, offsets and writes on the device; a chunk that would
+    // end past out_cap is not written, and the call reports the capacity error
+    DevBuf d_err;
+    if (int r = d_err.alloc(8, s)) return r;
+    if (int r = launch_fill(d_err.p, 0, 8, s)) return r;
+    if (int r = fc_encode_chunks_dev(d_values, starts, with_check, d_out, out_cap, d_offsets, d_err.as<uint32_t>(), s))
+        return r;
+    uint64_t total = 0;
+    uint32_t err[2] = {0, 0};
+    PS_HIP_CHECK(hipMemcpyAsync(&total, d_offsets + nchunks, 8, hipMemcpyDeviceToHost, s));
+    PS_HIP_CHECK(hipMemcpyAsync(err, d_err.p, 8, hipMemcpyDeviceToHost, s));
     if (int r = stream_wait(s)) return r;
+    if (err[0])
+        return fail(PS_ERR_INVALID_ARGUMENT, "%u chunks would be bitmaps but are not sorted; use the checked encoder",
+                    err[0]);
+    if (err[1] || total > out_cap)
+        return fail(PS_ERR_CAPACITY, "encoded size %llu exceeds output capacity %llu", (unsigned long long)total,
+                    (unsigned long long)out_cap);
     return PS_OK;
     PS_API_END
 }

@@ -405,7 +405,8 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
         if (int r = w->x_off.ensure((nchunks_x + 1) * 8, s2)) return r;
         if (int r = w->h_x_off.ensure((nchunks_x + 1) * 8)) return r;
         if (int r = fc_encode_chunks_dev(w->vals.as<uint32_t>(), cstarts, 1, w->xs_bytes.as<uint8_t>(),
-                                         w->x_off.as<uint64_t>(), reinterpret_cast<uint32_t*>(w->counters + 8), s2))
+                                         w->xs_bytes.cap, w->x_off.as<uint64_t>(),
+                                         reinterpret_cast<uint32_t*>(w->counters + 8), s2))
             return r;
         PS_HIP_CHECK(hipMemcpyAsync(w->h_x_off.p, w->x_off.p, (nchunks_x + 1) * 8, hipMemcpyDeviceToHost, s2));
         for (int b = 0; b < nb; ++b) {

@@ -249,7 +249,7 @@ int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur,
     if (int rc = w->fr_bytes.ensure(std::max<uint64_t>(fr_cap, 16), s)) return rc;
     if (int rc = w->fr_off.ensure(nfo * 8, s)) return rc;
     if (int rc = w->h_fr_off.ensure(nfo * 8)) return rc;
-    if (int rc = fc_encode_chunks_dev(w->out_idx.as<uint32_t>(), starts, 0, w->fr_bytes.as<uint8_t>(),
+    if (int rc = fc_encode_chunks_dev(w->out_idx.as<uint32_t>(), starts, 0, w->fr_bytes.as<uint8_t>(), w->fr_bytes.cap,
                                       w->fr_off.as<uint64_t>(), reinterpret_cast<uint32_t*>(w->counters + 7), s))
         return rc;
     PS_HIP_CHECK(hipMemcpyAsync(w->h_fr_off.p, w->fr_off.p, nfo * 8, hipMemcpyDeviceToHost, s));
@@ -309,7 +309,8 @@ int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur,
         if (int rc = w->x_off.ensure((nchunks_x + 1) * 8, s2)) return rc;
         if (int rc = w->h_x_off.ensure((nchunks_x + 1) * 8)) return rc;
         if (int rc = fc_encode_chunks_dev(w->vals.as<uint32_t>(), cstarts, 1, w->xs_bytes.as<uint8_t>(),
-                                          w->x_off.as<uint64_t>(), reinterpret_cast<uint32_t*>(w->counters + 7), s2))
+                                          w->xs_bytes.cap, w->x_off.as<uint64_t>(),
+                                          reinterpret_cast<uint32_t*>(w->counters + 7), s2))
             return rc;
         PS_HIP_CHECK(hipMemcpyAsync(w->h_x_off.p, w->x_off.p, (nchunks_x + 1) * 8, hipMemcpyDeviceToHost, s2));
         uint64_t vpos = 0;

@@ -182,6 +182,27 @@ def test_svint_encode_capacity_is_a_bound(C, ps):
     assert np.all(out[cap:] == 0xA5)
 
 
+def test_fc_encode_capacity_is_a_bound(C, ps):
+    # the same bound for FrontierCompression, whose chunks here are a bitmap
+    # built in LDS, a StreamVInt chunk and a bitmap wider than LDS
+    rng = np.random.default_rng(23)
+    chunks = [sorted_unique(rng, 4096, 16384), sorted_unique(rng, 3000, 1 << 31), sorted_unique(rng, 1 << 17, 1 << 18)]
+    data, off = C.fc_encode(chunks)
+    for cut in (1, 2):  # chunks [0, cut) fit
+        cap = int(off[cut]) + 5
+        d_v = ps.DeviceBuffer.from_array(np.concatenate(chunks))
+        d_st = ps.DeviceBuffer.from_array(np.cumsum([0] + [len(c) for c in chunks]).astype(np.uint64))
+        canary = np.full(len(data) + 64, 0x5A, np.uint8)
+        d_out = ps.DeviceBuffer.from_array(canary)
+        d_off = ps.DeviceBuffer(8 * 4)
+        with pytest.raises(ps.PSError, match="capacity") as e:
+            ps.check(ps.lib().ps_fc_encode(d_v.ptr, d_st.ptr, 3, 1, d_out.ptr, cap, d_off.ptr, None))
+        assert e.value.code == ps.PS_ERR_CAPACITY
+        out = d_out.download(np.uint8, len(canary))
+        assert np.array_equal(out[:int(off[cut])], data[:int(off[cut])])
+        assert np.all(out[cap:] == 0x5A)
+
+
 def test_svint_decode_capacity_is_a_bound(C, ps):
     # values_cap below the decoded count is refused, no value at or past
     # values_cap is written, and the chunks that fit still decode
