@@ -239,6 +239,13 @@ __device__ __forceinline__ void block_add3(unsigned long long a, unsigned long l
 
 // The 64 mark bytes (0 or 1) of one bit-array word as its bits, clearing
 // them; bytes are read and written as four 16-byte vectors.
+// puzzles that can name the slots whose child may leave the segment
+// (PZ::cross_slots(index, seg_bits)); the others expand every slot
+template <class PZ, class = void>
+struct HasCrossSlots : std::false_type {};
+template <class PZ>
+struct HasCrossSlots<PZ, std::void_t<decltype(PZ::cross_slots(0ull, 0))>> : std::true_type {};
+
 __device__ __forceinline__ uint64_t fold_marks(uint8_t* p) {
     uint4* v = reinterpret_cast<uint4*>(p);
     uint64_t bits = 0;

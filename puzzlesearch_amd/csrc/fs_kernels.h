@@ -117,13 +117,6 @@ __device__ __forceinline__ uint32_t fs_live(uint32_t w, const unsigned long long
     return nz;
 }
 
-// puzzles that can name the slots whose child may leave the segment
-// (PZ::cross_slots(index, seg_bits)); the others expand every slot
-template <class PZ, class = void>
-struct HasCrossSlots : std::false_type {};
-template <class PZ>
-struct HasCrossSlots<PZ, std::void_t<decltype(PZ::cross_slots(0ull, 0))>> : std::true_type {};
-
 // The in-segment children of one parent, by slot.
 template <int SLOTS>
 struct BinSink {

@@ -110,7 +110,10 @@ __global__ __launch_bounds__(TPB) void k_s3_mark_in(S3MarkArgs a) {
             continue;
         }
         if (a.cur) atomicOr(&a.cur[id >> 6], 1ull << (id & 63));
-        PZ::expand(a.base | id, 0u, a.prm, sink);
+        if constexpr (HasCrossSlots<PZ>::value)  // skip the moves that certainly leave the segment
+            PZ::expand(a.base | id, 0u, a.prm, sink, PZ::in_slots(a.base | id, a.seg_bits));
+        else
+            PZ::expand(a.base | id, 0u, a.prm, sink);
         sink.flush();
         sink.pend_w = ~0ull;
     }
@@ -254,16 +257,24 @@ __global__ __launch_bounds__(TPB) void k_s3_write(S3ScanArgs a) {
         for (int w = 0; w < NW; ++w) before += (w < wave) ? tot[par][w] : 0u;
         uint64_t pos = a.out_off[r] + before + incl - c;
         const uint64_t j0 = w0 + (uint64_t)threadIdx.x * 4;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            uint64_t v = live[e];
-            while (v) {
-                const int b = __ffsll((unsigned long long)v) - 1;
-                v &= v - 1;
-                a.out_idx[pos++] = (uint32_t)((j0 + e) * 64 + (uint64_t)b);
+        // one entry per iteration: a word is fetched when the last one is used up
+        uint32_t nz = (live[0] ? 1u : 0u) | (live[1] ? 2u : 0u) | (live[2] ? 4u : 0u) | (live[3] ? 8u : 0u);
+        uint64_t v = 0, wid = 0;
+        for (;;) {
+            if (!v) {
+                if (!nz) break;
+                const int e = __ffs(nz) - 1;
+                nz &= nz - 1u;
+                v = (e == 0) ? live[0] : (e == 1) ? live[1] : (e == 2) ? live[2] : live[3];
+                wid = (j0 + (uint64_t)e) * 64;
             }
-            if (nx[e]) a.next[j0 + e] = 0ull;
+            const int b = __ffsll((unsigned long long)v) - 1;
+            v &= v - 1;
+            a.out_idx[pos++] = (uint32_t)(wid + (uint64_t)b);
         }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (nx[e]) a.next[j0 + e] = 0ull;
         if (threadIdx.x == 0) a.dirty[r] = 0;  // every thread read it before the barrier above
         par ^= 1u;
     }
