@@ -41,7 +41,7 @@ static_assert(FS_COL_WAVES * FS_COL_ROWS == (int)FS_MAX_BIN_BLOCKS, "column pass
 struct FsBinArgs {
     psd::PuzzleParams prm;
     const uint32_t* idx;  // frontier, in-segment ids
-    const uint8_t* ops;   // their op bits
+    const uint8_t* ops;   // their op bits (NULL: none)
     uint64_t n;
     uint64_t base;  // segment << seg_bits
     uint32_t segment;
@@ -58,7 +58,9 @@ struct FsBinArgs {
                                   // place pass, its first position relative to the region's start
     const unsigned long long* rstart;  // place pass: nregions + 1 region starts in the mark list
     uint32_t* marks;              // place pass: (state in region << 2) | move
-    unsigned long long* counters; // [0] in-segment children, [6] ids outside the segment (count pass)
+    unsigned long long* children_out;  // count pass: += in-segment children of the frontier
+    unsigned long long* bad_out;       // count pass: += frontier ids outside the segment
+    uint32_t label_mask;               // 3: the move labels of the 4-bit op flags; 0: marks without labels
 };
 
 struct FsRegionArgs {
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(TPB) void k_fs_bin(FsBinArgs a) {
     const uint32_t lmask = (1u << a.rb) - 1u;
     auto put = [&](uint32_t state, uint32_t label) {
         uint32_t* c = &cur[state >> a.rb];
-        if constexpr (PLACE) a.marks[atomicAdd(c, 1u)] = ((state & lmask) << 2) | label;
+        if constexpr (PLACE) a.marks[atomicAdd(c, 1u)] = ((state & lmask) << 2) | (label & a.label_mask);
         else atomicAdd(c, 1u);
     };
     unsigned long long marks = 0, bad = 0;
@@ -163,10 +165,11 @@ __global__ __launch_bounds__(TPB) void k_fs_bin(FsBinArgs a) {
             continue;
         }
         if (!PLACE && a.odd) atomicOr(&a.odd[id >> 6], 1ull << (id & 63));
+        const uint32_t opb = a.ops ? (uint32_t)a.ops[i] : 0u;
         if constexpr (HasCrossSlots<PZ>::value)  // skip the moves that certainly leave the segment
-            PZ::expand(a.base | id, (uint32_t)a.ops[i], a.prm, sink, PZ::in_slots(a.base | id, a.seg_bits));
+            PZ::expand(a.base | id, opb, a.prm, sink, PZ::in_slots(a.base | id, a.seg_bits));
         else
-            PZ::expand(a.base | id, (uint32_t)a.ops[i], a.prm, sink);
+            PZ::expand(a.base | id, opb, a.prm, sink);
         marks += (unsigned long long)__popc(sink.valid);
 #pragma unroll
         for (int k = 0; k < PZ::SLOTS; ++k)
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(TPB) void k_fs_bin(FsBinArgs a) {
     if (!PLACE) {
         __syncthreads();
         for (uint32_t r = threadIdx.x; r < a.nregions; r += TPB) row[r] = cur[r];
-        block_add3<TPB>(marks, bad, 0ull, &a.counters[0], &a.counters[6], nullptr);
+        block_add3<TPB>(marks, bad, 0ull, a.children_out, a.bad_out, nullptr);
     }
 }
 

@@ -58,6 +58,11 @@ PS_DECLARE_FS_FAMILY(hanoi)
         });                                                                                                 \
     }
 
+// the column pass over the (workgroup, region) table of the bin passes
+// (k_fs_bin_columns): rcount[r] = marks of region r, and each entry replaced
+// by the workgroup's first position in its region
+int fs_bin_columns(uint32_t* table, uint32_t rows, uint32_t nregions, uint32_t* rcount, hipStream_t s);
+
 // place = 0: count the marks per (workgroup, region); 1: write them to their regions' ranges
 inline int fs_bin(const ps_puzzle& p, const psk::FsBinArgs& a, int place, uint32_t grid, hipStream_t s) {
     switch (family_of(p)) {

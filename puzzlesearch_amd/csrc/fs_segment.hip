@@ -180,6 +180,19 @@ int read_counters(ps_fs_segment* w, hipStream_t s) {
 }
 
 // the region kernels need more than 64 KB of LDS for regions of 2^18 states
+}  // namespace
+
+int fs_bin_columns(uint32_t* table, uint32_t rows, uint32_t nregions, uint32_t* rcount, hipStream_t s) {
+    if (rows > psk::FS_MAX_BIN_BLOCKS) return fail(PS_ERR_INTERNAL, "%u bin workgroups exceed %u", rows,
+                                                   psk::FS_MAX_BIN_BLOCKS);
+    hipLaunchKernelGGL(psk::k_fs_bin_columns, dim3((nregions + 63) / 64), dim3(psk::FS_COL_WAVES * 64), 0, s, table,
+                       rows, nregions, rcount);
+    PS_HIP_CHECK(hipGetLastError());
+    return PS_OK;
+}
+
+namespace {
+
 int region_lds_setup(size_t bytes) {
     static size_t set = 0;
     if (bytes <= 65536 || bytes <= set) return PS_OK;
@@ -228,7 +241,9 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
         a.xstarts = w->xseg_starts.as<uint64_t>();
     }
     a.odd = w->odd ? w->odd_bits : nullptr;
-    a.counters = w->counters;
+    a.children_out = w->counters + 0;
+    a.bad_out = w->counters + 6;
+    a.label_mask = 3u;
     // workgroups of the bin passes: two per CU, each with at least one pass of its threads
     const uint32_t grid_bin = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>({blocks_for(std::max(n, nx), psk::FS_BIN_TPB), (uint64_t)w->cus * 2,
@@ -238,9 +253,7 @@ int run_step(ps_fs_segment* w, uint32_t segment, const uint32_t* d_idx, const ui
     if (int r = w->bin_table.ensure((size_t)grid_bin * w->nregions * 4, s)) return r;
     a.table = w->bin_table.as<uint32_t>();
     if (int r = fs_bin(w->puzzle, a, 0, grid_bin, s)) return r;
-    hipLaunchKernelGGL(psk::k_fs_bin_columns, dim3((uint32_t)((w->nregions + 63) / 64)),
-                       dim3(psk::FS_COL_WAVES * 64), 0, s, a.table, grid_bin, (uint32_t)w->nregions, w->rcount);
-    PS_HIP_CHECK(hipGetLastError());
+    if (int r = fs_bin_columns(a.table, grid_bin, (uint32_t)w->nregions, w->rcount, s)) return r;
     if (int r = scan_excl_u32(w->rcount, w->nregions, w->rstart, s)) return r;
     if (int r = w->marks.ensure(std::max<uint64_t>(max_marks, 1) * 4, s)) return r;
     if (n || nx) {

@@ -30,6 +30,7 @@
 #include "codec_internal.h"
 #include "devbuf.h"
 #include "fs_kernels.h"
+#include "fs_launch.h"
 #include "ps_internal.h"
 #include "seg3_launch.h"
 #include "seg_sort.h"
@@ -64,6 +65,17 @@ struct ps_seg3 {
     unsigned long long* h_counters = nullptr;  // pinned
     ps::GrowBuf out_idx, fr_bytes, xseg_copy, slot_buf, compact, vals, xs_bytes, block_cnt, block_off, fr_off, x_off;
     ps::GrowHost h_fr_off, h_x_off;  // chunk offsets of the frontier and cross-segment streams (pinned)
+    // binned marks (puzzles expanded in full with at most 4 moves, no mark
+    // post-process): per region of 2^brb states, no NextArray pass in HBM
+    bool binned = false;
+    int brb = 0;
+    uint32_t bnreg = 0;
+    uint32_t* brcount = nullptr;              // marks per region
+    unsigned long long* brstart = nullptr;    // bnreg + 1
+    unsigned long long* bostart = nullptr;    // bnreg + 1: ranges of the old frontier
+    unsigned long long* bcnt_out = nullptr;   // bnreg
+    unsigned long long* bout_off = nullptr;   // bnreg + 1
+    ps::GrowBuf bin_table, bmarks;
     uint32_t cus = 256;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // the cross-segment children are found, grouped and encoded on `side`
@@ -85,13 +97,14 @@ int destroy_s3(ps_seg3* w) {
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     if (w->side) (void)hipStreamSynchronize(w->side);
     for (GrowBuf* g : {&w->out_idx, &w->fr_bytes, &w->xseg_copy, &w->slot_buf, &w->compact, &w->vals, &w->xs_bytes,
-                       &w->block_cnt, &w->block_off, &w->fr_off, &w->x_off})
+                       &w->block_cnt, &w->block_off, &w->fr_off, &w->x_off, &w->bin_table, &w->bmarks})
         g->release(w->stream);
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     w->h_fr_off.release();
     w->h_x_off.release();
     for (void* p : {(void*)w->next, (void*)w->cur, (void*)w->dirty, (void*)w->cnt_out, (void*)w->out_off,
-                    (void*)w->counters})
+                    (void*)w->counters, (void*)w->brcount, (void*)w->brstart, (void*)w->bostart, (void*)w->bcnt_out,
+                    (void*)w->bout_off})
         if (p) (void)hipFree(p);
     if (w->h_counters) (void)hipHostFree(w->h_counters);
     if (w->ev0) (void)hipEventDestroy(w->ev0);
@@ -112,6 +125,96 @@ int read_counters(ps_seg3* w, hipStream_t s) {
 
 uint32_t grid_for(const ps_seg3* w, uint64_t n, uint32_t tpb) {
     return (uint32_t)std::min<uint64_t>(blocks_for(n, tpb), (uint64_t)w->cus * 8);
+}
+
+// Steps 1-4 with binned marks: the frontier's in-segment children and the
+// incoming marks binned per region of 2^brb states (the frontier-search
+// step's bin passes, labels dropped), then per region the NextArray bits in
+// LDS, the old frontier cleared, the current frontier excluded (odd cycles;
+// the count pass sets CurArray) and the new layer counted and written in
+// order.  Returns with the counters read.
+int binned_scan(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur, const uint32_t* d_old,
+                uint64_t nold, const uint32_t* d_xseg, const uint64_t* xstarts, uint64_t nx, hipStream_t s) {
+    const uint64_t in_mask = (1ull << w->seg_bits) - 1ull;
+    const uint64_t max_marks = ncur * (uint64_t)w->slots_x + nx;  // slots per parent in full expansion
+    if (max_marks > 0xFFFFFFFFull)
+        return fail(PS_ERR_CAPACITY, "%llu marks in one step exceed 2^32", (unsigned long long)max_marks);
+    if (nx) {  // bad ids among the marks (the bin passes mask them)
+        hipLaunchKernelGGL(psk::k_s3_check_list<256>, dim3(grid_for(w, nx, 256)), dim3(256), 0, s, d_xseg, nx, in_mask,
+                           w->counters + 1, (unsigned long long*)nullptr);
+        PS_HIP_CHECK(hipGetLastError());
+    }
+    if (nold) {  // the old frontier's ranges per region need it ascending
+        hipLaunchKernelGGL(psk::k_s3_check_list<256>, dim3(grid_for(w, nold, 256)), dim3(256), 0, s, d_old, nold, in_mask,
+                           (unsigned long long*)nullptr, w->counters + 8);
+        PS_HIP_CHECK(hipGetLastError());
+    }
+    psk::FsBinArgs a{};
+    a.prm = params_of(w->puzzle);
+    a.idx = d_cur;
+    a.ops = nullptr;
+    a.n = ncur;
+    a.base = (uint64_t)segment << w->seg_bits;
+    a.segment = segment;
+    a.seg_bits = w->seg_bits;
+    a.rb = w->brb;
+    a.nregions = w->bnreg;
+    a.nx = nx;
+    a.nops = w->ops;
+    if (nx) {
+        if (int r = w->xseg_copy.ensure((size_t)(w->ops + 1) * 8, s)) return r;
+        if (int r = copy_to_device(w->xseg_copy.p, xstarts, (size_t)(w->ops + 1) * 8, s)) return r;
+        a.xidx = d_xseg;
+        a.xstarts = w->xseg_copy.as<uint64_t>();
+    }
+    a.odd = w->odd ? w->cur : nullptr;
+    a.children_out = w->counters + 0;
+    a.bad_out = w->counters + 1;
+    a.label_mask = 0u;
+    const uint32_t grid_bin = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>({blocks_for(std::max(ncur, nx), psk::FS_BIN_TPB), (uint64_t)w->cus * 2,
+                               (uint64_t)psk::FS_MAX_BIN_BLOCKS}));
+    a.per = (ncur + grid_bin - 1) / grid_bin;
+    a.perx = (nx + grid_bin - 1) / grid_bin;
+    if (int r = w->bin_table.ensure((size_t)grid_bin * w->bnreg * 4, s)) return r;
+    a.table = w->bin_table.as<uint32_t>();
+    if (int r = fs_bin(w->puzzle, a, 0, grid_bin, s)) return r;
+    if (int r = fs_bin_columns(a.table, grid_bin, w->bnreg, w->brcount, s)) return r;
+    if (int r = scan_excl_u32(w->brcount, w->bnreg, w->brstart, s)) return r;
+    if (int r = w->bmarks.ensure(std::max<uint64_t>(max_marks, 1) * 4, s)) return r;
+    a.odd = nullptr;
+    a.rstart = w->brstart;
+    a.marks = w->bmarks.as<uint32_t>();
+    if (int r = fs_bin(w->puzzle, a, 1, grid_bin, s)) return r;
+    // the old frontier's range in every region
+    hipLaunchKernelGGL(psk::k_s3_range_starts<256>, dim3((w->bnreg + 1 + 255) / 256), dim3(256), 0, s, d_old, nold, w->brb,
+                       w->bnreg, w->bostart);
+    PS_HIP_CHECK(hipGetLastError());
+    psk::S3RegionArgs ra{};
+    ra.nregions = w->bnreg;
+    ra.rb = w->brb;
+    ra.marks = w->bmarks.as<uint32_t>();
+    ra.rstart = w->brstart;
+    ra.old = d_old;
+    ra.ostart = w->bostart;
+    ra.cur = w->odd ? w->cur : nullptr;
+    ra.cnt_out = w->bcnt_out;
+    ra.counters = w->counters;
+    ra.out_off = w->bout_off;
+    const uint32_t grid_region = (uint32_t)std::min<uint64_t>(w->bnreg, (uint64_t)w->cus * 2);
+    hipLaunchKernelGGL((psk::k_s3_region<psk::S3_BIN_TPB, false>), dim3(grid_region), dim3(psk::S3_BIN_TPB), 0, s, ra);
+    PS_HIP_CHECK(hipGetLastError());
+    if (int r = scan_excl_u64(w->bcnt_out, w->bnreg, w->bout_off, s)) return r;
+    if (int r = read_counters(w, s)) return r;
+    if (w->h_counters[8])
+        return fail(PS_ERR_INVALID_ARGUMENT, "the old frontier is not ascending (%llu descents)",
+                    (unsigned long long)w->h_counters[8]);
+    const uint64_t count = w->h_counters[2];
+    if (int r = w->out_idx.ensure(std::max<uint64_t>(count, 1) * 4, s)) return r;
+    ra.out_idx = w->out_idx.as<uint32_t>();
+    hipLaunchKernelGGL((psk::k_s3_region<psk::S3_BIN_TPB, true>), dim3(grid_region), dim3(psk::S3_BIN_TPB), 0, s, ra);
+    PS_HIP_CHECK(hipGetLastError());
+    return PS_OK;
 }
 
 int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur, const uint32_t* d_old, uint64_t nold,
@@ -138,64 +241,68 @@ int run_step(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t ncur,
     const uint64_t base = (uint64_t)segment << w->seg_bits;
     const uint64_t in_mask = (1ull << w->seg_bits) - 1ull;
 
-    // 1 incoming cross-segment marks
-    if (nx) {
-        const uint32_t* marks = d_xseg;
-        if (w->post) {  // PancakeOptimizedGPU::CrossSegmentPostProcess on a copy
-            if (int rc = w->xseg_copy.ensure(nx * 4, s)) return rc;
-            PS_HIP_CHECK(hipMemcpyAsync(w->xseg_copy.p, d_xseg, nx * 4, hipMemcpyDeviceToDevice, s));
-            if (int rc = ps_pancake_xseg_postprocess(w->puzzle.size, (int32_t)segment, w->xseg_copy.as<uint32_t>(), nx,
-                                                     s))
-                return rc;
-            marks = w->xseg_copy.as<uint32_t>();
+    if (w->binned) {
+        if (int rc = binned_scan(w, segment, d_cur, ncur, d_old, nold, d_xseg, xstarts, nx, s)) return rc;
+    } else {
+        // 1 incoming cross-segment marks
+        if (nx) {
+            const uint32_t* marks = d_xseg;
+            if (w->post) {  // PancakeOptimizedGPU::CrossSegmentPostProcess on a copy
+                if (int rc = w->xseg_copy.ensure(nx * 4, s)) return rc;
+                PS_HIP_CHECK(hipMemcpyAsync(w->xseg_copy.p, d_xseg, nx * 4, hipMemcpyDeviceToDevice, s));
+                if (int rc = ps_pancake_xseg_postprocess(w->puzzle.size, (int32_t)segment, w->xseg_copy.as<uint32_t>(), nx,
+                                                         s))
+                    return rc;
+                marks = w->xseg_copy.as<uint32_t>();
+            }
+            hipLaunchKernelGGL(psk::k_s3_mark_list<256>, dim3(grid_for(w, nx, 256)), dim3(256), 0, s, marks, nx, in_mask,
+                               (uint32_t*)w->next, w->dirty, w->counters + 1);
+            PS_HIP_CHECK(hipGetLastError());
         }
-        hipLaunchKernelGGL(psk::k_s3_mark_list<256>, dim3(grid_for(w, nx, 256)), dim3(256), 0, s, marks, nx, in_mask,
-                           (uint32_t*)w->next, w->dirty, w->counters + 1);
+        // 2 in-segment children of the current frontier
+        if (ncur) {
+            psk::S3MarkArgs a{};
+            a.prm = params_of(w->puzzle);
+            a.idx = d_cur;
+            a.n = ncur;
+            a.base = base;
+            a.segment = segment;
+            a.seg_bits = w->seg_bits;
+            a.next = (uint32_t*)w->next;
+            a.dirty = w->dirty;
+            a.cur = w->odd ? w->cur : nullptr;
+            a.counters = w->counters;
+            if (int rc = s3_mark_in(w->puzzle, w->mode_in, a, grid_for(w, ncur, S3_TPB), s)) return rc;
+        }
+        // 3 the old frontier clears
+        if (nold) {
+            hipLaunchKernelGGL(psk::k_s3_clear_list<256>, dim3(grid_for(w, nold, 256)), dim3(256), 0, s, d_old, nold, in_mask,
+                               (uint32_t*)w->next);
+            PS_HIP_CHECK(hipGetLastError());
+        }
+        // 4 scan: count per region, prefix sum, ordered write + clear
+        psk::S3ScanArgs sa{};
+        sa.next = w->next;
+        sa.cur = w->odd ? w->cur : nullptr;
+        sa.dirty = w->dirty;
+        sa.nwords = w->nwords;
+        sa.cnt_out = w->cnt_out;
+        sa.counters = w->counters;
+        sa.out_off = w->out_off;
+        const uint32_t grid_scan = (uint32_t)std::min<uint64_t>(w->nregions, (uint64_t)w->cus * 8);
+        hipLaunchKernelGGL(psk::k_s3_count<S3_TPB>, dim3(grid_scan), dim3(S3_TPB), 0, s, sa);
+        PS_HIP_CHECK(hipGetLastError());
+        if (int rc = scan_excl_u64(w->cnt_out, w->nregions, w->out_off, s)) return rc;
+        if (int rc = read_counters(w, s)) return rc;
+        if (int rc = w->out_idx.ensure(std::max<uint64_t>(w->h_counters[2], 1) * 4, s)) return rc;
+        sa.out_idx = w->out_idx.as<uint32_t>();
+        hipLaunchKernelGGL(psk::k_s3_write<S3_TPB>, dim3(grid_scan), dim3(S3_TPB), 0, s, sa);
         PS_HIP_CHECK(hipGetLastError());
     }
-    // 2 in-segment children of the current frontier
-    if (ncur) {
-        psk::S3MarkArgs a{};
-        a.prm = params_of(w->puzzle);
-        a.idx = d_cur;
-        a.n = ncur;
-        a.base = base;
-        a.segment = segment;
-        a.seg_bits = w->seg_bits;
-        a.next = (uint32_t*)w->next;
-        a.dirty = w->dirty;
-        a.cur = w->odd ? w->cur : nullptr;
-        a.counters = w->counters;
-        if (int rc = s3_mark_in(w->puzzle, w->mode_in, a, grid_for(w, ncur, S3_TPB), s)) return rc;
-    }
-    // 3 the old frontier clears
-    if (nold) {
-        hipLaunchKernelGGL(psk::k_s3_clear_list<256>, dim3(grid_for(w, nold, 256)), dim3(256), 0, s, d_old, nold, in_mask,
-                           (uint32_t*)w->next);
-        PS_HIP_CHECK(hipGetLastError());
-    }
-    // 4 scan: count per region, prefix sum, ordered write + clear
-    psk::S3ScanArgs sa{};
-    sa.next = w->next;
-    sa.cur = w->odd ? w->cur : nullptr;
-    sa.dirty = w->dirty;
-    sa.nwords = w->nwords;
-    sa.cnt_out = w->cnt_out;
-    sa.counters = w->counters;
-    sa.out_off = w->out_off;
-    const uint32_t grid_scan = (uint32_t)std::min<uint64_t>(w->nregions, (uint64_t)w->cus * 8);
-    hipLaunchKernelGGL(psk::k_s3_count<S3_TPB>, dim3(grid_scan), dim3(S3_TPB), 0, s, sa);
-    PS_HIP_CHECK(hipGetLastError());
-    if (int rc = scan_excl_u64(w->cnt_out, w->nregions, w->out_off, s)) return rc;
-    if (int rc = read_counters(w, s)) return rc;
     const uint64_t children_in = w->h_counters[0];
     const uint64_t bad_ids = w->h_counters[1];
     const uint64_t count = w->h_counters[2];
     const uint64_t scanned = w->h_counters[3];
-    if (int rc = w->out_idx.ensure(std::max<uint64_t>(count, 1) * 4, s)) return rc;
-    sa.out_idx = w->out_idx.as<uint32_t>();
-    hipLaunchKernelGGL(psk::k_s3_write<S3_TPB>, dim3(grid_scan), dim3(S3_TPB), 0, s, sa);
-    PS_HIP_CHECK(hipGetLastError());
     if (w->odd && ncur) {
         hipLaunchKernelGGL(psk::k_s3_clear_cur<256>, dim3(grid_for(w, ncur, 256)), dim3(256), 0, s, d_cur, ncur, in_mask,
                            w->cur);
@@ -434,12 +541,25 @@ int ps_seg3_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st, ps_se
         }
         return PS_OK;
     };
-    if (int r = dalloc((void**)&w->next, w->nwords * 8)) return cleanup(r);
+    // binned marks for puzzles expanded in full with at most 4 moves (the bin
+    // passes of the frontier-search step) and no mark post-process
+    w->binned = w->mode_in == PS_EXPAND_FULL && !w->post && w->ops <= 4 && segment_bits >= 6;
+    if (w->binned) {
+        w->brb = std::min(psk::FS_BIN_BITS, segment_bits);
+        w->bnreg = (uint32_t)(seg_size >> w->brb);
+        if (int r = dalloc((void**)&w->brcount, (uint64_t)w->bnreg * 4)) return cleanup(r);
+        if (int r = dalloc((void**)&w->brstart, (uint64_t)w->bnreg * 8 + 8)) return cleanup(r);
+        if (int r = dalloc((void**)&w->bostart, (uint64_t)w->bnreg * 8 + 8)) return cleanup(r);
+        if (int r = dalloc((void**)&w->bcnt_out, (uint64_t)w->bnreg * 8)) return cleanup(r);
+        if (int r = dalloc((void**)&w->bout_off, (uint64_t)w->bnreg * 8 + 8)) return cleanup(r);
+    } else {
+        if (int r = dalloc((void**)&w->next, w->nwords * 8)) return cleanup(r);
+        if (int r = dalloc((void**)&w->dirty, w->nregions)) return cleanup(r);
+        if (int r = dalloc((void**)&w->cnt_out, w->nregions * 8)) return cleanup(r);
+        if (int r = dalloc((void**)&w->out_off, w->nregions * 8 + 8)) return cleanup(r);
+    }
     if (w->odd)
         if (int r = dalloc((void**)&w->cur, w->nwords * 8)) return cleanup(r);
-    if (int r = dalloc((void**)&w->dirty, w->nregions)) return cleanup(r);
-    if (int r = dalloc((void**)&w->cnt_out, w->nregions * 8)) return cleanup(r);
-    if (int r = dalloc((void**)&w->out_off, w->nregions * 8 + 8)) return cleanup(r);
     if (int r = dalloc((void**)&w->counters, psk::S3_NCOUNTERS * 8)) return cleanup(r);
     if (hipHostMalloc((void**)&w->h_counters, psk::S3_NCOUNTERS * 8, hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(PS_ERR_OUT_OF_MEMORY, "pinned host allocation failed"));
@@ -449,10 +569,12 @@ int ps_seg3_create(const ps_puzzle* p, int32_t segment_bits, ps_stream st, ps_se
         return cleanup(fail(PS_ERR_HIP, "event creation failed"));
     if (hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(PS_ERR_HIP, "stream creation failed"));
-    if (int r = launch_fill(w->next, 0, w->nwords * 8, w->stream)) return cleanup(r);
+    if (!w->binned) {
+        if (int r = launch_fill(w->next, 0, w->nwords * 8, w->stream)) return cleanup(r);
+        if (int r = launch_fill(w->dirty, 0, w->nregions, w->stream)) return cleanup(r);
+    }
     if (w->odd)
         if (int r = launch_fill(w->cur, 0, w->nwords * 8, w->stream)) return cleanup(r);
-    if (int r = launch_fill(w->dirty, 0, w->nregions, w->stream)) return cleanup(r);
     if (hipStreamSynchronize(w->stream) != hipSuccess) return cleanup(fail(PS_ERR_HIP, "stream synchronize failed"));
     *out = w;
     return PS_OK;
@@ -471,7 +593,9 @@ int ps_seg3_info(const ps_seg3* w, uint32_t* segments, int32_t* operators, uint6
     if (!w) return fail(PS_ERR_INVALID_ARGUMENT, "NULL segment workspace");
     if (segments) *segments = w->nseg;
     if (operators) *operators = w->ops;
-    if (footprint_bytes) *footprint_bytes = w->nwords * 8 * (w->odd ? 2 : 1) + w->nregions * 17;
+    if (footprint_bytes)
+        *footprint_bytes = w->binned ? (uint64_t)w->bnreg * 36 + (w->odd ? w->nwords * 8 : 0)
+                                     : w->nwords * 8 * (w->odd ? 2 : 1) + w->nregions * 17;
     return PS_OK;
     PS_API_END
 }

@@ -24,8 +24,9 @@ constexpr uint32_t S3_CROSS_PER_BLOCK = 4096;                    // parents per 
 // counters of a step: [0] in-segment children, [1] ids outside the segment,
 // [2] the layer count, [3] words scanned, [4] cross-segment children, [5]
 // children outside [0, nseg), [6] cross children written, [7] frontier chunks
-// that FrontierCompression::Encode would take as a bitmap but are not sorted
-constexpr int S3_NCOUNTERS = 8;
+// that FrontierCompression::Encode would take as a bitmap but are not sorted,
+// [8] descents in the old frontier (binned marks need it ascending)
+constexpr int S3_NCOUNTERS = 9;
 
 struct S3MarkArgs {
     psd::PuzzleParams prm;
@@ -340,6 +341,154 @@ __global__ __launch_bounds__(TPB) void k_s3_cross(S3CrossArgs a) {
     }
     block_add3<TPB>(nx, sink.bad, nx, &a.counters[4], &a.counters[5], &a.counters[6]);
     if (threadIdx.x == 0) a.block_count[blockIdx.x] = running;
+}
+
+// ---------------------------------------------------------------- binned marks
+
+// Without the NextArray in HBM (puzzles expanded in full, at most 4 moves):
+// the marks are binned per region of 2^rb states by the bin passes of the
+// frontier-search step (k_fs_bin, labels dropped), and one persistent
+// workgroup per CU builds each region's NextArray bits in LDS from them,
+// clears the old frontier's bits (its ids are sorted, so each region's are a
+// range found by binary search), excludes the current frontier's bits (odd
+// cycles, read from CurArray) and counts (WRITE = false) or writes the new
+// layer's ids in order (WRITE = true).  Thread t owns bit words [8t, 8t+8).
+struct S3RegionArgs {
+    uint32_t nregions;
+    int rb;
+    const uint32_t* marks;             // (state in region << 2), region by region
+    const unsigned long long* rstart;  // nregions + 1
+    const uint32_t* old;               // the old frontier, ascending in-segment ids
+    const unsigned long long* ostart;  // nregions + 1: each region's range of old
+    const unsigned long long* cur;     // CurArray (odd cycles), else NULL
+    unsigned long long* cnt_out;       // count pass: per region
+    unsigned long long* counters;      // count pass: [2] += count, [3] += words scanned
+    const unsigned long long* out_off; // write pass
+    uint32_t* out_idx;
+};
+
+constexpr int S3_BIN_TPB = 1024;
+constexpr int S3_BIN_WPT = 8;  // LDS bit words per thread: 2^18 states per region
+
+// ostart[r] = first index of `ids` (ascending) at or above r << rb, r = 0..nregions
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_s3_range_starts(const uint32_t* ids, uint64_t n, int rb, uint32_t nregions,
+                                                         unsigned long long* ostart) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nregions) return;
+    const uint64_t key = (uint64_t)r << rb;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if ((uint64_t)ids[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    ostart[r] = lo;
+}
+
+// ids above in_mask (counted in *bad) and, with `desc`, places where an id
+// is below its predecessor (counted in *desc)
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_s3_check_list(const uint32_t* ids, uint64_t n, uint64_t in_mask,
+                                                       unsigned long long* bad, unsigned long long* desc) {
+    unsigned long long nb = 0, nd = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t id = ids[i];
+        if ((uint64_t)id > in_mask) ++nb;
+        if (desc && i > 0 && id < ids[i - 1]) ++nd;
+    }
+    nb = wave_sum(nb);
+    nd = wave_sum(nd);
+    if (threadIdx.x % 64 == 0) {
+        if (nb && bad) atomicAdd(bad, nb);
+        if (nd) atomicAdd(desc, nd);
+    }
+}
+
+template <int TPB, bool WRITE>
+__global__ __launch_bounds__(TPB) void k_s3_region(S3RegionArgs a) {
+    constexpr int NW = TPB / 64;
+    constexpr int WPT = S3_BIN_WPT;
+    static_assert(TPB * WPT * 32 == (1 << FS_BIN_BITS), "a region's bit words fill the workgroup");
+    __shared__ uint32_t bits[TPB * WPT];
+    __shared__ uint32_t s_w[NW];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint32_t rwords = (1u << a.rb) / 32u;  // bit words of the region (<= TPB * WPT)
+    const uint32_t t8 = threadIdx.x * WPT;
+    for (uint32_t x = threadIdx.x; x < TPB * WPT; x += TPB) bits[x] = 0u;
+    unsigned long long total = 0, scanned = 0;  // thread 0
+    for (uint32_t r = blockIdx.x; r < a.nregions; r += gridDim.x) {
+        const uint64_t m0 = a.rstart[r], m1 = a.rstart[r + 1];
+        if (m0 == m1) {  // no mark: nothing in the new layer here
+            if (!WRITE && threadIdx.x == 0) a.cnt_out[r] = 0;
+            continue;
+        }
+        __syncthreads();  // the bits are clear (start, or cleared by their owners)
+        for (uint64_t j = m0 + threadIdx.x; j < m1; j += TPB) {
+            const uint32_t st = a.marks[j] >> 2;
+            lor(&bits[st >> 5], 1u << (st & 31u));
+        }
+        __syncthreads();
+        const uint32_t rmask = (1u << a.rb) - 1u;
+        for (uint64_t j = a.ostart[r] + threadIdx.x; j < a.ostart[r + 1]; j += TPB) {
+            const uint32_t st = a.old[j] & rmask;
+            atomicAnd(&bits[st >> 5], ~(1u << (st & 31u)));
+        }
+        __syncthreads();
+        uint32_t live[WPT];
+        uint32_t c = 0;
+        const uint64_t g0 = (uint64_t)r * rwords + t8;  // this thread's first bit word in the segment (32-bit words)
+#pragma unroll
+        for (int k = 0; k < WPT; ++k) {
+            live[k] = (t8 + (uint32_t)k < rwords) ? bits[t8 + k] : 0u;
+            if (a.cur && live[k]) live[k] &= ~(uint32_t)(a.cur[(g0 + (uint64_t)k) >> 1] >> (32 * ((g0 + (uint64_t)k) & 1)));
+            c += (uint32_t)__popc(live[k]);
+        }
+        if (!WRITE) {
+            const uint32_t wc = wave_sum(c);
+            if (lane == 0) s_w[wave] = wc;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t sum = 0;
+                for (int x = 0; x < NW; ++x) sum += s_w[x];
+                a.cnt_out[r] = sum;
+                total += sum;
+                scanned += rwords / 2;  // in 64-bit words
+            }
+        } else {
+            const uint32_t incl = wave_incl_scan(c, lane);
+            if (lane == 63) s_w[wave] = incl;
+            __syncthreads();
+            const uint32_t wt = lane < NW ? s_w[lane] : 0u;
+            const uint32_t wi = wave_incl_scan(wt, lane);
+            uint64_t pos = a.out_off[r] + __shfl(wi - wt, wave, 64) + incl - c;
+            // one entry per iteration: a word is fetched when the last one is used up
+            uint32_t nz = 0;
+#pragma unroll
+            for (int k = 0; k < WPT; ++k) nz |= live[k] ? (1u << k) : 0u;
+            uint32_t v = 0, base = 0;
+            for (;;) {
+                if (!v) {
+                    if (!nz) break;
+                    const int k = __ffs(nz) - 1;
+                    nz &= nz - 1u;
+                    uint32_t x = 0;
+#pragma unroll
+                    for (int q = 0; q < WPT; ++q) x = (q == k) ? live[q] : x;
+                    v = x;
+                    base = (uint32_t)((g0 + (uint64_t)k) * 32u);
+                }
+                const int b = __ffs(v) - 1;
+                v &= v - 1u;
+                a.out_idx[pos++] = base + (uint32_t)b;
+            }
+        }
+        // this thread's words are read only by this thread: clear them for the next region
+#pragma unroll
+        for (int k = 0; k < WPT; ++k)
+            if (t8 + (uint32_t)k < rwords) bits[t8 + k] = 0u;
+    }
+    if (!WRITE) block_add3<TPB>(total, scanned, 0ull, &a.counters[2], &a.counters[3], nullptr);
 }
 
 }  // namespace psk
