@@ -212,7 +212,8 @@ int binned_scan(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t nc
     const uint64_t count = w->h_counters[2];
     if (int r = w->out_idx.ensure(std::max<uint64_t>(count, 1) * 4, s)) return r;
     ra.out_idx = w->out_idx.as<uint32_t>();
-    hipLaunchKernelGGL((psk::k_s3_region<psk::S3_BIN_TPB, true>), dim3(grid_region), dim3(psk::S3_BIN_TPB), 0, s, ra);
+    hipLaunchKernelGGL((psk::k_s3_region<psk::S3_BIN_TPB, true>), dim3(grid_region), dim3(psk::S3_BIN_TPB),
+                       (size_t)psk::S3_STAGE_IDS * 4, s, ra);
     PS_HIP_CHECK(hipGetLastError());
     return PS_OK;
 }

@@ -353,6 +353,7 @@ __global__ __launch_bounds__(TPB) void k_s3_cross(S3CrossArgs a) {
 // range found by binary search), excludes the current frontier's bits (odd
 // cycles, read from CurArray) and counts (WRITE = false) or writes the new
 // layer's ids in order (WRITE = true).  Thread t owns bit words [8t, 8t+8).
+// The write pass needs S3_STAGE_IDS * 4 bytes of dynamic LDS.
 struct S3RegionArgs {
     uint32_t nregions;
     int rb;
@@ -369,6 +370,10 @@ struct S3RegionArgs {
 
 constexpr int S3_BIN_TPB = 1024;
 constexpr int S3_BIN_WPT = 8;  // LDS bit words per thread: 2^18 states per region
+// the write pass stages up to this many ids of a region in (dynamic) LDS and
+// stores them with consecutive lanes on consecutive ids; a region with more
+// ids is written straight from the owner threads
+constexpr uint32_t S3_STAGE_IDS = 11264;  // 44 KB: with the 32 KB of bits two workgroups fit a CU
 
 // ostart[r] = first index of `ids` (ascending) at or above r << rb, r = 0..nregions
 template <int TPB>
@@ -412,6 +417,7 @@ __global__ __launch_bounds__(TPB) void k_s3_region(S3RegionArgs a) {
     static_assert(TPB * WPT * 32 == (1 << FS_BIN_BITS), "a region's bit words fill the workgroup");
     __shared__ uint32_t bits[TPB * WPT];
     __shared__ uint32_t s_w[NW];
+    extern __shared__ uint32_t stage[];  // write pass: S3_STAGE_IDS ids
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const uint32_t rwords = (1u << a.rb) / 32u;  // bit words of the region (<= TPB * WPT)
     const uint32_t t8 = threadIdx.x * WPT;
@@ -461,7 +467,10 @@ __global__ __launch_bounds__(TPB) void k_s3_region(S3RegionArgs a) {
             __syncthreads();
             const uint32_t wt = lane < NW ? s_w[lane] : 0u;
             const uint32_t wi = wave_incl_scan(wt, lane);
-            uint64_t pos = a.out_off[r] + __shfl(wi - wt, wave, 64) + incl - c;
+            const uint64_t r0 = a.out_off[r], nr = a.out_off[r + 1] - r0;
+            const bool staged = nr <= S3_STAGE_IDS;  // the same for the whole workgroup
+            uint32_t* dst = staged ? stage : a.out_idx + r0;
+            uint64_t pos = __shfl(wi - wt, wave, 64) + incl - c;
             // one entry per iteration: a word is fetched when the last one is used up
             uint32_t nz = 0;
 #pragma unroll
@@ -480,7 +489,12 @@ __global__ __launch_bounds__(TPB) void k_s3_region(S3RegionArgs a) {
                 }
                 const int b = __ffs(v) - 1;
                 v &= v - 1u;
-                a.out_idx[pos++] = base + (uint32_t)b;
+                dst[pos++] = base + (uint32_t)b;
+            }
+            if (staged) {
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < (uint32_t)nr; i += TPB) a.out_idx[r0 + i] = stage[i];
+                __syncthreads();  // the stage is free for the next region
             }
         }
         // this thread's words are read only by this thread: clear them for the next region
