@@ -206,9 +206,16 @@ int binned_scan(ps_seg3* w, uint32_t segment, const uint32_t* d_cur, uint64_t nc
     PS_HIP_CHECK(hipGetLastError());
     if (int r = scan_excl_u64(w->bcnt_out, w->bnreg, w->bout_off, s)) return r;
     if (int r = read_counters(w, s)) return r;
-    if (w->h_counters[8])
+    if (w->h_counters[8]) {
+        if (w->odd && ncur) {  // the count pass set CurArray: leave it clear
+            hipLaunchKernelGGL(psk::k_s3_clear_cur<256>, dim3(grid_for(w, ncur, 256)), dim3(256), 0, s, d_cur, ncur,
+                               in_mask, w->cur);
+            PS_HIP_CHECK(hipGetLastError());
+            PS_HIP_CHECK(hipStreamSynchronize(s));
+        }
         return fail(PS_ERR_INVALID_ARGUMENT, "the old frontier is not ascending (%llu descents)",
                     (unsigned long long)w->h_counters[8]);
+    }
     const uint64_t count = w->h_counters[2];
     if (int r = w->out_idx.ensure(std::max<uint64_t>(count, 1) * 4, s)) return r;
     ra.out_idx = w->out_idx.as<uint32_t>();

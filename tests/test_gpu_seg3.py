@@ -163,3 +163,31 @@ def test_no_data_out_of_range_and_reuse(S, ps):
         assert [b[:3] for b in got["buckets"]] == [b[:3] for b in want["buckets"]]
         with pytest.raises(psa.PSError):
             w.step(_nseg(p, 12), cur, np.zeros(0, np.uint32))
+
+
+def test_binned_marks_edge_cases(S, ps):
+    # the binned path (the 3x3 sliding tile in 2^12-state segments): ids
+    # outside the segment among the marks are refused, an old frontier that
+    # is not ascending is refused, and after both the workspace still equals
+    # the oracle step, incoming marks and old frontier included
+    seg3, _ = S
+    p = O.sliding(3, 3)
+    nseg = _nseg(p, 12)
+    with seg3.Seg3(psa.sliding(3, 3), 12) as w:
+        xs = [np.zeros(0, np.uint32)] * 3 + [np.array([9, 1 << 12], np.uint32)]
+        with pytest.raises(psa.PSError, match="not below"):
+            w.step(2, np.array([5], np.uint32), np.zeros(0, np.uint32), xseg=xs)
+        with pytest.raises(psa.PSError, match="ascending"):
+            w.step(2, np.array([5, 700], np.uint32), np.array([30, 20], np.uint32))
+        rng = np.random.default_rng(5)
+        cur = np.sort(rng.choice(1 << 12, 300, replace=False)).astype(np.uint32)
+        old = np.setdiff1d(np.sort(rng.choice(1 << 12, 200, replace=False)), cur).astype(np.uint32)
+        xs = [np.sort(rng.choice(1 << 12, 40, replace=False)).astype(np.uint32) for _ in range(4)]
+        got = w.step(2, cur, old, xseg=xs)
+        want = O.bfs3_segment_step(p, 12, 2, nseg, cur, old, xs)
+        assert got["count"] == want["count"]
+        assert np.array_equal(got["fr_idx"], want["fr_idx"])
+        assert np.array_equal(got["fr_bytes"], want["fr_bytes"])
+        assert [(o, s, n) for o, s, n, _ in got["buckets"]] == [(o, s, n) for o, s, n, _ in want["buckets"]]
+        for (_, _, _, a), (_, _, _, b) in zip(got["buckets"], want["buckets"]):
+            assert np.array_equal(a, b)
