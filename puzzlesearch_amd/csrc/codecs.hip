@@ -88,10 +88,13 @@ __device__ __forceinline__ uint32_t delta_at(const uint32_t* v, uint64_t start, 
     return j ? x - v[start + j - 1] : x;
 }
 
-__global__ void k_svi_tuple_sizes(const uint32_t* v, SviPlan pl, uint32_t* sizes) {
+// skip (optional): chunks another encoder writes (bitmap chunks); their sizes,
+// and so their payload sums, are left undefined and unused
+__global__ void k_svi_tuple_sizes(const uint32_t* v, SviPlan pl, uint32_t* sizes, const uint32_t* skip) {
     uint64_t t;
     uint32_t c;
     if (!svi_tuple(pl, t, c)) return;
+    if (skip && skip[c]) return;
     const uint32_t lt = (uint32_t)(t - pl.tuple_base[c]);
     uint32_t sz = 1;
 #pragma unroll
@@ -632,9 +635,8 @@ __global__ void k_svi_chunk_bytes(SviPlan pl, const uint64_t* payload, int with_
 // a bitmap when count > 128 and the bitmap has fewer bits than 10 per value
 // (minus 8 without the check); an unchecked bitmap chunk that is not sorted
 // counts in *err.
-__global__ void k_fc_plan(SviPlan pl, const uint64_t* payload, const uint32_t* first, const uint32_t* last,
-                          const uint32_t* ok, int with_check, uint32_t* is_bitmap, uint32_t* words, uint64_t* bytes,
-                          uint32_t* err) {
+__global__ void k_fc_kind(SviPlan pl, const uint32_t* first, const uint32_t* last, const uint32_t* ok, int with_check,
+                          uint32_t* is_bitmap, uint32_t* words, uint32_t* err) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= pl.nchunks) return;
     const uint64_t count = pl.counts[c];
@@ -646,12 +648,21 @@ __global__ void k_fc_plan(SviPlan pl, const uint64_t* payload, const uint32_t* f
         const uint64_t bits = (l < f) ? ~0ull : ((l - f + 63) / 64 * 8 + 8) * 8;
         if (with_check) use_bitmap = bits < min_bits && ok[c];
         else use_bitmap = bits < min_bits - 8 && l >= f;
-        if (!with_check && use_bitmap && !ok[c]) atomicAdd(err, 1u);
+        if (!with_check && use_bitmap && !ok[c] && err) atomicAdd(err, 1u);
         if (use_bitmap) nw = (l - f + 63) / 64;
     }
     is_bitmap[c] = use_bitmap ? 1u : 0u;
     words[c] = (uint32_t)nw;
-    bytes[c] = use_bitmap ? 8 + nw * 8 : (count ? 4 + payload[c] : 0);
+}
+
+// chunk bytes: a bitmap chunk's header and words, a StreamVInt chunk's count
+// and payload, nothing for an empty chunk
+__global__ void k_fc_bytes(SviPlan pl, const uint64_t* payload, const uint32_t* is_bitmap, const uint32_t* words,
+                           uint64_t* bytes) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= pl.nchunks) return;
+    const uint64_t count = pl.counts[c];
+    bytes[c] = is_bitmap[c] ? 8 + (uint64_t)words[c] * 8 : (count ? 4 + payload[c] : 0);
 }
 
 // One block per chunk, bitmap chunks of at most FC_LDS_WORDS words only: the
@@ -949,11 +960,13 @@ int svi_plan_all(const std::vector<uint64_t>& starts, hipStream_t s, DevBuf& tab
 }
 
 // tuple sizes and the per-chunk payload sums of a plan
-int svi_payload(const uint32_t* d_values, const SviPlan& pl, hipStream_t s, DevBuf& d_toff, DevBuf& d_pay) {
+int svi_payload(const uint32_t* d_values, const SviPlan& pl, hipStream_t s, DevBuf& d_toff, DevBuf& d_pay,
+                const uint32_t* skip = nullptr) {
     if (int r = d_toff.alloc((pl.ntuples + 1) * 4, s)) return r;
     if (int r = d_pay.alloc(((uint64_t)pl.nchunks + 1) * 8, s)) return r;
     if (pl.ntuples)
-        hipLaunchKernelGGL(k_svi_tuple_sizes, svi_grid(pl), dim3(256), 0, s, d_values, pl, d_toff.as<uint32_t>());
+        hipLaunchKernelGGL(k_svi_tuple_sizes, svi_grid(pl), dim3(256), 0, s, d_values, pl, d_toff.as<uint32_t>(),
+                           skip);
     if (pl.nchunks)
         hipLaunchKernelGGL(k_chunk_scan_u32<256>, dim3(pl.nchunks), dim3(256), 0, s, d_toff.as<uint32_t>(),
                            pl.tuple_base, d_pay.as<uint64_t>());
@@ -999,14 +1012,19 @@ int fc_encode_chunks_dev(const uint32_t* d_values, const std::vector<uint64_t>& 
     uint32_t* is_bm = ok + n;
     uint32_t* words = is_bm + n;
     uint32_t* wide = words + n;
-    if (n)
+    if (n) {
         hipLaunchKernelGGL(k_fc_inspect<1024>, dim3(n), dim3(1024), 0, s, d_values, pl.starts, first, last, ok);
-    if (int r = svi_payload(d_values, pl, s, d_toff, d_pay)) return r;
+        hipLaunchKernelGGL(k_fc_kind, dim3(blocks_for(n, 256)), dim3(256), 0, s, pl, (const uint32_t*)first,
+                           (const uint32_t*)last, (const uint32_t*)ok, with_check, is_bm, words, d_err);
+        PS_HIP_CHECK(hipGetLastError());
+    }
+    // StreamVInt sizes of the other chunks only
+    if (int r = svi_payload(d_values, pl, s, d_toff, d_pay, is_bm)) return r;
     if (int r = d_bytes.alloc(((uint64_t)n + 1) * 8, s)) return r;
     if (n)
-        hipLaunchKernelGGL(k_fc_plan, dim3(blocks_for(n, 256)), dim3(256), 0, s, pl,
-                           (const uint64_t*)d_pay.as<uint64_t>(), (const uint32_t*)first, (const uint32_t*)last,
-                           (const uint32_t*)ok, with_check, is_bm, words, d_bytes.as<uint64_t>(), d_err);
+        hipLaunchKernelGGL(k_fc_bytes, dim3(blocks_for(n, 256)), dim3(256), 0, s, pl,
+                           (const uint64_t*)d_pay.as<uint64_t>(), (const uint32_t*)is_bm, (const uint32_t*)words,
+                           d_bytes.as<uint64_t>());
     PS_HIP_CHECK(hipGetLastError());
     if (int r = scan_excl_u64(d_bytes.as<unsigned long long>(), n, (unsigned long long*)d_offsets, s)) return r;
     if (pl.ntuples)
