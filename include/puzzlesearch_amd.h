@@ -480,7 +480,10 @@ typedef struct ps_bfs_options {
                                in an undirected graph children of layer L lie in L-1..L+1) */
     int32_t direction;      /* PS_DIR_TOP_DOWN (default), PS_DIR_AUTO or PS_DIR_BOTTOM_UP;
                                batched mode with an LDS window only */
-    int32_t reserved;
+    int32_t fused_layers;   /* 1: one launch per layer finalizes and expands each tile
+                               (k_bfs3_tile_step, four bit arrays instead of three);
+                               batched three-bit PS_CROSS_ATOMIC top-down searches without
+                               skip_visited whose tiles lie inside one segment */
 } ps_bfs_options;
 
 typedef struct ps_bfs_stats {

@@ -71,7 +71,7 @@ class BfsOptions(ctypes.Structure):
                 ("tile_bits_log2", ctypes.c_int32), ("block_threads", ctypes.c_int32),
                 ("grid_blocks", ctypes.c_int32), ("hash", ctypes.c_int32), ("graph", ctypes.c_int32),
                 ("graph_layers", ctypes.c_int32), ("cross_marks", ctypes.c_int32), ("skip_visited", ctypes.c_int32),
-                ("direction", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("direction", ctypes.c_int32), ("fused_layers", ctypes.c_int32)]
 
 
 class BfsStats(ctypes.Structure):

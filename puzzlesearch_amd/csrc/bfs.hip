@@ -195,7 +195,23 @@ psk::LayerEndArgs layer_end_args(ps_bfs* b) {
     e.ntiles = b->ntiles;
     e.tile_bits = b->tile_words * 64;
     e.seg_shift = b->seg_shift;
+    e.rot4 = b->tstep;
+    e.list_only = 0;
+    e.touched_par[0] = b->touched;
+    e.touched_par[1] = b->touched1;
     return e;
+}
+
+psk::TileStepArgs tile_step_args(ps_bfs* b) {
+    psk::TileStepArgs t{};
+    for (int i = 0; i < 4; ++i) t.bits[i] = b->arr[i];
+    t.seg_count = b->seg_count;
+    t.touched[0] = b->touched;
+    t.touched[1] = b->touched1;
+    t.seg_words = b->seg_words;
+    t.odd = b->info.has_odd_cycles;
+    t.hash = b->opt.hash;
+    return t;
 }
 
 }  // namespace ps
@@ -240,6 +256,23 @@ int launch_batched_layer(ps_bfs* b, bool timed) {
                                b->nwords, b->st);
             PS_HIP_CHECK(hipGetLastError());
         }
+        return launch_layer_end(b);
+    }
+    if (b->tstep) {
+        if (!b->primed)
+            if (int r = prime_tstep(b)) return r;
+        size_t t0 = 0, t1 = 0;
+        if (timed)
+            if (int r = record(b, &t0)) return r;
+        psk::ExpandArgs a = expand_args(b);
+        a.active = b->tile_list;
+        LaunchCfg ct{b->tpb_expand, b->grid_tstep, b->lds_expand, b->stream};
+        if (int r = bfs3_tile_step(b->puzzle, a, tile_step_args(b), ct, nullptr)) return r;
+        if (timed) {
+            if (int r = record(b, &t1)) return r;
+            if (b->collect_kernel_times) b->expand_spans.push_back({t0, t1});
+        }
+        b->stats.expand_launches++;
         return launch_layer_end(b);
     }
     if (b->fused) {
@@ -341,6 +374,27 @@ int record(ps_bfs* b, size_t* idx) {
 }
 
 }  // namespace
+
+int prime_tstep(ps_bfs* b) {
+    size_t f0 = 0, f1 = 0;
+    if (int r = record(b, &f0)) return r;
+    // the start layer, expanded from arr[0] into arr[1] (the three-array roles
+    // at layer 0), its marks flagged in the touched bits of odd mark layers
+    psk::ExpandArgs a = expand_args(b);
+    a.active = b->tile_list;
+    a.touched = b->touched1;
+    LaunchCfg ce{b->tpb_expand, b->grid_expand, b->lds_expand, b->stream};
+    if (int r = bfs3_expand(b->puzzle, a, ce, nullptr)) return r;
+    if (int r = record(b, &f1)) return r;
+    if (b->collect_kernel_times) b->expand_spans.push_back({f0, f1});
+    b->stats.expand_launches++;
+    psk::LayerEndArgs e = layer_end_args(b);
+    e.list_only = 1;
+    hipLaunchKernelGGL(psk::k_bfs_layer_end<1024>, dim3(1), dim3(1024), 0, b->stream, e);
+    PS_HIP_CHECK(hipGetLastError());
+    b->primed = true;
+    return PS_OK;
+}
 
 int prime_fused(ps_bfs* b) {
     size_t f0 = 0, f1 = 0;
@@ -459,6 +513,8 @@ int destroy_bfs(ps_bfs* b) {
     if (b->ev_run0) (void)hipEventDestroy(b->ev_run0);
     if (b->ev_run1) (void)hipEventDestroy(b->ev_run1);
     // the two-bit search aliases arr[1] and arr[2] to arr[0]
+    if (b->arr[3]) (void)hipFree(b->arr[3]);
+    if (b->touched1) (void)hipFree(b->touched1);
     for (int i = 0; i < 3; ++i) {
         bool alias = false;
         for (int j = 0; j < i; ++j) alias = alias || b->arr[j] == b->arr[i];
@@ -556,7 +612,7 @@ void list_geometry(int n, uint32_t* tiles, uint32_t* slots, uint64_t* entries) {
 }
 
 uint64_t footprint_bytes(const ps_puzzle* p, const ps_bfs_options* o, const Geometry& g, int max_layers) {
-    uint64_t arrays = (o->bits == 2 ? 1 : 3) * g.nwords * 8;
+    uint64_t arrays = (o->bits == 2 ? 1 : o->fused_layers ? 4 : 3) * g.nwords * 8;
     if (o->bits == 3 && o->cross_marks == PS_CROSS_BYTES) arrays += g.nwords * 64;
     if (o->bits == 3 && o->cross_marks == PS_CROSS_LISTS && p->kind == PS_PANCAKE && p->size >= 10 && p->size <= 12) {
         uint32_t tiles, slots;
@@ -575,7 +631,8 @@ uint64_t footprint_bytes(const ps_puzzle* p, const ps_bfs_options* o, const Geom
         const uint64_t tw = o->tile_bits_log2 > 0 && o->tile_bits_log2 < 12 ? 1ull << std::max(o->tile_bits_log2 - 6, 0) : 64;
         arrays += (g.nwords + tw - 1) / tw * 4;
     }
-    const uint64_t meta = 3ull * g.nseg * 8 + ((g.nseg + 31) / 32) * 4 + 2ull * max_layers * 8 + sizeof(BfsDevState);
+    const uint64_t meta = (o->fused_layers ? 4ull : 3ull) * g.nseg * 8 + ((g.nseg + 31) / 32) * 4 * (o->fused_layers ? 2 : 1) +
+                          2ull * max_layers * 8 + sizeof(BfsDevState);
     return arrays + meta;
 }
 
@@ -586,6 +643,8 @@ int enqueue_layers(ps_bfs* b, int n) {
     b->collect_kernel_times = false;
     if (b->fused && !b->primed)
         if (int r = prime_fused(b)) return r;
+    if (b->tstep && !b->primed)
+        if (int r = prime_tstep(b)) return r;
     if (!use_graph) {
         for (int i = 0; i < n; ++i)
             if (int r = launch_batched_layer(b, false)) return r;
@@ -595,7 +654,7 @@ int enqueue_layers(ps_bfs* b, int n) {
     for (int i = 0; i < n; i += b->graph_layers) {
         PS_HIP_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
         b->stats.expand_launches += b->graph_layers;
-        b->stats.finalize_launches += b->bits == 3 && !b->fused ? b->graph_layers : 0;
+        b->stats.finalize_launches += b->bits == 3 && !b->fused && !b->tstep ? b->graph_layers : 0;
         b->stats.apply_launches += b->lists && !b->fused ? b->graph_layers : 0;
     }
     return PS_OK;
@@ -729,6 +788,15 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         if (opt.tile_bits_log2 != 0)
             return fail(PS_ERR_UNSUPPORTED, "PS_CROSS_LISTS uses its own tiles of 9! states (tile_bits_log2 = 0)");
     }
+    if (opt.fused_layers != 0 && opt.fused_layers != 1) return fail(PS_ERR_INVALID_ARGUMENT, "fused_layers must be 0 or 1");
+    if (opt.fused_layers &&
+        (opt.bits != 3 || opt.mode != PS_BFS_BATCHED || opt.cross_marks != PS_CROSS_ATOMIC ||
+         opt.direction != PS_DIR_TOP_DOWN || opt.skip_visited || opt.tile_bits_log2 < 0))
+        // a tile's expansion reads only its own frontier words and ORs every
+        // child outside its window straight into the next array
+        return fail(PS_ERR_UNSUPPORTED,
+                    "fused_layers covers batched three-bit PS_CROSS_ATOMIC top-down searches with an LDS window "
+                    "and without skip_visited");
     // the puzzle must have a device expander
     if (int r = dispatch<FAM_ALL>(*p, PS_EXPAND_FULL, [](auto) { return PS_OK; })) return r;
 
@@ -789,7 +857,7 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
                 if (int r = dalloc((void**)&b->visited, b->nwords * 8)) return cleanup(r);
         }
     }
-    if (int r = dalloc((void**)&b->seg_count, (uint64_t)3 * b->nseg * 8)) return cleanup(r);
+    if (int r = dalloc((void**)&b->seg_count, (uint64_t)(opt.fused_layers ? 4 : 3) * b->nseg * 8)) return cleanup(r);
     if (int r = dalloc((void**)&b->touched, (uint64_t)((b->nseg + 31) / 32) * 4)) return cleanup(r);
     if (int r = dalloc((void**)&b->layer_counts, (uint64_t)b->max_layers * 8)) return cleanup(r);
     if (int r = dalloc((void**)&b->layer_hashes, (uint64_t)b->max_layers * 8)) return cleanup(r);
@@ -884,6 +952,21 @@ int ps_bfs_create(const ps_puzzle* p, const ps_bfs_options* opt_in, ps_stream s,
         b->grid_fin = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * 8, b->nchunks));
         if (opt.mode == PS_BFS_BATCHED && !b->lists)
             if (int r = dalloc((void**)&b->tile_list, b->ntiles * 4)) return cleanup(r);
+        if (opt.fused_layers) {
+            if (b->nseg > 1 && b->seg_words % b->tile_words != 0)
+                return cleanup(fail(PS_ERR_INVALID_ARGUMENT,
+                                    "fused_layers needs tiles inside one segment (%llu-word tiles, %llu-word segments)",
+                                    (unsigned long long)b->tile_words, (unsigned long long)b->seg_words));
+            b->tstep = 1;
+            if (int r = dalloc((void**)&b->arr[3], b->nwords * 8)) return cleanup(r);
+            if (int r = dalloc((void**)&b->touched1, (uint64_t)((b->nseg + 31) / 32) * 4)) return cleanup(r);
+            int occ_t = 0;
+            LaunchCfg ct{b->tpb_expand, 0, b->lds_expand, b->stream};
+            if (int r = bfs3_tile_step(*p, psk::ExpandArgs{}, psk::TileStepArgs{}, ct, &occ_t)) return cleanup(r);
+            if (occ_t < 1) return cleanup(fail(PS_ERR_UNSUPPORTED, "tile step kernel does not fit (LDS %zu bytes)", b->lds_expand));
+            const int64_t want_t = opt.grid_blocks > 0 ? (int64_t)opt.grid_blocks : (int64_t)occ_t * cus;
+            b->grid_tstep = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(want_t, (int64_t)b->ntiles));
+        }
     }
     *out = b;
     return PS_OK;
@@ -921,8 +1004,12 @@ int ps_bfs_reset(ps_bfs* b, uint64_t start) {
         if (b->visited)
             if (int r = launch_fill(b->visited, 0, b->nwords * 8, b->stream)) return r;
     }
+    if (b->tstep) {
+        if (int r = launch_fill(b->arr[3], 0, b->nwords * 8, b->stream)) return r;
+        if (int r = launch_fill(b->touched1, 0, (uint64_t)((b->nseg + 31) / 32) * 4, b->stream)) return r;
+    }
     b->primed = false;
-    if (int r = launch_fill(b->seg_count, 0, (uint64_t)3 * b->nseg * 8, b->stream)) return r;
+    if (int r = launch_fill(b->seg_count, 0, (uint64_t)(b->tstep ? 4 : 3) * b->nseg * 8, b->stream)) return r;
     if (int r = launch_fill(b->touched, 0, (uint64_t)((b->nseg + 31) / 32) * 4, b->stream)) return r;
     // start state, layer 0
     uint64_t word_idx, word;
@@ -1010,6 +1097,8 @@ int ps_bfs_run(ps_bfs* b, uint64_t start, uint64_t* counts, uint64_t* hashes, in
     // a fused search expands its start layer before the (captured) layer loop
     if (b->fused && !b->primed)
         if (int r = prime_fused(b)) return r;
+    if (b->tstep && !b->primed)
+        if (int r = prime_tstep(b)) return r;
     if (use_graph)
         if (int r = build_graph(b)) return r;
     PS_HIP_CHECK(hipEventRecord(b->ev_run0, b->stream));
@@ -1022,7 +1111,7 @@ int ps_bfs_run(ps_bfs* b, uint64_t start, uint64_t* counts, uint64_t* hashes, in
         if (use_graph) {
             PS_HIP_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
             b->stats.expand_launches += b->graph_layers;
-            b->stats.finalize_launches += b->bits == 3 && !b->fused ? b->graph_layers : 0;
+            b->stats.finalize_launches += b->bits == 3 && !b->fused && !b->tstep ? b->graph_layers : 0;
             b->stats.apply_launches += b->lists && !b->fused ? b->graph_layers : 0;
         } else {
             if (int r = launch_batched_layer(b, true)) return r;
@@ -1068,7 +1157,7 @@ int ps_bfs_frontier(const ps_bfs* b, const uint64_t** d_words, uint64_t* nwords)
         *d_words = b->arr[0];
     } else {
         const uint32_t L = b->h_st->layer;
-        *d_words = b->arr[L % 3];
+        *d_words = b->arr[b->tstep ? L % 4 : L % 3];
     }
     *nwords = b->nwords;
     return PS_OK;

@@ -22,13 +22,14 @@ struct ps_bfs {
     bool own_stream = false;
     int bits = 3;
     uint64_t nbits = 0, nwords = 0;
-    uint64_t* arr[3] = {nullptr, nullptr, nullptr};
+    uint64_t* arr[4] = {nullptr, nullptr, nullptr, nullptr};  // arr[3]: fused_layers only
     uint8_t* xmark = nullptr;  // byte per state, three-bit search with PS_CROSS_BYTES
     uint32_t nseg = 1;
     int seg_shift = 63;
     uint64_t seg_words = 0;
     unsigned long long* seg_count = nullptr;
     uint32_t* touched = nullptr;
+    uint32_t* touched1 = nullptr;  // fused_layers: touched of the odd mark layers
     unsigned long long* layer_counts = nullptr;
     unsigned long long* layer_hashes = nullptr;
     BfsDevState* st = nullptr;
@@ -63,6 +64,9 @@ struct ps_bfs {
     // lists of two parities; the start layer is expanded by a first launch
     int fused = 0;
     bool primed = false;
+    // batched PS_CROSS_ATOMIC with fused_layers: one k_bfs3_tile_step per layer
+    int tstep = 0;
+    uint32_t grid_tstep = 0;
     uint32_t* xlists1 = nullptr;  // parity 1 (parity 0: xlists, xcounts, xflags)
     uint64_t* visited = nullptr;  // fused lists with PS_DIR_AUTO: every state finalized so far
     uint32_t grid_pull = 0;
@@ -101,6 +105,8 @@ psk::StepArgs step_args(ps_bfs* b, bool first);
 psk::PullArgs pull_args(ps_bfs* b);
 // expands the start layer of a fused search (once after each reset)
 int prime_fused(ps_bfs* b);
+// expands the start layer of a fused_layers search and lists the first tiles
+int prime_tstep(ps_bfs* b);
 // reads the device state into b->stats (stream idle)
 int collect_stats(ps_bfs* b);
 // enqueues one complete layer (expand, finalize, bookkeeping) on b->stream

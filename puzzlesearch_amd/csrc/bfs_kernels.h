@@ -154,6 +154,12 @@ struct LayerEndArgs {
     uint64_t ntiles;
     uint64_t tile_bits;  // states per tile
     int32_t seg_shift;
+    // k_bfs3_tile_step searches (four arrays, TileStepArgs): 1; touched_par as
+    // in TileStepArgs::touched.  list_only: build the next launch's tile list
+    // and nothing else (after the start layer's expansion).
+    int32_t rot4;
+    int32_t list_only;
+    uint32_t* touched_par[2];
 };
 
 struct Bfs2Args {
@@ -292,17 +298,37 @@ struct ExpandTally {
     unsigned long long children = 0, parents = 0, scan = 0;
 };
 
+// The arrays one tile's expansion reads and writes.
+struct TileRoles {
+    const uint64_t* cur;               // the frontier expanded
+    uint32_t* nxt;                     // the next layer's marks
+    const uint64_t* old;               // the layer before cur (skip_visited)
+    const unsigned long long* sc_cur;  // per-segment counts of cur; NULL: the caller knows the tile holds states
+    uint32_t* touched;                 // segments that receive marks
+};
+
+// Roles of the three rotating arrays at layer L.
+__device__ __forceinline__ TileRoles roles3(const ExpandArgs& a, uint32_t L) {
+    TileRoles r;
+    r.cur = a.bits[L % 3];
+    r.nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
+    r.old = a.bits[(L + 2) % 3];
+    r.sc_cur = a.seg_count + (uint64_t)(L % 3) * a.nseg;
+    r.touched = a.touched;
+    return r;
+}
+
 // Expands every parent of one tile of one search.  All threads of the block
 // call it with the same tile; it starts and ends with a block barrier.
 template <class PZ, int TPB>
-__device__ __forceinline__ void expand_tile(const ExpandArgs& a, uint64_t tile, const ExpandLds& lds,
-                                            uint32_t* s_local, ExpandTally& tally) {
+__device__ __forceinline__ void expand_tile_roles(const ExpandArgs& a, const TileRoles& rl, uint64_t tile,
+                                                  const ExpandLds& lds, uint32_t* s_local, ExpandTally& tally) {
     BfsDevState* st = a.st;
-    const uint32_t L = st->layer;
-    const uint64_t* __restrict__ cur = a.bits[L % 3];
-    uint32_t* nxt = reinterpret_cast<uint32_t*>(a.bits[(L + 1) % 3]);
-    const uint64_t* __restrict__ old = a.bits[(L + 2) % 3];
-    const unsigned long long* sc_cur = a.seg_count + (uint64_t)(L % 3) * a.nseg;
+    const uint64_t* __restrict__ cur = rl.cur;
+    uint32_t* nxt = rl.nxt;
+    const uint64_t* __restrict__ old = rl.old;
+    const unsigned long long* sc_cur = rl.sc_cur;
+    uint32_t* touched = rl.touched;
     constexpr int NW = TPB / 64;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     uint32_t* win = lds.win;
@@ -314,7 +340,7 @@ __device__ __forceinline__ void expand_tile(const ExpandArgs& a, uint64_t tile, 
     const uint64_t pw0 = w0 > a.word_lo ? w0 : a.word_lo;
     const uint64_t pw1 = w1 < a.word_hi ? w1 : a.word_hi;
     bool live = pw0 < pw1;
-    if (live && track) {
+    if (live && track && sc_cur) {
         const uint64_t s0 = (pw0 * 64) >> a.seg_shift, s1 = (pw1 * 64 - 1) >> a.seg_shift;
         live = false;
         for (uint64_t s = s0; s <= s1; ++s) live |= sc_cur[s] != 0;
@@ -356,7 +382,7 @@ __device__ __forceinline__ void expand_tile(const ExpandArgs& a, uint64_t tile, 
             if (track) {
                 const uint64_t s = child >> a.seg_shift;
                 if (lds.tl_words) lor(&lds.tl[s >> 5], 1u << (s & 31));
-                else gor(&a.touched[s >> 5], 1u << (s & 31));
+                else gor(&touched[s >> 5], 1u << (s & 31));
             }
         }
     };
@@ -418,10 +444,16 @@ __device__ __forceinline__ void expand_tile(const ExpandArgs& a, uint64_t tile, 
         const uint64_t s0 = win_lo >> a.seg_shift, s1 = (win_hi - 1) >> a.seg_shift;
         for (uint64_t s = s0; s <= s1; ++s) {
             if (lds.tl_words) lor(&lds.tl[s >> 5], 1u << (s & 31));
-            else gor(&a.touched[s >> 5], 1u << (s & 31));
+            else gor(&touched[s >> 5], 1u << (s & 31));
         }
     }
     __syncthreads();
+}
+
+template <class PZ, int TPB>
+__device__ __forceinline__ void expand_tile(const ExpandArgs& a, uint64_t tile, const ExpandLds& lds,
+                                            uint32_t* s_local, ExpandTally& tally) {
+    expand_tile_roles<PZ, TPB>(a, roles3(a, a.st->layer), tile, lds, s_local, tally);
 }
 
 // Bottom-up expansion of one tile (direction-optimizing BFS): every unvisited
@@ -607,12 +639,12 @@ __device__ __forceinline__ uint64_t fin_word(uint64_t v, uint64_t o, uint64_t c,
 // allocations).  XM: fold the cross-window mark bytes into new.
 constexpr int FIN_UNROLL = 4;
 
-template <int TPB, bool XM>
+template <int TPB, bool XM, int UN = FIN_UNROLL>
 __device__ __forceinline__ void finalize_words(const uint64_t* __restrict__ cur, uint64_t* __restrict__ nw,
                                                uint64_t* __restrict__ old, uint8_t* __restrict__ xmark, uint64_t w0,
                                                uint64_t w1, bool odd, bool hash, unsigned long long& cnt,
                                                unsigned long long& h) {
-    constexpr int U = XM ? 2 : FIN_UNROLL;
+    constexpr int U = XM ? 2 : UN;
     const uint64_t npair = (w1 - w0) / 2;  // w0 is even: 16-byte accesses are aligned
     for (uint64_t p0 = threadIdx.x; p0 < npair; p0 += (uint64_t)TPB * U) {
         ulonglong2 vn[U], vo[U], vc[U];
@@ -754,6 +786,121 @@ __global__ __launch_bounds__(TPB) void k_bfs3_finalize(FinalizeArgs a) {
     flush_fin_tally(st, tally);
 }
 
+// One layer of a batched three-bit search whose children outside the LDS
+// window are OR'ed into the next array (PS_CROSS_ATOMIC), with the finalize
+// and the expansion of each tile in one launch.  Four arrays rotate: the
+// frontier F_j and the marks M_j of layer j live in bits[j % 4].  The launch
+// at layer L finalizes M_{L+1} in place into F_{L+1} (new &= ~F_L [odd],
+// new &= ~F_{L-1}; F_{L-1}'s words cleared) and expands F_{L+1} into
+// bits[(L+2) % 4], which the launch before cleared, so no expansion writes an
+// array that another tile's finalize reads.  touched[j % 2] flags the
+// segments holding marks of layer j, seg_count[j % 4] holds F_j's counts.
+// Each tile lies inside one segment (or there is one segment); the tile list
+// (ExpandArgs::active) names every tile with marks or F_{L-1} states.
+struct TileStepArgs {
+    uint64_t* bits[4];
+    unsigned long long* seg_count;  // [4][nseg]
+    uint32_t* touched[2];
+    uint64_t seg_words;
+    int32_t odd;
+    int32_t hash;
+};
+
+template <class PZ, int TPB>
+__global__ __launch_bounds__(TPB) void k_bfs3_tile_step(ExpandArgs a, TileStepArgs t) {
+    extern __shared__ uint32_t smem[];
+    constexpr int NW = TPB / 64;
+    BfsDevState* st = a.st;
+    if (st->done) return;
+    const int tid = threadIdx.x;
+    const uint32_t L = st->layer;
+    uint64_t* mk = t.bits[(L + 1) & 3u];        // M_{L+1}, finalized into F_{L+1}
+    const uint64_t* cur = t.bits[L & 3u];       // F_L
+    uint64_t* old = t.bits[(L + 3) & 3u];       // F_{L-1}, cleared
+    unsigned long long* sc_new = t.seg_count + (uint64_t)((L + 1) & 3u) * a.nseg;
+    const unsigned long long* sc_old = t.seg_count + (uint64_t)((L + 3) & 3u) * a.nseg;
+    const uint32_t* tin = t.touched[(L + 1) & 1u];
+    TileRoles rl;
+    rl.cur = mk;
+    rl.nxt = reinterpret_cast<uint32_t*>(t.bits[(L + 2) & 3u]);
+    rl.old = nullptr;
+    rl.sc_cur = nullptr;
+    rl.touched = t.touched[L & 1u];
+    const bool track = a.nseg > 1;
+    const bool odd = t.odd != 0, hash = t.hash != 0;
+    ExpandLds lds;
+    lds.win = smem;
+    lds.cwin = nullptr;
+    lds.queues = smem + (uint32_t)(a.tile_words * 2);
+    lds.tl = lds.queues + NW * QCAP;
+    lds.tl_words = (track && a.nseg <= TOUCH_LDS_MAX) ? (a.nseg + 31) / 32 : 0u;
+    __shared__ unsigned long long s_tile, s_cnt;
+    __shared__ unsigned long long red[2][NW];
+    __shared__ uint32_t s_local;
+    for (uint32_t i = tid; i < lds.tl_words; i += TPB) lds.tl[i] = 0;
+
+    ExpandTally tally;
+    FinTally ft;
+    for (;;) {
+        if (tid == 0) {
+            const unsigned long long k = atomicAdd(&st->tile_counter, 1ull);
+            s_tile = k < st->active_tiles ? (uint64_t)a.active[k] : a.tile_end;
+        }
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        if (tile >= a.tile_end) break;
+        const uint64_t w0 = tile * a.tile_words;
+        const uint64_t w1 = (w0 + a.tile_words < a.nwords) ? w0 + a.tile_words : a.nwords;
+        const uint64_t sg = track ? w0 / t.seg_words : 0;
+        const bool touched = !track || ((tin[sg >> 5] >> (sg & 31)) & 1u);
+        const bool old_live = !track || sc_old[sg] != 0;
+        if (touched) {
+            unsigned long long cnt = 0, h = 0;
+            // two steps in flight: the expansion's registers bound the kernel
+            finalize_words<TPB, false, 2>(cur, mk, old, nullptr, w0, w1, odd, hash, cnt, h);
+            cnt = wave_sum(cnt);
+            h = wave_sum(h);
+            const int wave = tid / 64, lane = tid % 64;
+            if (lane == 0) {
+                red[0][wave] = cnt;
+                red[1][wave] = h;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                unsigned long long sc = 0, sh = 0;
+                for (int w = 0; w < NW; ++w) {
+                    sc += red[0][w];
+                    sh += red[1][w];
+                }
+                if (sc) atomicAdd(&sc_new[sg], sc);
+                ft.cnt += sc;
+                ft.hash += sh;
+                ft.full += w1 - w0;
+                s_cnt = sc;
+            }
+        } else {
+            if (old_live)
+                for (uint64_t j = w0 + tid; j < w1; j += TPB) old[j] = 0;
+            if (tid == 0) {
+                if (old_live) ft.clear += w1 - w0;
+                s_cnt = 0;
+            }
+        }
+        // the tile's new frontier words, written above by this block, are read
+        // by the expansion after the barrier
+        __syncthreads();
+        if (s_cnt) expand_tile_roles<PZ, TPB>(a, rl, tile, lds, &s_local, tally);
+        else __syncthreads();
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < lds.tl_words; i += TPB) {
+        const uint32_t v = lds.tl[i];
+        if (v) gor(&rl.touched[i], v);
+    }
+    block_add3<TPB>(tally.children, tally.parents, tally.scan, &st->children, &st->expanded, &st->scan_words);
+    flush_fin_tally(st, ft);
+}
+
 // Bookkeeping at the end of a layer, run by one block: record the layer, rotate
 // the roles, reset the per-layer counters.
 template <int TPB>
@@ -762,14 +909,23 @@ __device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
     if (st->done) return;
     const uint32_t L = st->layer;
     __shared__ uint32_t s_list;
-    if (a.bits == 3) {
+    if (a.rot4 && !a.list_only) {
+        // F_{L-1} was cleared by the launch just ended: its counts next serve
+        // F_{L+3}; the marks of layer L+1 are finalized
+        unsigned long long* sc_free = a.seg_count + (uint64_t)((L + 3) & 3u) * a.nseg;
+        for (uint32_t i = threadIdx.x; i < a.nseg; i += TPB) sc_free[i] = 0;
+        if (a.nseg > 1) {
+            uint32_t* t = a.touched_par[(L + 1) & 1u];
+            for (uint32_t i = threadIdx.x; i < (a.nseg + 31) / 32; i += TPB) t[i] = 0;
+        }
+    } else if (a.bits == 3 && !a.rot4) {
         unsigned long long* sc_next_new = a.seg_count + (uint64_t)((L + 2) % 3) * a.nseg;
         for (uint32_t i = threadIdx.x; i < a.nseg; i += TPB) sc_next_new[i] = 0;
         if (a.nseg > 1)
             for (uint32_t i = threadIdx.x; i < (a.nseg + 31) / 32; i += TPB) a.touched[i] = 0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && !a.list_only) {
         const unsigned long long cnt = st->acc_count;
         st->visited += cnt;
         // the next layer goes bottom-up when its frontier outnumbers the states
@@ -787,6 +943,8 @@ __device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
         }
         st->acc_count = 0;
         st->acc_hash = 0;
+    }
+    if (threadIdx.x == 0) {
         st->tile_counter = 0;
         st->chunk_counter = 0;
         s_list = (a.active && !st->done && !st->pull) ? 1u : 0u;
@@ -798,6 +956,11 @@ __device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
     constexpr int NW = TPB / 64;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const unsigned long long* sc = a.seg_count + (uint64_t)((L + 1) % 3) * a.nseg;
+    // a four-array launch at layer Ln finalizes the segments with marks of
+    // layer Ln+1 and clears those holding F_{Ln-1}
+    const uint32_t Ln = a.list_only ? L : L + 1;
+    const uint32_t* tmark = a.rot4 ? a.touched_par[(Ln + 1) & 1u] : nullptr;
+    if (a.rot4) sc = a.seg_count + (uint64_t)((Ln + 3) & 3u) * a.nseg;
     __shared__ uint32_t s_wsum[NW];
     uint32_t base = 0;
     for (uint64_t t0 = 0; t0 < a.ntiles; t0 += TPB) {
@@ -812,7 +975,8 @@ __device__ __forceinline__ void layer_end_body(const LayerEndArgs& a) {
                 const uint64_t s0 = b0 >> a.seg_shift, s1 = b1 >> a.seg_shift;
                 if (s1 - s0 >= 64) live = true;  // many small segments: visit the tile
                 else
-                    for (uint64_t sg = s0; sg <= s1 && !live; ++sg) live = sc[sg] != 0;
+                    for (uint64_t sg = s0; sg <= s1 && !live; ++sg)
+                        live = sc[sg] != 0 || (tmark && ((tmark[sg >> 5] >> (sg & 31)) & 1u));
             }
         }
         const unsigned long long m = __ballot(live);

@@ -19,6 +19,8 @@ struct LaunchCfg {
 // raise the kernel's dynamic-LDS limit); otherwise launch.
 #define PS_DECLARE_BFS_FAMILY(NAME)                                                                   \
     int bfs3_expand_##NAME(const ps_puzzle& p, const psk::ExpandArgs& a, const LaunchCfg& c, int* occ); \
+    int bfs3_tile_step_##NAME(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::TileStepArgs& t,  \
+                              const LaunchCfg& c, int* occ);                                             \
     int bfs2_layer_##NAME(const ps_puzzle& p, const psk::Bfs2Args& a, const LaunchCfg& c, int* occ);
 PS_DECLARE_BFS_FAMILY(pancake)
 PS_DECLARE_BFS_FAMILY(sliding)
@@ -47,6 +49,17 @@ inline int bfs3_expand(const ps_puzzle& p, const psk::ExpandArgs& a, const Launc
     case FAM_SLIDING: return bfs3_expand_sliding(p, a, c, occ);
     case FAM_PACKED: return bfs3_expand_packed(p, a, c, occ);
     case FAM_HANOI: return bfs3_expand_hanoi(p, a, c, occ);
+    }
+    return fail(PS_ERR_UNSUPPORTED, "unknown puzzle family");
+}
+
+inline int bfs3_tile_step(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::TileStepArgs& t,
+                          const LaunchCfg& c, int* occ) {
+    switch (family_of(p)) {
+    case FAM_PANCAKE: return bfs3_tile_step_pancake(p, a, t, c, occ);
+    case FAM_SLIDING: return bfs3_tile_step_sliding(p, a, t, c, occ);
+    case FAM_PACKED: return bfs3_tile_step_packed(p, a, t, c, occ);
+    case FAM_HANOI: return bfs3_tile_step_hanoi(p, a, t, c, occ);
     }
     return fail(PS_ERR_UNSUPPORTED, "unknown puzzle family");
 }
@@ -86,6 +99,23 @@ int launch_or_query(K kern, const LaunchCfg& c, int* occ, const void* args_dummy
             switch (c.tpb) {                                                                                    \
             case 256: return go(psk::k_bfs3_expand<PZ, 256>);                                                   \
             case 512: return go(psk::k_bfs3_expand<PZ, 512>);                                                   \
+            }                                                                                                   \
+            return fail(PS_ERR_INVALID_ARGUMENT, "block_threads %d not in {256, 512}", c.tpb);                 \
+        });                                                                                                     \
+    }                                                                                                           \
+    int bfs3_tile_step_##NAME(const ps_puzzle& p, const psk::ExpandArgs& a, const psk::TileStepArgs& t,        \
+                              const LaunchCfg& c, int* occ) {                                                   \
+        return dispatch<FAMV>(p, PS_EXPAND_FULL, [&](auto tag) -> int {                                         \
+            using PZ = typename decltype(tag)::type;                                                            \
+            auto go = [&](auto kern) -> int {                                                                   \
+                if (occ) return launch_or_query(kern, c, occ, nullptr);                                         \
+                hipLaunchKernelGGL(kern, dim3(c.grid), dim3(c.tpb), c.lds, c.stream, a, t);                     \
+                PS_HIP_CHECK(hipGetLastError());                                                                \
+                return PS_OK;                                                                                   \
+            };                                                                                                  \
+            switch (c.tpb) {                                                                                    \
+            case 256: return go(psk::k_bfs3_tile_step<PZ, 256>);                                                \
+            case 512: return go(psk::k_bfs3_tile_step<PZ, 512>);                                                \
             }                                                                                                   \
             return fail(PS_ERR_INVALID_ARGUMENT, "block_threads %d not in {256, 512}", c.tpb);                 \
         });                                                                                                     \

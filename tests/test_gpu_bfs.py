@@ -38,6 +38,13 @@ MODES = {
     "batched_auto_dir": dict(mode=psa.BFS_BATCHED, direction=psa.DIR_AUTO),
     "batched_bottom_up": dict(mode=psa.BFS_BATCHED, direction=psa.DIR_BOTTOM_UP),
     "batched_bottom_up_seg12": dict(mode=psa.BFS_BATCHED, direction=psa.DIR_BOTTOM_UP, segment_bits=12),
+    # finalize and expansion of each tile in one launch (four rotating arrays):
+    # one segment; one tile per segment; four tiles per segment, eager
+    "batched_fused": dict(mode=psa.BFS_BATCHED, cross_marks=psa.CROSS_ATOMIC, fused_layers=1),
+    "batched_fused_tile12_seg12": dict(mode=psa.BFS_BATCHED, cross_marks=psa.CROSS_ATOMIC, fused_layers=1,
+                                       segment_bits=12, tile_bits_log2=12),
+    "batched_fused_tile12_seg14_eager": dict(mode=psa.BFS_BATCHED, cross_marks=psa.CROSS_ATOMIC, fused_layers=1,
+                                             segment_bits=14, tile_bits_log2=12, graph=0, block_threads=256),
 }
 
 
@@ -165,13 +172,16 @@ FIX = _fixtures()
 
 @pytest.mark.parametrize("name", [n for n in ("pancake_10", "pancake_11", "hanoi_12", "hanoi_12_sym", "hanoi_13",
                                                 "sliding_packed_5x2", "sliding_packed_4x3") if n in FIX])
-@pytest.mark.parametrize("mode", ["batched_graph", "plain_seg20"])
+@pytest.mark.parametrize("mode", ["batched_graph", "plain_seg20", "fused_seg20"])
 def test_medium_against_oracle_fixture(ps, name, mode):
     rec = FIX[name]
     p = psa.Puzzle(**rec["puzzle"])
     opts = dict(MODES.get(mode, {}))
     if mode == "plain_seg20":
         opts = dict(mode=psa.BFS_PLAIN, segment_bits=20)
+    if mode == "fused_seg20":
+        # tiles of 2^16 states inside 2^20-state segments
+        opts = dict(cross_marks=psa.CROSS_ATOMIC, fused_layers=1, segment_bits=20, tile_bits_log2=16)
     with psa.Bfs(p, index_count=rec["index_count"], **opts) as b:
         counts, hashes = b.run(rec["start"])
     assert counts == rec["counts"]
@@ -262,6 +272,21 @@ def test_sliding_4x3_full(ps, golden, shape, cross):
     assert counts == golden["sliding_4x3"]["counts"]
     if shape in FIX:
         assert [str(x) for x in hashes] == FIX[shape]["hashes"]
+    if cross == 1:
+        # top-down, finalize and expansion of each tile fused (9!-state tiles
+        # inside one 12!-index segment)
+        with psa.Bfs(p, cross_marks=cross, fused_layers=1) as b:
+            assert b.run(0) == (counts, hashes)
+
+
+def test_fused_layers_refused_outside_its_scope(ps):
+    """fused_layers needs PS_CROSS_ATOMIC top-down batched searches whose
+    tiles lie inside one segment."""
+    p = psa.pancake(9)
+    for bad in (dict(cross_marks=psa.CROSS_BYTES), dict(direction=psa.DIR_AUTO), dict(skip_visited=1),
+                dict(mode=psa.BFS_PLAIN), dict(tile_bits_log2=-1), dict(segment_bits=12, tile_bits_log2=13)):
+        with pytest.raises(psa.PSError):
+            psa.Bfs(p, **{"cross_marks": psa.CROSS_ATOMIC, "fused_layers": 1, **bad})
 
 
 def test_hanoi16_full(ps):
@@ -277,6 +302,8 @@ def test_hanoi16_full(ps):
     assert st.expanded == 4 ** 16
     assert counts[:11] == [1, 3, 6, 12, 30, 30, 66, 96, 126, 210, 330]  # shared prefix with the 21-disk vector
     with psa.Bfs(psa.hanoi(16), segment_bits=20, block_threads=256, cross_marks=psa.CROSS_ATOMIC, graph=1) as b:
+        assert b.run(0) == (counts, hashes)
+    with psa.Bfs(psa.hanoi(16), segment_bits=20, cross_marks=psa.CROSS_ATOMIC, fused_layers=1) as b:
         assert b.run(0) == (counts, hashes)
     if "hanoi_16" in FIX:
         assert counts == FIX["hanoi_16"]["counts"]
