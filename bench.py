@@ -321,7 +321,7 @@ def seg3_step_bench(psa, steps, ppm=200000, seed=1234, old_seed=4321):
 
 TUNE_CANDIDATES = {"segment_bits": [22, 20, 24, 26], "block_threads": [512, 256], "grid_blocks": [0, 256, 1024],
                    "graph_layers": [4, 8, 16], "cross_marks": [0, 1], "tile_bits_log2": [0, 16],
-                   "skip_visited": [0], "direction": [0, 1]}
+                   "skip_visited": [0], "direction": [0, 1], "fused_layers": [0, 1]}
 # 10..12 pancakes: cross-tile lists first (bottom-up and other windows do not combine with them)
 PANCAKE_CANDIDATES = dict(TUNE_CANDIDATES, cross_marks=[2, 0, 1], block_threads=[0, 512, 256])
 
@@ -334,7 +334,8 @@ def kernel_rooflines(psa, opts, st):
     per-unit figures of the design notes times the units it processed) over
     its summed HIP-event time.  Returns {kernel: {...}}, dominant first."""
     out = {}
-    main = "k_pkl_step" if opts.get("cross_marks") == psa.CROSS_LISTS else "k_bfs3_expand"
+    main = ("k_pkl_step" if opts.get("cross_marks") == psa.CROSS_LISTS else
+            "k_bfs3_tile_step" if opts.get("fused_layers") else "k_bfs3_expand")
     if st.expand_launches:
         out[main] = {"launches": st.expand_launches, "ms_total": st.expand_ms, "bytes_total": st.expand_bytes}
     if st.pull_launches:
@@ -361,7 +362,9 @@ def tuned(psa, p, start, index_count=0, candidates=None):
 PROFILE_SETS = {
     "sliding4x3": (lambda psa: psa.sliding(4, 3),
                    dict(block_threads=512, segment_bits=20, cross_marks=1, direction=1)),
-    "hanoi16": (lambda psa: psa.hanoi(16), dict(block_threads=512, segment_bits=20, cross_marks=1)),
+    "hanoi16": (lambda psa: psa.hanoi(16), dict(block_threads=512, segment_bits=20, cross_marks=1, fused_layers=1)),
+    # the same search with split expand and finalize launches
+    "hanoi16_split": (lambda psa: psa.hanoi(16), dict(block_threads=512, segment_bits=20, cross_marks=1)),
 }
 
 

@@ -34,6 +34,7 @@ CANDIDATES = {
     "skip_visited": [0, 1],
     "graph_layers": [4, 1, 8, 16],
     "direction": [0, 1],
+    "fused_layers": [0, 1],
 }
 DEFAULTS = {k: v[0] for k, v in CANDIDATES.items()}
 
